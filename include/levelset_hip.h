@@ -1,0 +1,121 @@
+/* C-ABI of the MI355X level-set hot path (seeds, FIM redistancing,
+ * narrowband construction, upwind advection) on one GPU.
+ *
+ * Plain pointers and sizes only. Host buffers are caller-owned and copied in
+ * and out; the context owns all device memory. Every call returns LS_OK (0)
+ * or a negative LS_ERR_* code, with a message in ls_last_error(); nothing
+ * exits the process or throws across this boundary. One context per host
+ * thread; calls on one context are serialized by the caller.
+ */
+#ifndef LEVELSET_HIP_H
+#define LEVELSET_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ls_ctx ls_ctx;
+
+enum {
+  LS_OK = 0,
+  LS_ERR_ARG = -1,      /* invalid argument */
+  LS_ERR_STATE = -2,    /* call out of order (e.g. step before mesh) */
+  LS_ERR_HIP = -3,      /* HIP runtime error */
+  LS_ERR_LIMIT = -4,    /* a patch exceeds device limits; use smaller patches */
+  LS_ERR_INTERNAL = -5
+};
+
+enum { LS_F32 = 0, LS_F64 = 1 };
+enum { LS_MODE_ADVECT = 0, LS_MODE_MCF = 1 };
+
+/* Per-step parameters. Replaces the argument list of
+ * meshFIM3d::GenerateData(char*, int nsteps, float timestep, int inside_niter,
+ * int nside, int block_size, float bandwidth, int part_type, int metis_size,
+ * bool verbose) (src/core3d/include/meshFIM3d.h:32-36; 2D takes double
+ * timestep/bandwidth, src/core2d/include/meshFIM2d.h:36-39). */
+typedef struct ls_step_params {
+  double dt;           /* timestep (timeStep_) */
+  double bandwidth;    /* narrowband width (bandwidth_) */
+  double tol;          /* FIM convergence tolerance (SMALLNUM) */
+  double lambda_eps;   /* interior tet-solve regulariser (SMALLNUM in localSolverTet1) */
+  int32_t niter;       /* sweeps per patch visit (NUM_ITER: 5 in 3D, 10 in 2D) */
+  int32_t inside_niter;/* advection passes per step (insideIterations_) */
+  int32_t mode;        /* LS_MODE_ADVECT */
+  int32_t reserved;
+} ls_step_params;
+
+typedef struct ls_stats {
+  int32_t fim_iters;       /* active-list iterations of the FIM loop */
+  int32_t nb_count;        /* narrowband patches after the step */
+  int64_t patch_visits;    /* sum of active-list sizes */
+  int64_t probes;          /* neighbour probes */
+  int64_t nb_verts;        /* vertex-updates of the step (owned vertices of narrowband patches) */
+  int32_t n_seed_patches;
+  int32_t reserved;
+  double ms_seed, ms_fim, ms_nb, ms_advect, ms_total; /* HIP-event timings */
+} ls_stats;
+
+typedef struct ls_layout_info {
+  int64_t n_verts, n_cells, n_copies;
+  int32_t n_patches, max_patch_copies, max_patch_verts, max_patch_halo;
+  int32_t max_vertex_slots, dim;
+} ls_layout_info;
+
+/* Context lifetime. Replaces LevelSet's device selection and reset
+ * (src/LevelSet.cu:146-166); no device reset is performed. */
+int ls_create(ls_ctx** out, int device);
+void ls_destroy(ls_ctx* ctx);
+const char* ls_last_error(const ls_ctx* ctx);
+
+/* Mesh: dim 2 (triangles, FP64 path) or 3 (tetrahedra, FP32 path).
+ * xyz is n_verts x 3, cells is n_cells x (dim+1). Replaces
+ * LevelSet::initializeMesh (src/LevelSet.cu:111-144). */
+int ls_set_mesh(ls_ctx* ctx, int dim, int64_t n_verts, const double* xyz, int64_t n_cells,
+                const int32_t* cells);
+
+/* Partition + patch layout (meshFIM3d::InitPatches/InitPatches2/
+ * GenerateBlockNeighbors, src/core3d/cuda/meshFIM3d.cu:423-570).
+ * ls_partition_grid restates GraphPartition_Square (:269-352) for an
+ * L x W x H structured vertex numbering; ls_partition_rcb is the METIS
+ * stand-in (Partition_METIS, :354-421) with exactly n_parts parts;
+ * ls_partition_labels takes caller labels in [0, n_parts). */
+int ls_partition_grid(ls_ctx* ctx, int L, int W, int H, int bl, int bw, int bh);
+int ls_partition_rcb(ls_ctx* ctx, int n_parts);
+int ls_partition_labels(ls_ctx* ctx, const int32_t* labels, int n_parts);
+int ls_get_layout_info(ls_ctx* ctx, ls_layout_info* info);
+/* Any pointer may be NULL. vert_perm: N (new -> original vertex);
+ * copy_src: F (original element of each copy); vert_off/ele_off: P+1;
+ * copy_verts: F x (dim+1) renumbered vertex ids. */
+int ls_get_layout(ls_ctx* ctx, int32_t* vert_perm, int32_t* copy_src, int32_t* vert_off,
+                  int32_t* ele_off, int32_t* copy_verts);
+
+/* Initial level set, one value per vertex in the caller's order
+ * (LevelSet::initializeVertices, src/LevelSet.cu:52-80). dtype LS_F32/LS_F64. */
+int ls_set_phi(ls_ctx* ctx, const void* phi, int dtype);
+/* One velocity (x,y,z) per element (LevelSet::initializeAdvection,
+ * src/LevelSet.cu:82-108). patch_sigma = 1 reproduces the reference's use of
+ * the first element copy's velocity for a whole patch
+ * (src/core3d/include/meshFIM_kernels.h:51-53); 0 uses each element's own. */
+int ls_set_velocity(ls_ctx* ctx, const double* vel, int patch_sigma);
+
+/* One timestep: seeds, sign save, FIM, narrowband + sign restore, advection
+ * (loop body of meshFIM3d::GenerateData, src/core3d/cuda/meshFIM3d.cu:639-678;
+ * redistance3d::FindSeedPoint/ReInitTsign/GenerateData,
+ * src/core3d/include/redistance3d.h:27-39). stats may be NULL. When the new
+ * narrowband is empty, no advection runs and stats->nb_count is 0 (the
+ * reference ends its time loop there, meshFIM3d.cu:655-659). */
+int ls_step(ls_ctx* ctx, const ls_step_params* params, ls_stats* stats);
+
+/* Current level set in the caller's vertex order (kernel_compute_vertT_before_permute3d,
+ * src/core3d/include/meshFIM_kernels.h:661-670). */
+int ls_get_phi(ls_ctx* ctx, void* out, int dtype);
+/* Narrowband patch ids, sorted ascending; ids may be NULL to query *n. */
+int ls_get_narrowband(ls_ctx* ctx, int32_t* ids, int32_t* n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LEVELSET_HIP_H */

@@ -1,0 +1,786 @@
+// CPU oracle for the level-set per-timestep hot path.
+//
+// TEST INFRASTRUCTURE ONLY. This file is the checker: tests, the smoke check
+// and the CPU-baseline leg of the benchmark load it; the product never does.
+//
+// A plain C++ restatement of the reference scheme, templated on the real type:
+//   * Solver<double> is the FP64 host build (the parity oracle);
+//   * Solver<float> keeps the GPU's FP32 operation order, FP64 only for the
+//     tet solver's quadratic discriminant and roots, so GPU results can be
+//     compared with it bit for bit.
+// Every stage cites the reference function it restates. The patch layout is
+// rebuilt here with the reference's own steps (stable sorts over SoA arrays)
+// so the product's layout builder is checked independently.
+//
+// Parity pinning (see tests/): local solvers against brute-force Hopf-Lax
+// minimisation and textbook point/segment/triangle distances; FIM against
+// exact distances to planes; advection against exact translation of linear
+// data and the discrete maximum principle; layout against brute-force
+// definitions of the orderings.
+#include <omp.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+namespace {
+
+constexpr double kLarge = 1e9;  // LARGENUM (value external to the reference; 1e9 is exact in FP32)
+
+enum Label { SeedPoint = 0, ActivePoint = 1, AlivePoint = 2, FarPoint = 3 };  // redistance3d.h:22-25
+
+// Discriminant/root type: FP64 for both instances.
+template <typename T> struct Prec { using D = double; };
+
+// Triangle Hopf-Lax local solve at C from A, B (src/core2d/include/redistance_kernels.h:287-334),
+// unit speed. The reference calls sqrtf on doubles there; this uses the type's own sqrt.
+template <typename T>
+T tri_solve(T TA, T TB, T LenAB, T LenBC, T LenAC) {
+  const T big = (T)kLarge;
+  if (TA >= big && TB >= big) return big;
+  T TC1 = big, TC2 = big;
+  const T TAB = TB - TA;
+  const T a = (LenAB * LenAB - TAB * TAB) * LenAB * LenAB;
+  const T EdgeTA = TA + LenAC;
+  const T EdgeTB = TB + LenBC;
+  if (a > (T)0) {
+    const T cosA = (LenAC * LenAC + LenAB * LenAB - LenBC * LenBC) / ((T)2 * LenAC * LenAB);
+    const T b = (T)2 * LenAB * LenAC * cosA * (TAB * TAB - LenAB * LenAB);
+    const T delta = (T)4 * LenAC * LenAC * a * TAB * TAB * ((T)1 - cosA * cosA);
+    const T sq = std::sqrt(delta);
+    const T l1 = (-b + sq) / ((T)2 * a);
+    const T l2 = (-b - sq) / ((T)2 * a);
+    if (l1 >= (T)0 && l1 <= (T)1) {
+      const T LenAD = l1 * LenAB;
+      const T LenCD = std::sqrt(LenAC * LenAC + LenAD * LenAD - (T)2 * LenAC * LenAD * cosA);
+      TC1 = l1 * TAB + TA + LenCD;
+    }
+    if (l2 >= (T)0 && l2 <= (T)1) {
+      const T LenAD = l2 * LenAB;
+      const T LenCD = std::sqrt(LenAC * LenAC + LenAD * LenAD - (T)2 * LenAC * LenAD * cosA);
+      TC2 = l2 * TAB + TA + LenCD;
+    }
+    return std::fmin(TC2, std::fmin(TC1, std::fmin(EdgeTA, EdgeTB)));
+  }
+  return std::fmin(EdgeTA, EdgeTB);
+}
+
+// Roots of p*x^2 + q*x + r in the discriminant type; false when delta < 0.
+template <typename T>
+bool roots(T p, T q, T r, T& x1, T& x2) {
+  using D = typename Prec<T>::D;
+  const D dp = p, dq = q, dr = r;
+  const D delta = dq * dq - (D)4 * dp * dr;
+  if (!(delta >= 0.0)) return false;
+  const D sq = std::sqrt(delta);
+  x1 = (T)((-dq + sq) / ((D)2 * dp));
+  x2 = (T)((-dq - sq) / ((D)2 * dp));
+  return true;
+}
+
+// Tet Hopf-Lax local solve for D from A, B, C given six edge dot products
+// (src/core3d/include/redistance_kernels3d.h:124-285): three face (edge)
+// quadratics, three vertex candidates, and one interior quadratic whose second
+// barycentric coordinate is (s*l1 + t) / (h + leps).
+template <typename T>
+T tet_solve(T TA, T TB, T TC, T ACAC, T ACBC, T ACCD, T BCBC, T BCCD, T CDCD, T leps) {
+  const T big = (T)kLarge;
+  if (TA >= big && TB >= big && TC >= big) return big;
+  T FaceTAC = big, FaceTAB = big, FaceTBC = big;
+  T TD = big;
+  const T TAC = TC - TA, TBC = TC - TB, TAB = TB - TA;
+  T x1, x2;
+
+  // Edge B-C (lambda1 = 0).
+  {
+    const T p = BCBC * TBC * TBC - BCBC * BCBC;
+    const T q = (BCCD + BCCD) * TBC * TBC - (T)2 * BCBC * BCCD;
+    const T r = TBC * TBC * CDCD - BCCD * BCCD;
+    if (roots(p, q, r, x1, x2)) {
+      const T cand[2] = {x1, x2};
+      for (int k = 0; k < 2; ++k) {
+        const T l2 = cand[k];
+        const T l3 = (T)1 - l2;
+        if (l2 >= (T)0 && l2 <= (T)1) {
+          const T TE = TB * l2 + TC * l3;
+          FaceTBC = std::fmin(FaceTBC, TE + std::sqrt((l2 * BCBC + BCCD) * l2 + (l2 * BCCD + CDCD)));
+        }
+      }
+    }
+    FaceTBC = std::fmin(FaceTBC, std::fmin(TB + std::sqrt((BCBC + BCCD) + (BCCD + CDCD)), TC + std::sqrt(CDCD)));
+  }
+  // Edge A-B (lambda3 = 0).
+  {
+    const T gx = ACAC - ACBC, gy = ACBC - BCBC, gz = ACCD - BCCD;
+    const T t2 = TAB * TAB;
+    const T p = (t2 * ACAC - gx * gx) + (BCBC * t2 - gy * gy) - ((ACBC + ACBC) * t2 - (T)2 * gx * gy);
+    const T q = -(BCBC * t2 - gy * gy) * (T)2 + ((ACBC + ACBC) * t2 - (T)2 * gx * gy) +
+                ((ACCD + ACCD) * t2 - (T)2 * gx * gz) - ((BCCD + BCCD) * t2 - (T)2 * gy * gz);
+    const T r = (t2 * BCBC - gy * gy) + ((BCCD + BCCD) * t2 - (T)2 * gy * gz) + (t2 * CDCD - gz * gz);
+    if (roots(p, q, r, x1, x2)) {
+      const T cand[2] = {x1, x2};
+      for (int k = 0; k < 2; ++k) {
+        const T l1 = cand[k];
+        const T l2 = (T)1 - l1;
+        if (l1 >= (T)0 && l1 <= (T)1) {
+          const T TE = TA * l1 + TB * l2;
+          FaceTAB = std::fmin(FaceTAB, TE + std::sqrt((l1 * ACAC + l2 * ACBC + ACCD) * l1 +
+                                                      (l1 * ACBC + l2 * BCBC + BCCD) * l2 +
+                                                      (l1 * ACCD + l2 * BCCD + CDCD)));
+        }
+      }
+    }
+    FaceTAB = std::fmin(FaceTAB, std::fmin(TB + std::sqrt((BCBC + BCCD) + (BCCD + CDCD)),
+                                           TA + std::sqrt((ACAC + ACCD) + (ACCD + CDCD))));
+  }
+  // Edge A-C (lambda2 = 0).
+  {
+    const T p = ACAC * TAC * TAC - ACAC * ACAC;
+    const T q = (ACCD + ACCD) * TAC * TAC - (T)2 * ACAC * ACCD;
+    const T r = TAC * TAC * CDCD - ACCD * ACCD;
+    if (roots(p, q, r, x1, x2)) {
+      const T cand[2] = {x1, x2};
+      for (int k = 0; k < 2; ++k) {
+        const T l1 = cand[k];
+        const T l3 = (T)1 - l1;
+        if (l1 >= (T)0 && l1 <= (T)1) {
+          const T TE = TA * l1 + TC * l3;
+          FaceTAC = std::fmin(FaceTAC, TE + std::sqrt((l1 * ACAC + ACCD) * l1 + (l1 * ACCD + CDCD)));
+        }
+      }
+    }
+    FaceTAC = std::fmin(FaceTAC, std::fmin(TA + std::sqrt((ACAC + ACCD) + (ACCD + CDCD)), TC + std::sqrt(CDCD)));
+  }
+  // Interior of face ABC.
+  {
+    const T s = TAC * ACBC - TBC * ACAC;
+    const T t = TAC * BCCD - TBC * ACCD;
+    const T h = -(TAC * BCBC - TBC * ACBC);
+    const T t2 = TAC * TAC;
+    const T c1 = t2 * ACAC - ACAC * ACAC;
+    const T c2 = BCBC * t2 - ACBC * ACBC;
+    const T c3 = (ACBC + ACBC) * t2 - (T)2 * ACAC * ACBC;
+    const T c4 = (ACCD + ACCD) * t2 - (T)2 * ACAC * ACCD;
+    const T c5 = (BCCD + BCCD) * t2 - (T)2 * ACBC * ACCD;
+    const T c6 = t2 * CDCD - ACCD * ACCD;
+    const T p = c1 * h * h + c2 * s * s + c3 * s * h;
+    const T q = c2 * (T)2 * s * t + c3 * t * h + c4 * h * h + c5 * s * h;
+    const T r = c2 * t * t + c5 * t * h + c6 * h * h;
+    if (roots(p, q, r, x1, x2)) {
+      const T cand[2] = {x1, x2};
+      for (int k = 0; k < 2; ++k) {
+        const T l1 = cand[k];
+        const T l2 = (s * l1 + t) / (h + leps);
+        const T l3 = (T)1 - l1 - l2;
+        if (l1 >= (T)0 && l1 <= (T)1 && l2 >= (T)0 && l2 <= (T)1 && l3 >= (T)0 && l3 <= (T)1) {
+          const T TE = TA * l1 + TB * l2 + TC * l3;
+          TD = std::fmin(TD, TE + std::sqrt((l1 * ACAC + l2 * ACBC + ACCD) * l1 +
+                                            (l1 * ACBC + l2 * BCBC + BCCD) * l2 +
+                                            (l1 * ACCD + l2 * BCCD + CDCD)));
+        }
+      }
+    }
+  }
+  return std::fmin(TD, std::fmin(FaceTBC, std::fmin(FaceTAB, FaceTAC)));
+}
+
+// Edge dot products from the six local coordinates and the four permuted
+// solves of one element (src/core3d/include/redistance_kernels3d.h:373-414).
+template <typename T>
+struct TetGeom {
+  T ACAC, ACBC, ACCD, BCBC, BCCD, CDCD, ADAD, ADBD, ADCD, BDBD, CDBD;
+  explicit TetGeom(const T* L) {
+    ACBC = L[1] * (L[1] - L[0]) + L[2] * L[2];
+    BCCD = (L[1] - L[0]) * (L[3] - L[1]) + L[2] * (L[4] - L[2]);
+    ACCD = L[1] * (L[3] - L[1]) + L[2] * (L[4] - L[2]);
+    ADBD = L[3] * (L[3] - L[0]) + L[4] * L[4] + L[5] * L[5];
+    const T ACAD = L[1] * L[3] + L[2] * L[4];
+    const T BCBD = (L[1] - L[0]) * (L[3] - L[0]) + L[2] * L[4];
+    const T ADBC = ACBC + BCCD;
+    const T ACBD = ACBC + ACCD;
+    CDBD = ADBD - ACBD;
+    ADCD = ADBD - ADBC;
+    CDCD = ADCD - ACCD;
+    ADAD = ADCD + ACAD;
+    ACAC = ACAD - ACCD;
+    BDBD = BCBD + CDBD;
+    BCBC = BCBD - BCCD;
+  }
+  void update(T* v, T leps) const {  // v = slot values of corners A,B,C,D
+    const T TA = v[0], TB = v[1], TC = v[2], TD = v[3];
+    v[3] = std::fmin(TD, tet_solve(TA, TB, TC, ACAC, ACBC, ACCD, BCBC, BCCD, CDCD, leps));
+    v[0] = std::fmin(TA, tet_solve(TB, TD, TC, BCBC, -BCCD, -ACBC, CDCD, ACCD, ACAC, leps));
+    v[1] = std::fmin(TB, tet_solve(TA, TD, TC, ACAC, -ACCD, -ACBC, CDCD, BCCD, BCBC, leps));
+    v[2] = std::fmin(TC, tet_solve(TA, TB, TD, ADAD, ADBD, -ADCD, BDBD, -CDBD, CDCD, leps));
+  }
+};
+
+// Triangle edge lengths from the three local coordinates and the three
+// permuted solves (src/core2d/include/redistance_kernels.h:441-462).
+template <typename T>
+struct TriGeom {
+  T LenAB, LenBC, LenCA;
+  explicit TriGeom(const T* c) {
+    LenAB = std::fabs(c[0]);
+    LenBC = std::sqrt(c[2] * c[2] + (c[1] - c[0]) * (c[1] - c[0]));
+    LenCA = std::sqrt(c[1] * c[1] + c[2] * c[2]);
+  }
+  void update(T* v, T) const {
+    const T TA = v[0], TB = v[1], TC = v[2];
+    v[2] = std::fmin(TC, tri_solve(TA, TB, LenAB, LenBC, LenCA));
+    v[0] = std::fmin(TA, tri_solve(TB, TC, LenBC, LenCA, LenAB));
+    v[1] = std::fmin(TB, tri_solve(TC, TA, LenCA, LenAB, LenBC));
+  }
+};
+
+}  // namespace
+
+extern "C" {
+struct lso_params {
+  double dt;            // timestep
+  double bandwidth;     // narrowband width
+  double tol;           // FIM convergence tolerance (SMALLNUM)
+  double lambda_eps;    // interior-solve regulariser (SMALLNUM at redistance_kernels3d.h:258)
+  int niter;            // sweeps per patch visit (NUM_ITER)
+  int inside_niter;     // advection passes per step
+  int mode;             // 0: advection, 1: mean-curvature flow
+  int nthreads;         // OpenMP threads (0 = runtime default)
+};
+struct lso_stats {
+  int fim_iters;
+  int nb_count;
+  int64_t patch_visits;
+  int64_t probes;
+  int64_t nb_verts;
+  int n_seed_patches;
+  int pad;
+};
+}
+
+namespace {
+
+struct Base {
+  virtual ~Base() {}
+  virtual int set_phi(const double*) = 0;
+  virtual int set_velocity(const double*, int) = 0;
+  virtual int step(const lso_params&, lso_stats&) = 0;
+  virtual void get_phi(double*) const = 0;
+  virtual int get_nb(int32_t*) const = 0;
+  virtual void get_layout(int32_t*, int32_t*, int32_t*, int32_t*, int32_t*) const = 0;
+  virtual int64_t ncopies() const = 0;
+  virtual int nparts() const = 0;
+};
+
+template <typename T>
+struct Solver : Base {
+  int dim, nc;
+  int64_t N, E, F;
+  int P;
+  std::vector<double> xyz_perm;       // renumbered coordinates
+  std::vector<int32_t> vert_permute;  // new -> old
+  std::vector<int32_t> vert_ipermute; // old -> new
+  std::vector<int32_t> vert_offsets, ele_offsets;
+  std::vector<int32_t> ele_permute;   // copy -> original element
+  std::vector<int32_t> ele;           // SoA: ele[i*F + c], renumbered vertex ids
+  std::vector<T> coords;              // SoA local coordinates: coords[k*F + c]
+  std::vector<double> frames;         // per copy, for velocity projection
+  std::vector<T> sigma;               // SoA per-copy velocity in the local frame
+  std::vector<std::vector<int32_t>> lmem;  // per renumbered vertex: patch-local slots
+  std::vector<int32_t> bxadj, badj;   // patch adjacency
+  // State (renumbered order).
+  std::vector<T> phi, DT, DT_out, phi_out;
+  std::vector<int32_t> label, con;
+  std::vector<char> tsign;
+  std::vector<int32_t> narrowband;
+
+  Solver(int dim_, int64_t N_, const double* xyz, int64_t E_, const int32_t* cells,
+         const int32_t* labels, int P_)
+      : dim(dim_), nc(dim_ + 1), N(N_), E(E_), F(0), P(P_) {
+    // InitPatches: per-element distinct patch count and first-appearance labels
+    // (meshFIM_kernels.h:502-561), then sort copies by patch (meshFIM3d.cu:469).
+    std::vector<int32_t> npart(E);
+    for (int64_t e = 0; e < E; ++e) {
+      int n = 1;
+      const int p0 = labels[cells[e * nc]];
+      const int p1 = labels[cells[e * nc + 1]];
+      const int p2 = labels[cells[e * nc + 2]];
+      if (p1 != p0) n++;
+      if (p2 != p0 && p2 != p1) n++;
+      if (nc == 4) {
+        const int p3 = labels[cells[e * nc + 3]];
+        if (p3 != p0 && p3 != p1 && p3 != p2) n++;
+      }
+      npart[e] = n;
+    }
+    std::vector<int64_t> eoff(E + 1, 0);
+    for (int64_t e = 0; e < E; ++e) eoff[e + 1] = eoff[e] + npart[e];
+    F = eoff[E];
+    std::vector<int32_t> full_label(F), perm_pre(F);
+    for (int64_t e = 0; e < E; ++e) {
+      int64_t s = eoff[e];
+      int ps[4];
+      for (int i = 0; i < nc; ++i) ps[i] = labels[cells[e * nc + i]];
+      for (int64_t j = 0; j < npart[e]; ++j) perm_pre[s + j] = (int32_t)e;
+      full_label[s] = ps[0];
+      int i = 1;
+      if (ps[1] != ps[0]) full_label[s + i++] = ps[1];
+      if (ps[2] != ps[0] && ps[2] != ps[1]) full_label[s + i++] = ps[2];
+      if (nc == 4 && ps[3] != ps[0] && ps[3] != ps[1] && ps[3] != ps[2]) full_label[s + i++] = ps[3];
+    }
+    std::vector<int64_t> order(F);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int64_t a, int64_t b) { return full_label[a] < full_label[b]; });
+    ele_permute.resize(F);
+    ele_offsets.assign(P + 1, 0);
+    for (int64_t c = 0; c < F; ++c) {
+      ele_permute[c] = perm_pre[order[c]];
+      ele_offsets[full_label[order[c]] + 1]++;
+    }
+    for (int p = 0; p < P; ++p) ele_offsets[p + 1] += ele_offsets[p];
+    // InitPatches2: stable sort of vertices by label (meshFIM3d.cu:503-511).
+    vert_permute.resize(N);
+    std::iota(vert_permute.begin(), vert_permute.end(), 0);
+    std::stable_sort(vert_permute.begin(), vert_permute.end(),
+                     [&](int32_t a, int32_t b) { return labels[a] < labels[b]; });
+    vert_ipermute.resize(N);
+    for (int64_t v = 0; v < N; ++v) vert_ipermute[vert_permute[v]] = (int32_t)v;
+    vert_offsets.assign(P + 1, 0);
+    for (int64_t v = 0; v < N; ++v) vert_offsets[labels[v] + 1]++;
+    for (int p = 0; p < P; ++p) vert_offsets[p + 1] += vert_offsets[p];
+    xyz_perm.resize(3 * N);
+    for (int64_t v = 0; v < N; ++v)
+      for (int a = 0; a < 3; ++a) xyz_perm[3 * v + a] = xyz[3 * (int64_t)vert_permute[v] + a];
+    // kernel_ele_and_vert3d (meshFIM_kernels.h:420-446): renumbered SoA connectivity.
+    ele.resize(nc * F);
+    for (int64_t c = 0; c < F; ++c)
+      for (int i = 0; i < nc; ++i)
+        ele[i * F + c] = vert_ipermute[cells[(int64_t)ele_permute[c] * nc + i]];
+    // kernel_compute_local_coords3d / 2d (meshFIM_kernels.h:448-500,
+    // meshFIM2d_kernels.h:477-520), evaluated in FP64.
+    const int ncoord = dim == 3 ? 6 : 3;
+    coords.resize(ncoord * F);
+    frames.resize((dim == 3 ? 9 : 6) * F);
+    for (int64_t c = 0; c < F; ++c) local_frame(c);
+    // Gather CSR (meshFIM3d.cu:537-542): slot indices i*F + c stably sorted by
+    // vertex, then filtered per patch as in redistance_kernels3d.h:311-329.
+    std::vector<int64_t> mem(nc * F);
+    std::iota(mem.begin(), mem.end(), 0);
+    std::stable_sort(mem.begin(), mem.end(), [&](int64_t a, int64_t b) { return ele[a] < ele[b]; });
+    std::vector<int32_t> vpatch(N);
+    for (int p = 0; p < P; ++p)
+      for (int32_t v = vert_offsets[p]; v < vert_offsets[p + 1]; ++v) vpatch[v] = p;
+    lmem.assign(N, {});
+    for (int64_t k = 0; k < nc * F; ++k) {
+      const int64_t idx = mem[k];
+      const int32_t v = ele[idx];
+      const int64_t c = idx % F;
+      const int i = (int)(idx / F);
+      const int p = vpatch[v];
+      if (c >= ele_offsets[p] && c < ele_offsets[p + 1])
+        lmem[v].push_back((int32_t)((c - ele_offsets[p]) * nc + i));
+    }
+    // GenerateBlockNeighbors (meshFIM3d.cu:546-570): patch pairs over mesh edges.
+    std::vector<std::pair<int32_t, int32_t>> pairs;
+    for (int64_t e = 0; e < E; ++e)
+      for (int i = 0; i < nc; ++i)
+        for (int j = 0; j < nc; ++j) {
+          const int a = labels[cells[e * nc + i]], b = labels[cells[e * nc + j]];
+          if (a != b) pairs.push_back({a, b});
+        }
+    std::sort(pairs.begin(), pairs.end());
+    pairs.erase(std::unique(pairs.begin(), pairs.end()), pairs.end());
+    bxadj.assign(P + 1, 0);
+    for (auto& pr : pairs) bxadj[pr.first + 1]++;
+    for (int p = 0; p < P; ++p) bxadj[p + 1] += bxadj[p];
+    badj.resize(pairs.size());
+    for (size_t k = 0; k < pairs.size(); ++k) badj[k] = pairs[k].second;
+
+    phi.assign(N, (T)0); DT.assign(N, (T)kLarge); DT_out.assign(N, (T)0); phi_out.assign(N, (T)0);
+    label.assign(N, FarPoint); con.assign(N, 0); tsign.assign(N, 1);
+    sigma.assign((dim == 3 ? 3 : 2) * F, (T)0);
+  }
+
+  void local_frame(int64_t c) {
+    auto P3 = [&](int i) { return &xyz_perm[3 * (int64_t)ele[i * F + c]]; };
+    const double *A = P3(0), *B = P3(1), *C = P3(2);
+    double AB[3] = {B[0] - A[0], B[1] - A[1], B[2] - A[2]};
+    double AC[3] = {C[0] - A[0], C[1] - A[1], C[2] - A[2]};
+    auto cross = [](const double* a, const double* b, double* o) {
+      o[0] = a[1] * b[2] - a[2] * b[1];
+      o[1] = a[2] * b[0] - a[0] * b[2];
+      o[2] = a[0] * b[1] - a[1] * b[0];
+    };
+    auto dot = [](const double* a, const double* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; };
+    const double lenAB = std::sqrt(AB[0] * AB[0] + AB[1] * AB[1] + AB[2] * AB[2]);
+    double nrm[3];
+    cross(AB, AC, nrm);
+    if (dim == 3) {
+      const double* D = P3(3);
+      double AD[3] = {D[0] - A[0], D[1] - A[1], D[2] - A[2]};
+      const double lenN = std::sqrt(nrm[0] * nrm[0] + nrm[1] * nrm[1] + nrm[2] * nrm[2]);
+      double Z[3] = {nrm[0] / lenN, nrm[1] / lenN, nrm[2] / lenN};
+      double X[3] = {AB[0] / lenAB, AB[1] / lenAB, AB[2] / lenAB};
+      double Y[3];
+      cross(Z, X, Y);
+      const double v[6] = {lenAB, dot(AC, X), dot(AC, Y), dot(AD, X), dot(AD, Y), dot(AD, Z)};
+      for (int k = 0; k < 6; ++k) coords[k * F + c] = (T)v[k];
+      for (int a = 0; a < 3; ++a) {
+        frames[9 * c + a] = X[a]; frames[9 * c + 3 + a] = Y[a]; frames[9 * c + 6 + a] = Z[a];
+      }
+    } else {
+      double Y[3];
+      cross(nrm, AB, Y);
+      const double lenY = std::sqrt(Y[0] * Y[0] + Y[1] * Y[1] + Y[2] * Y[2]);
+      coords[0 * F + c] = (T)lenAB;
+      coords[1 * F + c] = (T)(dot(AC, AB) / lenAB);
+      coords[2 * F + c] = (T)(dot(AC, Y) / lenY);
+      for (int a = 0; a < 3; ++a) { frames[6 * c + a] = AB[a] / lenAB; frames[6 * c + 3 + a] = Y[a] / lenY; }
+    }
+  }
+
+  int set_phi(const double* p) override {
+    for (int64_t v = 0; v < N; ++v) phi[v] = (T)p[vert_permute[v]];
+    narrowband.clear();
+    return 0;
+  }
+  // patch_sigma = 1 reproduces the reference's use of the patch's first copy
+  // velocity for every element of the patch (meshFIM_kernels.h:51-53).
+  int set_velocity(const double* vel, int patch_sigma) override {
+    const int ns = dim == 3 ? 3 : 2, nf = dim == 3 ? 9 : 6;
+    std::vector<int32_t> first(F);
+    for (int p = 0; p < P; ++p)
+      for (int32_t c = ele_offsets[p]; c < ele_offsets[p + 1]; ++c) first[c] = ele_offsets[p];
+    for (int64_t c = 0; c < F; ++c) {
+      const double* s = &vel[3 * (int64_t)ele_permute[patch_sigma ? first[c] : c]];
+      for (int a = 0; a < ns; ++a) {
+        const double* ax = &frames[nf * c + 3 * a];
+        sigma[a * F + c] = (T)(s[0] * ax[0] + s[1] * ax[1] + s[2] * ax[2]);
+      }
+    }
+    return 0;
+  }
+  void get_phi(double* out) const override {
+    for (int64_t v = 0; v < N; ++v) out[vert_permute[v]] = (double)phi[v];
+  }
+  int get_nb(int32_t* ids) const override {
+    if (ids) std::copy(narrowband.begin(), narrowband.end(), ids);
+    return (int)narrowband.size();
+  }
+  void get_layout(int32_t* vp, int32_t* ep, int32_t* vo, int32_t* eo, int32_t* el) const override {
+    if (vp) std::copy(vert_permute.begin(), vert_permute.end(), vp);
+    if (ep) std::copy(ele_permute.begin(), ele_permute.end(), ep);
+    if (vo) std::copy(vert_offsets.begin(), vert_offsets.end(), vo);
+    if (eo) std::copy(ele_offsets.begin(), ele_offsets.end(), eo);
+    if (el) std::copy(ele.begin(), ele.end(), el);
+  }
+  int64_t ncopies() const override { return F; }
+  int nparts() const override { return P; }
+
+  // Seeds (kernel_seedlabel / kernel_seedlabel_narrowband,
+  // redistance_kernels3d.h:623-770; FindSeedPoint redistance3d.cu:21-65).
+  // Returns the sorted list of patches holding a seed element.
+  std::vector<int32_t> find_seeds() {
+    std::fill(DT.begin(), DT.end(), (T)kLarge);
+    std::fill(label.begin(), label.end(), (int32_t)FarPoint);
+    std::vector<int32_t> scan;
+    if (narrowband.empty()) { scan.resize(P); std::iota(scan.begin(), scan.end(), 0); }
+    else scan = narrowband;
+    std::vector<char> seeded(P, 0);
+    for (int32_t p : scan) {
+      for (int32_t c = ele_offsets[p]; c < ele_offsets[p + 1]; ++c) {
+        int32_t vid[4]; T val[4];
+        bool allpos = true, allneg = true;
+        for (int i = 0; i < nc; ++i) {
+          vid[i] = ele[i * F + c];
+          val[i] = phi[vid[i]];
+          allpos = allpos && val[i] >= (T)0;
+          allneg = allneg && val[i] <= (T)0;
+        }
+        for (int i = 0; i < nc; ++i)
+          if (val[i] == (T)0) { label[vid[i]] = SeedPoint; seeded[p] = 1; DT[vid[i]] = std::fabs(val[i]); }
+        if (!(allpos || allneg)) {
+          for (int i = 0; i < nc; ++i) { label[vid[i]] = SeedPoint; DT[vid[i]] = std::fabs(val[i]); }
+          seeded[p] = 1;
+        }
+      }
+    }
+    std::vector<int32_t> act;
+    for (int p = 0; p < P; ++p) if (seeded[p]) act.push_back(p);
+    return act;
+  }
+
+  // FIM patch sweep (kernel_update_values / kernel_run_check_neghbor,
+  // redistance_kernels3d.h:287-621). Reads DT, writes con and (if write_out)
+  // DT_out for owned non-seed vertices.
+  void update_patch(int p, int niter, bool write_out, T tol, T leps) {
+    const int32_t vs = vert_offsets[p], ve = vert_offsets[p + 1];
+    const int32_t es = ele_offsets[p], ee = ele_offsets[p + 1];
+    const int ne = ee - es, nv = ve - vs;
+    std::vector<T> s(nc * ne);
+    for (int c = 0; c < ne; ++c)
+      for (int i = 0; i < nc; ++i) s[c * nc + i] = DT[ele[i * F + es + c]];
+    std::vector<T> oldT(nv), newT(nv);
+    for (int v = 0; v < nv; ++v) oldT[v] = newT[v] = DT[vs + v];
+    const int ncoord = dim == 3 ? 6 : 3;
+    for (int it = 0; it < niter; ++it) {
+      for (int v = 0; v < nv; ++v)
+        if (!lmem[vs + v].empty()) oldT[v] = s[lmem[vs + v][0]];
+      for (int c = 0; c < ne; ++c) {
+        T Lc[6];
+        for (int k = 0; k < ncoord; ++k) Lc[k] = coords[k * F + es + c];
+        if (dim == 3) TetGeom<T>(Lc).update(&s[c * nc], leps);
+        else TriGeom<T>(Lc).update(&s[c * nc], leps);
+      }
+      for (int v = 0; v < nv; ++v) {
+        const std::vector<int32_t>& lm = lmem[vs + v];
+        if (lm.empty()) continue;
+        T m = s[lm[0]];
+        for (size_t k = 1; k < lm.size(); ++k) m = std::fmin(s[lm[k]], m);
+        for (int32_t l : lm) s[l] = m;
+        newT[v] = m;
+      }
+    }
+    for (int v = 0; v < nv; ++v) {
+      if (label[vs + v] != SeedPoint) {
+        if (write_out) DT_out[vs + v] = newT[v];
+        con[vs + v] = (oldT[v] - newT[v] < tol) ? 1 : 0;
+      } else {
+        con[vs + v] = 1;
+      }
+    }
+  }
+
+  // Host active-list loop (redistance3d::GenerateData, redistance3d.cu:67-224).
+  void fim(std::vector<int32_t> active, const lso_params& prm, lso_stats& st) {
+    const T tol = (T)prm.tol, leps = (T)prm.lambda_eps, bw = (T)prm.bandwidth;
+    DT_out = DT;
+    std::fill(con.begin(), con.end(), 0);
+    std::vector<char> bcon(P, 0);
+    std::vector<T> bmin(P, (T)kLarge);
+    std::vector<int> blabel(P, FarPoint);
+    while (!active.empty()) {
+      st.fim_iters++;
+      st.patch_visits += (int64_t)active.size();
+      #pragma omp parallel for schedule(dynamic, 1)
+      for (int64_t k = 0; k < (int64_t)active.size(); ++k)
+        update_patch(active[k], prm.niter, true, tol, leps);
+      for (int32_t p : active) {  // CopyOutBack + run_reduction_bandwidth
+        int all = 1;
+        T m = (T)kLarge;
+        for (int32_t v = vert_offsets[p]; v < vert_offsets[p + 1]; ++v) {
+          DT[v] = DT_out[v];
+          all &= con[v] ? 1 : 0;
+          m = std::fmin(m, DT_out[v]);
+        }
+        bcon[p] = (char)all;
+        bmin[p] = m;
+      }
+      for (int32_t p : active) blabel[p] = bcon[p] ? FarPoint : ActivePoint;
+      std::vector<int32_t> fresh;
+      for (int32_t p : active) {
+        if (bcon[p] && bmin[p] < bw) {
+          for (int32_t k = bxadj[p]; k < bxadj[p + 1]; ++k) {
+            const int32_t q = badj[k];
+            if (blabel[q] == FarPoint) { blabel[q] = ActivePoint; fresh.push_back(q); }
+          }
+        }
+      }
+      std::vector<int32_t> next;
+      for (int32_t p : active) if (!bcon[p]) next.push_back(p);
+      if (!fresh.empty()) {
+        st.probes += (int64_t)fresh.size();
+        #pragma omp parallel for schedule(dynamic, 1)
+        for (int64_t k = 0; k < (int64_t)fresh.size(); ++k)
+          update_patch(fresh[k], 1, false, tol, leps);
+        for (int32_t q : fresh) {  // run_reduction3d
+          int all = 1;
+          for (int32_t v = vert_offsets[q]; v < vert_offsets[q + 1]; ++v) all &= con[v] ? 1 : 0;
+          bcon[q] = (char)all;
+        }
+        for (int32_t q : fresh) {
+          if (!bcon[q]) next.push_back(q);
+          else blabel[q] = FarPoint;
+        }
+      }
+      std::sort(next.begin(), next.end());
+      active.swap(next);
+    }
+  }
+
+  // Narrowband (kernel_compute_new_narrowband, redistance_kernels3d.h:22-55)
+  // and sign restore (redistance3d.cu:262-272).
+  void narrowband_and_restore(const lso_params& prm, lso_stats& st) {
+    const T bw = (T)prm.bandwidth;
+    narrowband.clear();
+    for (int p = 0; p < P; ++p) {
+      T m = (T)kLarge;
+      for (int32_t v = vert_offsets[p]; v < vert_offsets[p + 1]; ++v) m = std::fmin(m, DT[v]);
+      if (m <= bw) narrowband.push_back(p);
+    }
+    for (int64_t v = 0; v < N; ++v) phi[v] = DT[v] * (T)(2 * (int)tsign[v] - 1);
+    st.nb_count = (int)narrowband.size();
+    st.nb_verts = 0;
+    for (int32_t p : narrowband) st.nb_verts += vert_offsets[p + 1] - vert_offsets[p];
+  }
+
+  // Upwind residual-distribution advection (kernel_updateT_single_stage3d,
+  // meshFIM_kernels.h:36-408; 2D meshFIM2d_kernels.h:37-362). Shape-function
+  // gradients are taken in each copy's local frame, where they are
+  // closed-form, instead of from the 4x4 cofactor inverse of global
+  // coordinates; K_i = vol * sigma . grad N_i is frame independent.
+  void advect_patch(int p, const lso_params& prm) {
+    const int32_t vs = vert_offsets[p], ve = vert_offsets[p + 1];
+    const int32_t es = ele_offsets[p], ee = ele_offsets[p + 1];
+    const int ne = ee - es;
+    const T dt = (T)prm.dt;
+    std::vector<T> s_up(nc * ne), s_dn(nc * ne);
+    for (int c = 0; c < ne; ++c) {
+      const int64_t g = es + c;
+      T ph[4], at[4] = {0, 0, 0, 0}, H, vol;
+      for (int i = 0; i < nc; ++i) ph[i] = phi[ele[i * F + g]];
+      if (dim == 3) {
+        const T L0 = coords[0 * F + g], L1 = coords[1 * F + g], L2 = coords[2 * F + g];
+        const T L3 = coords[3 * F + g], L4 = coords[4 * F + g], L5 = coords[5 * F + g];
+        const T sg[3] = {sigma[0 * F + g], sigma[1 * F + g], sigma[2 * F + g]};
+        vol = std::fabs(L0 * L2 * L5) / (T)6;
+        T gr[4][3];
+        gr[1][0] = (T)1 / L0;
+        gr[1][1] = -L1 / (L0 * L2);
+        gr[1][2] = (L1 * L4 - L2 * L3) / (L0 * L2 * L5);
+        gr[2][0] = (T)0;
+        gr[2][1] = (T)1 / L2;
+        gr[2][2] = -L4 / (L2 * L5);
+        gr[3][0] = (T)0;
+        gr[3][1] = (T)0;
+        gr[3][2] = (T)1 / L5;
+        for (int k = 0; k < 3; ++k) gr[0][k] = -((gr[1][k] + gr[2][k]) + gr[3][k]);
+        T K[4];
+        for (int i = 0; i < 4; ++i) K[i] = vol * (sg[0] * gr[i][0] + sg[1] * gr[i][1] + sg[2] * gr[i][2]);
+        nscheme(K, ph, 4, H, at);
+      } else {
+        const T c0 = coords[0 * F + g], c1 = coords[1 * F + g], c2 = coords[2 * F + g];
+        const T sg[2] = {sigma[0 * F + g], sigma[1 * F + g]};
+        vol = std::fabs(c0 * c2) / (T)2;
+        T gr[3][2];
+        gr[1][0] = (T)1 / c0;
+        gr[1][1] = -c1 / (c0 * c2);
+        gr[2][0] = (T)0;
+        gr[2][1] = (T)1 / c2;
+        for (int k = 0; k < 2; ++k) gr[0][k] = -(gr[1][k] + gr[2][k]);
+        T K[3];
+        for (int i = 0; i < 3; ++i) K[i] = vol * (sg[0] * gr[i][0] + sg[1] * gr[i][1]);
+        nscheme(K, ph, 3, H, at);
+      }
+      for (int i = 0; i < nc; ++i) {
+        s_up[c * nc + i] = at[i] * H;
+        s_dn[c * nc + i] = at[i] * vol;
+      }
+    }
+    for (int32_t v = vs; v < ve; ++v) {
+      T up = (T)0, dn = (T)0;
+      for (int32_t l : lmem[v]) up += s_up[l];
+      for (int32_t l : lmem[v]) dn += s_dn[l];
+      const T old = phi[v];
+      phi_out[v] = (std::fabs(dn) > (T)1e-8) ? old - dt * (up / dn) : old;
+    }
+  }
+
+  // N-scheme distribution coefficients (meshFIM_kernels.h:227-273).
+  static void nscheme(const T* K, const T* ph, int n, T& H, T* at) {
+    T Kp[4], Km[4], beta = (T)0;
+    H = (T)0;
+    for (int i = 0; i < n; ++i) {
+      H += K[i] * ph[i];
+      Kp[i] = std::fmax(K[i], (T)0);
+      Km[i] = std::fmin(K[i], (T)0);
+      beta += Km[i];
+    }
+    beta = (T)1 / beta;
+    for (int i = 0; i < n; ++i) at[i] = (T)0;
+    if (std::fabs(H) > (T)1e-8) {
+      T alpha[4], theta = (T)0;
+      for (int i = 0; i < n; ++i) {
+        T acc = (T)0;
+        for (int j = 0; j < n; ++j) acc += Km[j] * (ph[i] - ph[j]);
+        alpha[i] = (Kp[i] * beta * acc) / H;
+      }
+      for (int i = 0; i < n; ++i) theta += std::fmax((T)0, alpha[i]);
+      for (int i = 0; i < n; ++i) at[i] = std::fmax(alpha[i], (T)0) / theta;
+    }
+  }
+
+  // One timestep (meshFIM3d::GenerateData loop body, meshFIM3d.cu:639-678).
+  int step(const lso_params& prm, lso_stats& st) override {
+    if (prm.nthreads > 0) omp_set_num_threads(prm.nthreads);
+    std::memset(&st, 0, sizeof(st));
+    std::vector<int32_t> act = find_seeds();
+    st.n_seed_patches = (int)act.size();
+    for (int64_t v = 0; v < N; ++v) tsign[v] = (phi[v] < (T)0) ? 0 : 1;  // kernel_reinit_Tsign
+    fim(act, prm, st);
+    narrowband_and_restore(prm, st);
+    if (narrowband.empty()) return 0;
+    for (int it = 0; it < prm.inside_niter; ++it) {
+      #pragma omp parallel for schedule(dynamic, 1)
+      for (int64_t k = 0; k < (int64_t)narrowband.size(); ++k) advect_patch(narrowband[k], prm);
+      for (int32_t p : narrowband)  // CopyOutBack_levelset3d
+        for (int32_t v = vert_offsets[p]; v < vert_offsets[p + 1]; ++v) phi[v] = phi_out[v];
+    }
+    return 0;
+  }
+};
+
+}  // namespace
+
+extern "C" {
+
+void* lso_create(int dim, int precision, int64_t N, const double* xyz, int64_t E,
+                 const int32_t* cells, const int32_t* labels, int nparts) {
+  if (dim != 2 && dim != 3) return nullptr;
+  if (precision == 64) return new Solver<double>(dim, N, xyz, E, cells, labels, nparts);
+  if (precision == 32) return new Solver<float>(dim, N, xyz, E, cells, labels, nparts);
+  return nullptr;
+}
+void lso_destroy(void* h) { delete static_cast<Base*>(h); }
+int lso_set_phi(void* h, const double* phi) { return static_cast<Base*>(h)->set_phi(phi); }
+int lso_set_velocity(void* h, const double* vel, int patch_sigma) {
+  return static_cast<Base*>(h)->set_velocity(vel, patch_sigma);
+}
+int lso_step(void* h, const lso_params* prm, lso_stats* st) { return static_cast<Base*>(h)->step(*prm, *st); }
+void lso_get_phi(void* h, double* out) { static_cast<Base*>(h)->get_phi(out); }
+int lso_get_narrowband(void* h, int32_t* ids) { return static_cast<Base*>(h)->get_nb(ids); }
+int64_t lso_num_copies(void* h) { return static_cast<Base*>(h)->ncopies(); }
+void lso_get_layout(void* h, int32_t* vert_perm, int32_t* ele_perm, int32_t* vert_off,
+                    int32_t* ele_off, int32_t* ele_soa) {
+  static_cast<Base*>(h)->get_layout(vert_perm, ele_perm, vert_off, ele_off, ele_soa);
+}
+
+// Unit-level local solves for tests. T = {TA, TB, TC}; d = {ACAC, ACBC, ACCD, BCBC, BCCD, CDCD}.
+double lso_tet_solve(int precision, const double* T, const double* d, double leps) {
+  if (precision == 32)
+    return tet_solve<float>((float)T[0], (float)T[1], (float)T[2], (float)d[0], (float)d[1],
+                            (float)d[2], (float)d[3], (float)d[4], (float)d[5], (float)leps);
+  return tet_solve<double>(T[0], T[1], T[2], d[0], d[1], d[2], d[3], d[4], d[5], leps);
+}
+double lso_tri_solve(int precision, double TA, double TB, double AB, double BC, double AC) {
+  if (precision == 32) return tri_solve<float>((float)TA, (float)TB, (float)AB, (float)BC, (float)AC);
+  return tri_solve<double>(TA, TB, AB, BC, AC);
+}
+// Element update of one tet from six local coordinates (v = 4 slot values, in/out).
+void lso_tet_element_update(int precision, const double* L, double* v, double leps) {
+  if (precision == 32) {
+    float Lf[6], vf[4];
+    for (int k = 0; k < 6; ++k) Lf[k] = (float)L[k];
+    for (int k = 0; k < 4; ++k) vf[k] = (float)v[k];
+    TetGeom<float>(Lf).update(vf, (float)leps);
+    for (int k = 0; k < 4; ++k) v[k] = vf[k];
+  } else {
+    TetGeom<double>(L).update(v, leps);
+  }
+}
+int lso_max_threads() { return omp_get_max_threads(); }
+
+}  // extern "C"

@@ -1,0 +1,153 @@
+"""ctypes binding of the CPU oracle (liblsoracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, the smoke check and the CPU
+baseline leg of bench.py. The product path never imports this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liblsoracle.so")
+
+
+class Params(C.Structure):
+    _fields_ = [("dt", C.c_double), ("bandwidth", C.c_double), ("tol", C.c_double),
+                ("lambda_eps", C.c_double), ("niter", C.c_int), ("inside_niter", C.c_int),
+                ("mode", C.c_int), ("nthreads", C.c_int)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("fim_iters", C.c_int), ("nb_count", C.c_int), ("patch_visits", C.c_int64),
+                ("probes", C.c_int64), ("nb_verts", C.c_int64), ("n_seed_patches", C.c_int),
+                ("pad", C.c_int)]
+
+
+def build() -> str:
+    subprocess.check_call(["make", "-s", "-C", _HERE, "liblsoracle.so"])
+    return LIB_PATH
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        dp = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+        ip = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+        L.lso_create.restype = vp
+        L.lso_create.argtypes = [C.c_int, C.c_int, C.c_int64, dp, C.c_int64, ip, ip, C.c_int]
+        L.lso_destroy.argtypes = [vp]
+        L.lso_set_phi.argtypes = [vp, dp]
+        L.lso_set_velocity.argtypes = [vp, dp, C.c_int]
+        L.lso_step.argtypes = [vp, C.POINTER(Params), C.POINTER(Stats)]
+        L.lso_get_phi.argtypes = [vp, dp]
+        L.lso_get_narrowband.argtypes = [vp, C.c_void_p]
+        L.lso_num_copies.restype = C.c_int64
+        L.lso_num_copies.argtypes = [vp]
+        L.lso_get_layout.argtypes = [vp] + [C.c_void_p] * 5
+        L.lso_tet_solve.restype = C.c_double
+        L.lso_tet_solve.argtypes = [C.c_int, dp, dp, C.c_double]
+        L.lso_tri_solve.restype = C.c_double
+        L.lso_tri_solve.argtypes = [C.c_int] + [C.c_double] * 5
+        L.lso_tet_element_update.argtypes = [C.c_int, dp, dp, C.c_double]
+        L.lso_max_threads.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def make_params(dt, bandwidth, niter, tol=1e-5, lambda_eps=1e-5, inside_niter=1, mode=0, nthreads=0):
+    return Params(dt, bandwidth, tol, lambda_eps, niter, inside_niter, mode, nthreads)
+
+
+class Oracle:
+    """One mesh + state, FP64 (precision=64) or the FP32 mirror (precision=32)."""
+
+    def __init__(self, dim, xyz, cells, labels, nparts, precision=64):
+        self.L = lib()
+        self.dim = dim
+        self.N = xyz.shape[0]
+        self.E = cells.shape[0]
+        self.nparts = int(nparts)
+        self._xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+        self._cells = np.ascontiguousarray(cells, dtype=np.int32)
+        self._labels = np.ascontiguousarray(labels, dtype=np.int32)
+        self.h = self.L.lso_create(dim, precision, self.N, self._xyz.ravel(), self.E,
+                                   self._cells.ravel(), self._labels, self.nparts)
+        if not self.h:
+            raise RuntimeError("lso_create failed")
+
+    def close(self):
+        if self.h:
+            self.L.lso_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_phi(self, phi):
+        self.L.lso_set_phi(self.h, np.ascontiguousarray(phi, dtype=np.float64))
+
+    def set_velocity(self, vel, patch_sigma=0):
+        self.L.lso_set_velocity(self.h, np.ascontiguousarray(vel, dtype=np.float64).ravel(), patch_sigma)
+
+    def step(self, params: Params) -> Stats:
+        st = Stats()
+        rc = self.L.lso_step(self.h, C.byref(params), C.byref(st))
+        if rc != 0:
+            raise RuntimeError(f"lso_step rc={rc}")
+        return st
+
+    def phi(self):
+        out = np.empty(self.N, dtype=np.float64)
+        self.L.lso_get_phi(self.h, out)
+        return out
+
+    def narrowband(self):
+        n = self.L.lso_get_narrowband(self.h, None)
+        ids = np.empty(max(n, 1), dtype=np.int32)
+        self.L.lso_get_narrowband(self.h, ids.ctypes.data)
+        return ids[:n].copy()
+
+    def layout(self):
+        F = self.L.lso_num_copies(self.h)
+        nc = self.dim + 1
+        vp = np.empty(self.N, np.int32)
+        ep = np.empty(F, np.int32)
+        vo = np.empty(self.nparts + 1, np.int32)
+        eo = np.empty(self.nparts + 1, np.int32)
+        el = np.empty(nc * F, np.int32)
+        self.L.lso_get_layout(self.h, vp.ctypes.data, ep.ctypes.data, vo.ctypes.data,
+                              eo.ctypes.data, el.ctypes.data)
+        return {"vert_perm": vp, "copy_src": ep, "vert_off": vo, "ele_off": eo,
+                "copy_verts": el.reshape(nc, F).T.copy()}
+
+
+def tet_solve(T, dots, precision=64, lambda_eps=1e-5):
+    return lib().lso_tet_solve(precision, np.asarray(T, np.float64), np.asarray(dots, np.float64), lambda_eps)
+
+
+def tri_solve(TA, TB, AB, BC, AC, precision=64):
+    return lib().lso_tri_solve(precision, TA, TB, AB, BC, AC)
+
+
+def tet_element_update(L6, vals, precision=64, lambda_eps=1e-5):
+    v = np.array(vals, dtype=np.float64)
+    lib().lso_tet_element_update(precision, np.asarray(L6, np.float64), v, lambda_eps)
+    return v
+
+
+def max_threads():
+    return lib().lso_max_threads()

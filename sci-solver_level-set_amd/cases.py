@@ -1,0 +1,257 @@
+"""Synthetic level-set workloads: meshes, initial signed distances, velocities.
+
+Mesh conventions follow the reference's own fixtures:
+  * triangle grids split along the (i,j)->(i+1,j+1) diagonal, as the faces
+    ``3 1 18 2`` / ``3 1 17 18`` of src/test/test_data/SquareMesh_size16.ply:267-268;
+  * tetrahedral grids split into the 6 Kuhn tetrahedra of each cube, vertex index
+    k*n*n + j*n + i (x fastest), the ordering GraphPartition_Square assumes
+    (src/core3d/cuda/meshFIM3d.cu:316-326).
+Velocities are one vector per element, as LevelSet::initializeAdvection takes
+them (src/LevelSet.h:35, size checked at src/LevelSet.cu:98).
+"""
+from __future__ import annotations
+
+import itertools
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+
+@dataclass
+class Case:
+    name: str
+    dim: int
+    xyz: np.ndarray          # (N, 3) float64
+    cells: np.ndarray        # (E, dim+1) int32
+    phi0: np.ndarray         # (N,) float64
+    vel: np.ndarray          # (E, 3) float64, one vector per element
+    dt: float
+    steps: int
+    bandwidth: float
+    h: float
+    labels: np.ndarray       # (N,) int32 patch labels
+    nparts: int
+    mode: int = 0            # 0 advection, 1 mean-curvature flow
+    niter: int = 5
+    meta: dict = field(default_factory=dict)
+
+
+def tri_grid(n: int):
+    """n x n vertices on [0,1]^2, cells split along the (i,j)->(i+1,j+1) diagonal."""
+    h = 1.0 / (n - 1)
+    j, i = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    xyz = np.stack([i.ravel() * h, j.ravel() * h, np.zeros(n * n)], axis=1)
+    ci, cj = np.meshgrid(np.arange(n - 1), np.arange(n - 1), indexing="xy")
+    v = (cj * n + ci).ravel()
+    t1 = np.stack([v, v + n + 1, v + 1], axis=1)
+    t2 = np.stack([v, v + n, v + n + 1], axis=1)
+    cells = np.empty((2 * v.size, 3), dtype=np.int32)
+    cells[0::2] = t1
+    cells[1::2] = t2
+    return xyz, cells, h
+
+
+def kuhn_grid(n: int):
+    """n^3 vertices on [0,1]^3, each cube split into 6 Kuhn tetrahedra, positively oriented."""
+    h = 1.0 / (n - 1)
+    k, j, i = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    xyz = np.stack([i.ravel() * h, j.ravel() * h, k.ravel() * h], axis=1)
+    m = n - 1
+    ck, cj, ci = np.meshgrid(np.arange(m), np.arange(m), np.arange(m), indexing="ij")
+    base = (ck * n * n + cj * n + ci).ravel().astype(np.int64)
+    step = np.array([1, n, n * n], dtype=np.int64)
+    tets = []
+    for perm in itertools.permutations(range(3)):
+        v0 = base
+        v1 = v0 + step[perm[0]]
+        v2 = v1 + step[perm[1]]
+        v3 = base + step.sum()
+        t = np.stack([v0, v1, v2, v3], axis=1)
+        # Orientation of this permutation's path tet: odd permutations flip sign.
+        sgn = _perm_sign(perm)
+        if sgn < 0:
+            t = t[:, [0, 2, 1, 3]]
+        tets.append(t)
+    cells = np.stack(tets, axis=1).reshape(-1, 4).astype(np.int32)
+    return xyz, cells, h
+
+
+def _perm_sign(p):
+    s = 1
+    p = list(p)
+    for a in range(len(p)):
+        for b in range(a + 1, len(p)):
+            if p[a] > p[b]:
+                s = -s
+    return s
+
+
+def jitter_interior(xyz: np.ndarray, h: float, amp: float, seed: int, dim: int) -> np.ndarray:
+    """Move every interior vertex by U(-amp*h, amp*h) per axis (boundary vertices stay)."""
+    rng = np.random.default_rng(seed)
+    d = rng.uniform(-amp * h, amp * h, size=(xyz.shape[0], 3))
+    eps = 1e-12
+    interior = np.ones(xyz.shape[0], dtype=bool)
+    for a in range(dim):
+        interior &= (xyz[:, a] > eps) & (xyz[:, a] < 1.0 - eps)
+    d[~interior] = 0.0
+    if dim == 2:
+        d[:, 2] = 0.0
+    return xyz + d
+
+
+def tet_volumes(xyz, cells):
+    a = xyz[cells[:, 1]] - xyz[cells[:, 0]]
+    b = xyz[cells[:, 2]] - xyz[cells[:, 0]]
+    c = xyz[cells[:, 3]] - xyz[cells[:, 0]]
+    return np.einsum("ij,ij->i", np.cross(a, b), c) / 6.0
+
+
+def sdf_circle(xyz, center=(0.5, 0.5), r=0.3):
+    return np.hypot(xyz[:, 0] - center[0], xyz[:, 1] - center[1]) - r
+
+
+def sdf_sphere(xyz, center=(0.5, 0.5, 0.5), r=0.3):
+    return np.linalg.norm(xyz - np.asarray(center)[None, :], axis=1) - r
+
+
+def sdf_torus(xyz, R=0.3, r=0.1, center=(0.5, 0.5, 0.5)):
+    p = xyz - np.asarray(center)[None, :]
+    q = np.hypot(p[:, 0], p[:, 1]) - R
+    return np.hypot(q, p[:, 2]) - r
+
+
+def sdf_plane(xyz, normal, offset):
+    nrm = np.asarray(normal, dtype=np.float64)
+    nrm = nrm / np.linalg.norm(nrm)
+    return xyz @ nrm - offset
+
+
+def sdf_spheres_union(xyz, centers, radii):
+    out = np.full(xyz.shape[0], np.inf)
+    for c, r in zip(centers, radii):
+        out = np.minimum(out, np.linalg.norm(xyz - c[None, :], axis=1) - r)
+    return out
+
+
+def ellipsoid_distance(p, axes, iters=60):
+    """Signed distance from points p (N,3), in the ellipsoid's own frame, to the
+    ellipsoid x^2/a^2 + y^2/b^2 + z^2/c^2 = 1. FP64 root of the Lagrange
+    condition sum (a_k p_k / (a_k^2 + t))^2 = 1 by safeguarded Newton/bisection."""
+    a = np.asarray(axes, dtype=np.float64)[None, :]
+    ap = np.abs(p)
+    inside = np.sum((ap / a) ** 2, axis=1) < 1.0
+    amin2 = float(np.min(a) ** 2)
+    # f(t) = sum (a p / (a^2 + t))^2 - 1 is decreasing on t > -amin^2.
+    lo = np.full(p.shape[0], -amin2 + 1e-300)
+    hi = np.maximum(np.linalg.norm(ap * a, axis=1), 1e-30)
+    hi = np.where(inside, 0.0, hi)
+    lo = np.where(inside, -amin2, 0.0)
+
+    def f(t):
+        return np.sum((a * ap / (a * a + t[:, None])) ** 2, axis=1) - 1.0
+
+    for _ in range(iters * 2):
+        mid = 0.5 * (lo + hi)
+        fm = f(mid)
+        lo = np.where(fm > 0, mid, lo)
+        hi = np.where(fm > 0, hi, mid)
+    t = 0.5 * (lo + hi)
+    x = a * a * ap / (a * a + t[:, None])
+    d = np.linalg.norm(x - ap, axis=1)
+    return np.where(inside, -d, d)
+
+
+def centroids(xyz, cells):
+    return xyz[cells].mean(axis=1)
+
+
+def velocity_rotation_z(xyz, cells, omega=2 * math.pi, center=(0.5, 0.5)):
+    c = centroids(xyz, cells)
+    v = np.zeros((cells.shape[0], 3))
+    v[:, 0] = -omega * (c[:, 1] - center[1])
+    v[:, 1] = omega * (c[:, 0] - center[0])
+    return v
+
+
+def partition_blocks(n: int, block, dim: int) -> tuple[np.ndarray, int]:
+    """Grid block partition (GraphPartition_Square, src/core3d/cuda/meshFIM3d.cu:269-352)."""
+    if dim == 2:
+        bl, bw = block
+        nbl, nbw = -(-n // bl), -(-n // bw)
+        j, i = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+        lab = (j // bw) * nbl + (i // bl)
+        return lab.ravel().astype(np.int32), nbl * nbw
+    bl, bw, bh = block
+    nbl, nbw, nbh = -(-n // bl), -(-n // bw), -(-n // bh)
+    k, j, i = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    lab = (k // bh) * nbl * nbw + (j // bw) * nbl + (i // bl)
+    return lab.ravel().astype(np.int32), nbl * nbw * nbh
+
+
+def partition_rcb(xyz: np.ndarray, nparts: int) -> np.ndarray:
+    """Recursive coordinate bisection, the same rule as the C++ partitioner:
+    split along the longest bounding-box axis at n*floor(k/2)/k, ties by id."""
+    labels = np.zeros(xyz.shape[0], dtype=np.int32)
+
+    def split(ids, k, base):
+        if k <= 1 or ids.size <= 1:
+            labels[ids] = base
+            return
+        pts = xyz[ids]
+        ext = pts.max(axis=0) - pts.min(axis=0)
+        axis = int(np.argmax(ext))
+        kl = k // 2
+        m = (ids.size * kl) // k
+        order = np.lexsort((ids, pts[:, axis]))
+        split(ids[order[:m]], kl, base)
+        split(ids[order[m:]], k - kl, base + kl)
+
+    split(np.arange(xyz.shape[0]), nparts, 0)
+    return labels
+
+
+def config1(n: int = 17, seed: int = 1) -> Case:
+    """2D circle: 17x17 tri grid, jitter 0.1h, circle r=0.3, sigma=(0.5,0), CFL 0.5,
+    10 steps, band 4h, ~64-vertex patches (P = N // 64, src/core2d/cuda/meshFIM2d.cu:380-382)."""
+    xyz, cells, h = tri_grid(n)
+    xyz = jitter_interior(xyz, h, 0.1, seed, 2)
+    phi0 = sdf_circle(xyz)
+    vel = np.zeros((cells.shape[0], 3))
+    vel[:, 0] = 0.5
+    P = max(2, xyz.shape[0] // 64)
+    labels = partition_rcb(xyz, P)
+    return Case("2d_circle_17", 2, xyz, cells, phi0, vel, dt=0.5 * h / 0.5, steps=10,
+                bandwidth=4 * h, h=h, labels=labels, nparts=P, niter=10)
+
+
+def config2(n: int = 64, seed: int = 2, block=(8, 8, 4), steps: int = 100) -> Case:
+    """3D torus revolve: n^3 Kuhn grid, jitter 0.1h, torus R=0.3 r=0.1, rigid rotation
+    about z (omega = 2 pi) at element centroids, CFL 0.5, band 6h, 256-vertex patches."""
+    xyz, cells, h = kuhn_grid(n)
+    xyz = jitter_interior(xyz, h, 0.1, seed, 3)
+    phi0 = sdf_torus(xyz)
+    vel = velocity_rotation_z(xyz, cells)
+    vmax = float(np.max(np.linalg.norm(vel, axis=1)))
+    labels, P = partition_blocks(n, block, 3)
+    return Case(f"3d_torus_revolve_{n}", 3, xyz, cells, phi0, vel, dt=0.5 * h / vmax,
+                steps=steps, bandwidth=6 * h, h=h, labels=labels, nparts=P, niter=5,
+                meta={"vmax": vmax, "block": block})
+
+
+def small3d(n: int = 12, seed: int = 5, block=(4, 4, 4), shape: str = "sphere", steps: int = 3) -> Case:
+    """A small tet case for parity tests (sphere or plane, rotation velocity)."""
+    xyz, cells, h = kuhn_grid(n)
+    xyz = jitter_interior(xyz, h, 0.1, seed, 3)
+    if shape == "sphere":
+        phi0 = sdf_sphere(xyz, r=0.3)
+    elif shape == "torus":
+        phi0 = sdf_torus(xyz)
+    else:
+        phi0 = sdf_plane(xyz, (1.0, 0.5, 0.25), 0.45)
+    vel = velocity_rotation_z(xyz, cells)
+    vmax = float(np.max(np.linalg.norm(vel, axis=1)))
+    labels, P = partition_blocks(n, block, 3)
+    return Case(f"3d_{shape}_{n}", 3, xyz, cells, phi0, vel, dt=0.5 * h / vmax, steps=steps,
+                bandwidth=6 * h, h=h, labels=labels, nparts=P, niter=5)

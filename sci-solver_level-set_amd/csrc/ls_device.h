@@ -1,0 +1,61 @@
+// Device-side data views and kernel launchers shared by the C-ABI layer.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace lsamd {
+
+constexpr int kBlock = 256;          // threads per workgroup (4 wave64)
+constexpr double kLargeNum = 1e9;    // LARGENUM: unreached distance (exact in FP32)
+
+enum : int8_t { kSeedPoint = 0, kActivePoint = 1, kAlivePoint = 2, kFarPoint = 3 };
+
+// Patch layout resident in HBM (see ls_layout.h for the orderings).
+template <typename T>
+struct DevMesh {
+  int P;
+  int64_t N;
+  int64_t F;
+  const int32_t* vert_off;     // P+1
+  const int32_t* ele_off;      // P+1
+  const int32_t* halo_off;     // P+1
+  const int32_t* halo_ids;     // renumbered global ids
+  const ushort4* copy_local;   // F, patch-local corner ids
+  const int32_t* vslot_off;    // N+1
+  const uint16_t* vslots;      // patch-local slot ids per owned vertex
+  const T* coords;             // F x (6 | 3), local coordinates
+  const T* sigma;              // F x (3 | 2), velocity in the local frame
+  const int32_t* padj_off;     // P+1
+  const int32_t* padj;
+};
+
+// Per-timestep state (renumbered vertex order).
+template <typename T>
+struct DevState {
+  T* phi;
+  T* phi_out;
+  T* DT;
+  T* DT_out;
+  int8_t* label;
+  int8_t* tsign;
+  int32_t* bcon;       // P, patch converged
+  T* bmin;             // P, patch min distance after a sweep
+  int32_t* act;        // P, member of the current active list
+  int32_t* fresh;      // P, activation flag for neighbour probes
+  int32_t* seedflag;   // P
+  int32_t* counters;   // small array of device counters
+};
+
+// Device counters layout.
+enum Counter : int {
+  kCntActive = 0,   // entries in the current active list
+  kCntFresh = 1,    // entries in the neighbour-probe list
+  kCntNext = 2,     // entries in the next active list
+  kCntNB = 3,       // narrowband patches
+  kCntSeed = 4,     // seed patches
+  kCntNBVerts = 5,  // vertices owned by narrowband patches
+  kNumCounters = 8
+};
+
+}  // namespace lsamd

@@ -1,0 +1,30 @@
+// Engine interface between the C-ABI layer and the templated HIP driver.
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "ls_layout.h"
+
+struct ls_step_params;
+struct ls_stats;
+
+namespace lsamd {
+
+class Engine {
+ public:
+  virtual ~Engine() {}
+  // Uploads the layout; geometry is rounded to the engine's working type.
+  virtual int upload(const Layout& L, std::string& err) = 0;
+  virtual int set_phi(const double* phi_user, std::string& err) = 0;
+  virtual int set_velocity(const std::vector<double>& sigma_local, std::string& err) = 0;
+  virtual int step(const ls_step_params& prm, ls_stats& st, std::string& err) = 0;
+  virtual int get_phi(double* phi_user, std::string& err) = 0;
+  virtual int get_narrowband(std::vector<int32_t>& ids, std::string& err) = 0;
+  virtual int working_bytes() const = 0;  // sizeof working real type
+};
+
+// dim 3 -> FP32 tet engine, dim 2 -> FP64 triangle engine.
+Engine* make_engine(int dim, int device, std::string& err);
+
+}  // namespace lsamd

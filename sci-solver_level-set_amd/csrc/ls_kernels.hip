@@ -1,0 +1,894 @@
+// gfx950 kernels for the level-set per-timestep hot path.
+//
+// One workgroup (4 x wave64) owns one patch at a time: the patch's owned and
+// halo vertex values, its element-copy slots and its vertex slot lists are
+// staged in LDS, so HBM sees each patch's arrays once per visit and every
+// halo vertex once per patch (the reference re-reads halo values per element
+// copy, src/core3d/include/redistance_kernels3d.h:342-358).
+//
+// Arithmetic follows the reference scheme operation by operation in the
+// working type T (FP32 for tets, FP64 for triangles), with contraction off,
+// correctly rounded division and square root, and FP64 for the tet solver's
+// quadratic discriminant and roots.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "../../include/levelset_hip.h"
+#include "ls_device.h"
+#include "ls_engine.h"
+
+namespace lsamd {
+namespace kern {
+
+__device__ __forceinline__ float d_sqrt(float x) { return __fsqrt_rn(x); }
+__device__ __forceinline__ double d_sqrt(double x) { return __dsqrt_rn(x); }
+__device__ __forceinline__ float d_div(float a, float b) { return __fdiv_rn(a, b); }
+__device__ __forceinline__ double d_div(double a, double b) { return a / b; }
+__device__ __forceinline__ float d_min(float a, float b) { return fminf(a, b); }
+__device__ __forceinline__ double d_min(double a, double b) { return fmin(a, b); }
+__device__ __forceinline__ float d_max(float a, float b) { return fmaxf(a, b); }
+__device__ __forceinline__ double d_max(double a, double b) { return fmax(a, b); }
+__device__ __forceinline__ float d_abs(float a) { return fabsf(a); }
+__device__ __forceinline__ double d_abs(double a) { return fabs(a); }
+
+// Roots of p x^2 + q x + r with the discriminant and roots in FP64.
+template <typename T>
+__device__ __forceinline__ bool quad_roots(T p, T q, T r, T& x1, T& x2) {
+  const double dp = p, dq = q, dr = r;
+  const double delta = dq * dq - 4.0 * dp * dr;
+  if (!(delta >= 0.0)) return false;
+  const double sq = __dsqrt_rn(delta);
+  x1 = (T)((-dq + sq) / (2.0 * dp));
+  x2 = (T)((-dq - sq) / (2.0 * dp));
+  return true;
+}
+
+// Tet Hopf-Lax solve for the fourth corner (scheme of
+// src/core3d/include/redistance_kernels3d.h:124-285).
+template <typename T>
+__device__ T tet_solve(T TA, T TB, T TC, T ACAC, T ACBC, T ACCD, T BCBC, T BCCD, T CDCD, T leps) {
+  const T big = (T)kLargeNum;
+  if (TA >= big && TB >= big && TC >= big) return big;
+  T FaceTAC = big, FaceTAB = big, FaceTBC = big;
+  T TD = big;
+  const T TAC = TC - TA, TBC = TC - TB, TAB = TB - TA;
+  T x1, x2;
+  {
+    const T p = BCBC * TBC * TBC - BCBC * BCBC;
+    const T q = (BCCD + BCCD) * TBC * TBC - (T)2 * BCBC * BCCD;
+    const T r = TBC * TBC * CDCD - BCCD * BCCD;
+    if (quad_roots(p, q, r, x1, x2)) {
+#pragma unroll
+      for (int k = 0; k < 2; ++k) {
+        const T l2 = k == 0 ? x1 : x2;
+        const T l3 = (T)1 - l2;
+        if (l2 >= (T)0 && l2 <= (T)1) {
+          const T TE = TB * l2 + TC * l3;
+          FaceTBC = d_min(FaceTBC, TE + d_sqrt((l2 * BCBC + BCCD) * l2 + (l2 * BCCD + CDCD)));
+        }
+      }
+    }
+    FaceTBC = d_min(FaceTBC, d_min(TB + d_sqrt((BCBC + BCCD) + (BCCD + CDCD)), TC + d_sqrt(CDCD)));
+  }
+  {
+    const T gx = ACAC - ACBC, gy = ACBC - BCBC, gz = ACCD - BCCD;
+    const T t2 = TAB * TAB;
+    const T p = (t2 * ACAC - gx * gx) + (BCBC * t2 - gy * gy) - ((ACBC + ACBC) * t2 - (T)2 * gx * gy);
+    const T q = -(BCBC * t2 - gy * gy) * (T)2 + ((ACBC + ACBC) * t2 - (T)2 * gx * gy) +
+                ((ACCD + ACCD) * t2 - (T)2 * gx * gz) - ((BCCD + BCCD) * t2 - (T)2 * gy * gz);
+    const T r = (t2 * BCBC - gy * gy) + ((BCCD + BCCD) * t2 - (T)2 * gy * gz) + (t2 * CDCD - gz * gz);
+    if (quad_roots(p, q, r, x1, x2)) {
+#pragma unroll
+      for (int k = 0; k < 2; ++k) {
+        const T l1 = k == 0 ? x1 : x2;
+        const T l2 = (T)1 - l1;
+        if (l1 >= (T)0 && l1 <= (T)1) {
+          const T TE = TA * l1 + TB * l2;
+          FaceTAB = d_min(FaceTAB, TE + d_sqrt((l1 * ACAC + l2 * ACBC + ACCD) * l1 +
+                                               (l1 * ACBC + l2 * BCBC + BCCD) * l2 +
+                                               (l1 * ACCD + l2 * BCCD + CDCD)));
+        }
+      }
+    }
+    FaceTAB = d_min(FaceTAB, d_min(TB + d_sqrt((BCBC + BCCD) + (BCCD + CDCD)),
+                                   TA + d_sqrt((ACAC + ACCD) + (ACCD + CDCD))));
+  }
+  {
+    const T p = ACAC * TAC * TAC - ACAC * ACAC;
+    const T q = (ACCD + ACCD) * TAC * TAC - (T)2 * ACAC * ACCD;
+    const T r = TAC * TAC * CDCD - ACCD * ACCD;
+    if (quad_roots(p, q, r, x1, x2)) {
+#pragma unroll
+      for (int k = 0; k < 2; ++k) {
+        const T l1 = k == 0 ? x1 : x2;
+        const T l3 = (T)1 - l1;
+        if (l1 >= (T)0 && l1 <= (T)1) {
+          const T TE = TA * l1 + TC * l3;
+          FaceTAC = d_min(FaceTAC, TE + d_sqrt((l1 * ACAC + ACCD) * l1 + (l1 * ACCD + CDCD)));
+        }
+      }
+    }
+    FaceTAC = d_min(FaceTAC, d_min(TA + d_sqrt((ACAC + ACCD) + (ACCD + CDCD)), TC + d_sqrt(CDCD)));
+  }
+  {
+    const T s = TAC * ACBC - TBC * ACAC;
+    const T t = TAC * BCCD - TBC * ACCD;
+    const T h = -(TAC * BCBC - TBC * ACBC);
+    const T t2 = TAC * TAC;
+    const T c1 = t2 * ACAC - ACAC * ACAC;
+    const T c2 = BCBC * t2 - ACBC * ACBC;
+    const T c3 = (ACBC + ACBC) * t2 - (T)2 * ACAC * ACBC;
+    const T c4 = (ACCD + ACCD) * t2 - (T)2 * ACAC * ACCD;
+    const T c5 = (BCCD + BCCD) * t2 - (T)2 * ACBC * ACCD;
+    const T c6 = t2 * CDCD - ACCD * ACCD;
+    const T p = c1 * h * h + c2 * s * s + c3 * s * h;
+    const T q = c2 * (T)2 * s * t + c3 * t * h + c4 * h * h + c5 * s * h;
+    const T r = c2 * t * t + c5 * t * h + c6 * h * h;
+    if (quad_roots(p, q, r, x1, x2)) {
+      const T hd = h + leps;
+#pragma unroll
+      for (int k = 0; k < 2; ++k) {
+        const T l1 = k == 0 ? x1 : x2;
+        const T l2 = d_div(s * l1 + t, hd);
+        const T l3 = (T)1 - l1 - l2;
+        if (l1 >= (T)0 && l1 <= (T)1 && l2 >= (T)0 && l2 <= (T)1 && l3 >= (T)0 && l3 <= (T)1) {
+          const T TE = TA * l1 + TB * l2 + TC * l3;
+          TD = d_min(TD, TE + d_sqrt((l1 * ACAC + l2 * ACBC + ACCD) * l1 +
+                                     (l1 * ACBC + l2 * BCBC + BCCD) * l2 +
+                                     (l1 * ACCD + l2 * BCCD + CDCD)));
+        }
+      }
+    }
+  }
+  return d_min(TD, d_min(FaceTBC, d_min(FaceTAB, FaceTAC)));
+}
+
+// Triangle Hopf-Lax solve (scheme of src/core2d/include/redistance_kernels.h:287-334).
+template <typename T>
+__device__ T tri_solve(T TA, T TB, T LenAB, T LenBC, T LenAC) {
+  const T big = (T)kLargeNum;
+  if (TA >= big && TB >= big) return big;
+  T TC1 = big, TC2 = big;
+  const T TAB = TB - TA;
+  const T a = (LenAB * LenAB - TAB * TAB) * LenAB * LenAB;
+  const T EdgeTA = TA + LenAC;
+  const T EdgeTB = TB + LenBC;
+  if (a > (T)0) {
+    const T cosA = d_div(LenAC * LenAC + LenAB * LenAB - LenBC * LenBC, (T)2 * LenAC * LenAB);
+    const T b = (T)2 * LenAB * LenAC * cosA * (TAB * TAB - LenAB * LenAB);
+    const T delta = (T)4 * LenAC * LenAC * a * TAB * TAB * ((T)1 - cosA * cosA);
+    const T sq = d_sqrt(delta);
+    const T l1 = d_div(-b + sq, (T)2 * a);
+    const T l2 = d_div(-b - sq, (T)2 * a);
+    if (l1 >= (T)0 && l1 <= (T)1) {
+      const T LenAD = l1 * LenAB;
+      const T LenCD = d_sqrt(LenAC * LenAC + LenAD * LenAD - (T)2 * LenAC * LenAD * cosA);
+      TC1 = l1 * TAB + TA + LenCD;
+    }
+    if (l2 >= (T)0 && l2 <= (T)1) {
+      const T LenAD = l2 * LenAB;
+      const T LenCD = d_sqrt(LenAC * LenAC + LenAD * LenAD - (T)2 * LenAC * LenAD * cosA);
+      TC2 = l2 * TAB + TA + LenCD;
+    }
+    return d_min(TC2, d_min(TC1, d_min(EdgeTA, EdgeTB)));
+  }
+  return d_min(EdgeTA, EdgeTB);
+}
+
+// One element update: every corner takes min(own, solve from the other corners),
+// all solves reading the values from before the update
+// (src/core3d/include/redistance_kernels3d.h:373-414; 2D redistance_kernels.h:441-462).
+template <typename T, int NC>
+struct ElemSolve;
+
+template <typename T>
+struct ElemSolve<T, 4> {
+  static constexpr int kCoords = 6;
+  __device__ static void update(const T* L, T* v, T leps) {
+    const T ACBC = L[1] * (L[1] - L[0]) + L[2] * L[2];
+    const T BCCD = (L[1] - L[0]) * (L[3] - L[1]) + L[2] * (L[4] - L[2]);
+    const T ACCD = L[1] * (L[3] - L[1]) + L[2] * (L[4] - L[2]);
+    const T ADBD = L[3] * (L[3] - L[0]) + L[4] * L[4] + L[5] * L[5];
+    const T ACAD = L[1] * L[3] + L[2] * L[4];
+    const T BCBD = (L[1] - L[0]) * (L[3] - L[0]) + L[2] * L[4];
+    const T ADBC = ACBC + BCCD;
+    const T ACBD = ACBC + ACCD;
+    const T CDBD = ADBD - ACBD;
+    const T ADCD = ADBD - ADBC;
+    const T CDCD = ADCD - ACCD;
+    const T ADAD = ADCD + ACAD;
+    const T ACAC = ACAD - ACCD;
+    const T BDBD = BCBD + CDBD;
+    const T BCBC = BCBD - BCCD;
+    const T TA = v[0], TB = v[1], TC = v[2], TD = v[3];
+    v[3] = d_min(TD, tet_solve(TA, TB, TC, ACAC, ACBC, ACCD, BCBC, BCCD, CDCD, leps));
+    v[0] = d_min(TA, tet_solve(TB, TD, TC, BCBC, -BCCD, -ACBC, CDCD, ACCD, ACAC, leps));
+    v[1] = d_min(TB, tet_solve(TA, TD, TC, ACAC, -ACCD, -ACBC, CDCD, BCCD, BCBC, leps));
+    v[2] = d_min(TC, tet_solve(TA, TB, TD, ADAD, ADBD, -ADCD, BDBD, -CDBD, CDCD, leps));
+  }
+};
+
+template <typename T>
+struct ElemSolve<T, 3> {
+  static constexpr int kCoords = 3;
+  __device__ static void update(const T* c, T* v, T) {
+    const T LenAB = d_abs(c[0]);
+    const T LenBC = d_sqrt(c[2] * c[2] + (c[1] - c[0]) * (c[1] - c[0]));
+    const T LenCA = d_sqrt(c[1] * c[1] + c[2] * c[2]);
+    const T TA = v[0], TB = v[1], TC = v[2];
+    v[2] = d_min(TC, tri_solve(TA, TB, LenAB, LenBC, LenCA));
+    v[0] = d_min(TA, tri_solve(TB, TC, LenBC, LenCA, LenAB));
+    v[1] = d_min(TB, tri_solve(TC, TA, LenCA, LenAB, LenBC));
+  }
+};
+
+template <typename T>
+__device__ __forceinline__ T wave_min(T v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = d_min(v, __shfl_xor(v, off, 64));
+  return v;
+}
+
+// Block-wide min; every thread must call it. Returns the result in all threads.
+template <typename T>
+__device__ __forceinline__ T block_min(T v, T* scratch) {
+  v = wave_min(v);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) scratch[wave] = v;
+  __syncthreads();
+  T r = scratch[0];
+#pragma unroll
+  for (int w = 1; w < kBlock / 64; ++w) r = d_min(r, scratch[w]);
+  __syncthreads();
+  return r;
+}
+
+__device__ __forceinline__ int corner(const ushort4& c, int i) {
+  return i == 0 ? c.x : (i == 1 ? c.y : (i == 2 ? c.z : c.w));
+}
+
+// ---------------------------------------------------------------------------
+// Per-step reset: DT := LARGENUM, Label := Far, Tsign := (phi >= 0)
+// (redistance3d.cu:31-35 and kernel_reinit_Tsign, redistance_kernels3d.h:785-792).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_init(int64_t N, DevState<T> s) {
+  for (int64_t v = (int64_t)blockIdx.x * kBlock + threadIdx.x; v < N; v += (int64_t)gridDim.x * kBlock) {
+    s.DT[v] = (T)kLargeNum;
+    s.label[v] = kFarPoint;
+    s.tsign[v] = (s.phi[v] < (T)0) ? 0 : 1;
+  }
+}
+
+__global__ void k_set_counters(int32_t* c, int n, int32_t val) {
+  if ((int)threadIdx.x < n) c[threadIdx.x] = val;
+}
+
+// Seeds over the scanned patches (kernel_seedlabel / _narrowband,
+// redistance_kernels3d.h:623-770): a corner with phi == 0 is a seed, and so
+// is every corner of a sign-changing element; seeds get DT = |phi|. A patch
+// holding such an element joins the active list.
+template <typename T, int NC>
+__global__ __launch_bounds__(kBlock) void k_seeds(DevMesh<T> m, DevState<T> s, const int32_t* scan,
+                                                  int32_t* active, int32_t* counters) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  T* s_vtx = (T*)smem;
+  const int p = scan[blockIdx.x];
+  const int vs = m.vert_off[p], ve = m.vert_off[p + 1];
+  const int es = m.ele_off[p], ee = m.ele_off[p + 1];
+  const int hs = m.halo_off[p], he = m.halo_off[p + 1];
+  const int nv = ve - vs, ne = ee - es, nh = he - hs;
+  for (int i = threadIdx.x; i < nv; i += kBlock) s_vtx[i] = s.phi[vs + i];
+  for (int i = threadIdx.x; i < nh; i += kBlock) s_vtx[nv + i] = s.phi[m.halo_ids[hs + i]];
+  __syncthreads();
+  int flag = 0;
+  for (int e = threadIdx.x; e < ne; e += kBlock) {
+    const ushort4 c = m.copy_local[es + e];
+    T val[NC];
+    bool allpos = true, allneg = true;
+#pragma unroll
+    for (int i = 0; i < NC; ++i) {
+      val[i] = s_vtx[corner(c, i)];
+      allpos = allpos && val[i] >= (T)0;
+      allneg = allneg && val[i] <= (T)0;
+    }
+    const bool mixed = !(allpos || allneg);
+#pragma unroll
+    for (int i = 0; i < NC; ++i) {
+      if (mixed || val[i] == (T)0) {
+        const int l = corner(c, i);
+        const int g = l < nv ? vs + l : m.halo_ids[hs + l - nv];
+        s.label[g] = kSeedPoint;
+        s.DT[g] = d_abs(val[i]);
+        flag = 1;
+      }
+    }
+  }
+  flag = __syncthreads_or(flag);
+  if (threadIdx.x == 0 && flag) active[atomicAdd(&counters[kCntSeed], 1)] = p;
+}
+
+// FIM patch visit (kernel_update_values / kernel_run_check_neghbor,
+// redistance_kernels3d.h:287-621): NITER Jacobi sweeps over the patch's
+// element copies in LDS; each sweep updates every element from its corner
+// values, then every owned vertex takes the min over its slots and writes it
+// back to them. An element whose four inputs are unchanged since its last
+// update would reproduce its values exactly, so it is skipped (dirty flags);
+// this changes no result. The last sweep decides convergence per vertex
+// (old - new < tol; seeds count as converged), and the workgroup reduces it
+// to the patch flag plus the patch min of the written distances
+// (run_reduction_bandwidth, redistance_kernels3d.h:57-97).
+template <typename T, int NC>
+__global__ __launch_bounds__(kBlock) void k_sweep(DevMesh<T> m, DevState<T> s, const int32_t* list,
+                                                  int niter, int write_out, T tol, T leps) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ T red[kBlock / 64];
+  const int p = list[blockIdx.x];
+  const int vs = m.vert_off[p], ve = m.vert_off[p + 1];
+  const int es = m.ele_off[p], ee = m.ele_off[p + 1];
+  const int hs = m.halo_off[p], he = m.halo_off[p + 1];
+  const int nv = ve - vs, ne = ee - es, nh = he - hs;
+  const int sbase = m.vslot_off[vs];
+  const int nslots = m.vslot_off[ve] - sbase;
+  T* s_slot = (T*)smem;
+  T* s_vtx = s_slot + NC * ne;
+  T* s_vval = s_vtx + nv + nh;
+  uint16_t* s_voff = (uint16_t*)(s_vval + nv);
+  uint16_t* s_vsl = s_voff + nv + 1;
+  uint8_t* s_dirty = (uint8_t*)(s_vsl + nslots);
+  const int tid = threadIdx.x;
+
+  for (int i = tid; i < nv; i += kBlock) {
+    const T v = s.DT[vs + i];
+    s_vtx[i] = v;
+    s_vval[i] = v;
+    s_voff[i] = (uint16_t)(m.vslot_off[vs + i] - sbase);
+  }
+  if (tid == 0) s_voff[nv] = (uint16_t)nslots;
+  for (int i = tid; i < nh; i += kBlock) s_vtx[nv + i] = s.DT[m.halo_ids[hs + i]];
+  for (int i = tid; i < nslots; i += kBlock) s_vsl[i] = m.vslots[sbase + i];
+  __syncthreads();
+  for (int e = tid; e < ne; e += kBlock) {
+    const ushort4 c = m.copy_local[es + e];
+#pragma unroll
+    for (int i = 0; i < NC; ++i) s_slot[e * NC + i] = s_vtx[corner(c, i)];
+    s_dirty[e] = 1;
+  }
+  __syncthreads();
+
+  constexpr int K = ElemSolve<T, NC>::kCoords;
+  int con_all = 1;
+  T vmin = (T)kLargeNum;
+  for (int it = 0; it < niter; ++it) {
+    for (int e = tid; e < ne; e += kBlock) {
+      if (!s_dirty[e]) continue;
+      T x[NC], y[NC], L[K];
+#pragma unroll
+      for (int i = 0; i < NC; ++i) x[i] = y[i] = s_slot[e * NC + i];
+      const T* gc = m.coords + (int64_t)(es + e) * K;
+#pragma unroll
+      for (int k = 0; k < K; ++k) L[k] = gc[k];
+      ElemSolve<T, NC>::update(L, y, leps);
+      int changed = 0;
+#pragma unroll
+      for (int i = 0; i < NC; ++i) {
+        if (y[i] != x[i]) { s_slot[e * NC + i] = y[i]; changed = 1; }
+      }
+      s_dirty[e] = (uint8_t)changed;
+    }
+    __syncthreads();
+    const bool last = (it == niter - 1);
+    for (int v = tid; v < nv; v += kBlock) {
+      const int o0 = s_voff[v], o1 = s_voff[v + 1];
+      const T old = s_vval[v];
+      T mn = old;
+      if (o1 > o0) {
+        mn = s_slot[s_vsl[o0]];
+        for (int k = o0 + 1; k < o1; ++k) mn = d_min(s_slot[s_vsl[k]], mn);
+        for (int k = o0; k < o1; ++k) {
+          const int sl = s_vsl[k];
+          if (mn < s_slot[sl]) { s_slot[sl] = mn; s_dirty[sl / NC] = 1; }
+        }
+        s_vval[v] = mn;
+      }
+      if (last) {
+        T outv;
+        int c;
+        if (s.label[vs + v] != kSeedPoint) {
+          outv = mn;
+          c = (old - mn < tol) ? 1 : 0;
+        } else {
+          outv = s_vtx[v];
+          c = 1;
+        }
+        if (write_out) s.DT_out[vs + v] = outv;
+        con_all &= c;
+        vmin = d_min(vmin, outv);
+      }
+    }
+    __syncthreads();
+  }
+  con_all = __syncthreads_and(con_all);
+  const T bm = block_min(vmin, red);
+  if (tid == 0) {
+    s.bcon[p] = con_all;
+    if (write_out) s.bmin[p] = bm;
+  }
+}
+
+// CopyOutBack (redistance_kernels3d.h:772-783) plus the active-list label of
+// the patch: it stays active iff it did not converge.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_copyback(DevMesh<T> m, DevState<T> s, const int32_t* list) {
+  const int p = list[blockIdx.x];
+  const int vs = m.vert_off[p], ve = m.vert_off[p + 1];
+  for (int v = vs + threadIdx.x; v < ve; v += kBlock) s.DT[v] = s.DT_out[v];
+  if (threadIdx.x == 0) s.act[p] = s.bcon[p] ? 0 : 1;
+}
+
+// Neighbour activation (redistance3d.cu:141-181): a converged patch whose
+// min is below the band activates every neighbour not currently active.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_activate(DevMesh<T> m, DevState<T> s, const int32_t* list, int n,
+                                                     T bw, int32_t* fresh_list) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const int p = list[i];
+  if (!(s.bcon[p] && s.bmin[p] < bw)) return;
+  for (int k = m.padj_off[p]; k < m.padj_off[p + 1]; ++k) {
+    const int q = m.padj[k];
+    if (s.act[q] == 0 && atomicExch(&s.fresh[q], 1) == 0)
+      fresh_list[atomicAdd(&s.counters[kCntFresh], 1)] = q;
+  }
+}
+
+// Next active list (redistance3d.cu:183-222): unconverged patches of the
+// current list, then probed neighbours that did not converge.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_finalize(DevState<T> s, const int32_t* list, int n,
+                                                     const int32_t* fresh_list, int nfresh, int32_t* next) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) {
+    const int p = list[i];
+    if (s.act[p]) next[atomicAdd(&s.counters[kCntNext], 1)] = p;
+  } else if (i < n + nfresh) {
+    const int q = fresh_list[i - n];
+    s.fresh[q] = 0;
+    if (!s.bcon[q]) next[atomicAdd(&s.counters[kCntNext], 1)] = q;
+  }
+}
+
+// Narrowband (kernel_compute_new_narrowband, redistance_kernels3d.h:22-55):
+// patches whose min distance is <= the band; fused with the sign restore
+// phi = DT * (2*Tsign - 1) (kernel_recover_Tsign_whole, :807-814).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_narrowband(DevMesh<T> m, DevState<T> s, T bw, int32_t* nb) {
+  __shared__ T red[kBlock / 64];
+  const int p = blockIdx.x;
+  const int vs = m.vert_off[p], ve = m.vert_off[p + 1];
+  T mn = (T)kLargeNum;
+  for (int v = vs + threadIdx.x; v < ve; v += kBlock) {
+    const T d = s.DT[v];
+    mn = d_min(mn, d);
+    s.phi[v] = d * (T)(2 * (int)s.tsign[v] - 1);
+  }
+  mn = block_min(mn, red);
+  if (threadIdx.x == 0 && mn <= bw) {
+    nb[atomicAdd(&s.counters[kCntNB], 1)] = p;
+    atomicAdd(&s.counters[kCntNBVerts], ve - vs);
+  }
+}
+
+// Shape-function gradients of a copy in its local frame.
+template <typename T, int NC>
+struct Grad;
+
+template <typename T>
+struct Grad<T, 4> {
+  // A=(0,0,0), B=(L0,0,0), C=(L1,L2,0), D=(L3,L4,L5).
+  __device__ static void eval(const T* L, T (*g)[3], T& vol) {
+    vol = d_div(d_abs(L[0] * L[2] * L[5]), (T)6);
+    g[1][0] = d_div((T)1, L[0]);
+    g[1][1] = d_div(-L[1], L[0] * L[2]);
+    g[1][2] = d_div(L[1] * L[4] - L[2] * L[3], L[0] * L[2] * L[5]);
+    g[2][0] = (T)0;
+    g[2][1] = d_div((T)1, L[2]);
+    g[2][2] = d_div(-L[4], L[2] * L[5]);
+    g[3][0] = (T)0;
+    g[3][1] = (T)0;
+    g[3][2] = d_div((T)1, L[5]);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) g[0][k] = -((g[1][k] + g[2][k]) + g[3][k]);
+  }
+};
+
+template <typename T>
+struct Grad<T, 3> {
+  // A=(0,0), B=(c0,0), C=(c1,c2).
+  __device__ static void eval(const T* c, T (*g)[3], T& vol) {
+    vol = d_div(d_abs(c[0] * c[2]), (T)2);
+    g[1][0] = d_div((T)1, c[0]);
+    g[1][1] = d_div(-c[1], c[0] * c[2]);
+    g[2][0] = (T)0;
+    g[2][1] = d_div((T)1, c[2]);
+#pragma unroll
+    for (int k = 0; k < 2; ++k) g[0][k] = -(g[1][k] + g[2][k]);
+  }
+};
+
+// Upwind residual-distribution (N-scheme) advection of one narrowband patch
+// (kernel_updateT_single_stage3d, meshFIM_kernels.h:36-408): per element copy
+// K_i = vol sigma.gradN_i, H = sum K_i phi_i, distribution coefficients
+// alpha~_i; per owned vertex phi -= dt * sum(alpha~ H) / sum(alpha~ vol).
+template <typename T, int NC>
+__global__ __launch_bounds__(kBlock) void k_advect(DevMesh<T> m, DevState<T> s, const int32_t* nb, T dt) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  constexpr int K = ElemSolve<T, NC>::kCoords;
+  constexpr int D = NC - 1;
+  const int p = nb[blockIdx.x];
+  const int vs = m.vert_off[p], ve = m.vert_off[p + 1];
+  const int es = m.ele_off[p], ee = m.ele_off[p + 1];
+  const int hs = m.halo_off[p], he = m.halo_off[p + 1];
+  const int nv = ve - vs, ne = ee - es, nh = he - hs;
+  T* s_up = (T*)smem;
+  T* s_dn = s_up + NC * ne;
+  T* s_vtx = s_dn + NC * ne;
+  for (int i = threadIdx.x; i < nv; i += kBlock) s_vtx[i] = s.phi[vs + i];
+  for (int i = threadIdx.x; i < nh; i += kBlock) s_vtx[nv + i] = s.phi[m.halo_ids[hs + i]];
+  __syncthreads();
+  for (int e = threadIdx.x; e < ne; e += kBlock) {
+    const ushort4 c = m.copy_local[es + e];
+    T ph[NC], L[K], sg[D], g[NC][3], vol;
+#pragma unroll
+    for (int i = 0; i < NC; ++i) ph[i] = s_vtx[corner(c, i)];
+    const T* gc = m.coords + (int64_t)(es + e) * K;
+#pragma unroll
+    for (int k = 0; k < K; ++k) L[k] = gc[k];
+    const T* gs = m.sigma + (int64_t)(es + e) * D;
+#pragma unroll
+    for (int k = 0; k < D; ++k) sg[k] = gs[k];
+    Grad<T, NC>::eval(L, g, vol);
+    T Kc[NC], Kp[NC], Km[NC], at[NC];
+#pragma unroll
+    for (int i = 0; i < NC; ++i) {
+      T d = sg[0] * g[i][0] + sg[1] * g[i][1];
+      if (D == 3) d = d + sg[D - 1] * g[i][2];
+      Kc[i] = vol * d;
+    }
+    T H = (T)0, beta = (T)0;
+#pragma unroll
+    for (int i = 0; i < NC; ++i) {
+      H += Kc[i] * ph[i];
+      Kp[i] = d_max(Kc[i], (T)0);
+      Km[i] = d_min(Kc[i], (T)0);
+      beta += Km[i];
+      at[i] = (T)0;
+    }
+    beta = d_div((T)1, beta);
+    if (d_abs(H) > (T)1e-8) {
+      T alpha[NC], theta = (T)0;
+#pragma unroll
+      for (int i = 0; i < NC; ++i) {
+        T acc = (T)0;
+#pragma unroll
+        for (int j = 0; j < NC; ++j) acc += Km[j] * (ph[i] - ph[j]);
+        alpha[i] = d_div(Kp[i] * beta * acc, H);
+      }
+#pragma unroll
+      for (int i = 0; i < NC; ++i) theta += d_max((T)0, alpha[i]);
+#pragma unroll
+      for (int i = 0; i < NC; ++i) at[i] = d_div(d_max(alpha[i], (T)0), theta);
+    }
+#pragma unroll
+    for (int i = 0; i < NC; ++i) {
+      s_up[e * NC + i] = at[i] * H;
+      s_dn[e * NC + i] = at[i] * vol;
+    }
+  }
+  __syncthreads();
+  for (int v = threadIdx.x; v < nv; v += kBlock) {
+    const int o0 = m.vslot_off[vs + v], o1 = m.vslot_off[vs + v + 1];
+    T up = (T)0, dn = (T)0;
+    for (int k = o0; k < o1; ++k) up += s_up[m.vslots[k]];
+    for (int k = o0; k < o1; ++k) dn += s_dn[m.vslots[k]];
+    const T old = s_vtx[v];
+    s.phi_out[vs + v] = (d_abs(dn) > (T)1e-8) ? old - dt * d_div(up, dn) : old;
+  }
+}
+
+// CopyOutBack_levelset3d (meshFIM_kernels.h:22-34).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_copy_phi(DevMesh<T> m, DevState<T> s, const int32_t* nb) {
+  const int p = nb[blockIdx.x];
+  for (int v = m.vert_off[p] + threadIdx.x; v < m.vert_off[p + 1]; v += kBlock) s.phi[v] = s.phi_out[v];
+}
+
+inline int grid_for(int64_t n) {
+  int64_t g = (n + kBlock - 1) / kBlock;
+  if (g > 65536) g = 65536;
+  return (int)(g < 1 ? 1 : g);
+}
+
+}  // namespace kern
+
+using namespace kern;
+
+
+// ---------------------------------------------------------------------------
+// Host driver.
+
+#define LS_HIP(expr)                                                         \
+  do {                                                                       \
+    hipError_t e_ = (expr);                                                  \
+    if (e_ != hipSuccess) {                                                  \
+      err = std::string(#expr) + ": " + hipGetErrorString(e_);               \
+      return LS_ERR_HIP;                                                     \
+    }                                                                        \
+  } while (0)
+
+template <typename U>
+static hipError_t dev_upload(U** dptr, const U* host, size_t n) {
+  if (*dptr) { (void)hipFree(*dptr); *dptr = nullptr; }
+  hipError_t e = hipMalloc((void**)dptr, std::max<size_t>(n, 1) * sizeof(U));
+  if (e != hipSuccess) return e;
+  if (n) e = hipMemcpy(*dptr, host, n * sizeof(U), hipMemcpyHostToDevice);
+  return e;
+}
+template <typename U>
+static hipError_t dev_alloc(U** dptr, size_t n) {
+  if (*dptr) { (void)hipFree(*dptr); *dptr = nullptr; }
+  return hipMalloc((void**)dptr, std::max<size_t>(n, 1) * sizeof(U));
+}
+
+template <typename T, int NC>
+class EngineImpl final : public Engine {
+ public:
+  explicit EngineImpl(int device) : device_(device) {}
+  ~EngineImpl() override { release(); }
+
+  int working_bytes() const override { return (int)sizeof(T); }
+
+  int upload(const Layout& L, std::string& err) override {
+    LS_HIP(hipSetDevice(device_));
+    if (!stream_) LS_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    if (!ev_[0]) for (auto& e : ev_) LS_HIP(hipEventCreate(&e));
+    P_ = L.P; N_ = L.N; F_ = L.F;
+    vert_off_h_ = L.vert_off;
+    vert_perm_h_ = L.vert_perm;
+    int lds_max = 0;
+    LS_HIP(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, device_));
+    // Exact per-patch LDS needs.
+    fim_lds_ = adv_lds_ = seed_lds_ = 0;
+    for (int p = 0; p < P_; ++p) {
+      const size_t ne = L.ele_off[p + 1] - L.ele_off[p], nv = L.vert_off[p + 1] - L.vert_off[p];
+      const size_t nh = L.halo_off[p + 1] - L.halo_off[p];
+      const size_t ns = L.vslot_off[L.vert_off[p + 1]] - L.vslot_off[L.vert_off[p]];
+      const size_t fim = NC * ne * sizeof(T) + (2 * nv + nh) * sizeof(T) + 2 * (nv + 1 + ns) + ne;
+      const size_t adv = 2 * NC * ne * sizeof(T) + (nv + nh) * sizeof(T);
+      fim_lds_ = std::max(fim_lds_, fim);
+      adv_lds_ = std::max(adv_lds_, adv);
+      seed_lds_ = std::max(seed_lds_, (nv + nh) * sizeof(T));
+    }
+    fim_lds_ = (fim_lds_ + 15) & ~size_t(15);
+    adv_lds_ = (adv_lds_ + 15) & ~size_t(15);
+    seed_lds_ = (seed_lds_ + 15) & ~size_t(15);
+    if (fim_lds_ > (size_t)lds_max || adv_lds_ > (size_t)lds_max) {
+      err = "patch needs " + std::to_string(std::max(fim_lds_, adv_lds_)) + " B of LDS, device allows " +
+            std::to_string(lds_max) + " B; use smaller patches";
+      return LS_ERR_LIMIT;
+    }
+
+    LS_HIP(dev_upload(&d_vert_off_, L.vert_off.data(), L.vert_off.size()));
+    LS_HIP(dev_upload(&d_ele_off_, L.ele_off.data(), L.ele_off.size()));
+    LS_HIP(dev_upload(&d_halo_off_, L.halo_off.data(), L.halo_off.size()));
+    LS_HIP(dev_upload(&d_halo_ids_, L.halo_ids.data(), L.halo_ids.size()));
+    LS_HIP(dev_upload(&d_copy_local_, reinterpret_cast<const ushort4*>(L.copy_local.data()), (size_t)F_));
+    LS_HIP(dev_upload(&d_vslot_off_, L.vslot_off.data(), L.vslot_off.size()));
+    LS_HIP(dev_upload(&d_vslots_, L.vslots.data(), L.vslots.size()));
+    LS_HIP(dev_upload(&d_padj_off_, L.padj_off.data(), L.padj_off.size()));
+    LS_HIP(dev_upload(&d_padj_, L.padj.data(), L.padj.size()));
+    {
+      std::vector<T> c(L.local_coords.size());
+      for (size_t i = 0; i < c.size(); ++i) c[i] = (T)L.local_coords[i];
+      LS_HIP(dev_upload(&d_coords_, c.data(), c.size()));
+    }
+    LS_HIP(dev_alloc(&d_sigma_, (size_t)F_ * (NC - 1)));
+    LS_HIP(hipMemset(d_sigma_, 0, (size_t)F_ * (NC - 1) * sizeof(T)));
+    LS_HIP(dev_alloc(&d_phi_, N_));
+    LS_HIP(dev_alloc(&d_phi_out_, N_));
+    LS_HIP(dev_alloc(&d_DT_, N_));
+    LS_HIP(dev_alloc(&d_DT_out_, N_));
+    LS_HIP(dev_alloc(&d_label_, N_));
+    LS_HIP(dev_alloc(&d_tsign_, N_));
+    LS_HIP(dev_alloc(&d_bcon_, P_));
+    LS_HIP(dev_alloc(&d_bmin_, P_));
+    LS_HIP(dev_alloc(&d_act_, P_));
+    LS_HIP(dev_alloc(&d_fresh_, P_));
+    LS_HIP(dev_alloc(&d_seedflag_, P_));
+    LS_HIP(dev_alloc(&d_counters_, kNumCounters));
+    for (auto& l : d_lists_) LS_HIP(dev_alloc(&l, P_));
+    LS_HIP(dev_alloc(&d_nb_, P_));
+    LS_HIP(dev_alloc(&d_scan_all_, P_));
+    {
+      std::vector<int32_t> iota(P_);
+      for (int p = 0; p < P_; ++p) iota[p] = p;
+      LS_HIP(hipMemcpy(d_scan_all_, iota.data(), P_ * sizeof(int32_t), hipMemcpyHostToDevice));
+    }
+    LS_HIP(hipMemset(d_act_, 0, P_ * sizeof(int32_t)));
+    LS_HIP(hipMemset(d_fresh_, 0, P_ * sizeof(int32_t)));
+    LS_HIP(hipMemset(d_phi_, 0, N_ * sizeof(T)));
+    nnb_ = 0;
+    uploaded_ = true;
+    return LS_OK;
+  }
+
+  int set_phi(const double* phi_user, std::string& err) override {
+    if (!uploaded_) { err = "layout not built"; return LS_ERR_STATE; }
+    std::vector<T> h(N_);
+    for (int64_t v = 0; v < N_; ++v) h[v] = (T)phi_user[vert_perm_h_[v]];
+    LS_HIP(hipMemcpy(d_phi_, h.data(), N_ * sizeof(T), hipMemcpyHostToDevice));
+    nnb_ = 0;  // a new initial condition seeds from every patch
+    return LS_OK;
+  }
+
+  int set_velocity(const std::vector<double>& sig, std::string& err) override {
+    if (!uploaded_) { err = "layout not built"; return LS_ERR_STATE; }
+    std::vector<T> h(sig.size());
+    for (size_t i = 0; i < sig.size(); ++i) h[i] = (T)sig[i];
+    LS_HIP(hipMemcpy(d_sigma_, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+    return LS_OK;
+  }
+
+  int step(const ls_step_params& prm, ls_stats& st, std::string& err) override {
+    if (!uploaded_) { err = "layout not built"; return LS_ERR_STATE; }
+    if (prm.niter <= 0 || prm.inside_niter < 0) { err = "niter must be > 0, inside_niter >= 0"; return LS_ERR_ARG; }
+    if (prm.mode != LS_MODE_ADVECT) { err = "only LS_MODE_ADVECT is implemented"; return LS_ERR_ARG; }
+    std::memset(&st, 0, sizeof(st));
+    const DevMesh<T> m = mesh();
+    const DevState<T> s = state();
+    const T tol = (T)prm.tol, leps = (T)prm.lambda_eps, bw = (T)prm.bandwidth, dt = (T)prm.dt;
+    hipStream_t q = stream_;
+    LS_HIP(hipEventRecord(ev_[0], q));
+    // Seeds over every patch on the first step, else over the previous narrowband.
+    k_init<T><<<grid_for(N_), kBlock, 0, q>>>(N_, s);
+    k_set_counters<<<1, 64, 0, q>>>(d_counters_, kNumCounters, 0);
+    const int32_t* scan = nnb_ > 0 ? d_nb_cur() : d_scan_all_;
+    const int nscan = nnb_ > 0 ? nnb_ : P_;
+    int32_t* cur = d_lists_[0];
+    k_seeds<T, NC><<<nscan, kBlock, seed_lds_, q>>>(m, s, scan, cur, d_counters_);
+    LS_HIP(hipGetLastError());
+    LS_HIP(hipEventRecord(ev_[1], q));
+    int32_t cnt[kNumCounters];
+    LS_HIP(hipMemcpyAsync(cnt, d_counters_, sizeof(cnt), hipMemcpyDeviceToHost, q));
+    LS_HIP(hipStreamSynchronize(q));
+    int nact = cnt[kCntSeed];
+    st.n_seed_patches = nact;
+    int32_t* fresh_list = d_lists_[1];
+    int32_t* next = d_lists_[2];
+    while (nact > 0) {
+      st.fim_iters++;
+      st.patch_visits += nact;
+      k_sweep<T, NC><<<nact, kBlock, fim_lds_, q>>>(m, s, cur, prm.niter, 1, tol, leps);
+      k_copyback<T><<<nact, kBlock, 0, q>>>(m, s, cur);
+      k_set_counters<<<1, 64, 0, q>>>(d_counters_ + kCntFresh, 2, 0);
+      k_activate<T><<<grid_for(nact), kBlock, 0, q>>>(m, s, cur, nact, bw, fresh_list);
+      LS_HIP(hipGetLastError());
+      int32_t nf = 0;
+      LS_HIP(hipMemcpyAsync(&nf, d_counters_ + kCntFresh, sizeof(int32_t), hipMemcpyDeviceToHost, q));
+      LS_HIP(hipStreamSynchronize(q));
+      if (nf > 0) {
+        st.probes += nf;
+        k_sweep<T, NC><<<nf, kBlock, fim_lds_, q>>>(m, s, fresh_list, 1, 0, tol, leps);
+      }
+      k_finalize<T><<<grid_for(nact + nf), kBlock, 0, q>>>(s, cur, nact, fresh_list, nf, next);
+      LS_HIP(hipGetLastError());
+      LS_HIP(hipMemcpyAsync(&nact, d_counters_ + kCntNext, sizeof(int32_t), hipMemcpyDeviceToHost, q));
+      LS_HIP(hipStreamSynchronize(q));
+      std::swap(cur, next);
+      if (st.fim_iters > 1000000) { err = "FIM did not converge"; return LS_ERR_INTERNAL; }
+    }
+    LS_HIP(hipEventRecord(ev_[2], q));
+    // Narrowband into the buffer not holding the previous one.
+    nb_sel_ ^= 1;
+    k_narrowband<T><<<P_, kBlock, 0, q>>>(m, s, bw, d_nb_cur());
+    LS_HIP(hipGetLastError());
+    LS_HIP(hipMemcpyAsync(cnt, d_counters_, sizeof(cnt), hipMemcpyDeviceToHost, q));
+    LS_HIP(hipStreamSynchronize(q));
+    nnb_ = cnt[kCntNB];
+    st.nb_count = nnb_;
+    st.nb_verts = cnt[kCntNBVerts];
+    LS_HIP(hipEventRecord(ev_[3], q));
+    if (nnb_ > 0) {
+      for (int it = 0; it < prm.inside_niter; ++it) {
+        k_advect<T, NC><<<nnb_, kBlock, adv_lds_, q>>>(m, s, d_nb_cur(), dt);
+        k_copy_phi<T><<<nnb_, kBlock, 0, q>>>(m, s, d_nb_cur());
+      }
+      LS_HIP(hipGetLastError());
+    }
+    LS_HIP(hipEventRecord(ev_[4], q));
+    LS_HIP(hipEventSynchronize(ev_[4]));
+    float ms;
+    LS_HIP(hipEventElapsedTime(&ms, ev_[0], ev_[1])); st.ms_seed = ms;
+    LS_HIP(hipEventElapsedTime(&ms, ev_[1], ev_[2])); st.ms_fim = ms;
+    LS_HIP(hipEventElapsedTime(&ms, ev_[2], ev_[3])); st.ms_nb = ms;
+    LS_HIP(hipEventElapsedTime(&ms, ev_[3], ev_[4])); st.ms_advect = ms;
+    LS_HIP(hipEventElapsedTime(&ms, ev_[0], ev_[4])); st.ms_total = ms;
+    return LS_OK;
+  }
+
+  int get_phi(double* out, std::string& err) override {
+    if (!uploaded_) { err = "layout not built"; return LS_ERR_STATE; }
+    std::vector<T> h(N_);
+    LS_HIP(hipMemcpy(h.data(), d_phi_, N_ * sizeof(T), hipMemcpyDeviceToHost));
+    for (int64_t v = 0; v < N_; ++v) out[vert_perm_h_[v]] = (double)h[v];
+    return LS_OK;
+  }
+
+  int get_narrowband(std::vector<int32_t>& ids, std::string& err) override {
+    ids.assign(nnb_, 0);
+    if (nnb_ > 0) LS_HIP(hipMemcpy(ids.data(), d_nb_cur(), nnb_ * sizeof(int32_t), hipMemcpyDeviceToHost));
+    std::sort(ids.begin(), ids.end());
+    return LS_OK;
+  }
+
+ private:
+  DevMesh<T> mesh() const {
+    DevMesh<T> m;
+    m.P = P_; m.N = N_; m.F = F_;
+    m.vert_off = d_vert_off_; m.ele_off = d_ele_off_; m.halo_off = d_halo_off_; m.halo_ids = d_halo_ids_;
+    m.copy_local = d_copy_local_; m.vslot_off = d_vslot_off_; m.vslots = d_vslots_;
+    m.coords = d_coords_; m.sigma = d_sigma_; m.padj_off = d_padj_off_; m.padj = d_padj_;
+    return m;
+  }
+  DevState<T> state() const {
+    DevState<T> s;
+    s.phi = d_phi_; s.phi_out = d_phi_out_; s.DT = d_DT_; s.DT_out = d_DT_out_;
+    s.label = d_label_; s.tsign = d_tsign_; s.bcon = d_bcon_; s.bmin = d_bmin_;
+    s.act = d_act_; s.fresh = d_fresh_; s.seedflag = d_seedflag_; s.counters = d_counters_;
+    return s;
+  }
+  int32_t* d_nb_cur() const { return nb_sel_ ? d_lists_[3] : d_nb_; }
+
+  void release() {
+    (void)hipSetDevice(device_);
+    void* ptrs[] = {d_vert_off_, d_ele_off_, d_halo_off_, d_halo_ids_, d_copy_local_, d_vslot_off_,
+                    d_vslots_, d_padj_off_, d_padj_, d_coords_, d_sigma_, d_phi_, d_phi_out_, d_DT_,
+                    d_DT_out_, d_label_, d_tsign_, d_bcon_, d_bmin_, d_act_, d_fresh_, d_seedflag_,
+                    d_counters_, d_nb_, d_scan_all_, d_lists_[0], d_lists_[1], d_lists_[2], d_lists_[3]};
+    for (void* p : ptrs) if (p) (void)hipFree(p);
+    for (auto& e : ev_) if (e) (void)hipEventDestroy(e);
+    if (stream_) (void)hipStreamDestroy(stream_);
+  }
+
+  int device_;
+  bool uploaded_ = false;
+  hipStream_t stream_ = nullptr;
+  hipEvent_t ev_[5] = {};
+  int P_ = 0;
+  int64_t N_ = 0, F_ = 0;
+  size_t fim_lds_ = 0, adv_lds_ = 0, seed_lds_ = 0;
+  std::vector<int32_t> vert_off_h_, vert_perm_h_;
+  int nnb_ = 0, nb_sel_ = 0;
+  int32_t *d_vert_off_ = nullptr, *d_ele_off_ = nullptr, *d_halo_off_ = nullptr, *d_halo_ids_ = nullptr;
+  ushort4* d_copy_local_ = nullptr;
+  int32_t* d_vslot_off_ = nullptr;
+  uint16_t* d_vslots_ = nullptr;
+  int32_t *d_padj_off_ = nullptr, *d_padj_ = nullptr;
+  T *d_coords_ = nullptr, *d_sigma_ = nullptr;
+  T *d_phi_ = nullptr, *d_phi_out_ = nullptr, *d_DT_ = nullptr, *d_DT_out_ = nullptr, *d_bmin_ = nullptr;
+  int8_t *d_label_ = nullptr, *d_tsign_ = nullptr;
+  int32_t *d_bcon_ = nullptr, *d_act_ = nullptr, *d_fresh_ = nullptr, *d_seedflag_ = nullptr, *d_counters_ = nullptr;
+  int32_t *d_nb_ = nullptr, *d_scan_all_ = nullptr;
+  int32_t* d_lists_[4] = {nullptr, nullptr, nullptr, nullptr};
+};
+
+Engine* make_engine(int dim, int device, std::string& err) {
+  if (dim == 3) return new EngineImpl<float, 4>(device);
+  if (dim == 2) return new EngineImpl<double, 3>(device);
+  err = "dim must be 2 or 3";
+  return nullptr;
+}
+
+}  // namespace lsamd

@@ -1,0 +1,70 @@
+// Patch layout for the level-set hot path (host side, built once per mesh).
+//
+// The layout fixes every ordering the per-timestep kernels depend on:
+//   * vertices are renumbered patch-contiguously (stable sort by patch label),
+//     following InitPatches2 in src/core3d/cuda/meshFIM3d.cu:495-512;
+//   * every element is copied once per distinct patch among its vertices, in
+//     first-appearance order, and the copies are stably sorted by patch, as in
+//     InitPatches src/core3d/cuda/meshFIM3d.cu:423-493 with the labelling of
+//     src/core3d/include/meshFIM_kernels.h:502-561;
+//   * each owned vertex lists its element-copy slots ordered by (corner, copy),
+//     which is the order the reference's gather CSR produces
+//     (src/core3d/cuda/meshFIM3d.cu:537-542, filtered per patch at
+//     src/core3d/include/redistance_kernels3d.h:311-329).
+// Unlike the reference, each patch also gets a sorted halo list so a kernel
+// fetches every outside vertex once per patch, element corners are stored as
+// 16-bit patch-local indices, and per-copy geometry is precomputed in FP64.
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace lsamd {
+
+constexpr int kMaxLocal = 65535;  // patch-local indices and slots are uint16
+
+struct Layout {
+  int dim = 0;        // 2 = triangles, 3 = tetrahedra
+  int nc = 0;         // corners per element (dim + 1)
+  int64_t N = 0;      // vertices
+  int64_t E = 0;      // elements
+  int64_t F = 0;      // element copies
+  int P = 0;          // patches
+  std::vector<int32_t> labels;      // N, patch of each ORIGINAL vertex
+  std::vector<int32_t> vert_perm;   // N, new -> original vertex id
+  std::vector<int32_t> vert_iperm;  // N, original -> new vertex id
+  std::vector<int32_t> vert_off;    // P+1
+  std::vector<int32_t> ele_off;     // P+1
+  std::vector<int32_t> copy_src;    // F, original element of each copy
+  std::vector<int32_t> copy_verts;  // F*nc, renumbered vertex ids (AoS)
+  std::vector<uint16_t> copy_local; // F*4, patch-local corner ids (padded to 4)
+  std::vector<int32_t> halo_off;    // P+1
+  std::vector<int32_t> halo_ids;    // renumbered ids of halo vertices, sorted per patch
+  std::vector<int32_t> vslot_off;   // N+1, by renumbered vertex (global offsets)
+  std::vector<uint16_t> vslots;     // patch-local slot = local_copy*nc + corner
+  std::vector<double> xyz;          // N*3, renumbered order
+  std::vector<double> local_coords; // F*6 (3D: |AB|, AC.X, AC.Y, AD.X, AD.Y, AD.Z) or F*3 (2D)
+  std::vector<double> frame;        // F*9 (3D: X,Y,Z axes) or F*6 (2D: X,Y axes) for velocity projection
+  std::vector<int32_t> padj_off;    // P+1 patch adjacency (sorted, unique)
+  std::vector<int32_t> padj;
+  int max_ep = 0, max_vp = 0, max_hp = 0, max_vslots = 0;
+};
+
+// GraphPartition_Square (src/core3d/cuda/meshFIM3d.cu:269-352): vertex index
+// k*L*W + i*L + j of an L x W x H grid goes to block (k/bh, i/bw, j/bl).
+int partition_grid_blocks(int64_t N, int L, int W, int H, int bl, int bw, int bh,
+                          std::vector<int32_t>& labels, int& nparts, std::string& err);
+
+// Recursive coordinate bisection into exactly `nparts` parts: the build's
+// deterministic stand-in for METIS_PartGraphKway (src/core3d/cuda/meshFIM3d.cu:401).
+int partition_rcb(int64_t N, const double* xyz, int nparts, std::vector<int32_t>& labels,
+                  std::string& err);
+
+int build_layout(int dim, int64_t N, const double* xyz, int64_t E, const int32_t* cells,
+                 const int32_t* labels, int nparts, Layout& L, std::string& err);
+
+// Per-copy velocity in the copy's local frame (3D: 3 values, 2D: 2 values),
+// from one velocity per ORIGINAL element (E x 3).
+void project_velocity(const Layout& L, const double* vel_per_element, std::vector<double>& out);
+
+}  // namespace lsamd

@@ -1,0 +1,188 @@
+"""ctypes binding of liblevelset_hip.so (include/levelset_hip.h).
+
+Loading fails loudly when the HIP library is missing: there is no CPU
+fallback on the product path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liblevelset_hip.so")
+
+LS_F32, LS_F64 = 0, 1
+LS_MODE_ADVECT, LS_MODE_MCF = 0, 1
+
+
+class StepParams(C.Structure):
+    _fields_ = [("dt", C.c_double), ("bandwidth", C.c_double), ("tol", C.c_double),
+                ("lambda_eps", C.c_double), ("niter", C.c_int32), ("inside_niter", C.c_int32),
+                ("mode", C.c_int32), ("reserved", C.c_int32)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("fim_iters", C.c_int32), ("nb_count", C.c_int32), ("patch_visits", C.c_int64),
+                ("probes", C.c_int64), ("nb_verts", C.c_int64), ("n_seed_patches", C.c_int32),
+                ("reserved", C.c_int32), ("ms_seed", C.c_double), ("ms_fim", C.c_double),
+                ("ms_nb", C.c_double), ("ms_advect", C.c_double), ("ms_total", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
+class LayoutInfo(C.Structure):
+    _fields_ = [("n_verts", C.c_int64), ("n_cells", C.c_int64), ("n_copies", C.c_int64),
+                ("n_patches", C.c_int32), ("max_patch_copies", C.c_int32),
+                ("max_patch_verts", C.c_int32), ("max_patch_halo", C.c_int32),
+                ("max_vertex_slots", C.c_int32), ("dim", C.c_int32)]
+
+
+EXPORTS = ["ls_create", "ls_destroy", "ls_last_error", "ls_set_mesh", "ls_partition_grid",
+           "ls_partition_rcb", "ls_partition_labels", "ls_get_layout_info", "ls_get_layout",
+           "ls_set_phi", "ls_set_velocity", "ls_step", "ls_get_phi", "ls_get_narrowband"]
+
+
+def build(verbose=False) -> str:
+    cmd = ["make", "-C", os.path.join(_HERE, "csrc")]
+    if not verbose:
+        cmd.insert(1, "-s")
+    subprocess.check_call(cmd)
+    return LIB_PATH
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        L.ls_create.argtypes = [C.POINTER(vp), C.c_int]
+        L.ls_destroy.argtypes = [vp]
+        L.ls_destroy.restype = None
+        L.ls_last_error.argtypes = [vp]
+        L.ls_last_error.restype = C.c_char_p
+        L.ls_set_mesh.argtypes = [vp, C.c_int, C.c_int64, vp, C.c_int64, vp]
+        L.ls_partition_grid.argtypes = [vp] + [C.c_int] * 6
+        L.ls_partition_rcb.argtypes = [vp, C.c_int]
+        L.ls_partition_labels.argtypes = [vp, vp, C.c_int]
+        L.ls_get_layout_info.argtypes = [vp, C.POINTER(LayoutInfo)]
+        L.ls_get_layout.argtypes = [vp] + [vp] * 5
+        L.ls_set_phi.argtypes = [vp, vp, C.c_int]
+        L.ls_set_velocity.argtypes = [vp, vp, C.c_int]
+        L.ls_step.argtypes = [vp, C.POINTER(StepParams), C.POINTER(Stats)]
+        L.ls_get_phi.argtypes = [vp, vp, C.c_int]
+        L.ls_get_narrowband.argtypes = [vp, vp, C.POINTER(C.c_int32)]
+        _lib = L
+    return _lib
+
+
+class LevelSetError(RuntimeError):
+    pass
+
+
+def step_params(dt, bandwidth, niter, tol=1e-5, lambda_eps=1e-5, inside_niter=1, mode=LS_MODE_ADVECT):
+    return StepParams(dt, bandwidth, tol, lambda_eps, niter, inside_niter, mode, 0)
+
+
+class Solver:
+    """One mesh on one GPU through the C-ABI."""
+
+    def __init__(self, device=0):
+        self.L = lib()
+        self.ctx = C.c_void_p()
+        self._check(self.L.ls_create(C.byref(self.ctx), device))
+        self.N = self.E = 0
+        self.dim = 0
+
+    def _check(self, rc):
+        if rc != 0:
+            msg = self.L.ls_last_error(self.ctx).decode() if self.ctx else ""
+            raise LevelSetError(f"levelset rc={rc}: {msg}")
+
+    def close(self):
+        if self.ctx:
+            self.L.ls_destroy(self.ctx)
+            self.ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_mesh(self, dim, xyz, cells):
+        self._xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+        self._cells = np.ascontiguousarray(cells, dtype=np.int32)
+        self.dim, self.N, self.E = dim, self._xyz.shape[0], self._cells.shape[0]
+        self._check(self.L.ls_set_mesh(self.ctx, dim, self.N, self._xyz.ctypes.data, self.E,
+                                       self._cells.ctypes.data))
+
+    def partition_labels(self, labels, nparts):
+        lab = np.ascontiguousarray(labels, dtype=np.int32)
+        self._check(self.L.ls_partition_labels(self.ctx, lab.ctypes.data, int(nparts)))
+
+    def partition_rcb(self, nparts):
+        self._check(self.L.ls_partition_rcb(self.ctx, int(nparts)))
+
+    def partition_grid(self, L, W, H, bl, bw, bh):
+        self._check(self.L.ls_partition_grid(self.ctx, L, W, H, bl, bw, bh))
+
+    def layout_info(self):
+        info = LayoutInfo()
+        self._check(self.L.ls_get_layout_info(self.ctx, C.byref(info)))
+        return {k: getattr(info, k) for k, _ in info._fields_}
+
+    def layout(self):
+        info = self.layout_info()
+        N, F, P, nc = info["n_verts"], info["n_copies"], info["n_patches"], info["dim"] + 1
+        vp = np.empty(N, np.int32)
+        cs = np.empty(F, np.int32)
+        vo = np.empty(P + 1, np.int32)
+        eo = np.empty(P + 1, np.int32)
+        cv = np.empty(F * nc, np.int32)
+        self._check(self.L.ls_get_layout(self.ctx, vp.ctypes.data, cs.ctypes.data, vo.ctypes.data,
+                                         eo.ctypes.data, cv.ctypes.data))
+        return {"vert_perm": vp, "copy_src": cs, "vert_off": vo, "ele_off": eo,
+                "copy_verts": cv.reshape(F, nc)}
+
+    def set_phi(self, phi):
+        a = np.ascontiguousarray(phi, dtype=np.float64)
+        self._check(self.L.ls_set_phi(self.ctx, a.ctypes.data, LS_F64))
+
+    def set_velocity(self, vel, patch_sigma=0):
+        a = np.ascontiguousarray(vel, dtype=np.float64)
+        self._check(self.L.ls_set_velocity(self.ctx, a.ctypes.data, int(patch_sigma)))
+
+    def step(self, params: StepParams) -> Stats:
+        st = Stats()
+        self._check(self.L.ls_step(self.ctx, C.byref(params), C.byref(st)))
+        return st
+
+    def phi(self):
+        out = np.empty(self.N, dtype=np.float64)
+        self._check(self.L.ls_get_phi(self.ctx, out.ctypes.data, LS_F64))
+        return out
+
+    def narrowband(self):
+        n = C.c_int32(0)
+        self._check(self.L.ls_get_narrowband(self.ctx, None, C.byref(n)))
+        ids = np.empty(max(n.value, 1), np.int32)
+        self._check(self.L.ls_get_narrowband(self.ctx, ids.ctypes.data, C.byref(n)))
+        return ids[: n.value].copy()
+
+
+def solver_for_case(case, device=0, partition="labels"):
+    s = Solver(device)
+    s.set_mesh(case.dim, case.xyz, case.cells)
+    s.partition_labels(case.labels, case.nparts)
+    s.set_phi(case.phi0)
+    s.set_velocity(case.vel)
+    return s

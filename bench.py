@@ -1,0 +1,217 @@
+#!/usr/bin/env python3
+"""Benchmark: narrowband vertex-updates/s of the level-set timestep on one MI355X.
+
+Workload: the 3D torus revolve -- a 64^3 Kuhn tet mesh (262,144 vertices,
+1,500,282 tets), jitter 0.1h, torus (R=0.3, r=0.1), rigid rotation about z
+(omega = 2 pi), CFL 0.5, band 6h, 256-vertex patches, 100 timesteps. A
+vertex-update is one owned vertex of a narrowband patch receiving its final
+phi for a step (the set CopyOutBack_levelset3d writes,
+src/core3d/include/meshFIM_kernels.h:22-34). One bench "step" is one full
+timestep (seeds, sign save, FIM, narrowband + sign restore, advection).
+
+Besides the JSON line's throughput it reports:
+  * roofline of the dominant kernel (the FIM patch sweep), from HIP-event
+    durations of its launches and the algorithmic bytes of the patches they
+    visited;
+  * the CPU baseline: the oracle's FP32 instance (same scheme, OpenMP over
+    patches, all host threads) on a bounded sample of the same run;
+  * parity: the GPU against the FP32 oracle (bit-exact) and the FP64 oracle
+    (L-inf on phi, narrowband membership diff) on the first timesteps.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+import __graft_entry__ as ge  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+
+
+def fim_bytes(st):
+    """Algorithmic bytes of FIM patch visits and probes: per visit
+    40*E + 4*(V+H) + 5*V (16 B connectivity + 24 B local coordinates per
+    element copy, distances in, distances out + 1 B flag); probes do not write
+    distances: 40*E + 4*(V+H) + V."""
+    visits = 40 * st.visit_copies + 4 * (st.visit_verts + st.visit_halo) + 5 * st.visit_verts
+    probes = 40 * st.probe_copies + 4 * (st.probe_verts + st.probe_halo) + 1 * st.probe_verts
+    return visits + probes
+
+
+def adv_bytes(st):
+    """Advection per narrowband patch: 28*E + 16*(V+H) + 4*V."""
+    return 28 * st.adv_copies + 16 * (st.adv_verts + st.adv_halo) + 4 * st.adv_verts
+
+
+def lockstep(case, oracle_mod, lsb, precision, steps):
+    """FP32 oracle: runs beside the GPU from phi0 (bit-exact check, timed as the
+    CPU baseline). FP64 oracle: restarted every step from the GPU's phi and
+    narrowband, so each step is compared from identical inputs; vertices whose
+    advection hinges on a threshold decision inside the FP32-storage window are
+    reported separately."""
+    g = lsb.solver_for_case(case)
+    o = oracle_mod.Oracle(case.dim, case.xyz, case.cells, case.labels, case.nparts, precision)
+    o.set_phi(case.phi0)
+    o.set_velocity(case.vel)
+    gp = lsb.step_params(case.dt, case.bandwidth, case.niter)
+    op = oracle_mod.make_params(case.dt, case.bandwidth, case.niter)
+    res = {"steps": 0, "linf": 0.0, "linf_all": 0.0, "ambiguous": 0, "nb_diff": 0, "bit_mismatch": 0,
+           "cpu_s": 0.0, "cpu_verts": 0}
+    gphi, gnb = case.phi0, None
+    for _ in range(steps):
+        if precision == 64:
+            o.set_phi(gphi)
+            if gnb is not None:
+                o.set_narrowband(gnb)
+        g.step(gp)
+        t0 = time.perf_counter()
+        ost = o.step(op)
+        res["cpu_s"] += time.perf_counter() - t0
+        res["cpu_verts"] += ost.nb_verts
+        gphi, ophi = g.phi(), o.phi()
+        gnb, onb = g.narrowband(), o.narrowband()
+        res["nb_diff"] += int(np.setxor1d(gnb, onb).size)
+        fin = (np.abs(ophi) < 1e8) & (np.abs(gphi) < 1e8)
+        d = np.abs(gphi - ophi)
+        res["linf_all"] = max(res["linf_all"], float(np.max(d[fin])) if fin.any() else 0.0)
+        if precision == 64:
+            amb = o.ambiguous()
+            res["ambiguous"] = max(res["ambiguous"], int(amb.sum()))
+            ok = fin & ~amb
+            res["linf"] = max(res["linf"], float(np.max(d[ok])) if ok.any() else 0.0)
+        res["bit_mismatch"] += int(np.count_nonzero(gphi.astype(np.float32) != ophi.astype(np.float32)))
+        res["steps"] += 1
+        if ost.nb_count == 0:
+            break
+    g.close()
+    o.close()
+    return res
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=8, help="timesteps of the CPU-baseline sample")
+    ap.add_argument("--parity-steps", type=int, default=4, help="timesteps checked against the FP64 oracle")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU path is not built.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    ge.load_package()
+    import levelset_amd.cases as cases
+    import levelset_amd.levelset as lsb
+
+    case = cases.config2(steps=args.steps)
+    g = lsb.solver_for_case(case)
+    info = g.layout_info()
+    prm = lsb.step_params(case.dt, case.bandwidth, case.niter)
+    for _ in range(args.warmup):
+        g.step(prm)
+    g.set_phi(case.phi0)  # the timed region is the configured run from phi0
+
+    tot = {"verts": 0, "sweep_ms": 0.0, "sweep_launches": 0, "sweep_bytes": 0, "adv_ms": 0.0,
+           "adv_bytes": 0, "adv_launches": 0, "ms_fim": 0.0, "ms_total": 0.0, "fim_iters": 0,
+           "visits": 0, "probes": 0}
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        st = g.step(prm)
+        tot["verts"] += st.nb_verts
+        tot["sweep_ms"] += st.ms_sweep_kernels
+        tot["sweep_launches"] += st.sweep_launches
+        tot["sweep_bytes"] += fim_bytes(st)
+        tot["adv_ms"] += st.ms_advect_kernels
+        tot["adv_bytes"] += adv_bytes(st)
+        tot["adv_launches"] += 1 if st.nb_count > 0 else 0
+        tot["ms_fim"] += st.ms_fim
+        tot["ms_total"] += st.ms_total
+        tot["fim_iters"] += st.fim_iters
+        tot["visits"] += st.patch_visits
+        tot["probes"] += st.probes
+    wall = time.perf_counter() - t0
+    g.close()
+
+    value = tot["verts"] / wall
+    sweep_avg_ms = tot["sweep_ms"] / max(tot["sweep_launches"], 1)
+    sweep_avg_bytes = tot["sweep_bytes"] / max(tot["sweep_launches"], 1)
+    achieved = sweep_avg_bytes / (sweep_avg_ms * 1e-3) / 1e9
+    adv_gbs = tot["adv_bytes"] / max(tot["adv_ms"] * 1e-3, 1e-12) / 1e9
+
+    out = {
+        "metric": "narrowband vertex-updates/s, learned warm-start vs plain FIM @ Linf<=1e-5",
+        "value": value,
+        "unit": "vertex-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * wall / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "3d_torus_revolve_64: 64^3 Kuhn tets (262144 verts, 1500282 tets), "
+                        "256-vertex patches, band 6h, CFL 0.5, rigid rotation, 100 timesteps",
+            "mode": "plain_fim",
+            "patches": info["n_patches"],
+            "element_copies": info["n_copies"],
+            "fim_iters_per_step": tot["fim_iters"] / args.steps,
+            "patch_visits_per_step": tot["visits"] / args.steps,
+            "probes_per_step": tot["probes"] / args.steps,
+            "gpu_event_ms_per_step": tot["ms_total"] / args.steps,
+            "fim_ms_per_step": tot["ms_fim"] / args.steps,
+        },
+        "roofline": {
+            "kernel": "k_sweep (FIM patch sweep)",
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "avg_launch_ms": sweep_avg_ms,
+            "avg_launch_algorithmic_bytes": sweep_avg_bytes,
+            "launches": tot["sweep_launches"],
+            "advect_achieved_gbs": adv_gbs,
+        },
+    }
+
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        fp32 = lockstep(case, O, lsb, 32, args.cpu_steps)
+        fp64 = lockstep(case, O, lsb, 64, args.parity_steps)
+        out["cpu_baseline"] = {
+            "value": fp32["cpu_verts"] / fp32["cpu_s"],
+            "unit": "vertex-updates/s",
+            "cores": O.max_threads(),
+            "kind": "port",
+            "sample": f"first {fp32['steps']} of the {args.steps} timesteps of the same workload, "
+                      "oracle FP32 instance, OpenMP over patches",
+        }
+        out["parity"] = {
+            "fp32_oracle_steps": fp32["steps"],
+            "fp32_bit_mismatches": fp32["bit_mismatch"],
+            "fp32_nb_membership_diff": fp32["nb_diff"],
+            "fp64_oracle_steps": fp64["steps"],
+            "linf_vs_fp64": fp64["linf"],
+            "linf_vs_fp64_incl_threshold_ambiguous": fp64["linf_all"],
+            "threshold_ambiguous_verts_max_per_step": fp64["ambiguous"],
+            "fp64_nb_membership_diff": fp64["nb_diff"],
+            "tolerance": 1e-5,
+        }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -43,7 +43,7 @@ typedef struct ls_step_params {
   int32_t niter;       /* sweeps per patch visit (NUM_ITER: 5 in 3D, 10 in 2D) */
   int32_t inside_niter;/* advection passes per step (insideIterations_) */
   int32_t mode;        /* LS_MODE_ADVECT */
-  int32_t reserved;
+  int32_t max_fim_iters;/* 0: run the FIM loop to convergence; > 0: stop after that many iterations */
 } ls_step_params;
 
 typedef struct ls_stats {
@@ -53,8 +53,15 @@ typedef struct ls_stats {
   int64_t probes;          /* neighbour probes */
   int64_t nb_verts;        /* vertex-updates of the step (owned vertices of narrowband patches) */
   int32_t n_seed_patches;
-  int32_t reserved;
-  double ms_seed, ms_fim, ms_nb, ms_advect, ms_total; /* HIP-event timings */
+  int32_t sweep_launches;  /* launches of the FIM patch-sweep kernel (visits + probes) */
+  double ms_seed, ms_fim, ms_nb, ms_advect, ms_total; /* HIP-event stage timings */
+  double ms_sweep_kernels;  /* summed durations of the FIM patch-sweep launches */
+  double ms_advect_kernels; /* summed durations of the advection launches */
+  /* Sums over patch visits of element copies, owned and halo vertices, for
+   * algorithmic-byte accounting: FIM visits, neighbour probes, advection. */
+  int64_t visit_copies, visit_verts, visit_halo;
+  int64_t probe_copies, probe_verts, probe_halo;
+  int64_t adv_copies, adv_verts, adv_halo;
 } ls_stats;
 
 typedef struct ls_layout_info {
@@ -74,6 +81,12 @@ const char* ls_last_error(const ls_ctx* ctx);
  * LevelSet::initializeMesh (src/LevelSet.cu:111-144). */
 int ls_set_mesh(ls_ctx* ctx, int dim, int64_t n_verts, const double* xyz, int64_t n_cells,
                 const int32_t* cells);
+
+/* Storage precision of the tet engine: LS_F32 (default; element arithmetic is
+ * FP64, distances and geometry are stored in FP32) or LS_F64 (FP64 storage,
+ * identical to an FP64 host build). Triangle meshes always use LS_F64.
+ * Call before partitioning. */
+int ls_set_precision(ls_ctx* ctx, int dtype);
 
 /* Partition + patch layout (meshFIM3d::InitPatches/InitPatches2/
  * GenerateBlockNeighbors, src/core3d/cuda/meshFIM3d.cu:423-570).

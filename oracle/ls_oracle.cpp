@@ -5,9 +5,9 @@
 //
 // A plain C++ restatement of the reference scheme, templated on the real type:
 //   * Solver<double> is the FP64 host build (the parity oracle);
-//   * Solver<float> keeps the GPU's FP32 operation order, FP64 only for the
-//     tet solver's quadratic discriminant and roots, so GPU results can be
-//     compared with it bit for bit.
+//   * Solver<float> stores distances and level-set values in FP32 exactly as
+//     the GPU does and evaluates every element update in FP64 with the GPU's
+//     operation order, so GPU results can be compared with it bit for bit.
 // Every stage cites the reference function it restates. The patch layout is
 // rebuilt here with the reference's own steps (stable sorts over SoA arrays)
 // so the product's layout builder is checked independently.
@@ -28,7 +28,10 @@
 
 namespace {
 
-constexpr double kLarge = 1e9;  // LARGENUM (value external to the reference; 1e9 is exact in FP32)
+constexpr double kLarge = 1e9;
+// Relative window for threshold-ambiguous advection decisions: well above
+// the FP32 storage error of distances (about 1e-7 relative).
+constexpr double kAmbRel = 1e-6;  // LARGENUM (value external to the reference; 1e9 is exact in FP32)
 
 enum Label { SeedPoint = 0, ActivePoint = 1, AlivePoint = 2, FarPoint = 3 };  // redistance3d.h:22-25
 
@@ -189,18 +192,24 @@ T tet_solve(T TA, T TB, T TC, T ACAC, T ACBC, T ACCD, T BCBC, T BCCD, T CDCD, T 
 
 // Edge dot products from the six local coordinates and the four permuted
 // solves of one element (src/core3d/include/redistance_kernels3d.h:373-414).
-template <typename T>
+// The element update is evaluated in FP64 whatever the storage type T: the
+// reference solver's quadratic coefficients cancel badly in FP32 and its
+// candidate acceptance is discontinuous, so FP32 arithmetic there drifts
+// 1e-4 from the FP64 scheme while FP32 storage alone stays within 1e-7.
 struct TetGeom {
-  T ACAC, ACBC, ACCD, BCBC, BCCD, CDCD, ADAD, ADBD, ADCD, BDBD, CDBD;
-  explicit TetGeom(const T* L) {
+  double ACAC, ACBC, ACCD, BCBC, BCCD, CDCD, ADAD, ADBD, ADCD, BDBD, CDBD;
+  template <typename T>
+  explicit TetGeom(const T* Lt) {
+    double L[6];
+    for (int k = 0; k < 6; ++k) L[k] = (double)Lt[k];
     ACBC = L[1] * (L[1] - L[0]) + L[2] * L[2];
     BCCD = (L[1] - L[0]) * (L[3] - L[1]) + L[2] * (L[4] - L[2]);
     ACCD = L[1] * (L[3] - L[1]) + L[2] * (L[4] - L[2]);
     ADBD = L[3] * (L[3] - L[0]) + L[4] * L[4] + L[5] * L[5];
-    const T ACAD = L[1] * L[3] + L[2] * L[4];
-    const T BCBD = (L[1] - L[0]) * (L[3] - L[0]) + L[2] * L[4];
-    const T ADBC = ACBC + BCCD;
-    const T ACBD = ACBC + ACCD;
+    const double ACAD = L[1] * L[3] + L[2] * L[4];
+    const double BCBD = (L[1] - L[0]) * (L[3] - L[0]) + L[2] * L[4];
+    const double ADBC = ACBC + BCCD;
+    const double ACBD = ACBC + ACCD;
     CDBD = ADBD - ACBD;
     ADCD = ADBD - ADBC;
     CDCD = ADCD - ACCD;
@@ -209,30 +218,33 @@ struct TetGeom {
     BDBD = BCBD + CDBD;
     BCBC = BCBD - BCCD;
   }
-  void update(T* v, T leps) const {  // v = slot values of corners A,B,C,D
-    const T TA = v[0], TB = v[1], TC = v[2], TD = v[3];
-    v[3] = std::fmin(TD, tet_solve(TA, TB, TC, ACAC, ACBC, ACCD, BCBC, BCCD, CDCD, leps));
-    v[0] = std::fmin(TA, tet_solve(TB, TD, TC, BCBC, -BCCD, -ACBC, CDCD, ACCD, ACAC, leps));
-    v[1] = std::fmin(TB, tet_solve(TA, TD, TC, ACAC, -ACCD, -ACBC, CDCD, BCCD, BCBC, leps));
-    v[2] = std::fmin(TC, tet_solve(TA, TB, TD, ADAD, ADBD, -ADCD, BDBD, -CDBD, CDCD, leps));
+  template <typename T>
+  void update(T* v, double leps) const {  // v = slot values of corners A,B,C,D
+    const double TA = v[0], TB = v[1], TC = v[2], TD = v[3];
+    v[3] = (T)std::fmin(TD, tet_solve(TA, TB, TC, ACAC, ACBC, ACCD, BCBC, BCCD, CDCD, leps));
+    v[0] = (T)std::fmin(TA, tet_solve(TB, TD, TC, BCBC, -BCCD, -ACBC, CDCD, ACCD, ACAC, leps));
+    v[1] = (T)std::fmin(TB, tet_solve(TA, TD, TC, ACAC, -ACCD, -ACBC, CDCD, BCCD, BCBC, leps));
+    v[2] = (T)std::fmin(TC, tet_solve(TA, TB, TD, ADAD, ADBD, -ADCD, BDBD, -CDBD, CDCD, leps));
   }
 };
 
 // Triangle edge lengths from the three local coordinates and the three
-// permuted solves (src/core2d/include/redistance_kernels.h:441-462).
-template <typename T>
+// permuted solves (src/core2d/include/redistance_kernels.h:441-462), FP64.
 struct TriGeom {
-  T LenAB, LenBC, LenCA;
-  explicit TriGeom(const T* c) {
+  double LenAB, LenBC, LenCA;
+  template <typename T>
+  explicit TriGeom(const T* ct) {
+    const double c[3] = {(double)ct[0], (double)ct[1], (double)ct[2]};
     LenAB = std::fabs(c[0]);
     LenBC = std::sqrt(c[2] * c[2] + (c[1] - c[0]) * (c[1] - c[0]));
     LenCA = std::sqrt(c[1] * c[1] + c[2] * c[2]);
   }
-  void update(T* v, T) const {
-    const T TA = v[0], TB = v[1], TC = v[2];
-    v[2] = std::fmin(TC, tri_solve(TA, TB, LenAB, LenBC, LenCA));
-    v[0] = std::fmin(TA, tri_solve(TB, TC, LenBC, LenCA, LenAB));
-    v[1] = std::fmin(TB, tri_solve(TC, TA, LenCA, LenAB, LenBC));
+  template <typename T>
+  void update(T* v, double) const {
+    const double TA = v[0], TB = v[1], TC = v[2];
+    v[2] = (T)std::fmin(TC, tri_solve(TA, TB, LenAB, LenBC, LenCA));
+    v[0] = (T)std::fmin(TA, tri_solve(TB, TC, LenBC, LenCA, LenAB));
+    v[1] = (T)std::fmin(TB, tri_solve(TC, TA, LenCA, LenAB, LenBC));
   }
 };
 
@@ -248,6 +260,8 @@ struct lso_params {
   int inside_niter;     // advection passes per step
   int mode;             // 0: advection, 1: mean-curvature flow
   int nthreads;         // OpenMP threads (0 = runtime default)
+  int max_fim_iters;    // 0: FIM to convergence; > 0: stop after that many iterations
+  int pad;
 };
 struct lso_stats {
   int fim_iters;
@@ -271,6 +285,9 @@ struct Base {
   virtual int get_nb(int32_t*) const = 0;
   virtual void get_layout(int32_t*, int32_t*, int32_t*, int32_t*, int32_t*) const = 0;
   virtual int64_t ncopies() const = 0;
+  virtual void set_nb(const int32_t*, int) = 0;
+  virtual void get_amb(uint8_t*) const = 0;
+  virtual void get_coords(double*) const = 0;
   virtual int nparts() const = 0;
 };
 
@@ -295,6 +312,10 @@ struct Solver : Base {
   std::vector<int32_t> label, con;
   std::vector<char> tsign;
   std::vector<int32_t> narrowband;
+  // Vertices whose last advection update depended on a threshold decision
+  // (|H| > 1e-8 per element, |sum alpha~ vol| > 1e-8 per vertex) that lies
+  // inside the window FP32 storage of the inputs can move it across.
+  std::vector<uint8_t> amb;
 
   Solver(int dim_, int64_t N_, const double* xyz, int64_t E_, const int32_t* cells,
          const int32_t* labels, int P_)
@@ -400,7 +421,7 @@ struct Solver : Base {
     for (size_t k = 0; k < pairs.size(); ++k) badj[k] = pairs[k].second;
 
     phi.assign(N, (T)0); DT.assign(N, (T)kLarge); DT_out.assign(N, (T)0); phi_out.assign(N, (T)0);
-    label.assign(N, FarPoint); con.assign(N, 0); tsign.assign(N, 1);
+    label.assign(N, FarPoint); con.assign(N, 0); tsign.assign(N, 1); amb.assign(N, 0);
     sigma.assign((dim == 3 ? 3 : 2) * F, (T)0);
   }
 
@@ -478,6 +499,15 @@ struct Solver : Base {
     if (el) std::copy(ele.begin(), ele.end(), el);
   }
   int64_t ncopies() const override { return F; }
+  void set_nb(const int32_t* ids, int n) override { narrowband.assign(ids, ids + n); }
+  void get_coords(double* out) const override {
+    const int nk = dim == 3 ? 6 : 3;
+    for (int64_t c = 0; c < F; ++c)
+      for (int k = 0; k < nk; ++k) out[c * nk + k] = (double)coords[k * F + c];
+  }
+  void get_amb(uint8_t* out) const override {
+    for (int64_t v = 0; v < N; ++v) out[vert_permute[v]] = amb[v];
+  }
   int nparts() const override { return P; }
 
   // Seeds (kernel_seedlabel / kernel_seedlabel_narrowband,
@@ -516,7 +546,7 @@ struct Solver : Base {
   // FIM patch sweep (kernel_update_values / kernel_run_check_neghbor,
   // redistance_kernels3d.h:287-621). Reads DT, writes con and (if write_out)
   // DT_out for owned non-seed vertices.
-  void update_patch(int p, int niter, bool write_out, T tol, T leps) {
+  void update_patch(int p, int niter, bool write_out, T tol, double leps) {
     const int32_t vs = vert_offsets[p], ve = vert_offsets[p + 1];
     const int32_t es = ele_offsets[p], ee = ele_offsets[p + 1];
     const int ne = ee - es, nv = ve - vs;
@@ -532,8 +562,8 @@ struct Solver : Base {
       for (int c = 0; c < ne; ++c) {
         T Lc[6];
         for (int k = 0; k < ncoord; ++k) Lc[k] = coords[k * F + es + c];
-        if (dim == 3) TetGeom<T>(Lc).update(&s[c * nc], leps);
-        else TriGeom<T>(Lc).update(&s[c * nc], leps);
+        if (dim == 3) TetGeom(Lc).update(&s[c * nc], leps);
+        else TriGeom(Lc).update(&s[c * nc], leps);
       }
       for (int v = 0; v < nv; ++v) {
         const std::vector<int32_t>& lm = lmem[vs + v];
@@ -556,13 +586,14 @@ struct Solver : Base {
 
   // Host active-list loop (redistance3d::GenerateData, redistance3d.cu:67-224).
   void fim(std::vector<int32_t> active, const lso_params& prm, lso_stats& st) {
-    const T tol = (T)prm.tol, leps = (T)prm.lambda_eps, bw = (T)prm.bandwidth;
+    const T tol = (T)prm.tol, bw = (T)prm.bandwidth;
+    const double leps = prm.lambda_eps;
     DT_out = DT;
     std::fill(con.begin(), con.end(), 0);
     std::vector<char> bcon(P, 0);
     std::vector<T> bmin(P, (T)kLarge);
     std::vector<int> blabel(P, FarPoint);
-    while (!active.empty()) {
+    while (!active.empty() && (prm.max_fim_iters <= 0 || st.fim_iters < prm.max_fim_iters)) {
       st.fim_iters++;
       st.patch_visits += (int64_t)active.size();
       #pragma omp parallel for schedule(dynamic, 1)
@@ -636,80 +667,95 @@ struct Solver : Base {
     const int32_t vs = vert_offsets[p], ve = vert_offsets[p + 1];
     const int32_t es = ele_offsets[p], ee = ele_offsets[p + 1];
     const int ne = ee - es;
-    const T dt = (T)prm.dt;
-    std::vector<T> s_up(nc * ne), s_dn(nc * ne);
+    const double dt = prm.dt;
+    std::vector<T> s_up(nc * ne), s_dn(nc * ne);  // per-slot products in the storage type
+    std::vector<uint8_t> eamb(ne, 0);
+    std::vector<double> s_dnabs(nc * ne, 0.0);
     for (int c = 0; c < ne; ++c) {
       const int64_t g = es + c;
-      T ph[4], at[4] = {0, 0, 0, 0}, H, vol;
-      for (int i = 0; i < nc; ++i) ph[i] = phi[ele[i * F + g]];
+      double ph[4], at[4] = {0, 0, 0, 0}, H, vol;
+      for (int i = 0; i < nc; ++i) ph[i] = (double)phi[ele[i * F + g]];
       if (dim == 3) {
-        const T L0 = coords[0 * F + g], L1 = coords[1 * F + g], L2 = coords[2 * F + g];
-        const T L3 = coords[3 * F + g], L4 = coords[4 * F + g], L5 = coords[5 * F + g];
-        const T sg[3] = {sigma[0 * F + g], sigma[1 * F + g], sigma[2 * F + g]};
-        vol = std::fabs(L0 * L2 * L5) / (T)6;
-        T gr[4][3];
-        gr[1][0] = (T)1 / L0;
+        const double L0 = coords[0 * F + g], L1 = coords[1 * F + g], L2 = coords[2 * F + g];
+        const double L3 = coords[3 * F + g], L4 = coords[4 * F + g], L5 = coords[5 * F + g];
+        const double sg[3] = {sigma[0 * F + g], sigma[1 * F + g], sigma[2 * F + g]};
+        vol = std::fabs(L0 * L2 * L5) / 6.0;
+        double gr[4][3];
+        gr[1][0] = 1.0 / L0;
         gr[1][1] = -L1 / (L0 * L2);
         gr[1][2] = (L1 * L4 - L2 * L3) / (L0 * L2 * L5);
-        gr[2][0] = (T)0;
-        gr[2][1] = (T)1 / L2;
+        gr[2][0] = 0.0;
+        gr[2][1] = 1.0 / L2;
         gr[2][2] = -L4 / (L2 * L5);
-        gr[3][0] = (T)0;
-        gr[3][1] = (T)0;
-        gr[3][2] = (T)1 / L5;
+        gr[3][0] = 0.0;
+        gr[3][1] = 0.0;
+        gr[3][2] = 1.0 / L5;
         for (int k = 0; k < 3; ++k) gr[0][k] = -((gr[1][k] + gr[2][k]) + gr[3][k]);
-        T K[4];
+        double K[4];
         for (int i = 0; i < 4; ++i) K[i] = vol * (sg[0] * gr[i][0] + sg[1] * gr[i][1] + sg[2] * gr[i][2]);
         nscheme(K, ph, 4, H, at);
+        eamb[c] = near_threshold(K, ph, 4, H);
       } else {
-        const T c0 = coords[0 * F + g], c1 = coords[1 * F + g], c2 = coords[2 * F + g];
-        const T sg[2] = {sigma[0 * F + g], sigma[1 * F + g]};
-        vol = std::fabs(c0 * c2) / (T)2;
-        T gr[3][2];
-        gr[1][0] = (T)1 / c0;
+        const double c0 = coords[0 * F + g], c1 = coords[1 * F + g], c2 = coords[2 * F + g];
+        const double sg[2] = {sigma[0 * F + g], sigma[1 * F + g]};
+        vol = std::fabs(c0 * c2) / 2.0;
+        double gr[3][2];
+        gr[1][0] = 1.0 / c0;
         gr[1][1] = -c1 / (c0 * c2);
-        gr[2][0] = (T)0;
-        gr[2][1] = (T)1 / c2;
+        gr[2][0] = 0.0;
+        gr[2][1] = 1.0 / c2;
         for (int k = 0; k < 2; ++k) gr[0][k] = -(gr[1][k] + gr[2][k]);
-        T K[3];
+        double K[3];
         for (int i = 0; i < 3; ++i) K[i] = vol * (sg[0] * gr[i][0] + sg[1] * gr[i][1]);
         nscheme(K, ph, 3, H, at);
+        eamb[c] = near_threshold(K, ph, 3, H);
       }
       for (int i = 0; i < nc; ++i) {
-        s_up[c * nc + i] = at[i] * H;
-        s_dn[c * nc + i] = at[i] * vol;
+        s_up[c * nc + i] = (T)(at[i] * H);
+        s_dn[c * nc + i] = (T)(at[i] * vol);
       }
     }
     for (int32_t v = vs; v < ve; ++v) {
-      T up = (T)0, dn = (T)0;
-      for (int32_t l : lmem[v]) up += s_up[l];
-      for (int32_t l : lmem[v]) dn += s_dn[l];
-      const T old = phi[v];
-      phi_out[v] = (std::fabs(dn) > (T)1e-8) ? old - dt * (up / dn) : old;
+      double up = 0.0, dn = 0.0, dscale = 0.0;
+      uint8_t a = 0;
+      for (int32_t l : lmem[v]) up += (double)s_up[l];
+      for (int32_t l : lmem[v]) { dn += (double)s_dn[l]; dscale += std::fabs((double)s_dn[l]); a |= eamb[l / nc]; }
+      if (std::fabs(std::fabs(dn) - 1e-8) <= kAmbRel * dscale) a = 1;
+      amb[v] = a;
+      const double old = (double)phi[v];
+      phi_out[v] = (T)((std::fabs(dn) > 1e-8) ? old - dt * (up / dn) : old);
     }
   }
 
-  // N-scheme distribution coefficients (meshFIM_kernels.h:227-273).
-  static void nscheme(const T* K, const T* ph, int n, T& H, T* at) {
-    T Kp[4], Km[4], beta = (T)0;
-    H = (T)0;
+  // True when |H| lies within the perturbation FP32 storage of the corner
+  // values can cause: sum |K_i| * |phi_i| * kAmbRel around the 1e-8 cut-off.
+  static uint8_t near_threshold(const double* K, const double* ph, int n, double H) {
+    double sc = 0.0;
+    for (int i = 0; i < n; ++i) sc += std::fabs(K[i]) * (std::fabs(ph[i]) + 1e-3);
+    return std::fabs(std::fabs(H) - 1e-8) <= kAmbRel * sc ? 1 : 0;
+  }
+
+  // N-scheme distribution coefficients (meshFIM_kernels.h:227-273), FP64.
+  static void nscheme(const double* K, const double* ph, int n, double& H, double* at) {
+    double Kp[4], Km[4], beta = 0.0;
+    H = 0.0;
     for (int i = 0; i < n; ++i) {
       H += K[i] * ph[i];
-      Kp[i] = std::fmax(K[i], (T)0);
-      Km[i] = std::fmin(K[i], (T)0);
+      Kp[i] = std::fmax(K[i], 0.0);
+      Km[i] = std::fmin(K[i], 0.0);
       beta += Km[i];
     }
-    beta = (T)1 / beta;
-    for (int i = 0; i < n; ++i) at[i] = (T)0;
-    if (std::fabs(H) > (T)1e-8) {
-      T alpha[4], theta = (T)0;
+    beta = 1.0 / beta;
+    for (int i = 0; i < n; ++i) at[i] = 0.0;
+    if (std::fabs(H) > 1e-8) {
+      double alpha[4], theta = 0.0;
       for (int i = 0; i < n; ++i) {
-        T acc = (T)0;
+        double acc = 0.0;
         for (int j = 0; j < n; ++j) acc += Km[j] * (ph[i] - ph[j]);
         alpha[i] = (Kp[i] * beta * acc) / H;
       }
-      for (int i = 0; i < n; ++i) theta += std::fmax((T)0, alpha[i]);
-      for (int i = 0; i < n; ++i) at[i] = std::fmax(alpha[i], (T)0) / theta;
+      for (int i = 0; i < n; ++i) theta += std::fmax(0.0, alpha[i]);
+      for (int i = 0; i < n; ++i) at[i] = std::fmax(alpha[i], 0.0) / theta;
     }
   }
 
@@ -717,6 +763,7 @@ struct Solver : Base {
   int step(const lso_params& prm, lso_stats& st) override {
     if (prm.nthreads > 0) omp_set_num_threads(prm.nthreads);
     std::memset(&st, 0, sizeof(st));
+    std::fill(amb.begin(), amb.end(), 0);
     std::vector<int32_t> act = find_seeds();
     st.n_seed_patches = (int)act.size();
     for (int64_t v = 0; v < N; ++v) tsign[v] = (phi[v] < (T)0) ? 0 : 1;  // kernel_reinit_Tsign
@@ -753,6 +800,12 @@ int lso_step(void* h, const lso_params* prm, lso_stats* st) { return static_cast
 void lso_get_phi(void* h, double* out) { static_cast<Base*>(h)->get_phi(out); }
 int lso_get_narrowband(void* h, int32_t* ids) { return static_cast<Base*>(h)->get_nb(ids); }
 int64_t lso_num_copies(void* h) { return static_cast<Base*>(h)->ncopies(); }
+// Replaces the previous-step narrowband (seeds scan these patches next step).
+void lso_set_narrowband(void* h, const int32_t* ids, int n) { static_cast<Base*>(h)->set_nb(ids, n); }
+// Per-vertex (caller order) flag of threshold-ambiguous advection updates.
+void lso_get_ambiguous(void* h, uint8_t* out) { static_cast<Base*>(h)->get_amb(out); }
+// Per-copy local coordinates in the storage type, copy-major.
+void lso_get_coords(void* h, double* out) { static_cast<Base*>(h)->get_coords(out); }
 void lso_get_layout(void* h, int32_t* vert_perm, int32_t* ele_perm, int32_t* vert_off,
                     int32_t* ele_off, int32_t* ele_soa) {
   static_cast<Base*>(h)->get_layout(vert_perm, ele_perm, vert_off, ele_off, ele_soa);
@@ -775,10 +828,10 @@ void lso_tet_element_update(int precision, const double* L, double* v, double le
     float Lf[6], vf[4];
     for (int k = 0; k < 6; ++k) Lf[k] = (float)L[k];
     for (int k = 0; k < 4; ++k) vf[k] = (float)v[k];
-    TetGeom<float>(Lf).update(vf, (float)leps);
+    TetGeom(Lf).update(vf, leps);
     for (int k = 0; k < 4; ++k) v[k] = vf[k];
   } else {
-    TetGeom<double>(L).update(v, leps);
+    TetGeom(L).update(v, leps);
   }
 }
 int lso_max_threads() { return omp_get_max_threads(); }

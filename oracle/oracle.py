@@ -18,7 +18,7 @@ LIB_PATH = os.path.join(_HERE, "liblsoracle.so")
 class Params(C.Structure):
     _fields_ = [("dt", C.c_double), ("bandwidth", C.c_double), ("tol", C.c_double),
                 ("lambda_eps", C.c_double), ("niter", C.c_int), ("inside_niter", C.c_int),
-                ("mode", C.c_int), ("nthreads", C.c_int)]
+                ("mode", C.c_int), ("nthreads", C.c_int), ("max_fim_iters", C.c_int), ("pad", C.c_int)]
 
 
 class Stats(C.Structure):
@@ -61,12 +61,16 @@ def lib():
         L.lso_tri_solve.argtypes = [C.c_int] + [C.c_double] * 5
         L.lso_tet_element_update.argtypes = [C.c_int, dp, dp, C.c_double]
         L.lso_max_threads.restype = C.c_int
+        L.lso_set_narrowband.argtypes = [vp, C.c_void_p, C.c_int]
+        L.lso_get_ambiguous.argtypes = [vp, C.c_void_p]
+        L.lso_get_coords.argtypes = [vp, C.c_void_p]
         _lib = L
     return _lib
 
 
-def make_params(dt, bandwidth, niter, tol=1e-5, lambda_eps=1e-5, inside_niter=1, mode=0, nthreads=0):
-    return Params(dt, bandwidth, tol, lambda_eps, niter, inside_niter, mode, nthreads)
+def make_params(dt, bandwidth, niter, tol=1e-5, lambda_eps=1e-5, inside_niter=1, mode=0, nthreads=0,
+                max_fim_iters=0):
+    return Params(dt, bandwidth, tol, lambda_eps, niter, inside_niter, mode, nthreads, max_fim_iters, 0)
 
 
 class Oracle:
@@ -114,6 +118,22 @@ class Oracle:
         out = np.empty(self.N, dtype=np.float64)
         self.L.lso_get_phi(self.h, out)
         return out
+
+    def set_narrowband(self, ids):
+        a = np.ascontiguousarray(ids, dtype=np.int32)
+        self.L.lso_set_narrowband(self.h, a.ctypes.data, int(a.size))
+
+    def ambiguous(self):
+        out = np.zeros(self.N, dtype=np.uint8)
+        self.L.lso_get_ambiguous(self.h, out.ctypes.data)
+        return out.astype(bool)
+
+    def coords(self):
+        F = self.L.lso_num_copies(self.h)
+        nk = 6 if self.dim == 3 else 3
+        out = np.empty(F * nk, np.float64)
+        self.L.lso_get_coords(self.h, out.ctypes.data)
+        return out.reshape(F, nk)
 
     def narrowband(self):
         n = self.L.lso_get_narrowband(self.h, None)

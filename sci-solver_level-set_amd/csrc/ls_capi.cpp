@@ -19,6 +19,7 @@ struct ls_ctx {
   std::vector<int32_t> cells;
   std::vector<double> vel;   // per element, kept to re-project after re-partitioning
   int patch_sigma = 0;
+  int precision = LS_F32;  // tet storage precision
   bool have_vel = false;
   lsamd::Layout layout;
   bool have_layout = false;
@@ -71,7 +72,8 @@ int finish_layout(ls_ctx* c, const std::vector<int32_t>& labels, int nparts) {
     c->have_layout = true;
     return LS_OK;
   }
-  std::unique_ptr<lsamd::Engine> eng(lsamd::make_engine(c->dim, c->device, err));
+  std::unique_ptr<lsamd::Engine> eng(
+      lsamd::make_engine(c->dim, c->dim == 2 ? LS_F64 : c->precision, c->device, err));
   if (!eng) return fail(c, LS_ERR_ARG, err);
   int rc = eng->upload(L, err);
   if (rc != LS_OK) return fail(c, rc, err);
@@ -114,6 +116,14 @@ int ls_set_mesh(ls_ctx* c, int dim, int64_t n_verts, const double* xyz, int64_t 
   c->have_layout = false;
   c->have_vel = false;
   c->engine.reset();
+  return LS_OK;
+}
+
+int ls_set_precision(ls_ctx* c, int dtype) {
+  if (!c) return LS_ERR_ARG;
+  if (dtype != LS_F32 && dtype != LS_F64) return fail(c, LS_ERR_ARG, "dtype must be LS_F32 or LS_F64");
+  if (c->have_layout) return fail(c, LS_ERR_STATE, "set the precision before partitioning");
+  c->precision = dtype;
   return LS_OK;
 }
 

@@ -45,6 +45,15 @@ struct DevState {
   int32_t* fresh;      // P, activation flag for neighbour probes
   int32_t* seedflag;   // P
   int32_t* counters;   // small array of device counters
+  unsigned long long* sizes;  // kNumSizes patch-size sums for byte accounting
+};
+
+// Patch-size sums (element copies, owned, halo) per launch kind.
+enum SizeCounter : int {
+  kSzVisit = 0,   // FIM visits: copies, verts, halo
+  kSzProbe = 3,   // neighbour probes
+  kSzAdvect = 6,  // advection
+  kNumSizes = 9
 };
 
 // Device counters layout.

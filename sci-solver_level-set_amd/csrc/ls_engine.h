@@ -24,7 +24,8 @@ class Engine {
   virtual int working_bytes() const = 0;  // sizeof working real type
 };
 
-// dim 3 -> FP32 tet engine, dim 2 -> FP64 triangle engine.
-Engine* make_engine(int dim, int device, std::string& err);
+// dim 3 -> tet engine with FP32 (LS_F32) or FP64 (LS_F64) storage,
+// dim 2 -> FP64 triangle engine.
+Engine* make_engine(int dim, int dtype, int device, std::string& err);
 
 }  // namespace lsamd

@@ -23,9 +23,11 @@
 namespace lsamd {
 namespace kern {
 
-__device__ __forceinline__ float d_sqrt(float x) { return __fsqrt_rn(x); }
-__device__ __forceinline__ double d_sqrt(double x) { return __dsqrt_rn(x); }
-__device__ __forceinline__ float d_div(float a, float b) { return __fdiv_rn(a, b); }
+// sqrtf and '/' are correctly rounded under -fhip-fp32-correctly-rounded-divide-sqrt;
+// __fsqrt_rn is not used because it maps to the native (approximate) sqrt.
+__device__ __forceinline__ float d_sqrt(float x) { return sqrtf(x); }
+__device__ __forceinline__ double d_sqrt(double x) { return sqrt(x); }
+__device__ __forceinline__ float d_div(float a, float b) { return a / b; }
 __device__ __forceinline__ double d_div(double a, double b) { return a / b; }
 __device__ __forceinline__ float d_min(float a, float b) { return fminf(a, b); }
 __device__ __forceinline__ double d_min(double a, double b) { return fmin(a, b); }
@@ -40,7 +42,7 @@ __device__ __forceinline__ bool quad_roots(T p, T q, T r, T& x1, T& x2) {
   const double dp = p, dq = q, dr = r;
   const double delta = dq * dq - 4.0 * dp * dr;
   if (!(delta >= 0.0)) return false;
-  const double sq = __dsqrt_rn(delta);
+  const double sq = sqrt(delta);
   x1 = (T)((-dq + sq) / (2.0 * dp));
   x2 = (T)((-dq - sq) / (2.0 * dp));
   return true;
@@ -181,47 +183,53 @@ __device__ T tri_solve(T TA, T TB, T LenAB, T LenBC, T LenAC) {
 // One element update: every corner takes min(own, solve from the other corners),
 // all solves reading the values from before the update
 // (src/core3d/include/redistance_kernels3d.h:373-414; 2D redistance_kernels.h:441-462).
+// Evaluated in FP64 from the stored values: the solver's quadratic coefficients
+// cancel badly in FP32 and its candidate acceptance is discontinuous.
 template <typename T, int NC>
 struct ElemSolve;
 
 template <typename T>
 struct ElemSolve<T, 4> {
   static constexpr int kCoords = 6;
-  __device__ static void update(const T* L, T* v, T leps) {
-    const T ACBC = L[1] * (L[1] - L[0]) + L[2] * L[2];
-    const T BCCD = (L[1] - L[0]) * (L[3] - L[1]) + L[2] * (L[4] - L[2]);
-    const T ACCD = L[1] * (L[3] - L[1]) + L[2] * (L[4] - L[2]);
-    const T ADBD = L[3] * (L[3] - L[0]) + L[4] * L[4] + L[5] * L[5];
-    const T ACAD = L[1] * L[3] + L[2] * L[4];
-    const T BCBD = (L[1] - L[0]) * (L[3] - L[0]) + L[2] * L[4];
-    const T ADBC = ACBC + BCCD;
-    const T ACBD = ACBC + ACCD;
-    const T CDBD = ADBD - ACBD;
-    const T ADCD = ADBD - ADBC;
-    const T CDCD = ADCD - ACCD;
-    const T ADAD = ADCD + ACAD;
-    const T ACAC = ACAD - ACCD;
-    const T BDBD = BCBD + CDBD;
-    const T BCBC = BCBD - BCCD;
-    const T TA = v[0], TB = v[1], TC = v[2], TD = v[3];
-    v[3] = d_min(TD, tet_solve(TA, TB, TC, ACAC, ACBC, ACCD, BCBC, BCCD, CDCD, leps));
-    v[0] = d_min(TA, tet_solve(TB, TD, TC, BCBC, -BCCD, -ACBC, CDCD, ACCD, ACAC, leps));
-    v[1] = d_min(TB, tet_solve(TA, TD, TC, ACAC, -ACCD, -ACBC, CDCD, BCCD, BCBC, leps));
-    v[2] = d_min(TC, tet_solve(TA, TB, TD, ADAD, ADBD, -ADCD, BDBD, -CDBD, CDCD, leps));
+  __device__ static void update(const T* Lt, T* v, double leps) {
+    double L[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) L[k] = (double)Lt[k];
+    const double ACBC = L[1] * (L[1] - L[0]) + L[2] * L[2];
+    const double BCCD = (L[1] - L[0]) * (L[3] - L[1]) + L[2] * (L[4] - L[2]);
+    const double ACCD = L[1] * (L[3] - L[1]) + L[2] * (L[4] - L[2]);
+    const double ADBD = L[3] * (L[3] - L[0]) + L[4] * L[4] + L[5] * L[5];
+    const double ACAD = L[1] * L[3] + L[2] * L[4];
+    const double BCBD = (L[1] - L[0]) * (L[3] - L[0]) + L[2] * L[4];
+    const double ADBC = ACBC + BCCD;
+    const double ACBD = ACBC + ACCD;
+    const double CDBD = ADBD - ACBD;
+    const double ADCD = ADBD - ADBC;
+    const double CDCD = ADCD - ACCD;
+    const double ADAD = ADCD + ACAD;
+    const double ACAC = ACAD - ACCD;
+    const double BDBD = BCBD + CDBD;
+    const double BCBC = BCBD - BCCD;
+    const double TA = v[0], TB = v[1], TC = v[2], TD = v[3];
+    v[3] = (T)d_min(TD, tet_solve(TA, TB, TC, ACAC, ACBC, ACCD, BCBC, BCCD, CDCD, leps));
+    v[0] = (T)d_min(TA, tet_solve(TB, TD, TC, BCBC, -BCCD, -ACBC, CDCD, ACCD, ACAC, leps));
+    v[1] = (T)d_min(TB, tet_solve(TA, TD, TC, ACAC, -ACCD, -ACBC, CDCD, BCCD, BCBC, leps));
+    v[2] = (T)d_min(TC, tet_solve(TA, TB, TD, ADAD, ADBD, -ADCD, BDBD, -CDBD, CDCD, leps));
   }
 };
 
 template <typename T>
 struct ElemSolve<T, 3> {
   static constexpr int kCoords = 3;
-  __device__ static void update(const T* c, T* v, T) {
-    const T LenAB = d_abs(c[0]);
-    const T LenBC = d_sqrt(c[2] * c[2] + (c[1] - c[0]) * (c[1] - c[0]));
-    const T LenCA = d_sqrt(c[1] * c[1] + c[2] * c[2]);
-    const T TA = v[0], TB = v[1], TC = v[2];
-    v[2] = d_min(TC, tri_solve(TA, TB, LenAB, LenBC, LenCA));
-    v[0] = d_min(TA, tri_solve(TB, TC, LenBC, LenCA, LenAB));
-    v[1] = d_min(TB, tri_solve(TC, TA, LenCA, LenAB, LenBC));
+  __device__ static void update(const T* ct, T* v, double) {
+    const double c[3] = {(double)ct[0], (double)ct[1], (double)ct[2]};
+    const double LenAB = d_abs(c[0]);
+    const double LenBC = d_sqrt(c[2] * c[2] + (c[1] - c[0]) * (c[1] - c[0]));
+    const double LenCA = d_sqrt(c[1] * c[1] + c[2] * c[2]);
+    const double TA = v[0], TB = v[1], TC = v[2];
+    v[2] = (T)d_min(TC, tri_solve(TA, TB, LenAB, LenBC, LenCA));
+    v[0] = (T)d_min(TA, tri_solve(TB, TC, LenBC, LenCA, LenAB));
+    v[1] = (T)d_min(TB, tri_solve(TC, TA, LenCA, LenAB, LenBC));
   }
 };
 
@@ -264,6 +272,10 @@ __global__ __launch_bounds__(kBlock) void k_init(int64_t N, DevState<T> s) {
 
 __global__ void k_set_counters(int32_t* c, int n, int32_t val) {
   if ((int)threadIdx.x < n) c[threadIdx.x] = val;
+}
+
+__global__ void k_clear_sizes(unsigned long long* c) {
+  if ((int)threadIdx.x < kNumSizes) c[threadIdx.x] = 0ull;
 }
 
 // Seeds over the scanned patches (kernel_seedlabel / _narrowband,
@@ -322,7 +334,7 @@ __global__ __launch_bounds__(kBlock) void k_seeds(DevMesh<T> m, DevState<T> s, c
 // (run_reduction_bandwidth, redistance_kernels3d.h:57-97).
 template <typename T, int NC>
 __global__ __launch_bounds__(kBlock) void k_sweep(DevMesh<T> m, DevState<T> s, const int32_t* list,
-                                                  int niter, int write_out, T tol, T leps) {
+                                                  int niter, int write_out, T tol, double leps) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ T red[kBlock / 64];
   const int p = list[blockIdx.x];
@@ -415,6 +427,10 @@ __global__ __launch_bounds__(kBlock) void k_sweep(DevMesh<T> m, DevState<T> s, c
   if (tid == 0) {
     s.bcon[p] = con_all;
     if (write_out) s.bmin[p] = bm;
+    unsigned long long* sz = s.sizes + (write_out ? kSzVisit : kSzProbe);
+    atomicAdd(sz + 0, (unsigned long long)ne);
+    atomicAdd(sz + 1, (unsigned long long)nv);
+    atomicAdd(sz + 2, (unsigned long long)nh);
   }
 }
 
@@ -481,38 +497,38 @@ __global__ __launch_bounds__(kBlock) void k_narrowband(DevMesh<T> m, DevState<T>
   }
 }
 
-// Shape-function gradients of a copy in its local frame.
-template <typename T, int NC>
+// Shape-function gradients of a copy in its local frame (FP64).
+template <int NC>
 struct Grad;
 
-template <typename T>
-struct Grad<T, 4> {
+template <>
+struct Grad<4> {
   // A=(0,0,0), B=(L0,0,0), C=(L1,L2,0), D=(L3,L4,L5).
-  __device__ static void eval(const T* L, T (*g)[3], T& vol) {
-    vol = d_div(d_abs(L[0] * L[2] * L[5]), (T)6);
-    g[1][0] = d_div((T)1, L[0]);
-    g[1][1] = d_div(-L[1], L[0] * L[2]);
-    g[1][2] = d_div(L[1] * L[4] - L[2] * L[3], L[0] * L[2] * L[5]);
-    g[2][0] = (T)0;
-    g[2][1] = d_div((T)1, L[2]);
-    g[2][2] = d_div(-L[4], L[2] * L[5]);
-    g[3][0] = (T)0;
-    g[3][1] = (T)0;
-    g[3][2] = d_div((T)1, L[5]);
+  __device__ static void eval(const double* L, double (*g)[3], double& vol) {
+    vol = d_abs(L[0] * L[2] * L[5]) / 6.0;
+    g[1][0] = 1.0 / L[0];
+    g[1][1] = -L[1] / (L[0] * L[2]);
+    g[1][2] = (L[1] * L[4] - L[2] * L[3]) / (L[0] * L[2] * L[5]);
+    g[2][0] = 0.0;
+    g[2][1] = 1.0 / L[2];
+    g[2][2] = -L[4] / (L[2] * L[5]);
+    g[3][0] = 0.0;
+    g[3][1] = 0.0;
+    g[3][2] = 1.0 / L[5];
 #pragma unroll
     for (int k = 0; k < 3; ++k) g[0][k] = -((g[1][k] + g[2][k]) + g[3][k]);
   }
 };
 
-template <typename T>
-struct Grad<T, 3> {
+template <>
+struct Grad<3> {
   // A=(0,0), B=(c0,0), C=(c1,c2).
-  __device__ static void eval(const T* c, T (*g)[3], T& vol) {
-    vol = d_div(d_abs(c[0] * c[2]), (T)2);
-    g[1][0] = d_div((T)1, c[0]);
-    g[1][1] = d_div(-c[1], c[0] * c[2]);
-    g[2][0] = (T)0;
-    g[2][1] = d_div((T)1, c[2]);
+  __device__ static void eval(const double* c, double (*g)[3], double& vol) {
+    vol = d_abs(c[0] * c[2]) / 2.0;
+    g[1][0] = 1.0 / c[0];
+    g[1][1] = -c[1] / (c[0] * c[2]);
+    g[2][0] = 0.0;
+    g[2][1] = 1.0 / c[2];
 #pragma unroll
     for (int k = 0; k < 2; ++k) g[0][k] = -(g[1][k] + g[2][k]);
   }
@@ -522,8 +538,9 @@ struct Grad<T, 3> {
 // (kernel_updateT_single_stage3d, meshFIM_kernels.h:36-408): per element copy
 // K_i = vol sigma.gradN_i, H = sum K_i phi_i, distribution coefficients
 // alpha~_i; per owned vertex phi -= dt * sum(alpha~ H) / sum(alpha~ vol).
+// Evaluated in FP64 from the stored values; the result is stored as T.
 template <typename T, int NC>
-__global__ __launch_bounds__(kBlock) void k_advect(DevMesh<T> m, DevState<T> s, const int32_t* nb, T dt) {
+__global__ __launch_bounds__(kBlock) void k_advect(DevMesh<T> m, DevState<T> s, const int32_t* nb, double dt) {
   extern __shared__ __align__(16) unsigned char smem[];
   constexpr int K = ElemSolve<T, NC>::kCoords;
   constexpr int D = NC - 1;
@@ -532,69 +549,74 @@ __global__ __launch_bounds__(kBlock) void k_advect(DevMesh<T> m, DevState<T> s, 
   const int es = m.ele_off[p], ee = m.ele_off[p + 1];
   const int hs = m.halo_off[p], he = m.halo_off[p + 1];
   const int nv = ve - vs, ne = ee - es, nh = he - hs;
-  T* s_up = (T*)smem;
+  T* s_up = (T*)smem;  // per-slot products, rounded to the storage type
   T* s_dn = s_up + NC * ne;
   T* s_vtx = s_dn + NC * ne;
   for (int i = threadIdx.x; i < nv; i += kBlock) s_vtx[i] = s.phi[vs + i];
   for (int i = threadIdx.x; i < nh; i += kBlock) s_vtx[nv + i] = s.phi[m.halo_ids[hs + i]];
+  if (threadIdx.x == 0) {
+    atomicAdd(s.sizes + kSzAdvect + 0, (unsigned long long)ne);
+    atomicAdd(s.sizes + kSzAdvect + 1, (unsigned long long)nv);
+    atomicAdd(s.sizes + kSzAdvect + 2, (unsigned long long)nh);
+  }
   __syncthreads();
   for (int e = threadIdx.x; e < ne; e += kBlock) {
     const ushort4 c = m.copy_local[es + e];
-    T ph[NC], L[K], sg[D], g[NC][3], vol;
+    double ph[NC], L[K], sg[D], g[NC][3], vol;
 #pragma unroll
-    for (int i = 0; i < NC; ++i) ph[i] = s_vtx[corner(c, i)];
+    for (int i = 0; i < NC; ++i) ph[i] = (double)s_vtx[corner(c, i)];
     const T* gc = m.coords + (int64_t)(es + e) * K;
 #pragma unroll
-    for (int k = 0; k < K; ++k) L[k] = gc[k];
+    for (int k = 0; k < K; ++k) L[k] = (double)gc[k];
     const T* gs = m.sigma + (int64_t)(es + e) * D;
 #pragma unroll
-    for (int k = 0; k < D; ++k) sg[k] = gs[k];
-    Grad<T, NC>::eval(L, g, vol);
-    T Kc[NC], Kp[NC], Km[NC], at[NC];
+    for (int k = 0; k < D; ++k) sg[k] = (double)gs[k];
+    Grad<NC>::eval(L, g, vol);
+    double Kc[NC], Kp[NC], Km[NC], at[NC];
 #pragma unroll
     for (int i = 0; i < NC; ++i) {
-      T d = sg[0] * g[i][0] + sg[1] * g[i][1];
+      double d = sg[0] * g[i][0] + sg[1] * g[i][1];
       if (D == 3) d = d + sg[D - 1] * g[i][2];
       Kc[i] = vol * d;
     }
-    T H = (T)0, beta = (T)0;
+    double H = 0.0, beta = 0.0;
 #pragma unroll
     for (int i = 0; i < NC; ++i) {
       H += Kc[i] * ph[i];
-      Kp[i] = d_max(Kc[i], (T)0);
-      Km[i] = d_min(Kc[i], (T)0);
+      Kp[i] = d_max(Kc[i], 0.0);
+      Km[i] = d_min(Kc[i], 0.0);
       beta += Km[i];
-      at[i] = (T)0;
+      at[i] = 0.0;
     }
-    beta = d_div((T)1, beta);
-    if (d_abs(H) > (T)1e-8) {
-      T alpha[NC], theta = (T)0;
+    beta = 1.0 / beta;
+    if (d_abs(H) > 1e-8) {
+      double alpha[NC], theta = 0.0;
 #pragma unroll
       for (int i = 0; i < NC; ++i) {
-        T acc = (T)0;
+        double acc = 0.0;
 #pragma unroll
         for (int j = 0; j < NC; ++j) acc += Km[j] * (ph[i] - ph[j]);
-        alpha[i] = d_div(Kp[i] * beta * acc, H);
+        alpha[i] = (Kp[i] * beta * acc) / H;
       }
 #pragma unroll
-      for (int i = 0; i < NC; ++i) theta += d_max((T)0, alpha[i]);
+      for (int i = 0; i < NC; ++i) theta += d_max(0.0, alpha[i]);
 #pragma unroll
-      for (int i = 0; i < NC; ++i) at[i] = d_div(d_max(alpha[i], (T)0), theta);
+      for (int i = 0; i < NC; ++i) at[i] = d_max(alpha[i], 0.0) / theta;
     }
 #pragma unroll
     for (int i = 0; i < NC; ++i) {
-      s_up[e * NC + i] = at[i] * H;
-      s_dn[e * NC + i] = at[i] * vol;
+      s_up[e * NC + i] = (T)(at[i] * H);
+      s_dn[e * NC + i] = (T)(at[i] * vol);
     }
   }
   __syncthreads();
   for (int v = threadIdx.x; v < nv; v += kBlock) {
     const int o0 = m.vslot_off[vs + v], o1 = m.vslot_off[vs + v + 1];
-    T up = (T)0, dn = (T)0;
-    for (int k = o0; k < o1; ++k) up += s_up[m.vslots[k]];
-    for (int k = o0; k < o1; ++k) dn += s_dn[m.vslots[k]];
-    const T old = s_vtx[v];
-    s.phi_out[vs + v] = (d_abs(dn) > (T)1e-8) ? old - dt * d_div(up, dn) : old;
+    double up = 0.0, dn = 0.0;
+    for (int k = o0; k < o1; ++k) up += (double)s_up[m.vslots[k]];
+    for (int k = o0; k < o1; ++k) dn += (double)s_dn[m.vslots[k]];
+    const double old = (double)s_vtx[v];
+    s.phi_out[vs + v] = (T)((d_abs(dn) > 1e-8) ? old - dt * (up / dn) : old);
   }
 }
 
@@ -708,6 +730,7 @@ class EngineImpl final : public Engine {
     LS_HIP(dev_alloc(&d_fresh_, P_));
     LS_HIP(dev_alloc(&d_seedflag_, P_));
     LS_HIP(dev_alloc(&d_counters_, kNumCounters));
+    LS_HIP(dev_alloc(&d_sizes_, kNumSizes));
     for (auto& l : d_lists_) LS_HIP(dev_alloc(&l, P_));
     LS_HIP(dev_alloc(&d_nb_, P_));
     LS_HIP(dev_alloc(&d_scan_all_, P_));
@@ -748,12 +771,15 @@ class EngineImpl final : public Engine {
     std::memset(&st, 0, sizeof(st));
     const DevMesh<T> m = mesh();
     const DevState<T> s = state();
-    const T tol = (T)prm.tol, leps = (T)prm.lambda_eps, bw = (T)prm.bandwidth, dt = (T)prm.dt;
+    const T tol = (T)prm.tol, bw = (T)prm.bandwidth;
+    const double leps = prm.lambda_eps, dt = prm.dt;
     hipStream_t q = stream_;
     LS_HIP(hipEventRecord(ev_[0], q));
     // Seeds over every patch on the first step, else over the previous narrowband.
+    kev_used_ = 0;
     k_init<T><<<grid_for(N_), kBlock, 0, q>>>(N_, s);
     k_set_counters<<<1, 64, 0, q>>>(d_counters_, kNumCounters, 0);
+    k_clear_sizes<<<1, 64, 0, q>>>(d_sizes_);
     const int32_t* scan = nnb_ > 0 ? d_nb_cur() : d_scan_all_;
     const int nscan = nnb_ > 0 ? nnb_ : P_;
     int32_t* cur = d_lists_[0];
@@ -767,10 +793,12 @@ class EngineImpl final : public Engine {
     st.n_seed_patches = nact;
     int32_t* fresh_list = d_lists_[1];
     int32_t* next = d_lists_[2];
-    while (nact > 0) {
+    while (nact > 0 && (prm.max_fim_iters <= 0 || st.fim_iters < prm.max_fim_iters)) {
       st.fim_iters++;
       st.patch_visits += nact;
+      int ke = kev_begin(0, q);
       k_sweep<T, NC><<<nact, kBlock, fim_lds_, q>>>(m, s, cur, prm.niter, 1, tol, leps);
+      kev_end(ke, q);
       k_copyback<T><<<nact, kBlock, 0, q>>>(m, s, cur);
       k_set_counters<<<1, 64, 0, q>>>(d_counters_ + kCntFresh, 2, 0);
       k_activate<T><<<grid_for(nact), kBlock, 0, q>>>(m, s, cur, nact, bw, fresh_list);
@@ -780,7 +808,9 @@ class EngineImpl final : public Engine {
       LS_HIP(hipStreamSynchronize(q));
       if (nf > 0) {
         st.probes += nf;
+        int kp = kev_begin(0, q);
         k_sweep<T, NC><<<nf, kBlock, fim_lds_, q>>>(m, s, fresh_list, 1, 0, tol, leps);
+        kev_end(kp, q);
       }
       k_finalize<T><<<grid_for(nact + nf), kBlock, 0, q>>>(s, cur, nact, fresh_list, nf, next);
       LS_HIP(hipGetLastError());
@@ -802,7 +832,9 @@ class EngineImpl final : public Engine {
     LS_HIP(hipEventRecord(ev_[3], q));
     if (nnb_ > 0) {
       for (int it = 0; it < prm.inside_niter; ++it) {
+        int ka = kev_begin(1, q);
         k_advect<T, NC><<<nnb_, kBlock, adv_lds_, q>>>(m, s, d_nb_cur(), dt);
+        kev_end(ka, q);
         k_copy_phi<T><<<nnb_, kBlock, 0, q>>>(m, s, d_nb_cur());
       }
       LS_HIP(hipGetLastError());
@@ -815,6 +847,16 @@ class EngineImpl final : public Engine {
     LS_HIP(hipEventElapsedTime(&ms, ev_[2], ev_[3])); st.ms_nb = ms;
     LS_HIP(hipEventElapsedTime(&ms, ev_[3], ev_[4])); st.ms_advect = ms;
     LS_HIP(hipEventElapsedTime(&ms, ev_[0], ev_[4])); st.ms_total = ms;
+    for (size_t i = 0; i < kev_used_; ++i) {
+      LS_HIP(hipEventElapsedTime(&ms, kev_[2 * i], kev_[2 * i + 1]));
+      if (kev_kind_[i] == 0) { st.ms_sweep_kernels += ms; st.sweep_launches++; }
+      else st.ms_advect_kernels += ms;
+    }
+    unsigned long long sz[kNumSizes];
+    LS_HIP(hipMemcpy(sz, d_sizes_, sizeof(sz), hipMemcpyDeviceToHost));
+    st.visit_copies = sz[kSzVisit]; st.visit_verts = sz[kSzVisit + 1]; st.visit_halo = sz[kSzVisit + 2];
+    st.probe_copies = sz[kSzProbe]; st.probe_verts = sz[kSzProbe + 1]; st.probe_halo = sz[kSzProbe + 2];
+    st.adv_copies = sz[kSzAdvect]; st.adv_verts = sz[kSzAdvect + 1]; st.adv_halo = sz[kSzAdvect + 2];
     return LS_OK;
   }
 
@@ -847,6 +889,7 @@ class EngineImpl final : public Engine {
     s.phi = d_phi_; s.phi_out = d_phi_out_; s.DT = d_DT_; s.DT_out = d_DT_out_;
     s.label = d_label_; s.tsign = d_tsign_; s.bcon = d_bcon_; s.bmin = d_bmin_;
     s.act = d_act_; s.fresh = d_fresh_; s.seedflag = d_seedflag_; s.counters = d_counters_;
+    s.sizes = d_sizes_;
     return s;
   }
   int32_t* d_nb_cur() const { return nb_sel_ ? d_lists_[3] : d_nb_; }
@@ -856,9 +899,11 @@ class EngineImpl final : public Engine {
     void* ptrs[] = {d_vert_off_, d_ele_off_, d_halo_off_, d_halo_ids_, d_copy_local_, d_vslot_off_,
                     d_vslots_, d_padj_off_, d_padj_, d_coords_, d_sigma_, d_phi_, d_phi_out_, d_DT_,
                     d_DT_out_, d_label_, d_tsign_, d_bcon_, d_bmin_, d_act_, d_fresh_, d_seedflag_,
-                    d_counters_, d_nb_, d_scan_all_, d_lists_[0], d_lists_[1], d_lists_[2], d_lists_[3]};
+                    d_counters_, d_sizes_, d_nb_, d_scan_all_, d_lists_[0], d_lists_[1], d_lists_[2],
+                    d_lists_[3]};
     for (void* p : ptrs) if (p) (void)hipFree(p);
     for (auto& e : ev_) if (e) (void)hipEventDestroy(e);
+    for (auto& e : kev_) if (e) (void)hipEventDestroy(e);
     if (stream_) (void)hipStreamDestroy(stream_);
   }
 
@@ -880,11 +925,35 @@ class EngineImpl final : public Engine {
   T *d_phi_ = nullptr, *d_phi_out_ = nullptr, *d_DT_ = nullptr, *d_DT_out_ = nullptr, *d_bmin_ = nullptr;
   int8_t *d_label_ = nullptr, *d_tsign_ = nullptr;
   int32_t *d_bcon_ = nullptr, *d_act_ = nullptr, *d_fresh_ = nullptr, *d_seedflag_ = nullptr, *d_counters_ = nullptr;
+  unsigned long long* d_sizes_ = nullptr;
+  std::vector<hipEvent_t> kev_;  // kernel-timing events, pairs (start, end)
+  std::vector<int> kev_kind_;    // 0 sweep, 1 advect, per pair
+  size_t kev_used_ = 0;
+
+  // Records a start event for a timed kernel launch; returns the pair index.
+  int kev_begin(int kind, hipStream_t q) {
+    if (2 * (kev_used_ + 1) > kev_.size()) {
+      for (int k = 0; k < 64; ++k) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return -1;
+        kev_.push_back(e);
+      }
+      kev_kind_.resize(kev_.size() / 2);
+    }
+    const int i = (int)kev_used_++;
+    kev_kind_[i] = kind;
+    (void)hipEventRecord(kev_[2 * i], q);
+    return i;
+  }
+  void kev_end(int i, hipStream_t q) {
+    if (i >= 0) (void)hipEventRecord(kev_[2 * i + 1], q);
+  }
   int32_t *d_nb_ = nullptr, *d_scan_all_ = nullptr;
   int32_t* d_lists_[4] = {nullptr, nullptr, nullptr, nullptr};
 };
 
-Engine* make_engine(int dim, int device, std::string& err) {
+Engine* make_engine(int dim, int dtype, int device, std::string& err) {
+  if (dim == 3 && dtype == LS_F64) return new EngineImpl<double, 4>(device);
   if (dim == 3) return new EngineImpl<float, 4>(device);
   if (dim == 2) return new EngineImpl<double, 3>(device);
   err = "dim must be 2 or 3";

@@ -21,17 +21,21 @@ LS_MODE_ADVECT, LS_MODE_MCF = 0, 1
 class StepParams(C.Structure):
     _fields_ = [("dt", C.c_double), ("bandwidth", C.c_double), ("tol", C.c_double),
                 ("lambda_eps", C.c_double), ("niter", C.c_int32), ("inside_niter", C.c_int32),
-                ("mode", C.c_int32), ("reserved", C.c_int32)]
+                ("mode", C.c_int32), ("max_fim_iters", C.c_int32)]
 
 
 class Stats(C.Structure):
     _fields_ = [("fim_iters", C.c_int32), ("nb_count", C.c_int32), ("patch_visits", C.c_int64),
                 ("probes", C.c_int64), ("nb_verts", C.c_int64), ("n_seed_patches", C.c_int32),
-                ("reserved", C.c_int32), ("ms_seed", C.c_double), ("ms_fim", C.c_double),
-                ("ms_nb", C.c_double), ("ms_advect", C.c_double), ("ms_total", C.c_double)]
+                ("sweep_launches", C.c_int32), ("ms_seed", C.c_double), ("ms_fim", C.c_double),
+                ("ms_nb", C.c_double), ("ms_advect", C.c_double), ("ms_total", C.c_double),
+                ("ms_sweep_kernels", C.c_double), ("ms_advect_kernels", C.c_double),
+                ("visit_copies", C.c_int64), ("visit_verts", C.c_int64), ("visit_halo", C.c_int64),
+                ("probe_copies", C.c_int64), ("probe_verts", C.c_int64), ("probe_halo", C.c_int64),
+                ("adv_copies", C.c_int64), ("adv_verts", C.c_int64), ("adv_halo", C.c_int64)]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class LayoutInfo(C.Structure):
@@ -41,7 +45,7 @@ class LayoutInfo(C.Structure):
                 ("max_vertex_slots", C.c_int32), ("dim", C.c_int32)]
 
 
-EXPORTS = ["ls_create", "ls_destroy", "ls_last_error", "ls_set_mesh", "ls_partition_grid",
+EXPORTS = ["ls_create", "ls_destroy", "ls_last_error", "ls_set_mesh", "ls_set_precision", "ls_partition_grid",
            "ls_partition_rcb", "ls_partition_labels", "ls_get_layout_info", "ls_get_layout",
            "ls_set_phi", "ls_set_velocity", "ls_step", "ls_get_phi", "ls_get_narrowband"]
 
@@ -70,6 +74,7 @@ def lib():
         L.ls_last_error.argtypes = [vp]
         L.ls_last_error.restype = C.c_char_p
         L.ls_set_mesh.argtypes = [vp, C.c_int, C.c_int64, vp, C.c_int64, vp]
+        L.ls_set_precision.argtypes = [vp, C.c_int]
         L.ls_partition_grid.argtypes = [vp] + [C.c_int] * 6
         L.ls_partition_rcb.argtypes = [vp, C.c_int]
         L.ls_partition_labels.argtypes = [vp, vp, C.c_int]
@@ -88,8 +93,9 @@ class LevelSetError(RuntimeError):
     pass
 
 
-def step_params(dt, bandwidth, niter, tol=1e-5, lambda_eps=1e-5, inside_niter=1, mode=LS_MODE_ADVECT):
-    return StepParams(dt, bandwidth, tol, lambda_eps, niter, inside_niter, mode, 0)
+def step_params(dt, bandwidth, niter, tol=1e-5, lambda_eps=1e-5, inside_niter=1, mode=LS_MODE_ADVECT,
+                max_fim_iters=0):
+    return StepParams(dt, bandwidth, tol, lambda_eps, niter, inside_niter, mode, max_fim_iters)
 
 
 class Solver:
@@ -124,6 +130,9 @@ class Solver:
         self.dim, self.N, self.E = dim, self._xyz.shape[0], self._cells.shape[0]
         self._check(self.L.ls_set_mesh(self.ctx, dim, self.N, self._xyz.ctypes.data, self.E,
                                        self._cells.ctypes.data))
+
+    def set_precision(self, dtype):
+        self._check(self.L.ls_set_precision(self.ctx, int(dtype)))
 
     def partition_labels(self, labels, nparts):
         lab = np.ascontiguousarray(labels, dtype=np.int32)
@@ -179,9 +188,10 @@ class Solver:
         return ids[: n.value].copy()
 
 
-def solver_for_case(case, device=0, partition="labels"):
+def solver_for_case(case, device=0, precision=LS_F32):
     s = Solver(device)
     s.set_mesh(case.dim, case.xyz, case.cells)
+    s.set_precision(precision)
     s.partition_labels(case.labels, case.nparts)
     s.set_phi(case.phi0)
     s.set_velocity(case.vel)

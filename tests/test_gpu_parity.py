@@ -1,9 +1,13 @@
 """GPU parity: the HIP path through the C-ABI against the oracle.
 
-* FP32 tets: bit-exact against the oracle's FP32 instance (same operation
-  order, FP64 discriminant) -- phi, narrowband membership and FIM schedule
-  statistics; within 1e-5 L-inf of the FP64 oracle.
-* FP64 triangles: bit-exact against the FP64 oracle.
+* FP32-storage tet engine: bit-exact against the oracle's FP32 instance
+  (phi, narrowband membership, FIM schedule statistics) over several steps.
+* FP64-storage tet engine and the FP64 triangle engine: bit-exact against the
+  FP64 oracle.
+* FP32 engine against the FP64 oracle, one step at a time from the GPU's own
+  state: L-inf <= 1e-5 on phi at every vertex whose advection update does not
+  hinge on a threshold decision (|H| > 1e-8, |sum alpha~ vol| > 1e-8) inside
+  the FP32-storage window; those vertices are counted and bounded.
 """
 import numpy as np
 import pytest
@@ -13,11 +17,11 @@ import levelset_amd.levelset as lsb
 
 pytestmark = pytest.mark.gpu
 
-TOL_FP64 = 1e-5  # L-inf tolerance on phi of the FP32 GPU path against the FP64 oracle
+TOL_FP64 = 1e-5  # L-inf on phi, FP32 engine vs FP64 oracle, non-ambiguous vertices
 
 
-def _run_pair(case, steps, oracle_mod, precision):
-    g = lsb.solver_for_case(case)
+def _lockstep(case, steps, oracle_mod, precision, engine=lsb.LS_F32):
+    g = lsb.solver_for_case(case, precision=engine)
     o = oracle_mod.Oracle(case.dim, case.xyz, case.cells, case.labels, case.nparts, precision)
     o.set_phi(case.phi0)
     o.set_velocity(case.vel)
@@ -35,36 +39,67 @@ def _run_pair(case, steps, oracle_mod, precision):
     return out
 
 
+def _same_schedule(gs, os_):
+    return (gs.fim_iters, gs.patch_visits, gs.probes, gs.nb_verts, gs.n_seed_patches) == \
+           (os_.fim_iters, os_.patch_visits, os_.probes, os_.nb_verts, os_.n_seed_patches)
+
+
 @pytest.mark.parametrize("shape,n,block", [("sphere", 12, (4, 4, 4)), ("torus", 20, (8, 8, 4)),
-                                           ("plane", 16, (4, 4, 4))])
+                                           ("plane", 16, (4, 4, 4)), ("torus", 32, (8, 8, 4))])
 def test_tet_fp32_bit_exact(shape, n, block, oracle_mod):
     case = cases.small3d(n=n, shape=shape, block=block, steps=4)
-    for k, gs, os_, gphi, ophi, gnb, onb in _run_pair(case, 4, oracle_mod, 32):
-        print(f"{shape} step {k}: gpu iters {gs.fim_iters} visits {gs.patch_visits} nb {gs.nb_count}")
+    for k, gs, os_, gphi, ophi, gnb, onb in _lockstep(case, 4, oracle_mod, 32):
+        print(f"{shape}{n} step {k}: iters {gs.fim_iters} visits {gs.patch_visits} nb {gs.nb_count}")
         np.testing.assert_array_equal(gnb, onb, err_msg=f"narrowband step {k}")
-        assert (gs.fim_iters, gs.patch_visits, gs.probes, gs.nb_verts) == \
-               (os_.fim_iters, os_.patch_visits, os_.probes, os_.nb_verts), f"schedule step {k}"
+        assert _same_schedule(gs, os_), f"schedule step {k}"
         np.testing.assert_array_equal(gphi.astype(np.float32), ophi.astype(np.float32),
                                       err_msg=f"phi step {k}")
 
 
-@pytest.mark.parametrize("shape,n,block", [("sphere", 12, (4, 4, 4)), ("torus", 20, (8, 8, 4))])
-def test_tet_fp32_vs_fp64_oracle(shape, n, block, oracle_mod):
-    case = cases.small3d(n=n, shape=shape, block=block, steps=3)
-    for k, gs, os_, gphi, ophi, gnb, onb in _run_pair(case, 3, oracle_mod, 64):
+@pytest.mark.parametrize("shape,n,block", [("sphere", 12, (4, 4, 4)), ("torus", 24, (8, 4, 4))])
+def test_tet_fp64_engine_bit_exact(shape, n, block, oracle_mod):
+    case = cases.small3d(n=n, shape=shape, block=block, steps=4)
+    for k, gs, os_, gphi, ophi, gnb, onb in _lockstep(case, 4, oracle_mod, 64, engine=lsb.LS_F64):
         np.testing.assert_array_equal(gnb, onb, err_msg=f"narrowband step {k}")
-        finite = np.abs(ophi) < 1e8
-        np.testing.assert_array_equal(np.abs(gphi) < 1e8, finite)
-        err = np.max(np.abs(gphi[finite] - ophi[finite]))
-        print(f"{shape} step {k}: Linf vs FP64 = {err:.3e}")
-        assert err <= TOL_FP64
+        assert _same_schedule(gs, os_), f"schedule step {k}"
+        np.testing.assert_array_equal(gphi, ophi, err_msg=f"phi step {k}")
+
+
+@pytest.mark.parametrize("shape,n,block", [("sphere", 12, (4, 4, 4)), ("torus", 32, (8, 8, 4))])
+def test_tet_fp32_vs_fp64_oracle_per_step(shape, n, block, oracle_mod):
+    case = cases.small3d(n=n, shape=shape, block=block, steps=4)
+    g = lsb.solver_for_case(case)
+    o = oracle_mod.Oracle(3, case.xyz, case.cells, case.labels, case.nparts, 64)
+    o.set_velocity(case.vel)
+    gp = lsb.step_params(case.dt, case.bandwidth, case.niter)
+    op = oracle_mod.make_params(case.dt, case.bandwidth, case.niter)
+    phi_prev, nb_prev = case.phi0, None
+    for k in range(4):
+        o.set_phi(phi_prev)
+        if nb_prev is not None:
+            o.set_narrowband(nb_prev)
+        gs = g.step(gp)
+        o.step(op)
+        gphi, ophi, amb = g.phi(), o.phi(), o.ambiguous()
+        np.testing.assert_array_equal(g.narrowband(), o.narrowband(), err_msg=f"narrowband step {k}")
+        fin = np.abs(ophi) < 1e8
+        np.testing.assert_array_equal(np.abs(gphi) < 1e8, fin)
+        err_ok = float(np.max(np.abs(gphi - ophi)[fin & ~amb]))
+        err_amb = float(np.max(np.abs(gphi - ophi)[fin & amb])) if (fin & amb).any() else 0.0
+        print(f"{shape}{n} step {k}: Linf {err_ok:.2e} (ambiguous: {int(amb.sum())} verts, {err_amb:.2e})")
+        assert err_ok <= TOL_FP64
+        assert amb.sum() <= 0.05 * gs.nb_verts
+        assert err_amb <= 1e-3
+        phi_prev, nb_prev = gphi, g.narrowband()
+    g.close()
+    o.close()
 
 
 def test_tri_fp64_bit_exact_config1(oracle_mod):
     case = cases.config1()
-    res = _run_pair(case, case.steps, oracle_mod, 64)
+    res = _lockstep(case, case.steps, oracle_mod, 64)
     assert len(res) == case.steps
     for k, gs, os_, gphi, ophi, gnb, onb in res:
         np.testing.assert_array_equal(gnb, onb)
-        assert (gs.fim_iters, gs.patch_visits, gs.probes) == (os_.fim_iters, os_.patch_visits, os_.probes)
+        assert _same_schedule(gs, os_)
         np.testing.assert_array_equal(gphi, ophi, err_msg=f"phi step {k}")

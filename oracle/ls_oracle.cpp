@@ -288,6 +288,7 @@ struct Base {
   virtual void set_nb(const int32_t*, int) = 0;
   virtual void get_amb(uint8_t*) const = 0;
   virtual void get_coords(double*) const = 0;
+  virtual void advect_all(double dt) = 0;
   virtual int nparts() const = 0;
 };
 
@@ -500,6 +501,14 @@ struct Solver : Base {
   }
   int64_t ncopies() const override { return F; }
   void set_nb(const int32_t* ids, int n) override { narrowband.assign(ids, ids + n); }
+  // One advection pass over every patch on the current phi (no seeds, no FIM).
+  void advect_all(double dt) override {
+    lso_params prm{};
+    prm.dt = dt;
+    #pragma omp parallel for schedule(dynamic, 1)
+    for (int p = 0; p < P; ++p) advect_patch(p, prm);
+    phi = phi_out;
+  }
   void get_coords(double* out) const override {
     const int nk = dim == 3 ? 6 : 3;
     for (int64_t c = 0; c < F; ++c)
@@ -804,6 +813,8 @@ int64_t lso_num_copies(void* h) { return static_cast<Base*>(h)->ncopies(); }
 void lso_set_narrowband(void* h, const int32_t* ids, int n) { static_cast<Base*>(h)->set_nb(ids, n); }
 // Per-vertex (caller order) flag of threshold-ambiguous advection updates.
 void lso_get_ambiguous(void* h, uint8_t* out) { static_cast<Base*>(h)->get_amb(out); }
+// Advection alone over all patches (tests of the advection scheme).
+void lso_advect_all(void* h, double dt) { static_cast<Base*>(h)->advect_all(dt); }
 // Per-copy local coordinates in the storage type, copy-major.
 void lso_get_coords(void* h, double* out) { static_cast<Base*>(h)->get_coords(out); }
 void lso_get_layout(void* h, int32_t* vert_perm, int32_t* ele_perm, int32_t* vert_off,

@@ -64,6 +64,7 @@ def lib():
         L.lso_set_narrowband.argtypes = [vp, C.c_void_p, C.c_int]
         L.lso_get_ambiguous.argtypes = [vp, C.c_void_p]
         L.lso_get_coords.argtypes = [vp, C.c_void_p]
+        L.lso_advect_all.argtypes = [vp, C.c_double]
         _lib = L
     return _lib
 
@@ -127,6 +128,9 @@ class Oracle:
         out = np.zeros(self.N, dtype=np.uint8)
         self.L.lso_get_ambiguous(self.h, out.ctypes.data)
         return out.astype(bool)
+
+    def advect_all(self, dt):
+        self.L.lso_advect_all(self.h, float(dt))
 
     def coords(self):
         F = self.L.lso_num_copies(self.h)

@@ -29,6 +29,10 @@ enum {
 
 enum { LS_F32 = 0, LS_F64 = 1 };
 enum { LS_MODE_ADVECT = 0, LS_MODE_MCF = 1 };
+/* Execution of a timestep: one persistent cooperative kernel (default), the
+ * same kernel replayed from a captured hipGraph, or one launch per phase with
+ * the FIM loop driven from the host. Results are identical. */
+enum { LS_EXEC_PERSISTENT = 0, LS_EXEC_KERNELS = 1, LS_EXEC_GRAPH = 2 };
 
 /* Per-step parameters. Replaces the argument list of
  * meshFIM3d::GenerateData(char*, int nsteps, float timestep, int inside_niter,
@@ -62,6 +66,14 @@ typedef struct ls_stats {
   int64_t visit_copies, visit_verts, visit_halo;
   int64_t probe_copies, probe_verts, probe_halo;
   int64_t adv_copies, adv_verts, adv_halo;
+  int64_t seed_copies, seed_verts, seed_halo;
+  double ms_step_kernel;    /* duration of the persistent step kernel (LS_EXEC_PERSISTENT/GRAPH) */
+  int32_t exec_mode;        /* execution mode used for the step */
+  int32_t reserved;
+  /* Persistent kernel phase times (device wall clock, barrier to barrier):
+   * reset+seeds, FIM visits, copyback+activation, probes, next-list,
+   * narrowband, advection. */
+  double ms_ph_seed, ms_ph_visit, ms_ph_copyback, ms_ph_probe, ms_ph_next, ms_ph_nb, ms_ph_advect;
 } ls_stats;
 
 typedef struct ls_layout_info {
@@ -87,6 +99,9 @@ int ls_set_mesh(ls_ctx* ctx, int dim, int64_t n_verts, const double* xyz, int64_
  * identical to an FP64 host build). Triangle meshes always use LS_F64.
  * Call before partitioning. */
 int ls_set_precision(ls_ctx* ctx, int dtype);
+
+/* Selects the execution mode (LS_EXEC_*); may be called at any time. */
+int ls_set_execution(ls_ctx* ctx, int mode);
 
 /* Partition + patch layout (meshFIM3d::InitPatches/InitPatches2/
  * GenerateBlockNeighbors, src/core3d/cuda/meshFIM3d.cu:423-570).

@@ -20,6 +20,7 @@ struct ls_ctx {
   std::vector<double> vel;   // per element, kept to re-project after re-partitioning
   int patch_sigma = 0;
   int precision = LS_F32;  // tet storage precision
+  int exec = LS_EXEC_PERSISTENT;
   bool have_vel = false;
   lsamd::Layout layout;
   bool have_layout = false;
@@ -79,6 +80,7 @@ int finish_layout(ls_ctx* c, const std::vector<int32_t>& labels, int nparts) {
   if (rc != LS_OK) return fail(c, rc, err);
   c->layout = std::move(L);
   c->engine = std::move(eng);
+  c->engine->set_exec(c->exec);
   c->have_layout = true;
   return c->have_vel ? apply_velocity(c) : LS_OK;
 }
@@ -124,6 +126,15 @@ int ls_set_precision(ls_ctx* c, int dtype) {
   if (dtype != LS_F32 && dtype != LS_F64) return fail(c, LS_ERR_ARG, "dtype must be LS_F32 or LS_F64");
   if (c->have_layout) return fail(c, LS_ERR_STATE, "set the precision before partitioning");
   c->precision = dtype;
+  return LS_OK;
+}
+
+int ls_set_execution(ls_ctx* c, int mode) {
+  if (!c) return LS_ERR_ARG;
+  if (mode != LS_EXEC_PERSISTENT && mode != LS_EXEC_KERNELS && mode != LS_EXEC_GRAPH)
+    return fail(c, LS_ERR_ARG, "mode must be LS_EXEC_PERSISTENT, LS_EXEC_KERNELS or LS_EXEC_GRAPH");
+  c->exec = mode;
+  if (c->engine) c->engine->set_exec(mode);
   return LS_OK;
 }
 

@@ -46,6 +46,33 @@ struct DevState {
   int32_t* seedflag;   // P
   int32_t* counters;   // small array of device counters
   unsigned long long* sizes;  // kNumSizes patch-size sums for byte accounting
+  // Persistent-step state: epoch-tagged list membership and probe marks,
+  // the lists, and counters that survive across steps.
+  int32_t* inA;        // P, epoch of the active list holding the patch
+  int32_t* freshE;     // P, epoch in which the patch was marked for a probe
+  int32_t* listA;      // P
+  int32_t* listB;      // P
+  int32_t* listF;      // P
+  int32_t* nb[2];      // P each, narrowband lists (previous / new)
+  int32_t* pc;         // kNumPc persistent-step counters
+  unsigned long long* ticks;  // kNumTicks phase times (device wall clock)
+  unsigned int* bar;          // grid barrier: [0] arrivals, [1] generation
+};
+
+enum TickIndex : int { kTkSeed = 0, kTkVisit, kTkCopyback, kTkProbe, kTkNext, kTkNb, kTkAdvect, kNumTicks = 8 };
+
+// Persistent-step counters.
+enum PcIndex : int {
+  kPcNb = 0,       // [0],[1]: narrowband sizes of nb[0], nb[1]
+  kPcSeed = 2,     // initial active-list size
+  kPcFresh = 3,    // [3],[4]: probe-list sizes by iteration parity
+  kPcNext = 5,     // [5],[6]: next active-list sizes by iteration parity
+  kPcNbVerts = 7,  // vertices of the new narrowband
+  kPcIters = 8,    // FIM iterations
+  kPcVisits = 9,   // patch visits
+  kPcProbes = 10,  // probes
+  kPcOverflow = 11,// FIM iteration cap hit
+  kNumPc = 16
 };
 
 // Patch-size sums (element copies, owned, halo) per launch kind.
@@ -53,7 +80,8 @@ enum SizeCounter : int {
   kSzVisit = 0,   // FIM visits: copies, verts, halo
   kSzProbe = 3,   // neighbour probes
   kSzAdvect = 6,  // advection
-  kNumSizes = 9
+  kSzSeed = 9,    // seed scan
+  kNumSizes = 12
 };
 
 // Device counters layout.

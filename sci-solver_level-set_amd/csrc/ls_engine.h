@@ -22,6 +22,7 @@ class Engine {
   virtual int get_phi(double* phi_user, std::string& err) = 0;
   virtual int get_narrowband(std::vector<int32_t>& ids, std::string& err) = 0;
   virtual int working_bytes() const = 0;  // sizeof working real type
+  virtual void set_exec(int mode) = 0;
 };
 
 // dim 3 -> tet engine with FP32 (LS_F32) or FP64 (LS_F64) storage,

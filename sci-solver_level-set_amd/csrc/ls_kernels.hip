@@ -10,6 +10,7 @@
 // working type T (FP32 for tets, FP64 for triangles), with contraction off,
 // correctly rounded division and square root, and FP64 for the tet solver's
 // quadratic discriminant and roots.
+#include <hip/hip_cooperative_groups.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -188,6 +189,16 @@ __device__ T tri_solve(T TA, T TB, T LenAB, T LenBC, T LenAC) {
 template <typename T, int NC>
 struct ElemSolve;
 
+// Each corner's solve only looks at points of the opposite face, so it is at
+// least min(other corners) + height of the corner over that face; when that
+// bound, less a margin far above FP64 rounding of the candidates, exceeds the
+// corner's value, min(value, solve) is the value and the solve is skipped.
+// The result is bit-identical to always solving.
+__device__ __forceinline__ bool must_solve(double Tk, double mo, double Mo, double hk, double scale) {
+  const double margin = 1e-12 * (Mo + hk + scale + 1.0);
+  return !(mo + hk - margin > Tk);
+}
+
 template <typename T>
 struct ElemSolve<T, 4> {
   static constexpr int kCoords = 6;
@@ -195,6 +206,28 @@ struct ElemSolve<T, 4> {
     double L[6];
 #pragma unroll
     for (int k = 0; k < 6; ++k) L[k] = (double)Lt[k];
+    const double Tv[4] = {(double)v[0], (double)v[1], (double)v[2], (double)v[3]};
+    // Heights of each corner over its opposite face, in the local frame
+    // A=(0,0,0), B=(L0,0,0), C=(L1,L2,0), D=(L3,L4,L5).
+    const double v6 = d_abs(L[0] * L[2] * L[5]);
+    const double nAx = L[2] * L[5], nAy = -(L[1] - L[0]) * L[5], nAz = (L[1] - L[0]) * L[4] - L[2] * (L[3] - L[0]);
+    const double nBx = L[2] * L[5], nBy = -L[1] * L[5], nBz = L[1] * L[4] - L[2] * L[3];
+    double h[4];
+    h[0] = v6 / d_sqrt(nAx * nAx + nAy * nAy + nAz * nAz);
+    h[1] = v6 / d_sqrt(nBx * nBx + nBy * nBy + nBz * nBz);
+    h[2] = d_abs(L[2] * L[5]) / d_sqrt(L[4] * L[4] + L[5] * L[5]);
+    h[3] = d_abs(L[5]);
+    const double scale = L[0] + d_abs(L[1]) + d_abs(L[2]) + d_abs(L[3]) + d_abs(L[4]) + d_abs(L[5]);
+    bool need[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      double mo = 1e300, Mo = 0.0;
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        if (j != k) { mo = d_min(mo, Tv[j]); Mo = d_max(Mo, Tv[j]); }
+      need[k] = must_solve(Tv[k], mo, Mo, h[k], scale);
+    }
+    if (!(need[0] || need[1] || need[2] || need[3])) return;
     const double ACBC = L[1] * (L[1] - L[0]) + L[2] * L[2];
     const double BCCD = (L[1] - L[0]) * (L[3] - L[1]) + L[2] * (L[4] - L[2]);
     const double ACCD = L[1] * (L[3] - L[1]) + L[2] * (L[4] - L[2]);
@@ -210,11 +243,11 @@ struct ElemSolve<T, 4> {
     const double ACAC = ACAD - ACCD;
     const double BDBD = BCBD + CDBD;
     const double BCBC = BCBD - BCCD;
-    const double TA = v[0], TB = v[1], TC = v[2], TD = v[3];
-    v[3] = (T)d_min(TD, tet_solve(TA, TB, TC, ACAC, ACBC, ACCD, BCBC, BCCD, CDCD, leps));
-    v[0] = (T)d_min(TA, tet_solve(TB, TD, TC, BCBC, -BCCD, -ACBC, CDCD, ACCD, ACAC, leps));
-    v[1] = (T)d_min(TB, tet_solve(TA, TD, TC, ACAC, -ACCD, -ACBC, CDCD, BCCD, BCBC, leps));
-    v[2] = (T)d_min(TC, tet_solve(TA, TB, TD, ADAD, ADBD, -ADCD, BDBD, -CDBD, CDCD, leps));
+    const double TA = Tv[0], TB = Tv[1], TC = Tv[2], TD = Tv[3];
+    if (need[3]) v[3] = (T)d_min(TD, tet_solve(TA, TB, TC, ACAC, ACBC, ACCD, BCBC, BCCD, CDCD, leps));
+    if (need[0]) v[0] = (T)d_min(TA, tet_solve(TB, TD, TC, BCBC, -BCCD, -ACBC, CDCD, ACCD, ACAC, leps));
+    if (need[1]) v[1] = (T)d_min(TB, tet_solve(TA, TD, TC, ACAC, -ACCD, -ACBC, CDCD, BCCD, BCBC, leps));
+    if (need[2]) v[2] = (T)d_min(TC, tet_solve(TA, TB, TD, ADAD, ADBD, -ADCD, BDBD, -CDBD, CDCD, leps));
   }
 };
 
@@ -226,10 +259,21 @@ struct ElemSolve<T, 3> {
     const double LenAB = d_abs(c[0]);
     const double LenBC = d_sqrt(c[2] * c[2] + (c[1] - c[0]) * (c[1] - c[0]));
     const double LenCA = d_sqrt(c[1] * c[1] + c[2] * c[2]);
-    const double TA = v[0], TB = v[1], TC = v[2];
-    v[2] = (T)d_min(TC, tri_solve(TA, TB, LenAB, LenBC, LenCA));
-    v[0] = (T)d_min(TA, tri_solve(TB, TC, LenBC, LenCA, LenAB));
-    v[1] = (T)d_min(TB, tri_solve(TC, TA, LenCA, LenAB, LenBC));
+    const double Tv[3] = {(double)v[0], (double)v[1], (double)v[2]};
+    // Heights over the opposite edge: A=(0,0), B=(c0,0), C=(c1,c2).
+    const double a2 = d_abs(c[0] * c[2]);
+    const double h[3] = {a2 / LenBC, a2 / LenCA, d_abs(c[2])};
+    const double scale = LenAB + LenBC + LenCA;
+    bool need[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const double o1 = Tv[(k + 1) % 3], o2 = Tv[(k + 2) % 3];
+      need[k] = must_solve(Tv[k], d_min(o1, o2), d_max(o1, o2), h[k], scale);
+    }
+    const double TA = Tv[0], TB = Tv[1], TC = Tv[2];
+    if (need[2]) v[2] = (T)d_min(TC, tri_solve(TA, TB, LenAB, LenBC, LenCA));
+    if (need[0]) v[0] = (T)d_min(TA, tri_solve(TB, TC, LenBC, LenCA, LenAB));
+    if (need[1]) v[1] = (T)d_min(TB, tri_solve(TC, TA, LenCA, LenAB, LenBC));
   }
 };
 
@@ -259,44 +303,41 @@ __device__ __forceinline__ int corner(const ushort4& c, int i) {
 }
 
 // ---------------------------------------------------------------------------
-// Per-step reset: DT := LARGENUM, Label := Far, Tsign := (phi >= 0)
-// (redistance3d.cu:31-35 and kernel_reinit_Tsign, redistance_kernels3d.h:785-792).
+// Patch bodies. Each runs on one workgroup of BS threads for one patch and
+// starts with a barrier so consecutive patches can reuse the LDS.
+
+// Per-step reset of one vertex range: DT := LARGENUM, Label := Far,
+// Tsign := (phi >= 0) (redistance3d.cu:31-35 and kernel_reinit_Tsign,
+// redistance_kernels3d.h:785-792).
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_init(int64_t N, DevState<T> s) {
-  for (int64_t v = (int64_t)blockIdx.x * kBlock + threadIdx.x; v < N; v += (int64_t)gridDim.x * kBlock) {
-    s.DT[v] = (T)kLargeNum;
-    s.label[v] = kFarPoint;
-    s.tsign[v] = (s.phi[v] < (T)0) ? 0 : 1;
-  }
+__device__ __forceinline__ void init_vertex(DevState<T>& s, int64_t v) {
+  s.DT[v] = (T)kLargeNum;
+  s.label[v] = kFarPoint;
+  s.tsign[v] = (s.phi[v] < (T)0) ? 0 : 1;
 }
 
-__global__ void k_set_counters(int32_t* c, int n, int32_t val) {
-  if ((int)threadIdx.x < n) c[threadIdx.x] = val;
-}
-
-__global__ void k_clear_sizes(unsigned long long* c) {
-  if ((int)threadIdx.x < kNumSizes) c[threadIdx.x] = 0ull;
-}
-
-// Seeds over the scanned patches (kernel_seedlabel / _narrowband,
+// Seeds of one patch (kernel_seedlabel / _narrowband,
 // redistance_kernels3d.h:623-770): a corner with phi == 0 is a seed, and so
-// is every corner of a sign-changing element; seeds get DT = |phi|. A patch
-// holding such an element joins the active list.
-template <typename T, int NC>
-__global__ __launch_bounds__(kBlock) void k_seeds(DevMesh<T> m, DevState<T> s, const int32_t* scan,
-                                                  int32_t* active, int32_t* counters) {
-  extern __shared__ __align__(16) unsigned char smem[];
+// is every corner of a sign-changing element; seeds get DT = |phi|. Returns
+// (to every thread) whether the patch holds such an element.
+template <typename T, int NC, int BS>
+__device__ int seeds_patch(const DevMesh<T>& m, DevState<T>& s, int p, unsigned char* smem) {
+  __syncthreads();
   T* s_vtx = (T*)smem;
-  const int p = scan[blockIdx.x];
   const int vs = m.vert_off[p], ve = m.vert_off[p + 1];
   const int es = m.ele_off[p], ee = m.ele_off[p + 1];
   const int hs = m.halo_off[p], he = m.halo_off[p + 1];
   const int nv = ve - vs, ne = ee - es, nh = he - hs;
-  for (int i = threadIdx.x; i < nv; i += kBlock) s_vtx[i] = s.phi[vs + i];
-  for (int i = threadIdx.x; i < nh; i += kBlock) s_vtx[nv + i] = s.phi[m.halo_ids[hs + i]];
+  for (int i = threadIdx.x; i < nv; i += BS) s_vtx[i] = s.phi[vs + i];
+  for (int i = threadIdx.x; i < nh; i += BS) s_vtx[nv + i] = s.phi[m.halo_ids[hs + i]];
+  if (threadIdx.x == 0) {
+    atomicAdd(s.sizes + kSzSeed + 0, (unsigned long long)ne);
+    atomicAdd(s.sizes + kSzSeed + 1, (unsigned long long)nv);
+    atomicAdd(s.sizes + kSzSeed + 2, (unsigned long long)nh);
+  }
   __syncthreads();
   int flag = 0;
-  for (int e = threadIdx.x; e < ne; e += kBlock) {
+  for (int e = threadIdx.x; e < ne; e += BS) {
     const ushort4 c = m.copy_local[es + e];
     T val[NC];
     bool allpos = true, allneg = true;
@@ -318,8 +359,21 @@ __global__ __launch_bounds__(kBlock) void k_seeds(DevMesh<T> m, DevState<T> s, c
       }
     }
   }
-  flag = __syncthreads_or(flag);
-  if (threadIdx.x == 0 && flag) active[atomicAdd(&counters[kCntSeed], 1)] = p;
+  return __syncthreads_or(flag);
+}
+
+// Block-wide min for BS threads; every thread must call it.
+template <typename T, int BS>
+__device__ __forceinline__ T block_min_n(T v, T* scratch) {
+  v = wave_min(v);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) scratch[wave] = v;
+  __syncthreads();
+  T r = scratch[0];
+#pragma unroll
+  for (int w = 1; w < BS / 64; ++w) r = d_min(r, scratch[w]);
+  __syncthreads();
+  return r;
 }
 
 // FIM patch visit (kernel_update_values / kernel_run_check_neghbor,
@@ -327,17 +381,15 @@ __global__ __launch_bounds__(kBlock) void k_seeds(DevMesh<T> m, DevState<T> s, c
 // element copies in LDS; each sweep updates every element from its corner
 // values, then every owned vertex takes the min over its slots and writes it
 // back to them. An element whose four inputs are unchanged since its last
-// update would reproduce its values exactly, so it is skipped (dirty flags);
-// this changes no result. The last sweep decides convergence per vertex
-// (old - new < tol; seeds count as converged), and the workgroup reduces it
-// to the patch flag plus the patch min of the written distances
-// (run_reduction_bandwidth, redistance_kernels3d.h:57-97).
-template <typename T, int NC>
-__global__ __launch_bounds__(kBlock) void k_sweep(DevMesh<T> m, DevState<T> s, const int32_t* list,
-                                                  int niter, int write_out, T tol, double leps) {
-  extern __shared__ __align__(16) unsigned char smem[];
-  __shared__ T red[kBlock / 64];
-  const int p = list[blockIdx.x];
+// update would reproduce its values exactly, so each sweep only visits the
+// elements marked dirty, compacted into an LDS work list. The last sweep
+// decides convergence per vertex (old - new < tol; seeds count as converged),
+// and the workgroup reduces it to the patch flag plus the patch min of the
+// written distances (run_reduction_bandwidth, redistance_kernels3d.h:57-97).
+template <typename T, int NC, int BS>
+__device__ void sweep_patch(const DevMesh<T>& m, DevState<T>& s, int p, int niter, int write_out, T tol,
+                            double leps, unsigned char* smem, T* red, int* s_ndirty) {
+  __syncthreads();
   const int vs = m.vert_off[p], ve = m.vert_off[p + 1];
   const int es = m.ele_off[p], ee = m.ele_off[p + 1];
   const int hs = m.halo_off[p], he = m.halo_off[p + 1];
@@ -349,39 +401,54 @@ __global__ __launch_bounds__(kBlock) void k_sweep(DevMesh<T> m, DevState<T> s, c
   T* s_vval = s_vtx + nv + nh;
   uint16_t* s_voff = (uint16_t*)(s_vval + nv);
   uint16_t* s_vsl = s_voff + nv + 1;
-  uint8_t* s_dirty = (uint8_t*)(s_vsl + nslots);
+  uint16_t* s_work = s_vsl + nslots;            // compacted dirty elements
+  uint8_t* s_dirty = (uint8_t*)(s_work + ne);
   const int tid = threadIdx.x;
 
-  for (int i = tid; i < nv; i += kBlock) {
+  for (int i = tid; i < nv; i += BS) {
     const T v = s.DT[vs + i];
     s_vtx[i] = v;
     s_vval[i] = v;
     s_voff[i] = (uint16_t)(m.vslot_off[vs + i] - sbase);
   }
   if (tid == 0) s_voff[nv] = (uint16_t)nslots;
-  for (int i = tid; i < nh; i += kBlock) s_vtx[nv + i] = s.DT[m.halo_ids[hs + i]];
-  for (int i = tid; i < nslots; i += kBlock) s_vsl[i] = m.vslots[sbase + i];
+  for (int i = tid; i < nh; i += BS) s_vtx[nv + i] = s.DT[m.halo_ids[hs + i]];
+  for (int i = tid; i < nslots; i += BS) s_vsl[i] = m.vslots[sbase + i];
   __syncthreads();
-  for (int e = tid; e < ne; e += kBlock) {
+  for (int e = tid; e < ne; e += BS) {
     const ushort4 c = m.copy_local[es + e];
 #pragma unroll
     for (int i = 0; i < NC; ++i) s_slot[e * NC + i] = s_vtx[corner(c, i)];
     s_dirty[e] = 1;
   }
-  __syncthreads();
 
   constexpr int K = ElemSolve<T, NC>::kCoords;
   int con_all = 1;
   T vmin = (T)kLargeNum;
   for (int it = 0; it < niter; ++it) {
-    for (int e = tid; e < ne; e += kBlock) {
-      if (!s_dirty[e]) continue;
+    if (tid == 0) *s_ndirty = 0;
+    __syncthreads();
+    // Wave-aggregated compaction; the loop bound is uniform across the block.
+    for (int base = 0; base < ne; base += BS) {
+      const int e = base + tid;
+      const bool d = e < ne && s_dirty[e];
+      const unsigned long long mask = __ballot(d);
+      const int lane = tid & 63;
+      int off = 0;
+      if (lane == 0 && mask) off = atomicAdd(s_ndirty, __popcll(mask));
+      off = __shfl(off, 0, 64);
+      if (d) s_work[off + __popcll(mask & ((1ull << lane) - 1ull))] = (uint16_t)e;
+    }
+    __syncthreads();
+    const int nd = *s_ndirty;
+    for (int k = tid; k < nd; k += BS) {
+      const int e = s_work[k];
       T x[NC], y[NC], L[K];
 #pragma unroll
       for (int i = 0; i < NC; ++i) x[i] = y[i] = s_slot[e * NC + i];
       const T* gc = m.coords + (int64_t)(es + e) * K;
 #pragma unroll
-      for (int k = 0; k < K; ++k) L[k] = gc[k];
+      for (int q = 0; q < K; ++q) L[q] = gc[q];
       ElemSolve<T, NC>::update(L, y, leps);
       int changed = 0;
 #pragma unroll
@@ -392,15 +459,15 @@ __global__ __launch_bounds__(kBlock) void k_sweep(DevMesh<T> m, DevState<T> s, c
     }
     __syncthreads();
     const bool last = (it == niter - 1);
-    for (int v = tid; v < nv; v += kBlock) {
+    for (int v = tid; v < nv; v += BS) {
       const int o0 = s_voff[v], o1 = s_voff[v + 1];
       const T old = s_vval[v];
       T mn = old;
       if (o1 > o0) {
         mn = s_slot[s_vsl[o0]];
-        for (int k = o0 + 1; k < o1; ++k) mn = d_min(s_slot[s_vsl[k]], mn);
-        for (int k = o0; k < o1; ++k) {
-          const int sl = s_vsl[k];
+        for (int q = o0 + 1; q < o1; ++q) mn = d_min(s_slot[s_vsl[q]], mn);
+        for (int q = o0; q < o1; ++q) {
+          const int sl = s_vsl[q];
           if (mn < s_slot[sl]) { s_slot[sl] = mn; s_dirty[sl / NC] = 1; }
         }
         s_vval[v] = mn;
@@ -420,10 +487,9 @@ __global__ __launch_bounds__(kBlock) void k_sweep(DevMesh<T> m, DevState<T> s, c
         vmin = d_min(vmin, outv);
       }
     }
-    __syncthreads();
   }
   con_all = __syncthreads_and(con_all);
-  const T bm = block_min(vmin, red);
+  const T bm = block_min_n<T, BS>(vmin, red);
   if (tid == 0) {
     s.bcon[p] = con_all;
     if (write_out) s.bmin[p] = bm;
@@ -434,67 +500,21 @@ __global__ __launch_bounds__(kBlock) void k_sweep(DevMesh<T> m, DevState<T> s, c
   }
 }
 
-// CopyOutBack (redistance_kernels3d.h:772-783) plus the active-list label of
-// the patch: it stays active iff it did not converge.
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_copyback(DevMesh<T> m, DevState<T> s, const int32_t* list) {
-  const int p = list[blockIdx.x];
-  const int vs = m.vert_off[p], ve = m.vert_off[p + 1];
-  for (int v = vs + threadIdx.x; v < ve; v += kBlock) s.DT[v] = s.DT_out[v];
-  if (threadIdx.x == 0) s.act[p] = s.bcon[p] ? 0 : 1;
-}
-
-// Neighbour activation (redistance3d.cu:141-181): a converged patch whose
-// min is below the band activates every neighbour not currently active.
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_activate(DevMesh<T> m, DevState<T> s, const int32_t* list, int n,
-                                                     T bw, int32_t* fresh_list) {
-  const int i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= n) return;
-  const int p = list[i];
-  if (!(s.bcon[p] && s.bmin[p] < bw)) return;
-  for (int k = m.padj_off[p]; k < m.padj_off[p + 1]; ++k) {
-    const int q = m.padj[k];
-    if (s.act[q] == 0 && atomicExch(&s.fresh[q], 1) == 0)
-      fresh_list[atomicAdd(&s.counters[kCntFresh], 1)] = q;
-  }
-}
-
-// Next active list (redistance3d.cu:183-222): unconverged patches of the
-// current list, then probed neighbours that did not converge.
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_finalize(DevState<T> s, const int32_t* list, int n,
-                                                     const int32_t* fresh_list, int nfresh, int32_t* next) {
-  const int i = blockIdx.x * kBlock + threadIdx.x;
-  if (i < n) {
-    const int p = list[i];
-    if (s.act[p]) next[atomicAdd(&s.counters[kCntNext], 1)] = p;
-  } else if (i < n + nfresh) {
-    const int q = fresh_list[i - n];
-    s.fresh[q] = 0;
-    if (!s.bcon[q]) next[atomicAdd(&s.counters[kCntNext], 1)] = q;
-  }
-}
-
-// Narrowband (kernel_compute_new_narrowband, redistance_kernels3d.h:22-55):
-// patches whose min distance is <= the band; fused with the sign restore
+// Narrowband test of one patch (kernel_compute_new_narrowband,
+// redistance_kernels3d.h:22-55) fused with the sign restore
 // phi = DT * (2*Tsign - 1) (kernel_recover_Tsign_whole, :807-814).
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_narrowband(DevMesh<T> m, DevState<T> s, T bw, int32_t* nb) {
-  __shared__ T red[kBlock / 64];
-  const int p = blockIdx.x;
+// Returns the patch min to every thread.
+template <typename T, int BS>
+__device__ T narrowband_patch(const DevMesh<T>& m, DevState<T>& s, int p, T* red) {
+  __syncthreads();
   const int vs = m.vert_off[p], ve = m.vert_off[p + 1];
   T mn = (T)kLargeNum;
-  for (int v = vs + threadIdx.x; v < ve; v += kBlock) {
+  for (int v = vs + threadIdx.x; v < ve; v += BS) {
     const T d = s.DT[v];
     mn = d_min(mn, d);
     s.phi[v] = d * (T)(2 * (int)s.tsign[v] - 1);
   }
-  mn = block_min(mn, red);
-  if (threadIdx.x == 0 && mn <= bw) {
-    nb[atomicAdd(&s.counters[kCntNB], 1)] = p;
-    atomicAdd(&s.counters[kCntNBVerts], ve - vs);
-  }
+  return block_min_n<T, BS>(mn, red);
 }
 
 // Shape-function gradients of a copy in its local frame (FP64).
@@ -538,29 +558,29 @@ struct Grad<3> {
 // (kernel_updateT_single_stage3d, meshFIM_kernels.h:36-408): per element copy
 // K_i = vol sigma.gradN_i, H = sum K_i phi_i, distribution coefficients
 // alpha~_i; per owned vertex phi -= dt * sum(alpha~ H) / sum(alpha~ vol).
-// Evaluated in FP64 from the stored values; the result is stored as T.
-template <typename T, int NC>
-__global__ __launch_bounds__(kBlock) void k_advect(DevMesh<T> m, DevState<T> s, const int32_t* nb, double dt) {
-  extern __shared__ __align__(16) unsigned char smem[];
+// Evaluated in FP64 from the stored values; per-slot products and the result
+// are stored as T.
+template <typename T, int NC, int BS>
+__device__ void advect_patch(const DevMesh<T>& m, DevState<T>& s, int p, double dt, unsigned char* smem) {
+  __syncthreads();
   constexpr int K = ElemSolve<T, NC>::kCoords;
   constexpr int D = NC - 1;
-  const int p = nb[blockIdx.x];
   const int vs = m.vert_off[p], ve = m.vert_off[p + 1];
   const int es = m.ele_off[p], ee = m.ele_off[p + 1];
   const int hs = m.halo_off[p], he = m.halo_off[p + 1];
   const int nv = ve - vs, ne = ee - es, nh = he - hs;
-  T* s_up = (T*)smem;  // per-slot products, rounded to the storage type
+  T* s_up = (T*)smem;
   T* s_dn = s_up + NC * ne;
   T* s_vtx = s_dn + NC * ne;
-  for (int i = threadIdx.x; i < nv; i += kBlock) s_vtx[i] = s.phi[vs + i];
-  for (int i = threadIdx.x; i < nh; i += kBlock) s_vtx[nv + i] = s.phi[m.halo_ids[hs + i]];
+  for (int i = threadIdx.x; i < nv; i += BS) s_vtx[i] = s.phi[vs + i];
+  for (int i = threadIdx.x; i < nh; i += BS) s_vtx[nv + i] = s.phi[m.halo_ids[hs + i]];
   if (threadIdx.x == 0) {
     atomicAdd(s.sizes + kSzAdvect + 0, (unsigned long long)ne);
     atomicAdd(s.sizes + kSzAdvect + 1, (unsigned long long)nv);
     atomicAdd(s.sizes + kSzAdvect + 2, (unsigned long long)nh);
   }
   __syncthreads();
-  for (int e = threadIdx.x; e < ne; e += kBlock) {
+  for (int e = threadIdx.x; e < ne; e += BS) {
     const ushort4 c = m.copy_local[es + e];
     double ph[NC], L[K], sg[D], g[NC][3], vol;
 #pragma unroll
@@ -610,7 +630,7 @@ __global__ __launch_bounds__(kBlock) void k_advect(DevMesh<T> m, DevState<T> s, 
     }
   }
   __syncthreads();
-  for (int v = threadIdx.x; v < nv; v += kBlock) {
+  for (int v = threadIdx.x; v < nv; v += BS) {
     const int o0 = m.vslot_off[vs + v], o1 = m.vslot_off[vs + v + 1];
     double up = 0.0, dn = 0.0;
     for (int k = o0; k < o1; ++k) up += (double)s_up[m.vslots[k]];
@@ -618,6 +638,103 @@ __global__ __launch_bounds__(kBlock) void k_advect(DevMesh<T> m, DevState<T> s, 
     const double old = (double)s_vtx[v];
     s.phi_out[vs + v] = (T)((d_abs(dn) > 1e-8) ? old - dt * (up / dn) : old);
   }
+}
+
+// ---------------------------------------------------------------------------
+// Host-driven path: one launch per phase.
+
+constexpr int kFimBlock = 1024;  // 16 waves: one patch fills a CU
+constexpr int kFimIterCap = 100000;  // FIM iteration cap when none is requested
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_init(int64_t N, DevState<T> s) {
+  for (int64_t v = (int64_t)blockIdx.x * kBlock + threadIdx.x; v < N; v += (int64_t)gridDim.x * kBlock)
+    init_vertex(s, v);
+}
+
+__global__ void k_set_counters(int32_t* c, int n, int32_t val) {
+  if ((int)threadIdx.x < n) c[threadIdx.x] = val;
+}
+
+__global__ void k_clear_sizes(unsigned long long* c) {
+  if ((int)threadIdx.x < kNumSizes) c[threadIdx.x] = 0ull;
+}
+
+template <typename T, int NC>
+__global__ __launch_bounds__(kBlock) void k_seeds(DevMesh<T> m, DevState<T> s, const int32_t* scan,
+                                                  int32_t* active, int32_t* counters) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int p = scan[blockIdx.x];
+  const int flag = seeds_patch<T, NC, kBlock>(m, s, p, smem);
+  if (threadIdx.x == 0 && flag) active[atomicAdd(&counters[kCntSeed], 1)] = p;
+}
+
+template <typename T, int NC, int BS>
+__global__ __launch_bounds__(BS) void k_sweep(DevMesh<T> m, DevState<T> s, const int32_t* list,
+                                              int niter, int write_out, T tol, double leps) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ T red[BS / 64];
+  __shared__ int s_ndirty;
+  sweep_patch<T, NC, BS>(m, s, list[blockIdx.x], niter, write_out, tol, leps, smem, red, &s_ndirty);
+}
+
+// CopyOutBack (redistance_kernels3d.h:772-783) plus the active-list label of
+// the patch: it stays active iff it did not converge.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_copyback(DevMesh<T> m, DevState<T> s, const int32_t* list) {
+  const int p = list[blockIdx.x];
+  const int vs = m.vert_off[p], ve = m.vert_off[p + 1];
+  for (int v = vs + threadIdx.x; v < ve; v += kBlock) s.DT[v] = s.DT_out[v];
+  if (threadIdx.x == 0) s.act[p] = s.bcon[p] ? 0 : 1;
+}
+
+// Neighbour activation (redistance3d.cu:141-181): a converged patch whose
+// min is below the band activates every neighbour not currently active.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_activate(DevMesh<T> m, DevState<T> s, const int32_t* list, int n,
+                                                     T bw, int32_t* fresh_list) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const int p = list[i];
+  if (!(s.bcon[p] && s.bmin[p] < bw)) return;
+  for (int k = m.padj_off[p]; k < m.padj_off[p + 1]; ++k) {
+    const int q = m.padj[k];
+    if (s.act[q] == 0 && atomicExch(&s.fresh[q], 1) == 0)
+      fresh_list[atomicAdd(&s.counters[kCntFresh], 1)] = q;
+  }
+}
+
+// Next active list (redistance3d.cu:183-222): unconverged patches of the
+// current list, then probed neighbours that did not converge.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_finalize(DevState<T> s, const int32_t* list, int n,
+                                                     const int32_t* fresh_list, int nfresh, int32_t* next) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) {
+    const int p = list[i];
+    if (s.act[p]) next[atomicAdd(&s.counters[kCntNext], 1)] = p;
+  } else if (i < n + nfresh) {
+    const int q = fresh_list[i - n];
+    s.fresh[q] = 0;
+    if (!s.bcon[q]) next[atomicAdd(&s.counters[kCntNext], 1)] = q;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_narrowband(DevMesh<T> m, DevState<T> s, T bw, int32_t* nb) {
+  __shared__ T red[kBlock / 64];
+  const int p = blockIdx.x;
+  const T mn = narrowband_patch<T, kBlock>(m, s, p, red);
+  if (threadIdx.x == 0 && mn <= bw) {
+    nb[atomicAdd(&s.counters[kCntNB], 1)] = p;
+    atomicAdd(&s.counters[kCntNBVerts], m.vert_off[p + 1] - m.vert_off[p]);
+  }
+}
+
+template <typename T, int NC>
+__global__ __launch_bounds__(kBlock) void k_advect(DevMesh<T> m, DevState<T> s, const int32_t* nb, double dt) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  advect_patch<T, NC, kBlock>(m, s, nb[blockIdx.x], dt, smem);
 }
 
 // CopyOutBack_levelset3d (meshFIM_kernels.h:22-34).
@@ -631,6 +748,204 @@ inline int grid_for(int64_t n) {
   int64_t g = (n + kBlock - 1) / kBlock;
   if (g > 65536) g = 65536;
   return (int)(g < 1 ? 1 : g);
+}
+
+// ---------------------------------------------------------------------------
+// Persistent path: one cooperative launch per timestep. Phases are separated
+// by grid barriers; the FIM active-list loop runs on the device, so a step
+// needs no host round trip.
+
+template <typename T>
+struct StepArgs {
+  int first;         // 1: seeds scan every patch; 0: the previous narrowband
+  int nb_in;         // nb buffer holding the previous narrowband
+  int niter;
+  int inside_niter;
+  int max_iters;     // FIM iteration cap (0 = none)
+  T tol;
+  T bw;
+  double leps;
+  double dt;
+};
+
+// Counters are read right after a grid barrier, whose acquire fence already
+// dropped stale lines; a relaxed agent-scope load suffices.
+__device__ __forceinline__ int ld_acquire(const int32_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Grid barrier for a co-resident grid (cooperative launch): one arrival
+// atomic per workgroup and a generation word the waiting workgroups poll.
+// The workgroup barrier makes every wave's stores visible to wave 0, whose
+// agent-scope fences then publish them (release) and drop stale cache lines
+// for the whole CU (acquire); one wave per workgroup pays for the fences.
+__device__ __forceinline__ void grid_barrier(unsigned int* bar, unsigned int nblocks) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();  // release this workgroup's writes
+    const unsigned int gen = __hip_atomic_load(&bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned int arrived =
+        __hip_atomic_fetch_add(&bar[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1u;
+    if (arrived == nblocks) {
+      __hip_atomic_store(&bar[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&bar[1], gen + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      while (__hip_atomic_load(&bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen)
+        __builtin_amdgcn_s_sleep(1);
+    }
+    __threadfence();  // acquire the other workgroups' writes
+  }
+  __syncthreads();
+}
+
+// Block 0 accumulates barrier-to-barrier phase times.
+struct PhaseClock {
+  unsigned long long t;
+  __device__ void start(bool on) { if (on) t = wall_clock64(); }
+  __device__ void lap(bool on, unsigned long long* acc) {
+    if (on) { const unsigned long long n = wall_clock64(); *acc += n - t; t = n; }
+  }
+};
+
+template <typename T, int NC, int BS>
+__global__ __launch_bounds__(BS) void k_step(DevMesh<T> m, DevState<T> s, StepArgs<T> a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ T red[BS / 64];
+  __shared__ int s_ndirty;
+  const int G = gridDim.x, b = blockIdx.x, tid = threadIdx.x;
+  const int64_t gtid = (int64_t)b * BS + tid, gsize = (int64_t)G * BS;
+  const int nb_out = 1 - a.nb_in;
+  const bool clk = (gtid == 0);
+  PhaseClock pcl;
+  pcl.start(clk);
+  unsigned long long tk[kNumTicks] = {0, 0, 0, 0, 0, 0, 0, 0};
+
+  // Phase 0: vertex reset and sign save; patch marks; counters.
+  for (int64_t v = gtid; v < m.N; v += gsize) init_vertex(s, v);
+  for (int64_t q = gtid; q < m.P; q += gsize) { s.inA[q] = 0; s.freshE[q] = 0; }
+  if (gtid < kNumSizes) s.sizes[gtid] = 0ull;
+  if (gtid == 0) {
+    s.pc[kPcSeed] = 0;
+    s.pc[kPcFresh] = s.pc[kPcFresh + 1] = 0;
+    s.pc[kPcNext] = s.pc[kPcNext + 1] = 0;
+    s.pc[kPcNbVerts] = 0;
+    s.pc[kPcIters] = s.pc[kPcVisits] = s.pc[kPcProbes] = 0;
+    s.pc[kPcOverflow] = 0;
+    s.pc[kPcNb + nb_out] = 0;
+  }
+  grid_barrier(s.bar, G);
+
+  // Phase 1: seeds over every patch (first step) or the previous narrowband.
+  const int nscan = a.first ? m.P : ld_acquire(&s.pc[kPcNb + a.nb_in]);
+  for (int i = b; i < nscan; i += G) {
+    const int p = a.first ? i : s.nb[a.nb_in][i];
+    if (seeds_patch<T, NC, BS>(m, s, p, smem) && tid == 0) {
+      s.listA[atomicAdd(&s.pc[kPcSeed], 1)] = p;
+      s.inA[p] = 1;
+    }
+  }
+  grid_barrier(s.bar, G);
+  pcl.lap(clk, &tk[kTkSeed]);
+
+  // Phase 2: FIM (redistance3d.cu:85-224). Epoch E tags the members of the
+  // current list (inA[p] == E) and the patches probed in this iteration
+  // (freshE[q] == E).
+  int32_t* cur = s.listA;
+  int32_t* nxt = s.listB;
+  int nA = ld_acquire(&s.pc[kPcSeed]);
+  int it = 0;
+  while (nA > 0) {
+    if (a.max_iters > 0 && it >= a.max_iters) {
+      if (gtid == 0) s.pc[kPcOverflow] = 1;
+      break;
+    }
+    const int E = it + 1, par = it & 1;
+    for (int i = b; i < nA; i += G)
+      sweep_patch<T, NC, BS>(m, s, cur[i], a.niter, 1, a.tol, a.leps, smem, red, &s_ndirty);
+    grid_barrier(s.bar, G);
+    pcl.lap(clk, &tk[kTkVisit]);
+    // Every block has read this iteration's list size, so the counters the
+    // next iteration appends to can be cleared.
+    if (gtid == 0) {
+      s.pc[kPcIters] += 1;
+      s.pc[kPcVisits] += nA;
+      s.pc[kPcFresh + (1 - par)] = 0;
+      s.pc[kPcNext + (1 - par)] = 0;
+    }
+    // CopyOutBack and neighbour activation: a converged patch below the band
+    // marks every neighbour that is not an unconverged member of the list.
+    for (int i = b; i < nA; i += G) {
+      const int p = cur[i];
+      const int vs = m.vert_off[p], ve = m.vert_off[p + 1];
+      for (int v = vs + tid; v < ve; v += BS) s.DT[v] = s.DT_out[v];
+      if (s.bcon[p] && s.bmin[p] < a.bw) {
+        for (int k = m.padj_off[p] + tid; k < m.padj_off[p + 1]; k += BS) {
+          const int q = m.padj[k];
+          const bool active_unconv = (s.inA[q] == E) && !s.bcon[q];
+          if (!active_unconv && atomicMax(&s.freshE[q], E) < E)
+            s.listF[atomicAdd(&s.pc[kPcFresh + par], 1)] = q;
+        }
+      }
+    }
+    grid_barrier(s.bar, G);
+    pcl.lap(clk, &tk[kTkCopyback]);
+    const int nF = ld_acquire(&s.pc[kPcFresh + par]);
+    if (gtid == 0) s.pc[kPcProbes] += nF;
+    for (int i = b; i < nF; i += G)
+      sweep_patch<T, NC, BS>(m, s, s.listF[i], 1, 0, a.tol, a.leps, smem, red, &s_ndirty);
+    grid_barrier(s.bar, G);
+    pcl.lap(clk, &tk[kTkProbe]);
+    // Next list: unconverged members not probed this iteration, then
+    // unconverged probed patches.
+    for (int64_t i = gtid; i < (int64_t)nA + nF; i += gsize) {
+      int p;
+      bool keep;
+      if (i < nA) {
+        p = cur[i];
+        keep = s.freshE[p] != E && !s.bcon[p];
+      } else {
+        p = s.listF[i - nA];
+        keep = !s.bcon[p];
+      }
+      if (keep) {
+        nxt[atomicAdd(&s.pc[kPcNext + par], 1)] = p;
+        s.inA[p] = E + 1;
+      }
+    }
+    grid_barrier(s.bar, G);
+    pcl.lap(clk, &tk[kTkNext]);
+    nA = ld_acquire(&s.pc[kPcNext + par]);
+    int32_t* t = cur;
+    cur = nxt;
+    nxt = t;
+    ++it;
+  }
+
+  // Phase 3: narrowband + sign restore over every patch.
+  for (int p = b; p < m.P; p += G) {
+    const T mn = narrowband_patch<T, BS>(m, s, p, red);
+    if (tid == 0 && mn <= a.bw) {
+      s.nb[nb_out][atomicAdd(&s.pc[kPcNb + nb_out], 1)] = p;
+      atomicAdd(&s.pc[kPcNbVerts], m.vert_off[p + 1] - m.vert_off[p]);
+    }
+  }
+  grid_barrier(s.bar, G);
+  pcl.lap(clk, &tk[kTkNb]);
+
+  // Phase 4: advection over the new narrowband, then CopyOutBack_levelset3d.
+  const int nnb = ld_acquire(&s.pc[kPcNb + nb_out]);
+  for (int pass = 0; pass < a.inside_niter && nnb > 0; ++pass) {
+    for (int i = b; i < nnb; i += G) advect_patch<T, NC, BS>(m, s, s.nb[nb_out][i], a.dt, smem);
+    grid_barrier(s.bar, G);
+    for (int i = b; i < nnb; i += G) {
+      const int p = s.nb[nb_out][i];
+      for (int v = m.vert_off[p] + tid; v < m.vert_off[p + 1]; v += BS) s.phi[v] = s.phi_out[v];
+    }
+    grid_barrier(s.bar, G);
+  }
+  pcl.lap(clk, &tk[kTkAdvect]);
+  if (clk)
+    for (int i = 0; i < kNumTicks; ++i) s.ticks[i] = tk[i];
 }
 
 }  // namespace kern
@@ -671,6 +986,7 @@ class EngineImpl final : public Engine {
   ~EngineImpl() override { release(); }
 
   int working_bytes() const override { return (int)sizeof(T); }
+  void set_exec(int mode) override { exec_ = mode; }
 
   int upload(const Layout& L, std::string& err) override {
     LS_HIP(hipSetDevice(device_));
@@ -687,21 +1003,31 @@ class EngineImpl final : public Engine {
       const size_t ne = L.ele_off[p + 1] - L.ele_off[p], nv = L.vert_off[p + 1] - L.vert_off[p];
       const size_t nh = L.halo_off[p + 1] - L.halo_off[p];
       const size_t ns = L.vslot_off[L.vert_off[p + 1]] - L.vslot_off[L.vert_off[p]];
-      const size_t fim = NC * ne * sizeof(T) + (2 * nv + nh) * sizeof(T) + 2 * (nv + 1 + ns) + ne;
+      const size_t fim = NC * ne * sizeof(T) + (2 * nv + nh) * sizeof(T) + 2 * (nv + 1 + ns) + 2 * ne + ne;
       const size_t adv = 2 * NC * ne * sizeof(T) + (nv + nh) * sizeof(T);
       fim_lds_ = std::max(fim_lds_, fim);
       adv_lds_ = std::max(adv_lds_, adv);
       seed_lds_ = std::max(seed_lds_, (nv + nh) * sizeof(T));
     }
     fim_lds_ = (fim_lds_ + 15) & ~size_t(15);
+    step_lds_ = std::max(fim_lds_, std::max(adv_lds_, seed_lds_));
+    step_lds_ = (step_lds_ + 15) & ~size_t(15);
     adv_lds_ = (adv_lds_ + 15) & ~size_t(15);
     seed_lds_ = (seed_lds_ + 15) & ~size_t(15);
-    if (fim_lds_ > (size_t)lds_max || adv_lds_ > (size_t)lds_max) {
-      err = "patch needs " + std::to_string(std::max(fim_lds_, adv_lds_)) + " B of LDS, device allows " +
+    if (step_lds_ > (size_t)lds_max) {
+      err = "patch needs " + std::to_string(step_lds_) + " B of LDS, device allows " +
             std::to_string(lds_max) + " B; use smaller patches";
       return LS_ERR_LIMIT;
     }
 
+    {
+      int cus = 0, per_cu = 0;
+      LS_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device_));
+      LS_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_step<T, NC, kFimBlock>,
+                                                          kFimBlock, step_lds_));
+      coop_grid_ = cus * per_cu;
+      if (coop_grid_ <= 0) { err = "persistent step kernel does not fit on a CU"; return LS_ERR_LIMIT; }
+    }
     LS_HIP(dev_upload(&d_vert_off_, L.vert_off.data(), L.vert_off.size()));
     LS_HIP(dev_upload(&d_ele_off_, L.ele_off.data(), L.ele_off.size()));
     LS_HIP(dev_upload(&d_halo_off_, L.halo_off.data(), L.halo_off.size()));
@@ -731,6 +1057,19 @@ class EngineImpl final : public Engine {
     LS_HIP(dev_alloc(&d_seedflag_, P_));
     LS_HIP(dev_alloc(&d_counters_, kNumCounters));
     LS_HIP(dev_alloc(&d_sizes_, kNumSizes));
+    LS_HIP(dev_alloc(&d_inA_, P_));
+    LS_HIP(dev_alloc(&d_freshE_, P_));
+    LS_HIP(dev_alloc(&d_listF_, P_));
+    LS_HIP(dev_alloc(&d_pc_, kNumPc));
+    LS_HIP(dev_alloc(&d_ticks_, kNumTicks));
+    LS_HIP(dev_alloc(&d_bar_, 2));
+    LS_HIP(hipMemset(d_bar_, 0, 2 * sizeof(unsigned int)));
+    {
+      int khz = 0;
+      LS_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device_));
+      wall_khz_ = khz > 0 ? khz : 100000;
+    }
+    LS_HIP(hipMemset(d_pc_, 0, kNumPc * sizeof(int32_t)));
     for (auto& l : d_lists_) LS_HIP(dev_alloc(&l, P_));
     LS_HIP(dev_alloc(&d_nb_, P_));
     LS_HIP(dev_alloc(&d_scan_all_, P_));
@@ -766,9 +1105,108 @@ class EngineImpl final : public Engine {
 
   int step(const ls_step_params& prm, ls_stats& st, std::string& err) override {
     if (!uploaded_) { err = "layout not built"; return LS_ERR_STATE; }
+    if (exec_ != LS_EXEC_KERNELS) return step_persistent(prm, st, err);
+    return step_kernels(prm, st, err);
+  }
+
+  // One cooperative launch per step (optionally replayed from a hipGraph).
+  int step_persistent(const ls_step_params& prm, ls_stats& st, std::string& err) {
     if (prm.niter <= 0 || prm.inside_niter < 0) { err = "niter must be > 0, inside_niter >= 0"; return LS_ERR_ARG; }
     if (prm.mode != LS_MODE_ADVECT) { err = "only LS_MODE_ADVECT is implemented"; return LS_ERR_ARG; }
     std::memset(&st, 0, sizeof(st));
+    st.exec_mode = exec_;
+    DevMesh<T> m = mesh();
+    DevState<T> s = state();
+    StepArgs<T> a;
+    a.first = nnb_ > 0 ? 0 : 1;
+    a.nb_in = nb_sel_;
+    a.niter = prm.niter;
+    a.inside_niter = prm.inside_niter;
+    // The device FIM loop always carries a cap so a step cannot run unbounded.
+    a.max_iters = prm.max_fim_iters > 0 ? prm.max_fim_iters : kFimIterCap;
+    a.tol = (T)prm.tol;
+    a.bw = (T)prm.bandwidth;
+    a.leps = prm.lambda_eps;
+    a.dt = prm.dt;
+    hipStream_t q = stream_;
+    void* args[] = {&m, &s, &a};
+    if (!pc_synced_) {  // the previous narrowband size, when the host-driven path ran last
+      pc_nb_host_ = nnb_;
+      LS_HIP(hipMemcpyAsync(d_pc_ + kPcNb + nb_sel_, &pc_nb_host_, sizeof(int32_t), hipMemcpyHostToDevice, q));
+      pc_synced_ = true;
+    }
+    LS_HIP(hipEventRecord(ev_[0], q));
+    if (exec_ == LS_EXEC_GRAPH) {
+      const int key[3] = {a.first, a.nb_in, 1};
+      const bool same = graph_exec_ && key[0] == graph_key_[0] && key[1] == graph_key_[1] &&
+                        std::memcmp(&graph_prm_, &prm, sizeof(prm)) == 0;
+      if (!same) {
+        if (graph_exec_) { LS_HIP(hipGraphExecDestroy(graph_exec_)); graph_exec_ = nullptr; }
+        if (graph_) { LS_HIP(hipGraphDestroy(graph_)); graph_ = nullptr; }
+        LS_HIP(hipStreamBeginCapture(q, hipStreamCaptureModeThreadLocal));
+        hipError_t le = hipLaunchCooperativeKernel((const void*)k_step<T, NC, kFimBlock>, dim3(coop_grid_),
+                                                   dim3(kFimBlock), args, step_lds_, q);
+        hipError_t ce = hipStreamEndCapture(q, &graph_);
+        if (le != hipSuccess || ce != hipSuccess) {
+          err = std::string("hipGraph capture of the step kernel failed: ") +
+                hipGetErrorString(le != hipSuccess ? le : ce);
+          return LS_ERR_HIP;
+        }
+        LS_HIP(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0));
+        graph_key_[0] = key[0]; graph_key_[1] = key[1]; graph_key_[2] = key[2];
+        graph_prm_ = prm;
+      }
+      LS_HIP(hipGraphLaunch(graph_exec_, q));
+    } else {
+      LS_HIP(hipLaunchCooperativeKernel((const void*)k_step<T, NC, kFimBlock>, dim3(coop_grid_),
+                                        dim3(kFimBlock), args, step_lds_, q));
+    }
+    LS_HIP(hipEventRecord(ev_[4], q));
+    int32_t pc[kNumPc];
+    unsigned long long sz[kNumSizes], tk[kNumTicks];
+    LS_HIP(hipMemcpyAsync(tk, d_ticks_, sizeof(tk), hipMemcpyDeviceToHost, q));
+    LS_HIP(hipMemcpyAsync(pc, d_pc_, sizeof(pc), hipMemcpyDeviceToHost, q));
+    LS_HIP(hipMemcpyAsync(sz, d_sizes_, sizeof(sz), hipMemcpyDeviceToHost, q));
+    LS_HIP(hipStreamSynchronize(q));
+    if (pc[kPcOverflow] && prm.max_fim_iters <= 0) {
+      err = "FIM did not converge within " + std::to_string(kFimIterCap) + " iterations";
+      return LS_ERR_INTERNAL;
+    }
+    const int nb_out = 1 - nb_sel_;
+    nb_sel_ = nb_out;
+    nnb_ = pc[kPcNb + nb_out];
+    st.fim_iters = pc[kPcIters];
+    st.patch_visits = pc[kPcVisits];
+    st.probes = pc[kPcProbes];
+    st.n_seed_patches = pc[kPcSeed];
+    st.nb_count = nnb_;
+    st.nb_verts = pc[kPcNbVerts];
+    float ms = 0.f;
+    LS_HIP(hipEventElapsedTime(&ms, ev_[0], ev_[4]));
+    st.ms_total = ms;
+    st.ms_step_kernel = ms;
+    fill_sizes(st, sz);
+    const double to_ms = 1.0 / (double)wall_khz_;
+    st.ms_ph_seed = tk[kTkSeed] * to_ms; st.ms_ph_visit = tk[kTkVisit] * to_ms;
+    st.ms_ph_copyback = tk[kTkCopyback] * to_ms; st.ms_ph_probe = tk[kTkProbe] * to_ms;
+    st.ms_ph_next = tk[kTkNext] * to_ms; st.ms_ph_nb = tk[kTkNb] * to_ms; st.ms_ph_advect = tk[kTkAdvect] * to_ms;
+    return LS_OK;
+  }
+
+  static void fill_sizes(ls_stats& st, const unsigned long long* sz) {
+    st.visit_copies = sz[kSzVisit]; st.visit_verts = sz[kSzVisit + 1]; st.visit_halo = sz[kSzVisit + 2];
+    st.probe_copies = sz[kSzProbe]; st.probe_verts = sz[kSzProbe + 1]; st.probe_halo = sz[kSzProbe + 2];
+    st.adv_copies = sz[kSzAdvect]; st.adv_verts = sz[kSzAdvect + 1]; st.adv_halo = sz[kSzAdvect + 2];
+    st.seed_copies = sz[kSzSeed]; st.seed_verts = sz[kSzSeed + 1]; st.seed_halo = sz[kSzSeed + 2];
+  }
+
+  // One launch per phase; the FIM loop is driven from the host.
+  int step_kernels(const ls_step_params& prm, ls_stats& st, std::string& err) {
+    if (prm.niter <= 0 || prm.inside_niter < 0) { err = "niter must be > 0, inside_niter >= 0"; return LS_ERR_ARG; }
+    if (prm.mode != LS_MODE_ADVECT) { err = "only LS_MODE_ADVECT is implemented"; return LS_ERR_ARG; }
+    std::memset(&st, 0, sizeof(st));
+    st.exec_mode = LS_EXEC_KERNELS;
+    pc_synced_ = false;
     const DevMesh<T> m = mesh();
     const DevState<T> s = state();
     const T tol = (T)prm.tol, bw = (T)prm.bandwidth;
@@ -797,7 +1235,7 @@ class EngineImpl final : public Engine {
       st.fim_iters++;
       st.patch_visits += nact;
       int ke = kev_begin(0, q);
-      k_sweep<T, NC><<<nact, kBlock, fim_lds_, q>>>(m, s, cur, prm.niter, 1, tol, leps);
+      k_sweep<T, NC, kFimBlock><<<nact, kFimBlock, fim_lds_, q>>>(m, s, cur, prm.niter, 1, tol, leps);
       kev_end(ke, q);
       k_copyback<T><<<nact, kBlock, 0, q>>>(m, s, cur);
       k_set_counters<<<1, 64, 0, q>>>(d_counters_ + kCntFresh, 2, 0);
@@ -809,7 +1247,7 @@ class EngineImpl final : public Engine {
       if (nf > 0) {
         st.probes += nf;
         int kp = kev_begin(0, q);
-        k_sweep<T, NC><<<nf, kBlock, fim_lds_, q>>>(m, s, fresh_list, 1, 0, tol, leps);
+        k_sweep<T, NC, kFimBlock><<<nf, kFimBlock, fim_lds_, q>>>(m, s, fresh_list, 1, 0, tol, leps);
         kev_end(kp, q);
       }
       k_finalize<T><<<grid_for(nact + nf), kBlock, 0, q>>>(s, cur, nact, fresh_list, nf, next);
@@ -817,7 +1255,7 @@ class EngineImpl final : public Engine {
       LS_HIP(hipMemcpyAsync(&nact, d_counters_ + kCntNext, sizeof(int32_t), hipMemcpyDeviceToHost, q));
       LS_HIP(hipStreamSynchronize(q));
       std::swap(cur, next);
-      if (st.fim_iters > 1000000) { err = "FIM did not converge"; return LS_ERR_INTERNAL; }
+      if (st.fim_iters > kFimIterCap) { err = "FIM did not converge"; return LS_ERR_INTERNAL; }
     }
     LS_HIP(hipEventRecord(ev_[2], q));
     // Narrowband into the buffer not holding the previous one.
@@ -854,9 +1292,7 @@ class EngineImpl final : public Engine {
     }
     unsigned long long sz[kNumSizes];
     LS_HIP(hipMemcpy(sz, d_sizes_, sizeof(sz), hipMemcpyDeviceToHost));
-    st.visit_copies = sz[kSzVisit]; st.visit_verts = sz[kSzVisit + 1]; st.visit_halo = sz[kSzVisit + 2];
-    st.probe_copies = sz[kSzProbe]; st.probe_verts = sz[kSzProbe + 1]; st.probe_halo = sz[kSzProbe + 2];
-    st.adv_copies = sz[kSzAdvect]; st.adv_verts = sz[kSzAdvect + 1]; st.adv_halo = sz[kSzAdvect + 2];
+    fill_sizes(st, sz);
     return LS_OK;
   }
 
@@ -890,6 +1326,12 @@ class EngineImpl final : public Engine {
     s.label = d_label_; s.tsign = d_tsign_; s.bcon = d_bcon_; s.bmin = d_bmin_;
     s.act = d_act_; s.fresh = d_fresh_; s.seedflag = d_seedflag_; s.counters = d_counters_;
     s.sizes = d_sizes_;
+    s.inA = d_inA_; s.freshE = d_freshE_;
+    s.listA = d_lists_[0]; s.listB = d_lists_[1]; s.listF = d_listF_;
+    s.nb[0] = d_nb_; s.nb[1] = d_lists_[3];
+    s.pc = d_pc_;
+    s.ticks = d_ticks_;
+    s.bar = d_bar_;
     return s;
   }
   int32_t* d_nb_cur() const { return nb_sel_ ? d_lists_[3] : d_nb_; }
@@ -900,7 +1342,9 @@ class EngineImpl final : public Engine {
                     d_vslots_, d_padj_off_, d_padj_, d_coords_, d_sigma_, d_phi_, d_phi_out_, d_DT_,
                     d_DT_out_, d_label_, d_tsign_, d_bcon_, d_bmin_, d_act_, d_fresh_, d_seedflag_,
                     d_counters_, d_sizes_, d_nb_, d_scan_all_, d_lists_[0], d_lists_[1], d_lists_[2],
-                    d_lists_[3]};
+                    d_lists_[3], d_inA_, d_freshE_, d_listF_, d_pc_, d_ticks_, d_bar_};
+    if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+    if (graph_) (void)hipGraphDestroy(graph_);
     for (void* p : ptrs) if (p) (void)hipFree(p);
     for (auto& e : ev_) if (e) (void)hipEventDestroy(e);
     for (auto& e : kev_) if (e) (void)hipEventDestroy(e);
@@ -950,6 +1394,19 @@ class EngineImpl final : public Engine {
   }
   int32_t *d_nb_ = nullptr, *d_scan_all_ = nullptr;
   int32_t* d_lists_[4] = {nullptr, nullptr, nullptr, nullptr};
+  int32_t *d_inA_ = nullptr, *d_freshE_ = nullptr, *d_listF_ = nullptr, *d_pc_ = nullptr;
+  unsigned long long* d_ticks_ = nullptr;
+  unsigned int* d_bar_ = nullptr;
+  int wall_khz_ = 100000;
+  int exec_ = LS_EXEC_PERSISTENT;
+  bool pc_synced_ = true;
+  int32_t pc_nb_host_ = 0;
+  int coop_grid_ = 0;
+  size_t step_lds_ = 0;
+  hipGraph_t graph_ = nullptr;
+  hipGraphExec_t graph_exec_ = nullptr;
+  int graph_key_[3] = {-1, -1, -1};  // (first, nb_in, params hash) the graph was captured for
+  ls_step_params graph_prm_{};
 };
 
 Engine* make_engine(int dim, int dtype, int device, std::string& err) {

@@ -16,6 +16,7 @@ LIB_PATH = os.path.join(_HERE, "liblevelset_hip.so")
 
 LS_F32, LS_F64 = 0, 1
 LS_MODE_ADVECT, LS_MODE_MCF = 0, 1
+LS_EXEC_PERSISTENT, LS_EXEC_KERNELS, LS_EXEC_GRAPH = 0, 1, 2
 
 
 class StepParams(C.Structure):
@@ -32,7 +33,12 @@ class Stats(C.Structure):
                 ("ms_sweep_kernels", C.c_double), ("ms_advect_kernels", C.c_double),
                 ("visit_copies", C.c_int64), ("visit_verts", C.c_int64), ("visit_halo", C.c_int64),
                 ("probe_copies", C.c_int64), ("probe_verts", C.c_int64), ("probe_halo", C.c_int64),
-                ("adv_copies", C.c_int64), ("adv_verts", C.c_int64), ("adv_halo", C.c_int64)]
+                ("adv_copies", C.c_int64), ("adv_verts", C.c_int64), ("adv_halo", C.c_int64),
+                ("seed_copies", C.c_int64), ("seed_verts", C.c_int64), ("seed_halo", C.c_int64),
+                ("ms_step_kernel", C.c_double), ("exec_mode", C.c_int32), ("reserved", C.c_int32),
+                ("ms_ph_seed", C.c_double), ("ms_ph_visit", C.c_double), ("ms_ph_copyback", C.c_double),
+                ("ms_ph_probe", C.c_double), ("ms_ph_next", C.c_double), ("ms_ph_nb", C.c_double),
+                ("ms_ph_advect", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -45,7 +51,7 @@ class LayoutInfo(C.Structure):
                 ("max_vertex_slots", C.c_int32), ("dim", C.c_int32)]
 
 
-EXPORTS = ["ls_create", "ls_destroy", "ls_last_error", "ls_set_mesh", "ls_set_precision", "ls_partition_grid",
+EXPORTS = ["ls_create", "ls_destroy", "ls_last_error", "ls_set_mesh", "ls_set_precision", "ls_set_execution", "ls_partition_grid",
            "ls_partition_rcb", "ls_partition_labels", "ls_get_layout_info", "ls_get_layout",
            "ls_set_phi", "ls_set_velocity", "ls_step", "ls_get_phi", "ls_get_narrowband"]
 
@@ -75,6 +81,7 @@ def lib():
         L.ls_last_error.restype = C.c_char_p
         L.ls_set_mesh.argtypes = [vp, C.c_int, C.c_int64, vp, C.c_int64, vp]
         L.ls_set_precision.argtypes = [vp, C.c_int]
+        L.ls_set_execution.argtypes = [vp, C.c_int]
         L.ls_partition_grid.argtypes = [vp] + [C.c_int] * 6
         L.ls_partition_rcb.argtypes = [vp, C.c_int]
         L.ls_partition_labels.argtypes = [vp, vp, C.c_int]
@@ -130,6 +137,9 @@ class Solver:
         self.dim, self.N, self.E = dim, self._xyz.shape[0], self._cells.shape[0]
         self._check(self.L.ls_set_mesh(self.ctx, dim, self.N, self._xyz.ctypes.data, self.E,
                                        self._cells.ctypes.data))
+
+    def set_execution(self, mode):
+        self._check(self.L.ls_set_execution(self.ctx, int(mode)))
 
     def set_precision(self, dtype):
         self._check(self.L.ls_set_precision(self.ctx, int(dtype)))
@@ -188,10 +198,11 @@ class Solver:
         return ids[: n.value].copy()
 
 
-def solver_for_case(case, device=0, precision=LS_F32):
+def solver_for_case(case, device=0, precision=LS_F32, execution=LS_EXEC_PERSISTENT):
     s = Solver(device)
     s.set_mesh(case.dim, case.xyz, case.cells)
     s.set_precision(precision)
+    s.set_execution(execution)
     s.partition_labels(case.labels, case.nparts)
     s.set_phi(case.phi0)
     s.set_velocity(case.vel)

@@ -20,8 +20,8 @@ pytestmark = pytest.mark.gpu
 TOL_FP64 = 1e-5  # L-inf on phi, FP32 engine vs FP64 oracle, non-ambiguous vertices
 
 
-def _lockstep(case, steps, oracle_mod, precision, engine=lsb.LS_F32):
-    g = lsb.solver_for_case(case, precision=engine)
+def _lockstep(case, steps, oracle_mod, precision, engine=lsb.LS_F32, execution=lsb.LS_EXEC_PERSISTENT):
+    g = lsb.solver_for_case(case, precision=engine, execution=execution)
     o = oracle_mod.Oracle(case.dim, case.xyz, case.cells, case.labels, case.nparts, precision)
     o.set_phi(case.phi0)
     o.set_velocity(case.vel)
@@ -54,6 +54,32 @@ def test_tet_fp32_bit_exact(shape, n, block, oracle_mod):
         assert _same_schedule(gs, os_), f"schedule step {k}"
         np.testing.assert_array_equal(gphi.astype(np.float32), ophi.astype(np.float32),
                                       err_msg=f"phi step {k}")
+
+
+@pytest.mark.parametrize("execution", [lsb.LS_EXEC_KERNELS, lsb.LS_EXEC_GRAPH])
+def test_tet_fp32_bit_exact_other_executions(execution, oracle_mod):
+    case = cases.small3d(n=20, shape="torus", block=(8, 8, 4), steps=4)
+    for k, gs, os_, gphi, ophi, gnb, onb in _lockstep(case, 4, oracle_mod, 32, execution=execution):
+        assert gs.exec_mode == execution
+        np.testing.assert_array_equal(gnb, onb, err_msg=f"narrowband step {k}")
+        assert _same_schedule(gs, os_), f"schedule step {k}"
+        np.testing.assert_array_equal(gphi.astype(np.float32), ophi.astype(np.float32))
+
+
+def test_execution_modes_switch_mid_run(oracle_mod):
+    case = cases.small3d(n=16, shape="sphere", block=(4, 4, 4), steps=6)
+    modes = [lsb.LS_EXEC_PERSISTENT, lsb.LS_EXEC_KERNELS, lsb.LS_EXEC_GRAPH, lsb.LS_EXEC_PERSISTENT,
+             lsb.LS_EXEC_GRAPH, lsb.LS_EXEC_KERNELS]
+    g = lsb.solver_for_case(case)
+    o = oracle_mod.Oracle(3, case.xyz, case.cells, case.labels, case.nparts, 32)
+    o.set_phi(case.phi0)
+    o.set_velocity(case.vel)
+    for k, mode in enumerate(modes):
+        g.set_execution(mode)
+        g.step(lsb.step_params(case.dt, case.bandwidth, case.niter))
+        o.step(oracle_mod.make_params(case.dt, case.bandwidth, case.niter))
+        np.testing.assert_array_equal(g.narrowband(), o.narrowband(), err_msg=f"step {k}")
+        np.testing.assert_array_equal(g.phi().astype(np.float32), o.phi().astype(np.float32))
 
 
 @pytest.mark.parametrize("shape,n,block", [("sphere", 12, (4, 4, 4)), ("torus", 24, (8, 4, 4))])

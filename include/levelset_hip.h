@@ -74,6 +74,8 @@ typedef struct ls_stats {
    * reset+seeds, FIM visits, copyback+activation, probes, next-list,
    * narrowband, advection. */
   double ms_ph_seed, ms_ph_visit, ms_ph_copyback, ms_ph_probe, ms_ph_next, ms_ph_nb, ms_ph_advect;
+  int64_t elem_updates;     /* element updates executed by FIM visits and probes */
+  int64_t local_solves;     /* local solves executed (the rest were skipped exactly) */
 } ls_stats;
 
 typedef struct ls_layout_info {

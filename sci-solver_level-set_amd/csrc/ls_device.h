@@ -81,7 +81,9 @@ enum SizeCounter : int {
   kSzProbe = 3,   // neighbour probes
   kSzAdvect = 6,  // advection
   kSzSeed = 9,    // seed scan
-  kNumSizes = 12
+  kSzElem = 12,   // element updates executed (FIM visits + probes)
+  kSzSolve = 13,  // local solves executed
+  kNumSizes = 14
 };
 
 // Device counters layout.

@@ -202,7 +202,7 @@ __device__ __forceinline__ bool must_solve(double Tk, double mo, double Mo, doub
 template <typename T>
 struct ElemSolve<T, 4> {
   static constexpr int kCoords = 6;
-  __device__ static void update(const T* Lt, T* v, double leps) {
+  __device__ static int update(const T* Lt, T* v, double leps) {
     double L[6];
 #pragma unroll
     for (int k = 0; k < 6; ++k) L[k] = (double)Lt[k];
@@ -227,7 +227,7 @@ struct ElemSolve<T, 4> {
         if (j != k) { mo = d_min(mo, Tv[j]); Mo = d_max(Mo, Tv[j]); }
       need[k] = must_solve(Tv[k], mo, Mo, h[k], scale);
     }
-    if (!(need[0] || need[1] || need[2] || need[3])) return;
+    if (!(need[0] || need[1] || need[2] || need[3])) return 0;
     const double ACBC = L[1] * (L[1] - L[0]) + L[2] * L[2];
     const double BCCD = (L[1] - L[0]) * (L[3] - L[1]) + L[2] * (L[4] - L[2]);
     const double ACCD = L[1] * (L[3] - L[1]) + L[2] * (L[4] - L[2]);
@@ -248,13 +248,14 @@ struct ElemSolve<T, 4> {
     if (need[0]) v[0] = (T)d_min(TA, tet_solve(TB, TD, TC, BCBC, -BCCD, -ACBC, CDCD, ACCD, ACAC, leps));
     if (need[1]) v[1] = (T)d_min(TB, tet_solve(TA, TD, TC, ACAC, -ACCD, -ACBC, CDCD, BCCD, BCBC, leps));
     if (need[2]) v[2] = (T)d_min(TC, tet_solve(TA, TB, TD, ADAD, ADBD, -ADCD, BDBD, -CDBD, CDCD, leps));
+    return (int)need[0] + (int)need[1] + (int)need[2] + (int)need[3];
   }
 };
 
 template <typename T>
 struct ElemSolve<T, 3> {
   static constexpr int kCoords = 3;
-  __device__ static void update(const T* ct, T* v, double) {
+  __device__ static int update(const T* ct, T* v, double) {
     const double c[3] = {(double)ct[0], (double)ct[1], (double)ct[2]};
     const double LenAB = d_abs(c[0]);
     const double LenBC = d_sqrt(c[2] * c[2] + (c[1] - c[0]) * (c[1] - c[0]));
@@ -274,6 +275,7 @@ struct ElemSolve<T, 3> {
     if (need[2]) v[2] = (T)d_min(TC, tri_solve(TA, TB, LenAB, LenBC, LenCA));
     if (need[0]) v[0] = (T)d_min(TA, tri_solve(TB, TC, LenBC, LenCA, LenAB));
     if (need[1]) v[1] = (T)d_min(TB, tri_solve(TC, TA, LenCA, LenAB, LenBC));
+    return (int)need[0] + (int)need[1] + (int)need[2];
   }
 };
 
@@ -425,6 +427,7 @@ __device__ void sweep_patch(const DevMesh<T>& m, DevState<T>& s, int p, int nite
   constexpr int K = ElemSolve<T, NC>::kCoords;
   int con_all = 1;
   T vmin = (T)kLargeNum;
+  int n_upd = 0, n_solve = 0;
   for (int it = 0; it < niter; ++it) {
     if (tid == 0) *s_ndirty = 0;
     __syncthreads();
@@ -449,7 +452,8 @@ __device__ void sweep_patch(const DevMesh<T>& m, DevState<T>& s, int p, int nite
       const T* gc = m.coords + (int64_t)(es + e) * K;
 #pragma unroll
       for (int q = 0; q < K; ++q) L[q] = gc[q];
-      ElemSolve<T, NC>::update(L, y, leps);
+      n_solve += ElemSolve<T, NC>::update(L, y, leps);
+      ++n_upd;
       int changed = 0;
 #pragma unroll
       for (int i = 0; i < NC; ++i) {
@@ -490,6 +494,15 @@ __device__ void sweep_patch(const DevMesh<T>& m, DevState<T>& s, int p, int nite
   }
   con_all = __syncthreads_and(con_all);
   const T bm = block_min_n<T, BS>(vmin, red);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    n_upd += __shfl_xor(n_upd, off, 64);
+    n_solve += __shfl_xor(n_solve, off, 64);
+  }
+  if ((tid & 63) == 0 && (n_upd | n_solve)) {
+    atomicAdd(s.sizes + kSzElem, (unsigned long long)n_upd);
+    atomicAdd(s.sizes + kSzSolve, (unsigned long long)n_solve);
+  }
   if (tid == 0) {
     s.bcon[p] = con_all;
     if (write_out) s.bmin[p] = bm;
@@ -1198,6 +1211,7 @@ class EngineImpl final : public Engine {
     st.probe_copies = sz[kSzProbe]; st.probe_verts = sz[kSzProbe + 1]; st.probe_halo = sz[kSzProbe + 2];
     st.adv_copies = sz[kSzAdvect]; st.adv_verts = sz[kSzAdvect + 1]; st.adv_halo = sz[kSzAdvect + 2];
     st.seed_copies = sz[kSzSeed]; st.seed_verts = sz[kSzSeed + 1]; st.seed_halo = sz[kSzSeed + 2];
+    st.elem_updates = sz[kSzElem]; st.local_solves = sz[kSzSolve];
   }
 
   // One launch per phase; the FIM loop is driven from the host.

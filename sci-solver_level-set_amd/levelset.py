@@ -38,7 +38,7 @@ class Stats(C.Structure):
                 ("ms_step_kernel", C.c_double), ("exec_mode", C.c_int32), ("reserved", C.c_int32),
                 ("ms_ph_seed", C.c_double), ("ms_ph_visit", C.c_double), ("ms_ph_copyback", C.c_double),
                 ("ms_ph_probe", C.c_double), ("ms_ph_next", C.c_double), ("ms_ph_nb", C.c_double),
-                ("ms_ph_advect", C.c_double)]
+                ("ms_ph_advect", C.c_double), ("elem_updates", C.c_int64), ("local_solves", C.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -28,10 +28,15 @@
 
 namespace {
 
-constexpr double kLarge = 1e9;
+constexpr double kLarge = 1e9;  // LARGENUM (value external to the reference; 1e9 is exact in FP32)
 // Relative window for threshold-ambiguous advection decisions: well above
 // the FP32 storage error of distances (about 1e-7 relative).
-constexpr double kAmbRel = 1e-6;  // LARGENUM (value external to the reference; 1e9 is exact in FP32)
+constexpr double kAmbRel = 1e-6;
+// A quadratic's discriminant within this fraction of q^2 + |4pr| of zero is
+// taken as zero (a double root). Equal values at both ends of an edge, or a
+// front parallel to a face, make the discriminant exactly zero; rounding
+// alone would otherwise decide whether the candidate exists.
+constexpr double kDiscRel = 1e-12;
 
 enum Label { SeedPoint = 0, ActivePoint = 1, AlivePoint = 2, FarPoint = 3 };  // redistance3d.h:22-25
 
@@ -71,12 +76,17 @@ T tri_solve(T TA, T TB, T LenAB, T LenBC, T LenAC) {
   return std::fmin(EdgeTA, EdgeTB);
 }
 
-// Roots of p*x^2 + q*x + r in the discriminant type; false when delta < 0.
+// Roots of p*x^2 + q*x + r in the discriminant type; false when delta < 0
+// beyond the rounding window kDiscRel. Every root in range yields a point of
+// the face whose Hopf-Lax value is an admissible candidate, so accepting a
+// double root that rounding pushed below zero never lowers the minimum below
+// the true one.
 template <typename T>
 bool roots(T p, T q, T r, T& x1, T& x2) {
   using D = typename Prec<T>::D;
   const D dp = p, dq = q, dr = r;
-  const D delta = dq * dq - (D)4 * dp * dr;
+  D delta = dq * dq - (D)4 * dp * dr;
+  if (delta < 0.0 && delta >= -kDiscRel * (dq * dq + std::fabs((D)4 * dp * dr))) delta = 0.0;
   if (!(delta >= 0.0)) return false;
   const D sq = std::sqrt(delta);
   x1 = (T)((-dq + sq) / ((D)2 * dp));

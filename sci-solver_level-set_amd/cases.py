@@ -255,3 +255,95 @@ def small3d(n: int = 12, seed: int = 5, block=(4, 4, 4), shape: str = "sphere", 
     labels, P = partition_blocks(n, block, 3)
     return Case(f"3d_{shape}_{n}", 3, xyz, cells, phi0, vel, dt=0.5 * h / vmax, steps=steps,
                 bandwidth=6 * h, h=h, labels=labels, nparts=P, niter=5)
+
+
+# --- mesh files -------------------------------------------------------------
+# Python readers for the formats the C++ LevelSet class loads (tetgen
+# .node/.ele, ASCII PLY). Coordinates are rounded to float32, as the mesh
+# classes store them (src/examples/example1.cu:88-95 works on float points).
+
+def _tetgen_rows(path):
+    rows = []
+    with open(path) as f:
+        for line in f:
+            line = line.split("#", 1)[0].split()
+            if line:
+                rows.append(line)
+    return rows
+
+
+def read_tetgen(basename: str):
+    """(xyz float32-valued float64 (N,3), tets int32 (E,4)); indices start at the
+    first point index of the .node file (0 or 1)."""
+    nrows = _tetgen_rows(basename + ".node")
+    nv = int(nrows[0][0])
+    pts = np.array([[float(v) for v in r[1:4]] for r in nrows[1:1 + nv]], dtype=np.float32)
+    first = int(nrows[1][0])
+    erows = _tetgen_rows(basename + ".ele")
+    ne = int(erows[0][0])
+    tets = np.array([[int(v) for v in r[1:5]] for r in erows[1:1 + ne]], dtype=np.int64) - first
+    return pts.astype(np.float64), tets.astype(np.int32)
+
+
+def read_ply(path: str):
+    """ASCII PLY: (xyz float32-valued float64 (N,3), triangles int32 (E,3)); polygons
+    are split into fans; a file whose faces name vertices 1..N is read as
+    1-based, and faces naming a vertex the file does not hold are dropped, as
+    the C++ reader does."""
+    with open(path) as f:
+        lines = f.read().split("\n")
+    nv = nf = 0
+    k = 0
+    while lines[k].strip() != "end_header":
+        w = lines[k].split()
+        if w[:2] == ["element", "vertex"]:
+            nv = int(w[2])
+        if w[:2] == ["element", "face"]:
+            nf = int(w[2])
+        k += 1
+    k += 1
+    pts = np.array([[float(v) for v in lines[k + i].split()[:3]] for i in range(nv)], dtype=np.float32)
+    faces = []
+    for i in range(nf):
+        w = [int(float(v)) for v in lines[k + nv + i].split()]
+        poly = w[1:1 + w[0]]
+        for q in range(2, len(poly)):
+            faces.append([poly[0], poly[q - 1], poly[q]])
+    faces = np.array(faces, dtype=np.int64).reshape(-1, 3)
+    if faces.size and faces.min() == 1 and faces.max() == nv:  # 1-based file
+        faces -= 1
+    faces = faces[((faces >= 0) & (faces < nv)).all(axis=1)]  # drop faces naming missing vertices
+    return pts.astype(np.float64), faces.astype(np.int32)
+
+
+def reorient_tets(xyz, tets):
+    """Exchange corners 1 and 2 of every tet with negative signed volume."""
+    t = tets.copy()
+    a, b, c, d = (xyz[t[:, i]] for i in range(4))
+    neg = np.einsum("ij,ij->i", np.cross(b - a, c - a), d - a) < 0.0
+    t[neg, 1], t[neg, 2] = tets[neg, 2], tets[neg, 1]
+    return t
+
+
+def levelset_default_x(xyz, ncells, dim):
+    """Initial values and velocities LevelSet::solveLevelSet fills in when the
+    caller gives none (src/LevelSet.cu:176-231): phi0 = x - midX (float32),
+    velocity (rangeX/100, 0, 0) per tet in 3D and zero in 2D."""
+    x = xyz[:, 0].astype(np.float32)
+    mn, mx = x.min(), x.max()
+    mid = float(np.float32(mn + mx)) / 2.0
+    rng = float(np.float32(mx - mn))
+    phi0 = (x.astype(np.float64) - mid).astype(np.float32)
+    vel = np.zeros((ncells, 3))
+    if dim == 3:
+        vel[:, 0] = float(np.float32(rng / 100.0))
+    return phi0, vel
+
+
+def metis_parts(n_verts: int, metis_size: int, dim: int) -> int:
+    """Patch count of Partition_METIS: ceil(n/size) in 3D
+    (src/core3d/cuda/meshFIM3d.cu:361), n//size with a minimum of 2 in 2D
+    (src/core2d/cuda/meshFIM2d.cu:380-382)."""
+    m = max(1, metis_size)
+    p = -(-n_verts // m) if dim == 3 else max(2, n_verts // m)
+    return max(1, min(p, n_verts))

@@ -37,11 +37,20 @@ __device__ __forceinline__ double d_max(double a, double b) { return fmax(a, b);
 __device__ __forceinline__ float d_abs(float a) { return fabsf(a); }
 __device__ __forceinline__ double d_abs(double a) { return fabs(a); }
 
-// Roots of p x^2 + q x + r with the discriminant and roots in FP64.
+// A discriminant within this fraction of q^2 + |4pr| of zero is a double
+// root: equal values at both ends of an edge, or a front parallel to a face,
+// make it exactly zero, and rounding alone must not decide whether the
+// candidate exists.
+constexpr double kDiscRel = 1e-12;
+
+// Roots of p x^2 + q x + r with the discriminant and roots in FP64. Every
+// root in range is a point of the face whose Hopf-Lax value is an admissible
+// candidate, so accepting a double root never undercuts the true minimum.
 template <typename T>
 __device__ __forceinline__ bool quad_roots(T p, T q, T r, T& x1, T& x2) {
   const double dp = p, dq = q, dr = r;
-  const double delta = dq * dq - 4.0 * dp * dr;
+  double delta = dq * dq - 4.0 * dp * dr;
+  if (delta < 0.0 && delta >= -kDiscRel * (dq * dq + d_abs(4.0 * dp * dr))) delta = 0.0;
   if (!(delta >= 0.0)) return false;
   const double sq = sqrt(delta);
   x1 = (T)((-dq + sq) / (2.0 * dp));

@@ -129,3 +129,70 @@ def test_tri_fp64_bit_exact_config1(oracle_mod):
         np.testing.assert_array_equal(gnb, onb)
         assert _same_schedule(gs, os_)
         np.testing.assert_array_equal(gphi, ophi, err_msg=f"phi step {k}")
+
+
+def _golden(name):
+    import os
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", f"{name}.npz")
+    with np.load(path, allow_pickle=False) as z:
+        return {k: z[k] for k in z.files}
+
+
+@pytest.mark.parametrize("name,engine,prec", [("pyramid", lsb.LS_F32, 32), ("torus", lsb.LS_F32, 32),
+                                              ("torus", lsb.LS_F64, 64), ("circle2d", lsb.LS_F64, 64)])
+def test_golden_vectors(name, engine, prec):
+    # The committed oracle vectors (reference test meshes, default LevelSet
+    # inputs; 2D circle workload), without running the oracle.
+    g = _golden(name)
+    s = lsb.Solver(device=0)
+    dim = int(g["dim"])
+    s.set_mesh(dim, g["xyz"], g["cells"])
+    if dim == 3:
+        s.set_precision(engine)
+    s.partition_labels(g["labels"], int(g["nparts"]))
+    s.set_phi(g["phi0"])
+    s.set_velocity(g["vel"])
+    prm = lsb.step_params(float(g["dt"]), float(g["bandwidth"]), int(g["niter"]))
+    off = 0
+    for k, ref in enumerate(g[f"phi_fp{prec}"]):
+        st = s.step(prm)
+        n = int(g[f"nb_len_fp{prec}"][k])
+        np.testing.assert_array_equal(s.narrowband(), g[f"nb_fp{prec}"][off:off + n], err_msg=f"nb step {k}")
+        off += n
+        got = s.phi()
+        if prec == 32:
+            np.testing.assert_array_equal(got.astype(np.float32), ref.astype(np.float32), err_msg=f"step {k}")
+        else:
+            np.testing.assert_array_equal(got, ref, err_msg=f"step {k}")
+        if st.nb_count == 0:
+            break
+    s.close()
+
+
+def test_unstructured_fp32_vs_fp64_oracle(oracle_mod):
+    # The reference's torus mesh is mirror-symmetric, so many edges carry equal
+    # values at both ends (discriminant exactly 0): FP32 storage must still
+    # track the FP64 scheme.
+    g = _golden("torus")
+    s = lsb.Solver(device=0)
+    s.set_mesh(3, g["xyz"], g["cells"])
+    s.partition_labels(g["labels"], int(g["nparts"]))
+    s.set_velocity(g["vel"])
+    o = oracle_mod.Oracle(3, g["xyz"], g["cells"], g["labels"], int(g["nparts"]), 64)
+    o.set_velocity(g["vel"])
+    s.set_phi(g["phi0"])
+    phi, nb = g["phi0"], None
+    for k in range(3):
+        o.set_phi(phi)
+        if nb is not None:
+            o.set_narrowband(nb)
+        s.step(lsb.step_params(1.0, 16.0, 5))
+        o.step(oracle_mod.make_params(1.0, 16.0, 5))
+        gphi, ophi, amb = s.phi(), o.phi(), o.ambiguous()
+        err = float(np.max(np.abs(gphi - ophi)[~amb]))
+        print(f"torus mesh step {k}: Linf {err:.2e}, ambiguous {int(amb.sum())}")
+        assert err <= TOL_FP64
+        assert amb.sum() <= 0.05 * amb.size
+        phi, nb = gphi, s.narrowband()
+    s.close()
+    o.close()

@@ -46,7 +46,7 @@ typedef struct ls_step_params {
   double lambda_eps;   /* interior tet-solve regulariser (SMALLNUM in localSolverTet1) */
   int32_t niter;       /* sweeps per patch visit (NUM_ITER: 5 in 3D, 10 in 2D) */
   int32_t inside_niter;/* advection passes per step (insideIterations_) */
-  int32_t mode;        /* LS_MODE_ADVECT */
+  int32_t mode;        /* LS_MODE_ADVECT, or LS_MODE_MCF: mean-curvature flow phi_t = kappa |grad phi| */
   int32_t max_fim_iters;/* 0: run the FIM loop to convergence; > 0: stop after that many iterations */
 } ls_step_params;
 

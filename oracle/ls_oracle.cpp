@@ -683,6 +683,7 @@ struct Solver : Base {
   // closed-form, instead of from the 4x4 cofactor inverse of global
   // coordinates; K_i = vol * sigma . grad N_i is frame independent.
   void advect_patch(int p, const lso_params& prm) {
+    if (prm.mode == 1) return curvature_patch(p, prm);
     const int32_t vs = vert_offsets[p], ve = vert_offsets[p + 1];
     const int32_t es = ele_offsets[p], ee = ele_offsets[p + 1];
     const int ne = ee - es;
@@ -743,6 +744,86 @@ struct Solver : Base {
       amb[v] = a;
       const double old = (double)phi[v];
       phi_out[v] = (T)((std::fabs(dn) > 1e-8) ? old - dt * (up / dn) : old);
+    }
+  }
+
+  // Shape-function gradients of copy g in its local frame (the closed forms
+  // advect_patch uses) and the copy's volume.
+  void local_gradients(int64_t g, double (*gr)[3], double& vol) const {
+    if (dim == 3) {
+      const double L0 = coords[0 * F + g], L1 = coords[1 * F + g], L2 = coords[2 * F + g];
+      const double L3 = coords[3 * F + g], L4 = coords[4 * F + g], L5 = coords[5 * F + g];
+      vol = std::fabs(L0 * L2 * L5) / 6.0;
+      gr[1][0] = 1.0 / L0;
+      gr[1][1] = -L1 / (L0 * L2);
+      gr[1][2] = (L1 * L4 - L2 * L3) / (L0 * L2 * L5);
+      gr[2][0] = 0.0;
+      gr[2][1] = 1.0 / L2;
+      gr[2][2] = -L4 / (L2 * L5);
+      gr[3][0] = 0.0;
+      gr[3][1] = 0.0;
+      gr[3][2] = 1.0 / L5;
+      for (int k = 0; k < 3; ++k) gr[0][k] = -((gr[1][k] + gr[2][k]) + gr[3][k]);
+    } else {
+      const double c0 = coords[0 * F + g], c1 = coords[1 * F + g], c2 = coords[2 * F + g];
+      vol = std::fabs(c0 * c2) / 2.0;
+      gr[1][0] = 1.0 / c0;
+      gr[1][1] = -c1 / (c0 * c2);
+      gr[2][0] = 0.0;
+      gr[2][1] = 1.0 / c2;
+      for (int k = 0; k < 2; ++k) gr[0][k] = -(gr[1][k] + gr[2][k]);
+      for (int i = 0; i < 3; ++i) gr[i][2] = 0.0;
+    }
+  }
+
+  // Mean-curvature flow phi_t = kappa |grad phi| (the scheme the reference
+  // leaves as dead CPU code, meshFIM3d.cu:243-266), with P1 elements and a
+  // lumped mass M_i = sum_e vol_e / (d+1):
+  //   kappa_i = -sum_e vol_e gradN_i . n_e / M_i,  n_e = grad phi_e / |grad phi_e|,
+  //   |grad phi|_i = sum_e vol_e |grad phi_e| / sum_e vol_e,
+  //   phi_i += dt * kappa_i * |grad phi|_i.
+  // Elements touching an unreached vertex (|phi| >= LARGENUM / 2) and
+  // elements with |grad phi_e| <= 1e-12 contribute nothing. Per-slot
+  // products are stored in T; sums are FP64.
+  void curvature_patch(int p, const lso_params& prm) {
+    const int32_t vs = vert_offsets[p], ve = vert_offsets[p + 1];
+    const int32_t es = ele_offsets[p], ee = ele_offsets[p + 1];
+    const int ne = ee - es;
+    std::vector<T> s_cv(nc * ne), s_vol(nc * ne), s_eik(ne);
+    for (int c = 0; c < ne; ++c) {
+      const int64_t g = es + c;
+      double ph[4], gr[4][3], vol;
+      bool fin = true;
+      for (int i = 0; i < nc; ++i) {
+        ph[i] = (double)phi[ele[i * F + g]];
+        fin = fin && std::fabs(ph[i]) < 0.5 * kLarge;
+      }
+      local_gradients(g, gr, vol);
+      double gp[3] = {0.0, 0.0, 0.0};
+      for (int i = 0; i < nc; ++i)
+        for (int k = 0; k < 3; ++k) gp[k] = gp[k] + ph[i] * gr[i][k];
+      const double gn = std::sqrt((gp[0] * gp[0] + gp[1] * gp[1]) + gp[2] * gp[2]);
+      const bool use = fin && gn > 1e-12;
+      for (int i = 0; i < nc; ++i) {
+        const double d = (gr[i][0] * gp[0] + gr[i][1] * gp[1]) + gr[i][2] * gp[2];
+        s_cv[c * nc + i] = (T)(use ? vol * d / gn : 0.0);
+        s_vol[c * nc + i] = (T)(use ? vol : 0.0);
+      }
+      s_eik[c] = (T)(use ? vol * gn : 0.0);
+    }
+    for (int32_t v = vs; v < ve; ++v) {
+      double cv = 0.0, sv = 0.0, se = 0.0;
+      for (int32_t l : lmem[v]) cv += (double)s_cv[l];
+      for (int32_t l : lmem[v]) sv += (double)s_vol[l];
+      for (int32_t l : lmem[v]) se += (double)s_eik[l / nc];
+      amb[v] = 0;
+      const double old = (double)phi[v];
+      if (sv > 0.0) {
+        const double kappa = -cv / (sv / (double)nc);
+        phi_out[v] = (T)(old + prm.dt * kappa * (se / sv));
+      } else {
+        phi_out[v] = (T)old;
+      }
     }
   }
 

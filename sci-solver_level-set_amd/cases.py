@@ -257,6 +257,56 @@ def small3d(n: int = 12, seed: int = 5, block=(4, 4, 4), shape: str = "sphere", 
                 bandwidth=6 * h, h=h, labels=labels, nparts=P, niter=5)
 
 
+# Explicit mean-curvature flow with P1 elements and a lumped mass is stable
+# for dt up to about h^2/6 on Kuhn grids (the largest eigenvalue of the
+# lumped P1 Laplacian there is 12/h^2); 0.5 h^2 diverges within ten steps.
+MCF_DT_FACTOR = 0.125
+
+
+def mcf_sphere(n: int = 24, r0: float = 0.3, block=(8, 8, 8), steps: int = 24, seed: int = 0) -> Case:
+    """A shrinking sphere under mean-curvature flow (closed form R(t)^2 = R0^2 - 4t)."""
+    xyz, cells, h = kuhn_grid(n)
+    if seed:
+        xyz = jitter_interior(xyz, h, 0.05, seed, 3)
+    labels, P = partition_blocks(n, block, 3)
+    return Case(f"3d_mcf_sphere_{n}", 3, xyz, cells, sdf_sphere(xyz, r=r0), np.zeros((cells.shape[0], 3)),
+                dt=MCF_DT_FACTOR * h * h, steps=steps, bandwidth=8 * h, h=h, labels=labels, nparts=P,
+                mode=1, niter=5, meta={"r0": r0})
+
+
+def mcf_circle(n: int = 33, r0: float = 0.3, block=(8, 8), steps: int = 40) -> Case:
+    """A shrinking circle under curve-shortening flow (R(t)^2 = R0^2 - 2t)."""
+    xyz, cells, h = tri_grid(n)
+    labels, P = partition_blocks(n, block, 2)
+    return Case(f"2d_mcf_circle_{n}", 2, xyz, cells, sdf_circle(xyz, r=r0), np.zeros((cells.shape[0], 3)),
+                dt=MCF_DT_FACTOR * h * h, steps=steps, bandwidth=8 * h, h=h, labels=labels, nparts=P,
+                mode=1, niter=10, meta={"r0": r0})
+
+
+def config3(n: int = 192, seed: int = 3, block=(8, 8, 8), steps: int = 50) -> Case:
+    """Mean-curvature flow of 8 spheres: n^3 Kuhn grid, jitter 0.05h, centres
+    U[0.25,0.75]^3, radii U[0.08,0.15], phi0 = min of the sphere distances,
+    band 8h, dt = h^2/8 (explicitly stable), 512-vertex block patches."""
+    xyz, cells, h = kuhn_grid(n)
+    xyz = jitter_interior(xyz, h, 0.05, seed, 3)
+    rng = np.random.default_rng(seed)
+    centres = rng.uniform(0.25, 0.75, size=(8, 3))
+    radii = rng.uniform(0.08, 0.15, size=8)
+    phi0 = sdf_spheres_union(xyz, centres, radii)
+    labels, P = partition_blocks(n, block, 3)
+    return Case(f"3d_mcf_spheres_{n}", 3, xyz, cells, phi0, np.zeros((cells.shape[0], 3)),
+                dt=MCF_DT_FACTOR * h * h, steps=steps, bandwidth=8 * h, h=h, labels=labels, nparts=P,
+                mode=1, niter=5, meta={"centres": centres, "radii": radii, "block": block})
+
+
+def interface_radius(xyz, phi, centre, h, dim):
+    """Mean radius of the zero level set of a distance-like phi about centre,
+    from the vertices within h of it (radius = |x - c| - phi)."""
+    r = np.linalg.norm(xyz[:, :dim] - np.asarray(centre)[:dim], axis=1)
+    near = np.abs(phi) < h
+    return float(np.mean(r[near] - phi[near]))
+
+
 # --- mesh files -------------------------------------------------------------
 # Python readers for the formats the C++ LevelSet class loads (tetgen
 # .node/.ele, ASCII PLY). Coordinates are rounded to float32, as the mesh

@@ -573,8 +573,99 @@ struct Grad<3> {
     g[2][1] = 1.0 / c[2];
 #pragma unroll
     for (int k = 0; k < 2; ++k) g[0][k] = -(g[1][k] + g[2][k]);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) g[i][2] = 0.0;
   }
 };
+
+// Mean-curvature flow phi_t = kappa |grad phi| on one narrowband patch (the
+// scheme the reference keeps as dead CPU code, meshFIM3d.cu:243-266), with
+// P1 elements and a lumped mass M_i = sum_e vol_e / (d+1):
+//   kappa_i = -sum_e vol_e gradN_i . n_e / M_i,  n_e = grad phi_e / |grad phi_e|,
+//   |grad phi|_i = sum_e vol_e |grad phi_e| / sum_e vol_e,
+//   phi_i += dt * kappa_i * |grad phi|_i.
+// Elements touching an unreached vertex (|phi| >= LARGENUM / 2) or with
+// |grad phi_e| <= 1e-12 contribute nothing. Gradients are taken in each
+// copy's local frame; only frame-independent quantities are summed.
+// Per-slot products are stored as T, sums are FP64.
+template <typename T, int NC, int BS>
+__device__ void curvature_patch(const DevMesh<T>& m, DevState<T>& s, int p, double dt, unsigned char* smem) {
+  __syncthreads();
+  constexpr int K = ElemSolve<T, NC>::kCoords;
+  const int vs = m.vert_off[p], ve = m.vert_off[p + 1];
+  const int es = m.ele_off[p], ee = m.ele_off[p + 1];
+  const int hs = m.halo_off[p], he = m.halo_off[p + 1];
+  const int nv = ve - vs, ne = ee - es, nh = he - hs;
+  T* s_cv = (T*)smem;
+  T* s_vol = s_cv + NC * ne;
+  T* s_vtx = s_vol + NC * ne;
+  T* s_eik = s_vtx + nv + nh;
+  for (int i = threadIdx.x; i < nv; i += BS) s_vtx[i] = s.phi[vs + i];
+  for (int i = threadIdx.x; i < nh; i += BS) s_vtx[nv + i] = s.phi[m.halo_ids[hs + i]];
+  if (threadIdx.x == 0) {
+    atomicAdd(s.sizes + kSzAdvect + 0, (unsigned long long)ne);
+    atomicAdd(s.sizes + kSzAdvect + 1, (unsigned long long)nv);
+    atomicAdd(s.sizes + kSzAdvect + 2, (unsigned long long)nh);
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < ne; e += BS) {
+    const ushort4 c = m.copy_local[es + e];
+    double ph[NC], L[K], g[NC][3], vol;
+    bool fin = true;
+#pragma unroll
+    for (int i = 0; i < NC; ++i) {
+      ph[i] = (double)s_vtx[corner(c, i)];
+      fin = fin && d_abs(ph[i]) < 0.5 * kLargeNum;
+    }
+    const T* gc = m.coords + (int64_t)(es + e) * K;
+#pragma unroll
+    for (int k = 0; k < K; ++k) L[k] = (double)gc[k];
+    Grad<NC>::eval(L, g, vol);
+    double gp[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int i = 0; i < NC; ++i)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) gp[k] = gp[k] + ph[i] * g[i][k];
+    const double gn = d_sqrt((gp[0] * gp[0] + gp[1] * gp[1]) + gp[2] * gp[2]);
+    const bool use = fin && gn > 1e-12;
+#pragma unroll
+    for (int i = 0; i < NC; ++i) {
+      const double d = (g[i][0] * gp[0] + g[i][1] * gp[1]) + g[i][2] * gp[2];
+      s_cv[e * NC + i] = (T)(use ? vol * d / gn : 0.0);
+      s_vol[e * NC + i] = (T)(use ? vol : 0.0);
+    }
+    s_eik[e] = (T)(use ? vol * gn : 0.0);
+  }
+  __syncthreads();
+  for (int v = threadIdx.x; v < nv; v += BS) {
+    const int o0 = m.vslot_off[vs + v], o1 = m.vslot_off[vs + v + 1];
+    double cv = 0.0, sv = 0.0, se = 0.0;
+    for (int k = o0; k < o1; ++k) cv += (double)s_cv[m.vslots[k]];
+    for (int k = o0; k < o1; ++k) sv += (double)s_vol[m.vslots[k]];
+    for (int k = o0; k < o1; ++k) se += (double)s_eik[m.vslots[k] / NC];
+    const double old = (double)s_vtx[v];
+    if (sv > 0.0) {
+      const double kappa = -cv / (sv / (double)NC);
+      s.phi_out[vs + v] = (T)(old + dt * kappa * (se / sv));
+    } else {
+      s.phi_out[vs + v] = (T)old;
+    }
+  }
+}
+
+// Narrowband update of one patch: advection (LS_MODE_ADVECT) or mean-curvature
+// flow (LS_MODE_MCF).
+template <typename T, int NC, int BS>
+__device__ void advect_patch(const DevMesh<T>& m, DevState<T>& s, int p, double dt, unsigned char* smem);
+
+template <typename T, int NC, int BS>
+__device__ __forceinline__ void update_patch(const DevMesh<T>& m, DevState<T>& s, int p, double dt, int mode,
+                                             unsigned char* smem) {
+  if (mode == LS_MODE_MCF)
+    curvature_patch<T, NC, BS>(m, s, p, dt, smem);
+  else
+    advect_patch<T, NC, BS>(m, s, p, dt, smem);
+}
 
 // Upwind residual-distribution (N-scheme) advection of one narrowband patch
 // (kernel_updateT_single_stage3d, meshFIM_kernels.h:36-408): per element copy
@@ -754,9 +845,10 @@ __global__ __launch_bounds__(kBlock) void k_narrowband(DevMesh<T> m, DevState<T>
 }
 
 template <typename T, int NC>
-__global__ __launch_bounds__(kBlock) void k_advect(DevMesh<T> m, DevState<T> s, const int32_t* nb, double dt) {
+__global__ __launch_bounds__(kBlock) void k_advect(DevMesh<T> m, DevState<T> s, const int32_t* nb, double dt,
+                                                   int mode) {
   extern __shared__ __align__(16) unsigned char smem[];
-  advect_patch<T, NC, kBlock>(m, s, nb[blockIdx.x], dt, smem);
+  update_patch<T, NC, kBlock>(m, s, nb[blockIdx.x], dt, mode, smem);
 }
 
 // CopyOutBack_levelset3d (meshFIM_kernels.h:22-34).
@@ -788,6 +880,7 @@ struct StepArgs {
   T bw;
   double leps;
   double dt;
+  int mode;          // LS_MODE_ADVECT or LS_MODE_MCF
 };
 
 // Counters are read right after a grid barrier, whose acquire fence already
@@ -957,7 +1050,7 @@ __global__ __launch_bounds__(BS) void k_step(DevMesh<T> m, DevState<T> s, StepAr
   // Phase 4: advection over the new narrowband, then CopyOutBack_levelset3d.
   const int nnb = ld_acquire(&s.pc[kPcNb + nb_out]);
   for (int pass = 0; pass < a.inside_niter && nnb > 0; ++pass) {
-    for (int i = b; i < nnb; i += G) advect_patch<T, NC, BS>(m, s, s.nb[nb_out][i], a.dt, smem);
+    for (int i = b; i < nnb; i += G) update_patch<T, NC, BS>(m, s, s.nb[nb_out][i], a.dt, a.mode, smem);
     grid_barrier(s.bar, G);
     for (int i = b; i < nnb; i += G) {
       const int p = s.nb[nb_out][i];
@@ -1026,7 +1119,7 @@ class EngineImpl final : public Engine {
       const size_t nh = L.halo_off[p + 1] - L.halo_off[p];
       const size_t ns = L.vslot_off[L.vert_off[p + 1]] - L.vslot_off[L.vert_off[p]];
       const size_t fim = NC * ne * sizeof(T) + (2 * nv + nh) * sizeof(T) + 2 * (nv + 1 + ns) + 2 * ne + ne;
-      const size_t adv = 2 * NC * ne * sizeof(T) + (nv + nh) * sizeof(T);
+      const size_t adv = 2 * NC * ne * sizeof(T) + (nv + nh + ne) * sizeof(T);
       fim_lds_ = std::max(fim_lds_, fim);
       adv_lds_ = std::max(adv_lds_, adv);
       seed_lds_ = std::max(seed_lds_, (nv + nh) * sizeof(T));
@@ -1127,14 +1220,28 @@ class EngineImpl final : public Engine {
 
   int step(const ls_step_params& prm, ls_stats& st, std::string& err) override {
     if (!uploaded_) { err = "layout not built"; return LS_ERR_STATE; }
-    if (exec_ != LS_EXEC_KERNELS) return step_persistent(prm, st, err);
+    if (exec_ != LS_EXEC_KERNELS && coop_ok_) return step_persistent(prm, st, err);
     return step_kernels(prm, st, err);
+  }
+
+  // A cooperative launch is refused while another process holds the
+  // device's cooperative queue. The step then runs, and later steps keep
+  // running, on the per-phase path, which computes the same results;
+  // ls_stats.exec_mode reports the path taken.
+  bool coop_refused(hipError_t e) {
+    if (e != hipErrorIllegalState && e != hipErrorCooperativeLaunchTooLarge) return false;
+    (void)hipGetLastError();
+    coop_ok_ = false;
+    return true;
   }
 
   // One cooperative launch per step (optionally replayed from a hipGraph).
   int step_persistent(const ls_step_params& prm, ls_stats& st, std::string& err) {
     if (prm.niter <= 0 || prm.inside_niter < 0) { err = "niter must be > 0, inside_niter >= 0"; return LS_ERR_ARG; }
-    if (prm.mode != LS_MODE_ADVECT) { err = "only LS_MODE_ADVECT is implemented"; return LS_ERR_ARG; }
+    if (prm.mode != LS_MODE_ADVECT && prm.mode != LS_MODE_MCF) {
+      err = "mode must be LS_MODE_ADVECT or LS_MODE_MCF";
+      return LS_ERR_ARG;
+    }
     std::memset(&st, 0, sizeof(st));
     st.exec_mode = exec_;
     DevMesh<T> m = mesh();
@@ -1150,6 +1257,7 @@ class EngineImpl final : public Engine {
     a.bw = (T)prm.bandwidth;
     a.leps = prm.lambda_eps;
     a.dt = prm.dt;
+    a.mode = prm.mode;
     hipStream_t q = stream_;
     void* args[] = {&m, &s, &a};
     if (!pc_synced_) {  // the previous narrowband size, when the host-driven path ran last
@@ -1178,10 +1286,14 @@ class EngineImpl final : public Engine {
         graph_key_[0] = key[0]; graph_key_[1] = key[1]; graph_key_[2] = key[2];
         graph_prm_ = prm;
       }
-      LS_HIP(hipGraphLaunch(graph_exec_, q));
+      const hipError_t ge = hipGraphLaunch(graph_exec_, q);
+      if (coop_refused(ge)) return step_kernels(prm, st, err);
+      LS_HIP(ge);
     } else {
-      LS_HIP(hipLaunchCooperativeKernel((const void*)k_step<T, NC, kFimBlock>, dim3(coop_grid_),
-                                        dim3(kFimBlock), args, step_lds_, q));
+      const hipError_t le = hipLaunchCooperativeKernel((const void*)k_step<T, NC, kFimBlock>, dim3(coop_grid_),
+                                                       dim3(kFimBlock), args, step_lds_, q);
+      if (coop_refused(le)) return step_kernels(prm, st, err);
+      LS_HIP(le);
     }
     LS_HIP(hipEventRecord(ev_[4], q));
     int32_t pc[kNumPc];
@@ -1226,7 +1338,10 @@ class EngineImpl final : public Engine {
   // One launch per phase; the FIM loop is driven from the host.
   int step_kernels(const ls_step_params& prm, ls_stats& st, std::string& err) {
     if (prm.niter <= 0 || prm.inside_niter < 0) { err = "niter must be > 0, inside_niter >= 0"; return LS_ERR_ARG; }
-    if (prm.mode != LS_MODE_ADVECT) { err = "only LS_MODE_ADVECT is implemented"; return LS_ERR_ARG; }
+    if (prm.mode != LS_MODE_ADVECT && prm.mode != LS_MODE_MCF) {
+      err = "mode must be LS_MODE_ADVECT or LS_MODE_MCF";
+      return LS_ERR_ARG;
+    }
     std::memset(&st, 0, sizeof(st));
     st.exec_mode = LS_EXEC_KERNELS;
     pc_synced_ = false;
@@ -1294,7 +1409,7 @@ class EngineImpl final : public Engine {
     if (nnb_ > 0) {
       for (int it = 0; it < prm.inside_niter; ++it) {
         int ka = kev_begin(1, q);
-        k_advect<T, NC><<<nnb_, kBlock, adv_lds_, q>>>(m, s, d_nb_cur(), dt);
+        k_advect<T, NC><<<nnb_, kBlock, adv_lds_, q>>>(m, s, d_nb_cur(), dt, prm.mode);
         kev_end(ka, q);
         k_copy_phi<T><<<nnb_, kBlock, 0, q>>>(m, s, d_nb_cur());
       }
@@ -1381,6 +1496,7 @@ class EngineImpl final : public Engine {
   int P_ = 0;
   int64_t N_ = 0, F_ = 0;
   size_t fim_lds_ = 0, adv_lds_ = 0, seed_lds_ = 0;
+  bool coop_ok_ = true;  // cooperative launches accepted so far
   std::vector<int32_t> vert_off_h_, vert_perm_h_;
   int nnb_ = 0, nb_sel_ = 0;
   int32_t *d_vert_off_ = nullptr, *d_ele_off_ = nullptr, *d_halo_off_ = nullptr, *d_halo_ids_ = nullptr;

@@ -144,6 +144,28 @@ int ls_get_phi(ls_ctx* ctx, void* out, int dtype);
 /* Narrowband patch ids, sorted ascending; ids may be NULL to query *n. */
 int ls_get_narrowband(ls_ctx* ctx, int32_t* ids, int32_t* n);
 
+/* Batches of independent meshes, advanced together by one launch per step.
+ * No reference counterpart: LevelSet solves one mesh per object and resets
+ * the device for every solve (src/LevelSet.cu:146-166). Each mesh keeps the
+ * results it would have alone; per-mesh timestep and narrowband width
+ * replace the uniform ls_step_params values.
+ *
+ * Mesh m owns vertices [vert_off[m], vert_off[m+1]) and cells
+ * [cell_off[m], cell_off[m+1]) of the concatenated arrays; cell entries are
+ * global vertex ids inside the mesh's own range. Offsets have n_meshes+1
+ * entries starting at 0. Partition with ls_partition_batch, or with
+ * ls_partition_labels when no patch spans two meshes. */
+int ls_set_mesh_batch(ls_ctx* ctx, int dim, int n_meshes, const int64_t* vert_off, const double* xyz,
+                      const int64_t* cell_off, const int32_t* cells);
+/* Recursive bisection of every mesh into ceil(n_m / patch_verts) patches. */
+int ls_partition_batch(ls_ctx* ctx, int patch_verts);
+/* Per-mesh timestep and narrowband width (n_meshes values each); both NULL
+ * restores the uniform ls_step_params values. */
+int ls_set_batch_params(ls_ctx* ctx, const double* dt, const double* bandwidth);
+/* Narrowband patch count of every mesh after the last step (n_meshes values;
+ * 0 marks a mesh whose narrowband emptied). */
+int ls_get_batch_nb_counts(ls_ctx* ctx, int32_t* counts);
+
 #ifdef __cplusplus
 }
 #endif

@@ -13,6 +13,7 @@ from __future__ import annotations
 
 import itertools
 import math
+import os
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -135,32 +136,54 @@ def sdf_spheres_union(xyz, centers, radii):
     return out
 
 
-def ellipsoid_distance(p, axes, iters=60):
+def ellipsoid_distance(p, axes, bisect=16, newton=6):
     """Signed distance from points p (N,3), in the ellipsoid's own frame, to the
-    ellipsoid x^2/a^2 + y^2/b^2 + z^2/c^2 = 1. FP64 root of the Lagrange
-    condition sum (a_k p_k / (a_k^2 + t))^2 = 1 by safeguarded Newton/bisection."""
-    a = np.asarray(axes, dtype=np.float64)[None, :]
-    ap = np.abs(p)
-    inside = np.sum((ap / a) ** 2, axis=1) < 1.0
-    amin2 = float(np.min(a) ** 2)
-    # f(t) = sum (a p / (a^2 + t))^2 - 1 is decreasing on t > -amin^2.
-    lo = np.full(p.shape[0], -amin2 + 1e-300)
-    hi = np.maximum(np.linalg.norm(ap * a, axis=1), 1e-30)
-    hi = np.where(inside, 0.0, hi)
-    lo = np.where(inside, -amin2, 0.0)
+    ellipsoid x^2/a^2 + y^2/b^2 + z^2/c^2 = 1 (negative inside), in FP64.
+
+    The closest point is x_k = a_k^2 p_k / (a_k^2 + t) with t the root of
+    f(t) = sum (a_k p_k / (a_k^2 + t))^2 - 1 on t > -min(a)^2, where f is
+    decreasing and convex: bisection narrows the bracket, then Newton from
+    its left end (f > 0) converges monotonically. Interior points with no
+    root there (on the plane of the shortest axis, e.g. the centre) take the
+    degenerate closest point x_k = a_k^2 p_k / (a_k^2 - a_min^2) (k not the
+    shortest axis), x_min = a_min sqrt(1 - sum (x_k / a_k)^2)."""
+    a = np.abs(np.asarray(axes, dtype=np.float64))
+    ap = np.abs(np.asarray(p, dtype=np.float64))
+    kmin = int(np.argmin(a))
+    amin2 = float(a[kmin] ** 2)
+    inside = np.sum((ap / a[None, :]) ** 2, axis=1) < 1.0
+    a2 = (a * a)[None, :]
 
     def f(t):
-        return np.sum((a * ap / (a * a + t[:, None])) ** 2, axis=1) - 1.0
+        return np.sum((a[None, :] * ap / (a2 + t[:, None])) ** 2, axis=1) - 1.0
 
-    for _ in range(iters * 2):
+    def df(t):
+        return -2.0 * np.sum((a[None, :] * ap) ** 2 / (a2 + t[:, None]) ** 3, axis=1)
+
+    eps = 1e-12 * amin2
+    lo = np.where(inside, -amin2 + eps, 0.0)
+    hi = np.where(inside, 0.0, np.maximum(np.linalg.norm(ap * a[None, :], axis=1), 1e-300))
+    degenerate = inside & ~(f(lo) > 0.0)
+    for _ in range(bisect):
         mid = 0.5 * (lo + hi)
-        fm = f(mid)
-        lo = np.where(fm > 0, mid, lo)
-        hi = np.where(fm > 0, hi, mid)
-    t = 0.5 * (lo + hi)
-    x = a * a * ap / (a * a + t[:, None])
-    d = np.linalg.norm(x - ap, axis=1)
-    return np.where(inside, -d, d)
+        pos = f(mid) > 0.0
+        lo = np.where(pos, mid, lo)
+        hi = np.where(pos, hi, mid)
+    t = lo
+    for _ in range(newton):
+        d = df(t)
+        step = np.where(d < 0.0, -f(t) / np.where(d < 0.0, d, -1.0), 0.0)
+        t = np.minimum(t + np.maximum(step, 0.0), hi)
+    x = a2 * ap / (a2 + t[:, None])
+    if degenerate.any():
+        shortest = (a2 - amin2)[0] <= 0.0  # every axis as short as the shortest
+        den = np.where(shortest, 1.0, a2[0] - amin2)[None, :]
+        xd = np.where(shortest[None, :], 0.0, a2 * ap[degenerate] / den)
+        rest = np.sum((xd / a[None, :]) ** 2, axis=1)
+        xd[:, kmin] = a[kmin] * np.sqrt(np.maximum(0.0, 1.0 - rest))
+        x[degenerate] = xd
+    dist = np.linalg.norm(x - ap, axis=1)
+    return np.where(inside, -dist, dist)
 
 
 def centroids(xyz, cells):
@@ -305,6 +328,86 @@ def interface_radius(xyz, phi, centre, h, dim):
     r = np.linalg.norm(xyz[:, :dim] - np.asarray(centre)[:dim], axis=1)
     near = np.abs(phi) < h
     return float(np.mean(r[near] - phi[near]))
+
+
+# --- batches of independent meshes ---------------------------------------
+
+
+@dataclass
+class Batch:
+    name: str
+    dim: int
+    meshes: list             # list of Case, one per mesh (labels unused: the batch is partitioned whole)
+    patch_verts: int = 256
+    niter: int = 5
+    mode: int = 0
+    meta: dict = field(default_factory=dict)
+
+    @property
+    def n_verts(self):
+        return sum(m.xyz.shape[0] for m in self.meshes)
+
+    @property
+    def n_cells(self):
+        return sum(m.cells.shape[0] for m in self.meshes)
+
+
+def random_rotation(rng):
+    """Haar-distributed rotation (QR of a Gaussian matrix, signs fixed)."""
+    q, r = np.linalg.qr(rng.standard_normal((3, 3)))
+    q = q * np.sign(np.diag(r))[None, :]
+    if np.linalg.det(q) < 0:
+        q[:, 0] = -q[:, 0]
+    return q
+
+
+def ellipsoid_mesh(n: int, rng, cfl: float = 0.5, band_h: float = 6.0) -> Case:
+    """n^3 Kuhn grid on [0,1]^3; phi0 = exact signed distance to an ellipsoid
+    centred at (0.5,0.5,0.5) with semi-axes U[0.1,0.35] and a Haar rotation;
+    affine velocity A x + b (entries U[-1,1]) at element centroids; CFL dt."""
+    xyz, cells, h = kuhn_grid(n)
+    axes = rng.uniform(0.1, 0.35, size=3)
+    Q = random_rotation(rng)
+    A = rng.uniform(-1.0, 1.0, size=(3, 3))
+    b = rng.uniform(-1.0, 1.0, size=3)
+    local = (xyz - 0.5) @ Q  # coordinates in the ellipsoid frame (columns of Q are its axes)
+    phi0 = ellipsoid_distance(local, axes)
+    vel = centroids(xyz, cells) @ A.T + b[None, :]
+    vmax = float(np.max(np.linalg.norm(vel, axis=1)))
+    return Case(f"ellipsoid_{n}", 3, xyz, cells, phi0, vel, dt=cfl * h / vmax, steps=0, bandwidth=band_h * h,
+                h=h, labels=np.zeros(xyz.shape[0], np.int32), nparts=1, niter=5,
+                meta={"axes": axes, "rotation": Q, "A": A, "b": b, "vmax": vmax})
+
+
+def _ellipsoid_mesh_seeded(args):
+    n, seed = args
+    return ellipsoid_mesh(n, np.random.default_rng(seed))
+
+
+def config4(n_meshes: int = 512, seed: int = 4, lo: int = 2000, hi: int = 200000, steps: int = 50,
+            patch_verts: int = 256, workers: int = 0) -> Batch:
+    """512 Kuhn meshes with vertex counts log-uniform in [2k, 200k] (side
+    round(N^(1/3))), random ellipsoids and affine velocities, CFL 0.5, band 6h.
+    Mesh m draws its ellipsoid and velocity from its own seed (seed, m), so
+    the batch is the same whatever the number of worker processes."""
+    rng = np.random.default_rng(seed)
+    counts = np.exp(rng.uniform(np.log(lo), np.log(hi), size=n_meshes))
+    sides = np.maximum(3, np.round(np.cbrt(counts)).astype(int))
+    jobs = [(int(n), [seed, m]) for m, n in enumerate(sides)]
+    workers = workers or min(8, os.cpu_count() or 1)
+    if workers > 1 and n_meshes > 8:
+        import concurrent.futures as cf
+        with cf.ProcessPoolExecutor(max_workers=workers) as ex:
+            meshes = list(ex.map(_ellipsoid_mesh_seeded, jobs, chunksize=4))
+    else:
+        meshes = [_ellipsoid_mesh_seeded(j) for j in jobs]
+    return Batch(f"batch_{n_meshes}_ellipsoids", 3, meshes, patch_verts=patch_verts, niter=5, mode=0,
+                 meta={"sides": sides, "steps": steps, "seed": seed})
+
+
+def small_batch(n_meshes: int = 6, seed: int = 11, lo: int = 300, hi: int = 6000) -> Batch:
+    """A few uneven ellipsoid meshes for tests."""
+    return config4(n_meshes, seed=seed, lo=lo, hi=hi, steps=4, patch_verts=128)
 
 
 # --- mesh files -------------------------------------------------------------

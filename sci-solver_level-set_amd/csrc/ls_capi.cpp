@@ -24,6 +24,12 @@ struct ls_ctx {
   bool have_vel = false;
   lsamd::Layout layout;
   bool have_layout = false;
+  // Batch of independent meshes (n_meshes > 0): mesh m owns vertices
+  // [mesh_voff[m], mesh_voff[m+1]) and cells [mesh_coff[m], mesh_coff[m+1]).
+  int n_meshes = 0;
+  std::vector<int64_t> mesh_voff, mesh_coff;
+  std::vector<double> mesh_dt, mesh_bw;  // per mesh; empty: the step's uniform values
+  std::vector<int32_t> patch_mesh;       // P, mesh of each patch
   std::unique_ptr<lsamd::Engine> engine;
   std::string err;
 };
@@ -61,8 +67,49 @@ int apply_velocity(ls_ctx* c) {
   return rc == LS_OK ? rc : fail(c, rc, err);
 }
 
+// Per-patch timestep and band from the per-mesh values.
+int apply_batch_params(ls_ctx* c) {
+  if (!c->engine || !c->have_layout) return LS_OK;
+  std::vector<double> dt, bw;
+  if (c->n_meshes > 0 && !c->mesh_dt.empty()) {
+    const int P = c->layout.P;
+    dt.resize(P);
+    bw.resize(P);
+    for (int p = 0; p < P; ++p) {
+      dt[p] = c->mesh_dt[c->patch_mesh[p]];
+      bw[p] = c->mesh_bw[c->patch_mesh[p]];
+    }
+  }
+  std::string err;
+  const int rc = c->engine->set_patch_params(dt, bw, err);
+  return rc == LS_OK ? rc : fail(c, rc, err);
+}
+
+// Mesh of every patch; a patch must not span two meshes of a batch.
+int map_patches(ls_ctx* c, const std::vector<int32_t>& labels, int nparts) {
+  c->patch_mesh.assign(nparts, -1);
+  if (c->n_meshes == 0) {
+    std::fill(c->patch_mesh.begin(), c->patch_mesh.end(), 0);
+    return LS_OK;
+  }
+  for (int m = 0; m < c->n_meshes; ++m)
+    for (int64_t v = c->mesh_voff[m]; v < c->mesh_voff[m + 1]; ++v) {
+      int32_t& pm = c->patch_mesh[labels[v]];
+      if (pm < 0) pm = m;
+      else if (pm != m)
+        return fail(c, LS_ERR_ARG, "patch " + std::to_string(labels[v]) + " spans meshes " + std::to_string(pm) +
+                                       " and " + std::to_string(m));
+    }
+  for (int p = 0; p < nparts; ++p)
+    if (c->patch_mesh[p] < 0) return fail(c, LS_ERR_ARG, "patch " + std::to_string(p) + " has no vertex");
+  return LS_OK;
+}
+
 int finish_layout(ls_ctx* c, const std::vector<int32_t>& labels, int nparts) {
   std::string err;
+  for (int32_t l : labels)
+    if (l < 0 || l >= nparts) return fail(c, LS_ERR_ARG, "patch label out of range");
+  if (map_patches(c, labels, nparts) != LS_OK) return LS_ERR_ARG;
   lsamd::Layout L;
   if (lsamd::build_layout(c->dim, c->N, c->xyz.data(), c->E, c->cells.data(), labels.data(), nparts,
                           L, err) != 0)
@@ -82,6 +129,8 @@ int finish_layout(ls_ctx* c, const std::vector<int32_t>& labels, int nparts) {
   c->engine = std::move(eng);
   c->engine->set_exec(c->exec);
   c->have_layout = true;
+  rc = apply_batch_params(c);
+  if (rc != LS_OK) return rc;
   return c->have_vel ? apply_velocity(c) : LS_OK;
 }
 
@@ -118,6 +167,84 @@ int ls_set_mesh(ls_ctx* c, int dim, int64_t n_verts, const double* xyz, int64_t 
   c->have_layout = false;
   c->have_vel = false;
   c->engine.reset();
+  c->n_meshes = 0;
+  c->mesh_voff.clear();
+  c->mesh_coff.clear();
+  c->mesh_dt.clear();
+  c->mesh_bw.clear();
+  return LS_OK;
+}
+
+int ls_set_mesh_batch(ls_ctx* c, int dim, int n_meshes, const int64_t* vert_off, const double* xyz,
+                      const int64_t* cell_off, const int32_t* cells) {
+  if (!c) return LS_ERR_ARG;
+  if (n_meshes <= 0 || !vert_off || !cell_off) return fail(c, LS_ERR_ARG, "n_meshes > 0 and offsets required");
+  if (vert_off[0] != 0 || cell_off[0] != 0) return fail(c, LS_ERR_ARG, "offsets must start at 0");
+  for (int m = 0; m < n_meshes; ++m)
+    if (vert_off[m + 1] <= vert_off[m] || cell_off[m + 1] <= cell_off[m])
+      return fail(c, LS_ERR_ARG, "mesh " + std::to_string(m) + " is empty");
+  const int nc = dim + 1;
+  for (int m = 0; m < n_meshes; ++m)
+    for (int64_t t = cell_off[m] * nc; t < cell_off[m + 1] * nc; ++t)
+      if (cells && (cells[t] < vert_off[m] || cells[t] >= vert_off[m + 1]))
+        return fail(c, LS_ERR_ARG, "a cell of mesh " + std::to_string(m) + " names a vertex of another mesh");
+  const int rc = ls_set_mesh(c, dim, vert_off[n_meshes], xyz, cell_off[n_meshes], cells);
+  if (rc != LS_OK) return rc;
+  c->n_meshes = n_meshes;
+  c->mesh_voff.assign(vert_off, vert_off + n_meshes + 1);
+  c->mesh_coff.assign(cell_off, cell_off + n_meshes + 1);
+  return LS_OK;
+}
+
+int ls_partition_batch(ls_ctx* c, int patch_verts) {
+  if (!c) return LS_ERR_ARG;
+  if (c->n_meshes == 0) return fail(c, LS_ERR_STATE, "set a mesh batch first");
+  if (patch_verts <= 0) return fail(c, LS_ERR_ARG, "patch_verts must be > 0");
+  const int nm = c->n_meshes;
+  std::vector<int> parts(nm), base(nm + 1, 0);
+  for (int m = 0; m < nm; ++m) {
+    const int64_t n = c->mesh_voff[m + 1] - c->mesh_voff[m];
+    parts[m] = (int)((n + patch_verts - 1) / patch_verts);
+    base[m + 1] = base[m] + parts[m];
+  }
+  std::vector<int32_t> labels(c->N);
+  std::vector<std::string> errs(nm);
+  #pragma omp parallel for schedule(dynamic, 1)
+  for (int m = 0; m < nm; ++m) {
+    const int64_t v0 = c->mesh_voff[m], n = c->mesh_voff[m + 1] - v0;
+    std::vector<int32_t> lab;
+    if (lsamd::partition_rcb(n, c->xyz.data() + 3 * v0, parts[m], lab, errs[m]) == 0)
+      for (int64_t v = 0; v < n; ++v) labels[v0 + v] = base[m] + lab[v];
+  }
+  for (int m = 0; m < nm; ++m)
+    if (!errs[m].empty()) return fail(c, LS_ERR_ARG, "mesh " + std::to_string(m) + ": " + errs[m]);
+  return finish_layout(c, labels, base[nm]);
+}
+
+int ls_set_batch_params(ls_ctx* c, const double* dt, const double* bandwidth) {
+  if (!c) return LS_ERR_ARG;
+  if (c->n_meshes == 0) return fail(c, LS_ERR_STATE, "set a mesh batch first");
+  if (!dt && !bandwidth) {
+    c->mesh_dt.clear();
+    c->mesh_bw.clear();
+  } else {
+    if (!dt || !bandwidth) return fail(c, LS_ERR_ARG, "give both dt and bandwidth, or neither");
+    c->mesh_dt.assign(dt, dt + c->n_meshes);
+    c->mesh_bw.assign(bandwidth, bandwidth + c->n_meshes);
+  }
+  return apply_batch_params(c);
+}
+
+int ls_get_batch_nb_counts(ls_ctx* c, int32_t* counts) {
+  if (!c || !counts) return LS_ERR_ARG;
+  if (c->n_meshes == 0) return fail(c, LS_ERR_STATE, "set a mesh batch first");
+  if (!c->have_layout || !c->engine) return fail(c, LS_ERR_STATE, "partition the mesh first");
+  std::vector<int32_t> v;
+  std::string err;
+  const int rc = c->engine->get_narrowband(v, err);
+  if (rc != LS_OK) return fail(c, rc, err);
+  std::fill(counts, counts + c->n_meshes, 0);
+  for (int32_t p : v) counts[c->patch_mesh[p]]++;
   return LS_OK;
 }
 

@@ -28,7 +28,20 @@ struct DevMesh {
   const T* sigma;              // F x (3 | 2), velocity in the local frame
   const int32_t* padj_off;     // P+1
   const int32_t* padj;
+  // Per-patch timestep and narrowband width for batches of meshes with
+  // their own parameters; null: the step's uniform values.
+  const double* patch_dt;      // P or null
+  const T* patch_bw;           // P or null
 };
+
+template <typename T>
+__device__ __forceinline__ T band_of(const DevMesh<T>& m, int p, T bw) {
+  return m.patch_bw ? m.patch_bw[p] : bw;
+}
+template <typename T>
+__device__ __forceinline__ double dt_of(const DevMesh<T>& m, int p, double dt) {
+  return m.patch_dt ? m.patch_dt[p] : dt;
+}
 
 // Per-timestep state (renumbered vertex order).
 template <typename T>

@@ -18,6 +18,10 @@ class Engine {
   virtual int upload(const Layout& L, std::string& err) = 0;
   virtual int set_phi(const double* phi_user, std::string& err) = 0;
   virtual int set_velocity(const std::vector<double>& sigma_local, std::string& err) = 0;
+  // Per-patch timestep and narrowband width (P each); empty vectors restore
+  // the step's uniform values.
+  virtual int set_patch_params(const std::vector<double>& dt, const std::vector<double>& bw,
+                               std::string& err) = 0;
   virtual int step(const ls_step_params& prm, ls_stats& st, std::string& err) = 0;
   virtual int get_phi(double* phi_user, std::string& err) = 0;
   virtual int get_narrowband(std::vector<int32_t>& ids, std::string& err) = 0;

@@ -662,9 +662,9 @@ template <typename T, int NC, int BS>
 __device__ __forceinline__ void update_patch(const DevMesh<T>& m, DevState<T>& s, int p, double dt, int mode,
                                              unsigned char* smem) {
   if (mode == LS_MODE_MCF)
-    curvature_patch<T, NC, BS>(m, s, p, dt, smem);
+    curvature_patch<T, NC, BS>(m, s, p, dt_of(m, p, dt), smem);
   else
-    advect_patch<T, NC, BS>(m, s, p, dt, smem);
+    advect_patch<T, NC, BS>(m, s, p, dt_of(m, p, dt), smem);
 }
 
 // Upwind residual-distribution (N-scheme) advection of one narrowband patch
@@ -809,7 +809,7 @@ __global__ __launch_bounds__(kBlock) void k_activate(DevMesh<T> m, DevState<T> s
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
   const int p = list[i];
-  if (!(s.bcon[p] && s.bmin[p] < bw)) return;
+  if (!(s.bcon[p] && s.bmin[p] < band_of(m, p, bw))) return;
   for (int k = m.padj_off[p]; k < m.padj_off[p + 1]; ++k) {
     const int q = m.padj[k];
     if (s.act[q] == 0 && atomicExch(&s.fresh[q], 1) == 0)
@@ -838,7 +838,7 @@ __global__ __launch_bounds__(kBlock) void k_narrowband(DevMesh<T> m, DevState<T>
   __shared__ T red[kBlock / 64];
   const int p = blockIdx.x;
   const T mn = narrowband_patch<T, kBlock>(m, s, p, red);
-  if (threadIdx.x == 0 && mn <= bw) {
+  if (threadIdx.x == 0 && mn <= band_of(m, p, bw)) {
     nb[atomicAdd(&s.counters[kCntNB], 1)] = p;
     atomicAdd(&s.counters[kCntNBVerts], m.vert_off[p + 1] - m.vert_off[p]);
   }
@@ -993,7 +993,7 @@ __global__ __launch_bounds__(BS) void k_step(DevMesh<T> m, DevState<T> s, StepAr
       const int p = cur[i];
       const int vs = m.vert_off[p], ve = m.vert_off[p + 1];
       for (int v = vs + tid; v < ve; v += BS) s.DT[v] = s.DT_out[v];
-      if (s.bcon[p] && s.bmin[p] < a.bw) {
+      if (s.bcon[p] && s.bmin[p] < band_of(m, p, a.bw)) {
         for (int k = m.padj_off[p] + tid; k < m.padj_off[p + 1]; k += BS) {
           const int q = m.padj[k];
           const bool active_unconv = (s.inA[q] == E) && !s.bcon[q];
@@ -1039,7 +1039,7 @@ __global__ __launch_bounds__(BS) void k_step(DevMesh<T> m, DevState<T> s, StepAr
   // Phase 3: narrowband + sign restore over every patch.
   for (int p = b; p < m.P; p += G) {
     const T mn = narrowband_patch<T, BS>(m, s, p, red);
-    if (tid == 0 && mn <= a.bw) {
+    if (tid == 0 && mn <= band_of(m, p, a.bw)) {
       s.nb[nb_out][atomicAdd(&s.pc[kPcNb + nb_out], 1)] = p;
       atomicAdd(&s.pc[kPcNbVerts], m.vert_off[p + 1] - m.vert_off[p]);
     }
@@ -1218,6 +1218,25 @@ class EngineImpl final : public Engine {
     return LS_OK;
   }
 
+  int set_patch_params(const std::vector<double>& dt, const std::vector<double>& bw, std::string& err) override {
+    if (!uploaded_) { err = "layout not built"; return LS_ERR_STATE; }
+    if (dt.empty() && bw.empty()) {
+      use_patch_prm_ = false;
+      return LS_OK;
+    }
+    if (dt.size() != (size_t)P_ || bw.size() != (size_t)P_) { err = "per-patch parameters need P values"; return LS_ERR_ARG; }
+    if (!d_patch_dt_) {
+      LS_HIP(dev_alloc(&d_patch_dt_, (size_t)P_));
+      LS_HIP(dev_alloc(&d_patch_bw_, (size_t)P_));
+    }
+    std::vector<T> hb(bw.size());
+    for (size_t i = 0; i < bw.size(); ++i) hb[i] = (T)bw[i];
+    LS_HIP(hipMemcpy(d_patch_dt_, dt.data(), dt.size() * sizeof(double), hipMemcpyHostToDevice));
+    LS_HIP(hipMemcpy(d_patch_bw_, hb.data(), hb.size() * sizeof(T), hipMemcpyHostToDevice));
+    use_patch_prm_ = true;
+    return LS_OK;
+  }
+
   int step(const ls_step_params& prm, ls_stats& st, std::string& err) override {
     if (!uploaded_) { err = "layout not built"; return LS_ERR_STATE; }
     if (exec_ != LS_EXEC_KERNELS && coop_ok_) return step_persistent(prm, st, err);
@@ -1267,7 +1286,7 @@ class EngineImpl final : public Engine {
     }
     LS_HIP(hipEventRecord(ev_[0], q));
     if (exec_ == LS_EXEC_GRAPH) {
-      const int key[3] = {a.first, a.nb_in, 1};
+      const int key[3] = {a.first, a.nb_in, use_patch_prm_ ? 2 : 1};
       const bool same = graph_exec_ && key[0] == graph_key_[0] && key[1] == graph_key_[1] &&
                         std::memcmp(&graph_prm_, &prm, sizeof(prm)) == 0;
       if (!same) {
@@ -1456,6 +1475,8 @@ class EngineImpl final : public Engine {
     m.vert_off = d_vert_off_; m.ele_off = d_ele_off_; m.halo_off = d_halo_off_; m.halo_ids = d_halo_ids_;
     m.copy_local = d_copy_local_; m.vslot_off = d_vslot_off_; m.vslots = d_vslots_;
     m.coords = d_coords_; m.sigma = d_sigma_; m.padj_off = d_padj_off_; m.padj = d_padj_;
+    m.patch_dt = use_patch_prm_ ? d_patch_dt_ : nullptr;
+    m.patch_bw = use_patch_prm_ ? d_patch_bw_ : nullptr;
     return m;
   }
   DevState<T> state() const {
@@ -1480,7 +1501,8 @@ class EngineImpl final : public Engine {
                     d_vslots_, d_padj_off_, d_padj_, d_coords_, d_sigma_, d_phi_, d_phi_out_, d_DT_,
                     d_DT_out_, d_label_, d_tsign_, d_bcon_, d_bmin_, d_act_, d_fresh_, d_seedflag_,
                     d_counters_, d_sizes_, d_nb_, d_scan_all_, d_lists_[0], d_lists_[1], d_lists_[2],
-                    d_lists_[3], d_inA_, d_freshE_, d_listF_, d_pc_, d_ticks_, d_bar_};
+                    d_lists_[3], d_inA_, d_freshE_, d_listF_, d_pc_, d_ticks_, d_bar_,
+                    d_patch_dt_, d_patch_bw_};
     if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
     if (graph_) (void)hipGraphDestroy(graph_);
     for (void* p : ptrs) if (p) (void)hipFree(p);
@@ -1505,6 +1527,9 @@ class EngineImpl final : public Engine {
   uint16_t* d_vslots_ = nullptr;
   int32_t *d_padj_off_ = nullptr, *d_padj_ = nullptr;
   T *d_coords_ = nullptr, *d_sigma_ = nullptr;
+  double* d_patch_dt_ = nullptr;
+  T* d_patch_bw_ = nullptr;
+  bool use_patch_prm_ = false;
   T *d_phi_ = nullptr, *d_phi_out_ = nullptr, *d_DT_ = nullptr, *d_DT_out_ = nullptr, *d_bmin_ = nullptr;
   int8_t *d_label_ = nullptr, *d_tsign_ = nullptr;
   int32_t *d_bcon_ = nullptr, *d_act_ = nullptr, *d_fresh_ = nullptr, *d_seedflag_ = nullptr, *d_counters_ = nullptr;

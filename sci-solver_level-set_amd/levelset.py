@@ -53,7 +53,8 @@ class LayoutInfo(C.Structure):
 
 EXPORTS = ["ls_create", "ls_destroy", "ls_last_error", "ls_set_mesh", "ls_set_precision", "ls_set_execution", "ls_partition_grid",
            "ls_partition_rcb", "ls_partition_labels", "ls_get_layout_info", "ls_get_layout",
-           "ls_set_phi", "ls_set_velocity", "ls_step", "ls_get_phi", "ls_get_narrowband"]
+           "ls_set_phi", "ls_set_velocity", "ls_step", "ls_get_phi", "ls_get_narrowband",
+           "ls_set_mesh_batch", "ls_partition_batch", "ls_set_batch_params", "ls_get_batch_nb_counts"]
 
 
 def build(verbose=False) -> str:
@@ -92,6 +93,10 @@ def lib():
         L.ls_step.argtypes = [vp, C.POINTER(StepParams), C.POINTER(Stats)]
         L.ls_get_phi.argtypes = [vp, vp, C.c_int]
         L.ls_get_narrowband.argtypes = [vp, vp, C.POINTER(C.c_int32)]
+        L.ls_set_mesh_batch.argtypes = [vp, C.c_int, C.c_int, vp, vp, vp, vp]
+        L.ls_partition_batch.argtypes = [vp, C.c_int]
+        L.ls_set_batch_params.argtypes = [vp, vp, vp]
+        L.ls_get_batch_nb_counts.argtypes = [vp, vp]
         _lib = L
     return _lib
 
@@ -190,6 +195,41 @@ class Solver:
         self._check(self.L.ls_get_phi(self.ctx, out.ctypes.data, LS_F64))
         return out
 
+    # -- batches of independent meshes ------------------------------------
+    def set_mesh_batch(self, dim, meshes):
+        """meshes: list of (xyz (n_m, 3), cells (e_m, dim+1)) with mesh-local
+        vertex ids; they are concatenated in order."""
+        voff = np.zeros(len(meshes) + 1, np.int64)
+        coff = np.zeros(len(meshes) + 1, np.int64)
+        for m, (x, c) in enumerate(meshes):
+            voff[m + 1] = voff[m] + x.shape[0]
+            coff[m + 1] = coff[m] + c.shape[0]
+        self._xyz = np.ascontiguousarray(np.concatenate([x for x, _ in meshes]), dtype=np.float64)
+        self._cells = np.ascontiguousarray(
+            np.concatenate([np.asarray(c, np.int64) + voff[m] for m, (_, c) in enumerate(meshes)]), dtype=np.int32)
+        self._voff, self._coff = voff, coff
+        self.dim, self.N, self.E = dim, int(voff[-1]), int(coff[-1])
+        self.n_meshes = len(meshes)
+        self._check(self.L.ls_set_mesh_batch(self.ctx, dim, self.n_meshes, voff.ctypes.data, self._xyz.ctypes.data,
+                                             coff.ctypes.data, self._cells.ctypes.data))
+
+    def partition_batch(self, patch_verts):
+        self._check(self.L.ls_partition_batch(self.ctx, int(patch_verts)))
+
+    def set_batch_params(self, dt, bandwidth):
+        self._bdt = np.ascontiguousarray(dt, dtype=np.float64)
+        self._bbw = np.ascontiguousarray(bandwidth, dtype=np.float64)
+        self._check(self.L.ls_set_batch_params(self.ctx, self._bdt.ctypes.data, self._bbw.ctypes.data))
+
+    def batch_nb_counts(self):
+        out = np.zeros(self.n_meshes, np.int32)
+        self._check(self.L.ls_get_batch_nb_counts(self.ctx, out.ctypes.data))
+        return out
+
+    def split(self, values):
+        """Per-mesh views of a concatenated per-vertex array."""
+        return [values[self._voff[m]:self._voff[m + 1]] for m in range(self.n_meshes)]
+
     def narrowband(self):
         n = C.c_int32(0)
         self._check(self.L.ls_get_narrowband(self.ctx, None, C.byref(n)))
@@ -207,3 +247,48 @@ def solver_for_case(case, device=0, precision=LS_F32, execution=LS_EXEC_PERSISTE
     s.set_phi(case.phi0)
     s.set_velocity(case.vel)
     return s
+
+
+def solver_for_batch(batch, device=0, execution=LS_EXEC_GRAPH, patch_verts=None, labels=None):
+    """A Solver holding a cases.Batch; patches from recursive bisection
+    (patch_verts) or from caller labels (labels, nparts)."""
+    s = Solver(device)
+    s.set_mesh_batch(batch.dim, [(m.xyz, m.cells) for m in batch.meshes])
+    s.set_execution(execution)
+    if labels is not None:
+        s.partition_labels(labels[0], labels[1])
+    else:
+        s.partition_batch(patch_verts or batch.patch_verts)
+    s.set_batch_params([m.dt for m in batch.meshes], [m.bandwidth for m in batch.meshes])
+    s.set_phi(np.concatenate([m.phi0 for m in batch.meshes]))
+    s.set_velocity(np.concatenate([m.vel for m in batch.meshes]))
+    return s
+
+
+def autotune_batch(batch, candidates, executions=(LS_EXEC_GRAPH, LS_EXEC_PERSISTENT, LS_EXEC_KERNELS),
+                   trial_steps=3, device=0, log=None):
+    """Times `trial_steps` steps of the batch for every patching candidate and
+    execution mode and returns the fastest as (candidate, execution, ms/step,
+    table). A candidate is either an int (recursive-bisection patch size) or
+    a (name, labels, nparts) tuple (caller block partition). Every trial
+    starts from the batch's initial state."""
+    table = []
+    best = None
+    for cand in candidates:
+        for ex in executions:
+            kw = {"patch_verts": cand} if isinstance(cand, int) else {"labels": (cand[1], cand[2])}
+            s = solver_for_batch(batch, device=device, execution=ex, **kw)
+            prm = step_params(1.0, 1.0, batch.niter, mode=batch.mode)
+            s.step(prm)  # first step: graph capture and warm caches
+            ms = 0.0
+            for _ in range(trial_steps):
+                ms += s.step(prm).ms_total
+            s.close()
+            ms /= trial_steps
+            name = cand if isinstance(cand, int) else cand[0]
+            table.append((name, ex, ms))
+            if log:
+                log(f"autotune: patches {name} exec {ex}: {ms:.3f} ms/step")
+            if best is None or ms < best[2]:
+                best = (cand, ex, ms)
+    return best[0], best[1], best[2], table

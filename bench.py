@@ -101,6 +101,148 @@ def lockstep(case, oracle_mod, lsb, precision, steps):
     return res
 
 
+def timed_run(g, prm, steps):
+    """Times `steps` timesteps; sums the per-step statistics."""
+    tot = {"verts": 0, "sweep_ms": 0.0, "sweep_launches": 0, "sweep_bytes": 0, "adv_ms": 0.0,
+           "adv_bytes": 0, "adv_launches": 0, "ms_fim": 0.0, "ms_total": 0.0, "fim_iters": 0,
+           "visits": 0, "probes": 0, "step_ms": 0.0, "step_bytes": 0, "exec_modes": set()}
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        st = g.step(prm)
+        tot["verts"] += st.nb_verts
+        tot["sweep_ms"] += st.ms_sweep_kernels
+        tot["sweep_launches"] += st.sweep_launches
+        tot["sweep_bytes"] += fim_bytes(st)
+        tot["adv_ms"] += st.ms_advect_kernels
+        tot["adv_bytes"] += adv_bytes(st)
+        tot["adv_launches"] += 1 if st.nb_count > 0 else 0
+        tot["ms_fim"] += st.ms_fim
+        tot["ms_total"] += st.ms_total
+        tot["fim_iters"] += st.fim_iters
+        tot["visits"] += st.patch_visits
+        tot["probes"] += st.probes
+        tot["step_ms"] += st.ms_step_kernel
+        tot["step_bytes"] += step_bytes(st)
+        tot["exec_modes"].add(st.exec_mode)
+    tot["wall"] = time.perf_counter() - t0
+    return tot
+
+
+def roofline(tot, steps, kernels_mode):
+    if kernels_mode:
+        kname = "k_sweep (FIM patch sweep)"
+        launches = tot["sweep_launches"]
+        avg_ms = tot["sweep_ms"] / max(launches, 1)
+        avg_bytes = tot["sweep_bytes"] / max(launches, 1)
+    else:
+        kname = "k_step (persistent timestep: seeds, FIM, narrowband, advection)"
+        launches = steps
+        avg_ms = tot["step_ms"] / steps
+        avg_bytes = tot["step_bytes"] / steps
+    achieved = avg_bytes / (avg_ms * 1e-3) / 1e9
+    return {"kernel": kname, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": None, "avg_launch_ms": avg_ms,
+            "avg_launch_algorithmic_bytes": avg_bytes, "launches": launches}
+
+
+def batch_workload(args, lsb, cases):
+    """512 uneven ellipsoid meshes in one launch per timestep (hipGraph), the
+    patch size and execution mode chosen by the autotuner on a 64-mesh
+    subset. CPU baseline and parity on a sample of the meshes."""
+    b = cases.config4(steps=args.steps)
+    print(f"batch: {len(b.meshes)} meshes, {b.n_verts} vertices, {b.n_cells} tets", file=sys.stderr, flush=True)
+    sub = cases.Batch(b.name + "_tune", 3, b.meshes[:64], patch_verts=b.patch_verts)
+    cand, ex, ms, table = lsb.autotune_batch(sub, [64, 128, 256, 512], trial_steps=2,
+                                             log=lambda m: print(m, file=sys.stderr, flush=True))
+    g = lsb.solver_for_batch(b, execution=ex, patch_verts=cand)
+    info = g.layout_info()
+    prm = lsb.step_params(1.0, 1.0, 5)
+    for _ in range(args.warmup):
+        g.step(prm)
+    g.set_phi(np.concatenate([m.phi0 for m in b.meshes]))
+    tot = timed_run(g, prm, args.steps)
+    g.close()
+    out = {"workload": f"{b.name}: {len(b.meshes)} Kuhn meshes, {b.n_verts} verts, {b.n_cells} tets, "
+                       f"ellipsoids, affine velocity, CFL 0.5, band 6h, {args.steps} timesteps",
+           "patch_verts": cand, "execution": {lsb.LS_EXEC_GRAPH: "graph", lsb.LS_EXEC_PERSISTENT: "persistent",
+                                              lsb.LS_EXEC_KERNELS: "kernels"}[ex],
+           "autotune_ms_per_step": {f"{c}/{e}": m for c, e, m in table},
+           "patches": info["n_patches"], "element_copies": info["n_copies"]}
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        sample = cases.Batch(b.name + "_sample", 3, b.meshes[:24], patch_verts=cand)
+        gs = lsb.solver_for_batch(sample, execution=ex, patch_verts=cand)
+        orc = []
+        for m in sample.meshes:
+            nparts = -(-m.xyz.shape[0] // cand)
+            o = O.Oracle(3, m.xyz, m.cells, cases.partition_rcb(m.xyz, nparts), nparts, 32)
+            o.set_phi(m.phi0)
+            o.set_velocity(m.vel)
+            orc.append(o)
+        cpu_s, cpu_v, mism, nbd = 0.0, 0, 0, 0
+        for _ in range(args.cpu_steps):
+            gs.step(prm)
+            counts = gs.batch_nb_counts()
+            for k, (o, phi) in enumerate(zip(orc, gs.split(gs.phi()))):
+                t0 = time.perf_counter()
+                ost = o.step(O.make_params(sample.meshes[k].dt, sample.meshes[k].bandwidth, 5))
+                cpu_s += time.perf_counter() - t0
+                cpu_v += ost.nb_verts
+                mism += int(np.count_nonzero(phi.astype(np.float32) != o.phi().astype(np.float32)))
+                nbd += abs(int(counts[k]) - ost.nb_count)
+        for o in orc:
+            o.close()
+        gs.close()
+        cpu = ({"value": cpu_v / cpu_s, "unit": "vertex-updates/s", "cores": O.max_threads(), "kind": "port",
+                "sample": f"first {args.cpu_steps} timesteps of the first 24 meshes, oracle FP32 instance, "
+                          "OpenMP over patches"},
+               {"fp32_oracle_meshes": 24, "fp32_oracle_steps": args.cpu_steps, "fp32_bit_mismatches": mism,
+                "nb_count_diff": nbd, "tolerance": 1e-5})
+    return out, tot, ex == lsb.LS_EXEC_KERNELS, cpu
+
+
+def mcf_workload(args, lsb, cases):
+    """Mean-curvature flow of 8 spheres on a 192^3 Kuhn grid (41.8M tets)."""
+    case = cases.config3(steps=args.steps)
+    print(f"mcf: {case.xyz.shape[0]} vertices, {case.cells.shape[0]} tets", file=sys.stderr, flush=True)
+    ex = {"graph": lsb.LS_EXEC_GRAPH, "persistent": lsb.LS_EXEC_PERSISTENT, "kernels": lsb.LS_EXEC_KERNELS}[args.exec]
+    g = lsb.solver_for_case(case, execution=ex)
+    info = g.layout_info()
+    prm = lsb.step_params(case.dt, case.bandwidth, case.niter, mode=lsb.LS_MODE_MCF)
+    for _ in range(args.warmup):
+        g.step(prm)
+    g.set_phi(case.phi0)
+    tot = timed_run(g, prm, args.steps)
+    g.close()
+    out = {"workload": f"{case.name}: 192^3 Kuhn tets ({case.xyz.shape[0]} verts, {case.cells.shape[0]} tets), "
+                       f"8 spheres, mean-curvature flow, dt = h^2/8, band 8h, {args.steps} timesteps",
+           "execution": args.exec, "patches": info["n_patches"], "element_copies": info["n_copies"]}
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        g = lsb.solver_for_case(case, execution=ex)
+        o = O.Oracle(3, case.xyz, case.cells, case.labels, case.nparts, 32)
+        o.set_phi(case.phi0)
+        o.set_velocity(case.vel)
+        cpu_s, cpu_v, mism = 0.0, 0, 0
+        for _ in range(args.cpu_steps):
+            g.step(prm)
+            t0 = time.perf_counter()
+            ost = o.step(O.make_params(case.dt, case.bandwidth, case.niter, mode=1))
+            cpu_s += time.perf_counter() - t0
+            cpu_v += ost.nb_verts
+            mism += int(np.count_nonzero(g.phi().astype(np.float32) != o.phi().astype(np.float32)))
+        o.close()
+        g.close()
+        cpu = ({"value": cpu_v / cpu_s, "unit": "vertex-updates/s", "cores": O.max_threads(), "kind": "port",
+                "sample": f"first {args.cpu_steps} timesteps of the same workload, oracle FP32 instance"},
+               {"fp32_oracle_steps": args.cpu_steps, "fp32_bit_mismatches": mism, "tolerance": 1e-5})
+    return out, tot, ex == lsb.LS_EXEC_KERNELS, cpu
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -111,6 +253,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--exec", default="graph", choices=["graph", "persistent", "kernels"],
                     help="graph: persistent step kernel replayed from a hipGraph (default)")
+    ap.add_argument("--workload", default="torus", choices=["torus", "batch", "mcf"],
+                    help="torus: 3D torus revolve, 64^3 (default); batch: 512 ellipsoid meshes; "
+                         "mcf: 8-sphere mean-curvature flow, 192^3")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU path is not built.",
@@ -120,6 +265,22 @@ def main():
     ge.load_package()
     import levelset_amd.cases as cases
     import levelset_amd.levelset as lsb
+
+    if args.workload != "torus":
+        fn = batch_workload if args.workload == "batch" else mcf_workload
+        cfg, tot, kmode, cpu = fn(args, lsb, cases)
+        out = {"metric": "narrowband vertex-updates/s, learned warm-start vs plain FIM @ Linf<=1e-5",
+               "value": tot["verts"] / tot["wall"], "unit": "vertex-updates/s", "n_gpus": 1, "steps": args.steps,
+               "warmup": args.warmup, "ms_per_step": 1e3 * tot["wall"] / args.steps, "higher_is_better": True,
+               "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+               "config": dict(cfg, mode="plain_fim", gpu_event_ms_per_step=tot["ms_total"] / args.steps,
+                              fim_iters_per_step=tot["fim_iters"] / args.steps,
+                              exec_modes_used=sorted(tot["exec_modes"])),
+               "roofline": roofline(tot, args.steps, kmode)}
+        if cpu:
+            out["cpu_baseline"], out["parity"] = cpu
+        print(json.dumps(out))
+        return
 
     case = cases.config2(steps=args.steps)
     exec_mode = {"graph": lsb.LS_EXEC_GRAPH, "persistent": lsb.LS_EXEC_PERSISTENT,

@@ -33,6 +33,12 @@ enum { LS_MODE_ADVECT = 0, LS_MODE_MCF = 1 };
  * same kernel replayed from a captured hipGraph, or one launch per phase with
  * the FIM loop driven from the host. Results are identical. */
 enum { LS_EXEC_PERSISTENT = 0, LS_EXEC_KERNELS = 1, LS_EXEC_GRAPH = 2 };
+/* Warm start of the FIM (no reference counterpart): none, |phi| after the
+ * previous step (identity predictor), or the learned predictor's distances
+ * (ls_mlp_predict). Warm values apply to non-seed vertices of the previous
+ * narrowband; a step whose fixed-point residual over the band exceeds
+ * warm_res_tol re-runs plain in the same launch. */
+enum { LS_WARM_NONE = 0, LS_WARM_IDENTITY = 1, LS_WARM_LEARNED = 2 };
 
 /* Per-step parameters. Replaces the argument list of
  * meshFIM3d::GenerateData(char*, int nsteps, float timestep, int inside_niter,
@@ -48,6 +54,10 @@ typedef struct ls_step_params {
   int32_t inside_niter;/* advection passes per step (insideIterations_) */
   int32_t mode;        /* LS_MODE_ADVECT, or LS_MODE_MCF: mean-curvature flow phi_t = kappa |grad phi| */
   int32_t max_fim_iters;/* 0: run the FIM loop to convergence; > 0: stop after that many iterations */
+  int32_t warm;         /* LS_WARM_*; needs LS_EXEC_PERSISTENT or LS_EXEC_GRAPH */
+  int32_t reserved0;
+  double warm_margin;   /* warm value = predicted distance * (1 + warm_margin) */
+  double warm_res_tol;  /* residual above which a warm step re-runs plain (0: tol) */
 } ls_step_params;
 
 typedef struct ls_stats {
@@ -76,6 +86,9 @@ typedef struct ls_stats {
   double ms_ph_seed, ms_ph_visit, ms_ph_copyback, ms_ph_probe, ms_ph_next, ms_ph_nb, ms_ph_advect;
   int64_t elem_updates;     /* element updates executed by FIM visits and probes */
   int64_t local_solves;     /* local solves executed (the rest were skipped exactly) */
+  double warm_residual;     /* warm start: max |DT - F(DT)| over non-seed band vertices */
+  int32_t warm_patches;     /* warm start: patches that received warm values */
+  int32_t warm_fallback;    /* warm start: 1 when the residual check failed and the step re-ran plain */
 } ls_stats;
 
 typedef struct ls_layout_info {
@@ -165,6 +178,38 @@ int ls_set_batch_params(ls_ctx* ctx, const double* dt, const double* bandwidth);
 /* Narrowband patch count of every mesh after the last step (n_meshes values;
  * 0 marks a mesh whose narrowband emptied). */
 int ls_get_batch_nb_counts(ls_ctx* ctx, int32_t* counts);
+
+/* Unsigned distances after the last step's redistancing, before advection
+ * (the DT array of redistance3d::GenerateData, src/core3d/cuda/redistance3d.cu:225-273),
+ * in the caller's vertex order; LARGENUM where the FIM did not reach. */
+int ls_get_distances(ls_ctx* ctx, double* out);
+
+/* Learned warm start (no reference counterpart; tetrahedral meshes). A
+ * 22-64-64-4 ReLU MLP per element copy: inputs are the four corner phi values
+ * and the six edge vectors in the copy's local frame, divided by |AB|;
+ * outputs y_i = (D_i - |phi_i|) / |AB| with D_i the redistanced distance.
+ * Inference and training run BF16 MFMA with FP32 accumulation. Parameters
+ * are a flat FP32 vector: W1 (22x64, by input), b1 (64), W2 (64x64), b2,
+ * W3 (64x4), b3 (4). */
+#define LS_MLP_PARAMS 5892
+int ls_mlp_set_params(ls_ctx* ctx, const float* params, int n);
+int ls_mlp_get_params(ls_ctx* ctx, float* params, int n);
+/* Warm values for the next ls_step with warm = LS_WARM_LEARNED: per vertex
+ * of the current narrowband, the mean predicted distance over its copies. */
+int ls_mlp_predict(ls_ctx* ctx);
+/* Training pairs the method generates: phase 0 before ls_step records the
+ * inputs of every copy of the current narrowband, phase 1 after it records
+ * their targets from the step's distances. The first call sizes the device
+ * dataset to `capacity` pairs; pairs beyond it are dropped. */
+int ls_mlp_collect(ls_ctx* ctx, int phase, int64_t capacity);
+/* Collected pairs: *n always; X (n x 22) and Y (n x 4) when not NULL. */
+int ls_mlp_dataset(ls_ctx* ctx, float* X, float* Y, int64_t* n);
+/* Sum of squared errors over n caller samples (X n x 22, Y n x 4) and the
+ * gradient of their mean (sum over the 4 outputs, mean over samples). */
+int ls_mlp_gradient(ls_ctx* ctx, const float* X, const float* Y, int64_t n, float* grad, double* loss);
+/* Adam on the GPU over the collected pairs (shuffled each epoch with seed);
+ * *loss is the last epoch's mean squared error per output. */
+int ls_mlp_train(ls_ctx* ctx, int epochs, int batch, double lr, uint64_t seed, double* loss);
 
 #ifdef __cplusplus
 }

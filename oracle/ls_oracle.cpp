@@ -236,6 +236,18 @@ struct TetGeom {
     v[1] = (T)std::fmin(TB, tet_solve(TA, TD, TC, ACAC, -ACCD, -ACBC, CDCD, BCCD, BCBC, leps));
     v[2] = (T)std::fmin(TC, tet_solve(TA, TB, TD, ADAD, ADBD, -ADCD, BDBD, -CDBD, CDCD, leps));
   }
+  // Corner k's value from the other three corners alone (the same solve as
+  // update(), without the min with corner k's own value).
+  template <typename T>
+  T corner_value(const T* v, int k, double leps) const {
+    const double TA = v[0], TB = v[1], TC = v[2], TD = v[3];
+    double r;
+    if (k == 3) r = tet_solve(TA, TB, TC, ACAC, ACBC, ACCD, BCBC, BCCD, CDCD, leps);
+    else if (k == 0) r = tet_solve(TB, TD, TC, BCBC, -BCCD, -ACBC, CDCD, ACCD, ACAC, leps);
+    else if (k == 1) r = tet_solve(TA, TD, TC, ACAC, -ACCD, -ACBC, CDCD, BCCD, BCBC, leps);
+    else r = tet_solve(TA, TB, TD, ADAD, ADBD, -ADCD, BDBD, -CDBD, CDCD, leps);
+    return (T)std::fmin(kLarge, r);
+  }
 };
 
 // Triangle edge lengths from the three local coordinates and the three
@@ -256,6 +268,15 @@ struct TriGeom {
     v[0] = (T)std::fmin(TA, tri_solve(TB, TC, LenBC, LenCA, LenAB));
     v[1] = (T)std::fmin(TB, tri_solve(TC, TA, LenCA, LenAB, LenBC));
   }
+  template <typename T>
+  T corner_value(const T* v, int k, double) const {
+    const double TA = v[0], TB = v[1], TC = v[2];
+    double r;
+    if (k == 2) r = tri_solve(TA, TB, LenAB, LenBC, LenCA);
+    else if (k == 0) r = tri_solve(TB, TC, LenBC, LenCA, LenAB);
+    else r = tri_solve(TC, TA, LenCA, LenAB, LenBC);
+    return (T)std::fmin(kLarge, r);
+  }
 };
 
 }  // namespace
@@ -271,7 +292,9 @@ struct lso_params {
   int mode;             // 0: advection, 1: mean-curvature flow
   int nthreads;         // OpenMP threads (0 = runtime default)
   int max_fim_iters;    // 0: FIM to convergence; > 0: stop after that many iterations
-  int pad;
+  int warm;             // 0: plain; 1: warm start from |phi|; 2: from the values given by set_warm
+  double warm_margin;   // warm value = prediction * (1 + warm_margin)
+  double warm_res_tol;  // residual above which a warm step re-runs plain (0: tol)
 };
 struct lso_stats {
   int fim_iters;
@@ -280,6 +303,9 @@ struct lso_stats {
   int64_t probes;
   int64_t nb_verts;
   int n_seed_patches;
+  int warm_patches;
+  double warm_residual;
+  int warm_fallback;
   int pad;
 };
 }
@@ -300,6 +326,7 @@ struct Base {
   virtual void get_coords(double*) const = 0;
   virtual void advect_all(double dt) = 0;
   virtual int nparts() const = 0;
+  virtual void set_warm(const double*) = 0;
 };
 
 template <typename T>
@@ -859,15 +886,103 @@ struct Solver : Base {
     }
   }
 
+  // Warm start (no reference counterpart): over the previous narrowband,
+  // every non-seed vertex whose warm value prediction * (1 + margin) is below
+  // LARGENUM/2 starts the FIM there instead of at LARGENUM, and every patch
+  // receiving one joins the first active list. The prediction is |phi|
+  // (warm 1) or the values given by set_warm (warm 2).
+  std::vector<double> warm;  // renumbered order, for warm 2
+  void set_warm(const double* w) override {
+    if (!w) { warm.clear(); return; }
+    warm.resize(N);
+    for (int64_t v = 0; v < N; ++v) warm[vert_ipermute[v]] = w[v];
+  }
+  int apply_warm(const std::vector<int32_t>& prev_nb, const lso_params& prm, std::vector<int32_t>& act) {
+    std::vector<char> in(P, 0);
+    for (int32_t p : act) in[p] = 1;
+    int npatch = 0;
+    for (int32_t p : prev_nb) {
+      bool any = false;
+      for (int32_t v = vert_offsets[p]; v < vert_offsets[p + 1]; ++v) {
+        if (label[v] == SeedPoint) continue;
+        const double base = prm.warm == 2 ? (double)(T)warm[v] : std::fabs((double)phi[v]);
+        const double w = base * (1.0 + prm.warm_margin);
+        if (w < 0.5 * kLarge) { DT[v] = (T)w; any = true; }
+      }
+      if (any) { npatch++; in[p] = 1; }
+    }
+    act.clear();
+    for (int p = 0; p < P; ++p) if (in[p]) act.push_back(p);
+    return npatch;
+  }
+
+  // Fixed-point residual over the band: max over non-seed vertices with
+  // DT <= band of |DT_v - F_v|, F_v the least value any element copy gives v
+  // from its other corners.
+  float residual(const lso_params& prm) {
+    const T bw = (T)prm.bandwidth;
+    float r = 0.f;
+    for (int p = 0; p < P; ++p) {
+      const int32_t vs = vert_offsets[p], ve = vert_offsets[p + 1];
+      T mn = (T)kLarge;
+      for (int32_t v = vs; v < ve; ++v) mn = std::fmin(mn, DT[v]);
+      if (!(mn <= bw)) continue;
+      std::vector<T> F(ve - vs, (T)kLarge);
+      std::vector<char> near_seed(ve - vs, 0);
+      for (int32_t c = ele_offsets[p]; c < ele_offsets[p + 1]; ++c) {
+        T val[4], L[6];
+        int32_t vid[4];
+        bool seeded = false;
+        for (int i = 0; i < nc; ++i) {
+          vid[i] = ele[i * F_of() + c];
+          val[i] = DT[vid[i]];
+          seeded = seeded || label[vid[i]] == SeedPoint;
+        }
+        if (seeded) {
+          for (int i = 0; i < nc; ++i)
+            if (vid[i] >= vs && vid[i] < ve) near_seed[vid[i] - vs] = 1;
+          continue;
+        }
+        for (int k = 0; k < (dim == 3 ? 6 : 3); ++k) L[k] = coords[k * F_of() + c];
+        for (int k = 0; k < nc; ++k) {
+          const int32_t v = vid[k];
+          if (v < vs || v >= ve || !(val[k] <= bw) || label[v] == SeedPoint) continue;
+          const T f = dim == 3 ? TetGeom(L).corner_value(val, k, prm.lambda_eps)
+                               : TriGeom(L).corner_value(val, k, prm.lambda_eps);
+          F[v - vs] = std::fmin(F[v - vs], f);
+        }
+      }
+      for (int32_t v = vs; v < ve; ++v) {
+        if (!(DT[v] <= bw) || label[v] == SeedPoint || near_seed[v - vs]) continue;
+        r = std::fmax(r, (float)std::fabs((double)DT[v] - (double)F[v - vs]));
+      }
+    }
+    return r;
+  }
+  int64_t F_of() const { return F; }
+
   // One timestep (meshFIM3d::GenerateData loop body, meshFIM3d.cu:639-678).
   int step(const lso_params& prm, lso_stats& st) override {
     if (prm.nthreads > 0) omp_set_num_threads(prm.nthreads);
     std::memset(&st, 0, sizeof(st));
     std::fill(amb.begin(), amb.end(), 0);
+    const std::vector<int32_t> prev_nb = narrowband;
     std::vector<int32_t> act = find_seeds();
     st.n_seed_patches = (int)act.size();
     for (int64_t v = 0; v < N; ++v) tsign[v] = (phi[v] < (T)0) ? 0 : 1;  // kernel_reinit_Tsign
+    const bool use_warm = prm.warm != 0 && !prev_nb.empty();
+    if (use_warm) st.warm_patches = apply_warm(prev_nb, prm, act);
     fim(act, prm, st);
+    if (use_warm) {
+      const float r = residual(prm);
+      st.warm_residual = r;
+      const float rt = (float)(prm.warm_res_tol > 0 ? prm.warm_res_tol : prm.tol);
+      if (!(r <= rt)) {  // the warm start is rejected: the plain method from the same phi
+        st.warm_fallback = 1;
+        act = find_seeds();
+        fim(act, prm, st);
+      }
+    }
     narrowband_and_restore(prm, st);
     if (narrowband.empty()) return 0;
     for (int it = 0; it < prm.inside_niter; ++it) {
@@ -906,6 +1021,8 @@ void lso_set_narrowband(void* h, const int32_t* ids, int n) { static_cast<Base*>
 void lso_get_ambiguous(void* h, uint8_t* out) { static_cast<Base*>(h)->get_amb(out); }
 // Advection alone over all patches (tests of the advection scheme).
 void lso_advect_all(void* h, double dt) { static_cast<Base*>(h)->advect_all(dt); }
+// Warm-start distances for the next step only (caller order; >= LARGENUM/2: none).
+void lso_set_warm(void* h, const double* warm) { static_cast<Base*>(h)->set_warm(warm); }
 // Per-copy local coordinates in the storage type, copy-major.
 void lso_get_coords(void* h, double* out) { static_cast<Base*>(h)->get_coords(out); }
 void lso_get_layout(void* h, int32_t* vert_perm, int32_t* ele_perm, int32_t* vert_off,

@@ -306,10 +306,10 @@ def mcf_circle(n: int = 33, r0: float = 0.3, block=(8, 8), steps: int = 40) -> C
                 mode=1, niter=10, meta={"r0": r0})
 
 
-def config3(n: int = 192, seed: int = 3, block=(8, 8, 8), steps: int = 50) -> Case:
+def config3(n: int = 192, seed: int = 3, block=(8, 8, 4), steps: int = 50) -> Case:
     """Mean-curvature flow of 8 spheres: n^3 Kuhn grid, jitter 0.05h, centres
     U[0.25,0.75]^3, radii U[0.08,0.15], phi0 = min of the sphere distances,
-    band 8h, dt = h^2/8 (explicitly stable), 512-vertex block patches."""
+    band 8h, dt = h^2/8 (explicitly stable), 256-vertex block patches."""
     xyz, cells, h = kuhn_grid(n)
     xyz = jitter_interior(xyz, h, 0.05, seed, 3)
     rng = np.random.default_rng(seed)

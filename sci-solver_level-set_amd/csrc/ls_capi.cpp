@@ -389,4 +389,83 @@ int ls_get_narrowband(ls_ctx* c, int32_t* ids, int32_t* n) {
   return LS_OK;
 }
 
+// Learned warm start: network parameters, training pairs, training,
+// gradients and prediction (the engine's tetrahedral path).
+namespace {
+int engine_ready(ls_ctx* c) {
+  if (!c->have_layout) return fail(c, LS_ERR_STATE, "partition the mesh first");
+  if (!c->engine) return fail(c, LS_ERR_STATE, "host-only context (device < 0) has no device state");
+  return LS_OK;
+}
+}  // namespace
+
+int ls_mlp_set_params(ls_ctx* c, const float* params, int n) {
+  if (!c || !params) return LS_ERR_ARG;
+  if (n != LS_MLP_PARAMS) return fail(c, LS_ERR_ARG, "expected LS_MLP_PARAMS parameters");
+  if (int rc = engine_ready(c)) return rc;
+  std::string err;
+  const int rc = c->engine->mlp_set_params(params, err);
+  return rc == LS_OK ? rc : fail(c, rc, err);
+}
+
+int ls_mlp_get_params(ls_ctx* c, float* params, int n) {
+  if (!c || !params) return LS_ERR_ARG;
+  if (n != LS_MLP_PARAMS) return fail(c, LS_ERR_ARG, "expected LS_MLP_PARAMS parameters");
+  if (int rc = engine_ready(c)) return rc;
+  std::string err;
+  const int rc = c->engine->mlp_get_params(params, err);
+  return rc == LS_OK ? rc : fail(c, rc, err);
+}
+
+int ls_mlp_predict(ls_ctx* c) {
+  if (!c) return LS_ERR_ARG;
+  if (int rc = engine_ready(c)) return rc;
+  std::string err;
+  const int rc = c->engine->mlp_predict(err);
+  return rc == LS_OK ? rc : fail(c, rc, err);
+}
+
+int ls_mlp_collect(ls_ctx* c, int phase, int64_t capacity) {
+  if (!c) return LS_ERR_ARG;
+  if (phase != 0 && phase != 1) return fail(c, LS_ERR_ARG, "phase must be 0 (before a step) or 1 (after it)");
+  if (int rc = engine_ready(c)) return rc;
+  std::string err;
+  const int rc = c->engine->mlp_collect(phase, capacity, err);
+  return rc == LS_OK ? rc : fail(c, rc, err);
+}
+
+int ls_mlp_dataset(ls_ctx* c, float* X, float* Y, int64_t* n) {
+  if (!c || !n) return LS_ERR_ARG;
+  if (int rc = engine_ready(c)) return rc;
+  *n = c->engine->mlp_dataset_size();
+  if (!X && !Y) return LS_OK;
+  std::string err;
+  const int rc = c->engine->mlp_get_dataset(X, Y, err);
+  return rc == LS_OK ? rc : fail(c, rc, err);
+}
+
+int ls_mlp_gradient(ls_ctx* c, const float* X, const float* Y, int64_t n, float* grad, double* loss) {
+  if (!c || !X || !Y || !grad || !loss) return LS_ERR_ARG;
+  if (int rc = engine_ready(c)) return rc;
+  std::string err;
+  const int rc = c->engine->mlp_gradient(X, Y, n, grad, loss, err);
+  return rc == LS_OK ? rc : fail(c, rc, err);
+}
+
+int ls_mlp_train(ls_ctx* c, int epochs, int batch, double lr, uint64_t seed, double* loss) {
+  if (!c || !loss) return LS_ERR_ARG;
+  if (int rc = engine_ready(c)) return rc;
+  std::string err;
+  const int rc = c->engine->mlp_train(epochs, batch, lr, seed, loss, err);
+  return rc == LS_OK ? rc : fail(c, rc, err);
+}
+
+int ls_get_distances(ls_ctx* c, double* out) {
+  if (!c || !out) return LS_ERR_ARG;
+  if (int rc = engine_ready(c)) return rc;
+  std::string err;
+  const int rc = c->engine->get_distances(out, err);
+  return rc == LS_OK ? rc : fail(c, rc, err);
+}
+
 }  // extern "C"

@@ -70,6 +70,7 @@ struct DevState {
   int32_t* pc;         // kNumPc persistent-step counters
   unsigned long long* ticks;  // kNumTicks phase times (device wall clock)
   unsigned int* bar;          // grid barrier: [0] arrivals, [1] generation
+  const T* warm;              // N warm-start distances (learned predictor), or null
 };
 
 enum TickIndex : int { kTkSeed = 0, kTkVisit, kTkCopyback, kTkProbe, kTkNext, kTkNb, kTkAdvect, kNumTicks = 8 };
@@ -85,6 +86,9 @@ enum PcIndex : int {
   kPcVisits = 9,   // patch visits
   kPcProbes = 10,  // probes
   kPcOverflow = 11,// FIM iteration cap hit
+  kPcResid = 12,   // warm start: max fixed-point residual over band vertices (float bits)
+  kPcFallback = 13,// warm start: the residual check failed and the step re-ran plain
+  kPcWarm = 14,    // warm start: patches that received warm values
   kNumPc = 16
 };
 

@@ -26,6 +26,17 @@ class Engine {
   virtual int get_phi(double* phi_user, std::string& err) = 0;
   virtual int get_narrowband(std::vector<int32_t>& ids, std::string& err) = 0;
   virtual int working_bytes() const = 0;  // sizeof working real type
+  // Learned warm start (tetrahedral engines).
+  virtual int mlp_set_params(const float* prm, std::string& err) = 0;
+  virtual int mlp_get_params(float* prm, std::string& err) = 0;
+  virtual int mlp_predict(std::string& err) = 0;
+  virtual int mlp_collect(int phase, int64_t capacity, std::string& err) = 0;
+  virtual int64_t mlp_dataset_size() const = 0;
+  virtual int mlp_get_dataset(float* X, float* Y, std::string& err) = 0;
+  virtual int mlp_gradient(const float* X, const float* Y, int64_t n, float* grad, double* loss,
+                           std::string& err) = 0;
+  virtual int mlp_train(int epochs, int batch, double lr, uint64_t seed, double* loss, std::string& err) = 0;
+  virtual int get_distances(double* out_user, std::string& err) = 0;
   virtual void set_exec(int mode) = 0;
 };
 

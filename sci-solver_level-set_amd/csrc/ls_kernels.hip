@@ -14,12 +14,14 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <vector>
 
 #include "../../include/levelset_hip.h"
 #include "ls_device.h"
 #include "ls_engine.h"
+#include "ls_mlp.h"
 
 namespace lsamd {
 namespace kern {
@@ -208,6 +210,39 @@ __device__ __forceinline__ bool must_solve(double Tk, double mo, double Mo, doub
   return !(mo + hk - margin > Tk);
 }
 
+// Edge dot products of a tet from its six local coordinates
+// (redistance_kernels3d.h:381-396) and the permuted local solve for each
+// corner (:411-414), all in FP64.
+struct TetDots {
+  double ACAC, ACBC, ACCD, BCBC, BCCD, CDCD, ADAD, ADBD, ADCD, BDBD, CDBD;
+  __device__ explicit TetDots(const double* L) {
+    ACBC = L[1] * (L[1] - L[0]) + L[2] * L[2];
+    BCCD = (L[1] - L[0]) * (L[3] - L[1]) + L[2] * (L[4] - L[2]);
+    ACCD = L[1] * (L[3] - L[1]) + L[2] * (L[4] - L[2]);
+    ADBD = L[3] * (L[3] - L[0]) + L[4] * L[4] + L[5] * L[5];
+    const double ACAD = L[1] * L[3] + L[2] * L[4];
+    const double BCBD = (L[1] - L[0]) * (L[3] - L[0]) + L[2] * L[4];
+    const double ADBC = ACBC + BCCD;
+    const double ACBD = ACBC + ACCD;
+    CDBD = ADBD - ACBD;
+    ADCD = ADBD - ADBC;
+    CDCD = ADCD - ACCD;
+    ADAD = ADCD + ACAD;
+    ACAC = ACAD - ACCD;
+    BDBD = BCBD + CDBD;
+    BCBC = BCBD - BCCD;
+  }
+  __device__ double solve(int k, const double* Tv, double leps) const {
+    const double TA = Tv[0], TB = Tv[1], TC = Tv[2], TD = Tv[3];
+    switch (k) {
+      case 3: return tet_solve(TA, TB, TC, ACAC, ACBC, ACCD, BCBC, BCCD, CDCD, leps);
+      case 0: return tet_solve(TB, TD, TC, BCBC, -BCCD, -ACBC, CDCD, ACCD, ACAC, leps);
+      case 1: return tet_solve(TA, TD, TC, ACAC, -ACCD, -ACBC, CDCD, BCCD, BCBC, leps);
+      default: return tet_solve(TA, TB, TD, ADAD, ADBD, -ADCD, BDBD, -CDBD, CDCD, leps);
+    }
+  }
+};
+
 template <typename T>
 struct ElemSolve<T, 4> {
   static constexpr int kCoords = 6;
@@ -237,27 +272,23 @@ struct ElemSolve<T, 4> {
       need[k] = must_solve(Tv[k], mo, Mo, h[k], scale);
     }
     if (!(need[0] || need[1] || need[2] || need[3])) return 0;
-    const double ACBC = L[1] * (L[1] - L[0]) + L[2] * L[2];
-    const double BCCD = (L[1] - L[0]) * (L[3] - L[1]) + L[2] * (L[4] - L[2]);
-    const double ACCD = L[1] * (L[3] - L[1]) + L[2] * (L[4] - L[2]);
-    const double ADBD = L[3] * (L[3] - L[0]) + L[4] * L[4] + L[5] * L[5];
-    const double ACAD = L[1] * L[3] + L[2] * L[4];
-    const double BCBD = (L[1] - L[0]) * (L[3] - L[0]) + L[2] * L[4];
-    const double ADBC = ACBC + BCCD;
-    const double ACBD = ACBC + ACCD;
-    const double CDBD = ADBD - ACBD;
-    const double ADCD = ADBD - ADBC;
-    const double CDCD = ADCD - ACCD;
-    const double ADAD = ADCD + ACAD;
-    const double ACAC = ACAD - ACCD;
-    const double BDBD = BCBD + CDBD;
-    const double BCBC = BCBD - BCCD;
+    const TetDots d(L);
     const double TA = Tv[0], TB = Tv[1], TC = Tv[2], TD = Tv[3];
-    if (need[3]) v[3] = (T)d_min(TD, tet_solve(TA, TB, TC, ACAC, ACBC, ACCD, BCBC, BCCD, CDCD, leps));
-    if (need[0]) v[0] = (T)d_min(TA, tet_solve(TB, TD, TC, BCBC, -BCCD, -ACBC, CDCD, ACCD, ACAC, leps));
-    if (need[1]) v[1] = (T)d_min(TB, tet_solve(TA, TD, TC, ACAC, -ACCD, -ACBC, CDCD, BCCD, BCBC, leps));
-    if (need[2]) v[2] = (T)d_min(TC, tet_solve(TA, TB, TD, ADAD, ADBD, -ADCD, BDBD, -CDBD, CDCD, leps));
+    if (need[3]) v[3] = (T)d_min(TD, d.solve(3, Tv, leps));
+    if (need[0]) v[0] = (T)d_min(TA, d.solve(0, Tv, leps));
+    if (need[1]) v[1] = (T)d_min(TB, d.solve(1, Tv, leps));
+    if (need[2]) v[2] = (T)d_min(TC, d.solve(2, Tv, leps));
     return (int)need[0] + (int)need[1] + (int)need[2] + (int)need[3];
+  }
+  // The value the element gives corner k from the other three (no min with
+  // corner k's own value, no skip): what update() stores when it solves k.
+  __device__ static T corner_value(const T* Lt, const T* v, int k, double leps) {
+    double L[6];
+#pragma unroll
+    for (int j = 0; j < 6; ++j) L[j] = (double)Lt[j];
+    const double Tv[4] = {(double)v[0], (double)v[1], (double)v[2], (double)v[3]};
+    const TetDots d(L);
+    return (T)d_min(kLargeNum, d.solve(k, Tv, leps));
   }
 };
 
@@ -285,6 +316,18 @@ struct ElemSolve<T, 3> {
     if (need[0]) v[0] = (T)d_min(TA, tri_solve(TB, TC, LenBC, LenCA, LenAB));
     if (need[1]) v[1] = (T)d_min(TB, tri_solve(TC, TA, LenCA, LenAB, LenBC));
     return (int)need[0] + (int)need[1] + (int)need[2];
+  }
+  __device__ static T corner_value(const T* ct, const T* v, int k, double) {
+    const double c[3] = {(double)ct[0], (double)ct[1], (double)ct[2]};
+    const double LenAB = d_abs(c[0]);
+    const double LenBC = d_sqrt(c[2] * c[2] + (c[1] - c[0]) * (c[1] - c[0]));
+    const double LenCA = d_sqrt(c[1] * c[1] + c[2] * c[2]);
+    const double TA = (double)v[0], TB = (double)v[1], TC = (double)v[2];
+    double r;
+    if (k == 2) r = tri_solve(TA, TB, LenAB, LenBC, LenCA);
+    else if (k == 0) r = tri_solve(TB, TC, LenBC, LenCA, LenAB);
+    else r = tri_solve(TC, TA, LenCA, LenAB, LenBC);
+    return (T)d_min(kLargeNum, r);
   }
 };
 
@@ -537,6 +580,96 @@ __device__ T narrowband_patch(const DevMesh<T>& m, DevState<T>& s, int p, T* red
     s.phi[v] = d * (T)(2 * (int)s.tsign[v] - 1);
   }
   return block_min_n<T, BS>(mn, red);
+}
+
+// Bit patterns of non-negative floating-point values order as the values,
+// so an integer atomicMin over them is a floating-point min.
+template <typename T>
+struct OrderedBits;
+template <>
+struct OrderedBits<float> {
+  using type = unsigned int;
+  __device__ static type of(float x) { return __float_as_uint(x); }
+  __device__ static float value(type b) { return __uint_as_float(b); }
+};
+template <>
+struct OrderedBits<double> {
+  using type = unsigned long long;
+  __device__ static type of(double x) { return (type)__double_as_longlong(x); }
+  __device__ static double value(type b) { return __longlong_as_double((long long)b); }
+};
+
+// Warm start of one patch of the previous narrowband: every non-seed vertex
+// whose warm value is below LARGENUM/2 starts the FIM there instead of at
+// LARGENUM. The warm value is the predictor's distance times (1 + margin):
+// the learned predictor's values (s.warm) or |phi| (identity predictor).
+// Returns (to every thread) whether the patch received any warm value.
+template <typename T, int BS>
+__device__ bool warm_patch(const DevMesh<T>& m, DevState<T>& s, int p, int mode, double margin) {
+  __syncthreads();
+  const int vs = m.vert_off[p], ve = m.vert_off[p + 1];
+  int any = 0;
+  for (int v = vs + threadIdx.x; v < ve; v += BS) {
+    if (s.label[v] == kSeedPoint) continue;
+    const double base = mode == 2 ? (double)s.warm[v] : d_abs((double)s.phi[v]);
+    const double w = base * (1.0 + margin);
+    if (w < 0.5 * kLargeNum) {
+      s.DT[v] = (T)w;
+      any = 1;
+    }
+  }
+  return __syncthreads_or(any) != 0;
+}
+
+// Fixed-point residual of one patch: for every owned, non-seed vertex with
+// DT <= band, |DT_v - F_v| where F_v is the least value any element copy of
+// v gives it from its other corners (no min with DT_v, no skip). At the
+// scheme's fixed point F_v == DT_v; a warm value left below it (F_v > DT_v)
+// or above it shows up alike. The block maximum is merged into
+// pc[kPcResid] (non-negative floats order as their bit patterns).
+template <typename T, int NC, int BS>
+__device__ void residual_patch(const DevMesh<T>& m, DevState<T>& s, int p, T bw, double leps, unsigned char* smem,
+                               T* red) {
+  __syncthreads();
+  constexpr int K = ElemSolve<T, NC>::kCoords;
+  const int vs = m.vert_off[p], ve = m.vert_off[p + 1];
+  const int es = m.ele_off[p], ee = m.ele_off[p + 1];
+  const int hs = m.halo_off[p], he = m.halo_off[p + 1];
+  const int nv = ve - vs, ne = ee - es, nh = he - hs;
+  T mn = (T)kLargeNum;
+  for (int v = vs + threadIdx.x; v < ve; v += BS) mn = d_min(mn, s.DT[v]);
+  if (!(block_min_n<T, BS>(mn, red) <= bw)) return;  // no band vertex here
+  using Bits = typename OrderedBits<T>::type;
+  T* s_vtx = (T*)smem;
+  Bits* s_F = (Bits*)(s_vtx + nv + nh);
+  const Bits big = OrderedBits<T>::of((T)kLargeNum);
+  for (int i = threadIdx.x; i < nv; i += BS) {
+    s_vtx[i] = s.DT[vs + i];
+    s_F[i] = big;
+  }
+  for (int i = threadIdx.x; i < nh; i += BS) s_vtx[nv + i] = s.DT[m.halo_ids[hs + i]];
+  __syncthreads();
+  for (int e = threadIdx.x; e < ne; e += BS) {
+    const ushort4 c = m.copy_local[es + e];
+    T val[NC];
+#pragma unroll
+    for (int i = 0; i < NC; ++i) val[i] = s_vtx[corner(c, i)];
+    const T* gc = m.coords + (int64_t)(es + e) * K;
+#pragma unroll
+    for (int k = 0; k < NC; ++k) {
+      const int lv = corner(c, k);
+      if (lv >= nv || !(val[k] <= bw) || s.label[vs + lv] == kSeedPoint) continue;
+      atomicMin(&s_F[lv], OrderedBits<T>::of(ElemSolve<T, NC>::corner_value(gc, val, k, leps)));
+    }
+  }
+  __syncthreads();
+  float r = 0.f;
+  for (int v = threadIdx.x; v < nv; v += BS) {
+    if (!(s_vtx[v] <= bw) || s.label[vs + v] == kSeedPoint) continue;
+    r = fmaxf(r, (float)d_abs((double)s_vtx[v] - (double)OrderedBits<T>::value(s_F[v])));
+  }
+  r = -block_min_n<float, BS>(-r, (float*)red);
+  if (threadIdx.x == 0 && r > 0.f) atomicMax((unsigned int*)&s.pc[kPcResid], __float_as_uint(r));
 }
 
 // Shape-function gradients of a copy in its local frame (FP64).
@@ -865,6 +998,338 @@ inline int grid_for(int64_t n) {
 }
 
 // ---------------------------------------------------------------------------
+// Learned warm start: inference over the previous narrowband, training-pair
+// collection, gradients and Adam (ls_mlp.h describes the network).
+
+constexpr int kMlpBlock = 256;  // 4 waves, each 16 samples per tile
+static_assert(mlp::kNumParams == LS_MLP_PARAMS, "parameter count of the C-ABI");
+
+// Patch of element copy c (copies are sorted by patch).
+template <typename T>
+__device__ inline int copy_patch(const DevMesh<T>& m, int64_t c) {
+  int lo = 0, hi = m.P;  // ele_off[lo] <= c < ele_off[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (m.ele_off[mid] <= c) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// Warm values of one narrowband patch: per element copy the predicted
+// corner distances, per owned vertex their mean over its copies.
+template <typename T>
+__global__ __launch_bounds__(kMlpBlock) void k_mlp_predict(DevMesh<T> m, DevState<T> s, const int32_t* nb,
+                                                           mlp::Weights w, T* warm) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int p = nb[blockIdx.x];
+  const int vs = m.vert_off[p], ve = m.vert_off[p + 1];
+  const int es = m.ele_off[p], ee = m.ele_off[p + 1];
+  const int hs = m.halo_off[p], he = m.halo_off[p + 1];
+  const int nv = ve - vs, ne = ee - es, nh = he - hs;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, r = lane & 15;
+  float* tile = (float*)smem + wave * mlp::kWaveTile;
+  float* X = tile;
+  float* H1 = X + 16 * mlp::kInPad;
+  float* H2 = H1 + 16 * mlp::kHid;
+  float* O = H2 + 16 * mlp::kHid;
+  float* s_pred = (float*)smem + (kMlpBlock / 64) * mlp::kWaveTile;  // 4 per copy
+  float* s_vtx = s_pred + 4 * ne;
+  for (int i = threadIdx.x; i < nv; i += kMlpBlock) s_vtx[i] = (float)s.phi[vs + i];
+  for (int i = threadIdx.x; i < nh; i += kMlpBlock) s_vtx[nv + i] = (float)s.phi[m.halo_ids[hs + i]];
+  __syncthreads();
+  const int ntiles = (ne + 15) / 16;
+  for (int t = wave; t < ntiles; t += kMlpBlock / 64) {
+    const int e = 16 * t + r;
+    float ph[4] = {0.f, 0.f, 0.f, 0.f}, sc = 1.f;
+    if (lane < 16) {
+      float x[mlp::kIn];
+      if (e < ne) {
+        const ushort4 c = m.copy_local[es + e];
+        float L[6];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) ph[i] = s_vtx[corner(c, i)];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) L[k] = (float)m.coords[(int64_t)(es + e) * 6 + k];
+        sc = mlp::features(ph, L, x);
+      } else {
+#pragma unroll
+        for (int k = 0; k < mlp::kIn; ++k) x[k] = 0.f;
+      }
+#pragma unroll
+      for (int k = 0; k < mlp::kInPad; ++k) X[r * mlp::kInPad + k] = k < mlp::kIn ? x[k] : 0.f;
+    }
+    mlp::wave_sync();
+    mlp::forward16(w, X, H1, H2, O, lane);
+    if (lane < 16 && e < ne) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const float d = fabsf(ph[i]) + sc * O[r * mlp::kOutPad + i];
+        s_pred[e * 4 + i] = d > 0.f ? d : 0.f;
+      }
+    }
+    mlp::wave_sync();
+  }
+  __syncthreads();
+  for (int v = threadIdx.x; v < nv; v += kMlpBlock) {
+    const int o0 = m.vslot_off[vs + v], o1 = m.vslot_off[vs + v + 1];
+    float acc = 0.f;
+    for (int k = o0; k < o1; ++k) acc += s_pred[m.vslots[k]];
+    warm[vs + v] = o1 > o0 ? (T)(acc / (float)(o1 - o0)) : (T)kLargeNum;
+  }
+}
+
+// Training inputs of the copies listed in `copies`: padded features from
+// the current phi (before the step) into X[i][kInPad].
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_mlp_collect_x(DevMesh<T> m, DevState<T> s, const int32_t* copies,
+                                                          int64_t n, float* X) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const int64_t c = copies[i];
+  // The copy's patch-local corners index its patch's vertices, then halo.
+  const int p = copy_patch(m, c);
+  const int vs = m.vert_off[p], nv = m.vert_off[p + 1] - vs, hs = m.halo_off[p];
+  const ushort4 cl = m.copy_local[c];
+  float ph[4], L[6], x[mlp::kIn];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int lv = corner(cl, k);
+    ph[k] = (float)s.phi[lv < nv ? vs + lv : m.halo_ids[hs + lv - nv]];
+  }
+#pragma unroll
+  for (int k = 0; k < 6; ++k) L[k] = (float)m.coords[c * 6 + k];
+  mlp::features(ph, L, x);
+#pragma unroll
+  for (int k = 0; k < mlp::kInPad; ++k) X[i * mlp::kInPad + k] = k < mlp::kIn ? x[k] : 0.f;
+}
+
+// Training targets after the step: y_k = D_k / s - |x_k| with D the
+// redistanced distances (DT) at the copy's corners.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_mlp_collect_y(DevMesh<T> m, DevState<T> s, const int32_t* copies,
+                                                          int64_t n, const float* X, float* Y) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const int64_t c = copies[i];
+  const int p = copy_patch(m, c);
+  const int vs = m.vert_off[p], nv = m.vert_off[p + 1] - vs, hs = m.halo_off[p];
+  const ushort4 cl = m.copy_local[c];
+  const float inv = 1.0f / (float)m.coords[c * 6];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int lv = corner(cl, k);
+    const float D = (float)s.DT[lv < nv ? vs + lv : m.halo_ids[hs + lv - nv]];
+    Y[i * mlp::kOut + k] = D < 0.5f * (float)kLargeNum ? D * inv - fabsf(X[i * mlp::kInPad + k]) : 0.f;
+  }
+}
+
+// Loss sum (O - Y)^2 over valid samples and its gradient for every parameter
+// (flat layout of ls_mlp.h, scaled by 1/n_total), accumulated with atomics.
+// Samples are rows idx[i] of X/Y when idx is given, else rows i.
+__global__ __launch_bounds__(kMlpBlock) void k_mlp_grad(const float* X, const float* Y, const int32_t* idx,
+                                                        int64_t n, float inv_total, mlp::Weights w, float* grad,
+                                                        float* loss) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, g = lane >> 4, r = lane & 15;
+  float* tile = (float*)smem + wave * mlp::kWaveTile;
+  float* Xs = tile;
+  float* H1 = Xs + 16 * mlp::kInPad;
+  float* H2 = H1 + 16 * mlp::kHid;
+  float* O = H2 + 16 * mlp::kHid;
+  float* red = (float*)smem + (kMlpBlock / 64) * mlp::kWaveTile;  // kNumParams
+  for (int i = threadIdx.x; i < mlp::kNumParams; i += kMlpBlock) red[i] = 0.f;
+  __syncthreads();
+  mlp::f32x4 a1[2][4], a2[4][4], a3[4];
+  float gb1[4] = {0.f, 0.f, 0.f, 0.f}, gb2[4] = {0.f, 0.f, 0.f, 0.f}, gb3 = 0.f, lsum = 0.f;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    a3[i] = mlp::f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int j = 0; j < 4; ++j) a2[i][j] = mlp::f32x4{0.f, 0.f, 0.f, 0.f};
+    a1[0][i] = a1[1][i] = mlp::f32x4{0.f, 0.f, 0.f, 0.f};
+  }
+  const int64_t ntiles = (n + 15) / 16;
+  for (int64_t t = (int64_t)blockIdx.x * (kMlpBlock / 64) + wave; t < ntiles; t += (int64_t)gridDim.x * (kMlpBlock / 64)) {
+    const int64_t i = 16 * t + r;
+    const bool ok = i < n;
+    const int64_t row = ok ? (idx ? (int64_t)idx[i] : i) : 0;
+    // Inputs: lane group g copies columns 8g..8g+7 of the sample's row.
+#pragma unroll
+    for (int k = 0; k < 8; ++k) Xs[r * mlp::kInPad + 8 * g + k] = ok ? X[row * mlp::kInPad + 8 * g + k] : 0.f;
+    mlp::wave_sync();
+    mlp::forward16(w, Xs, H1, H2, O, lane);
+    // dO = 2 (O - Y) / n_total over the 4 outputs of valid samples; the
+    // output tile is overwritten with dO. Lane (g, r) handles O[r][g].
+    {
+      const float y = ok ? Y[row * mlp::kOut + g] : 0.f;
+      const float o = O[r * mlp::kOutPad + g];
+      const float d = ok ? o - y : 0.f;
+      lsum += d * d;
+      mlp::wave_sync();
+      O[r * mlp::kOutPad + g] = 2.f * d * inv_total;
+#pragma unroll
+      for (int k = 4; k < mlp::kOutPad; k += 4) O[r * mlp::kOutPad + g + k] = 0.f;
+    }
+    mlp::wave_sync();
+    // db3, dW3 = H2^T dO, dH2 = (dO W3^T) * [H2 > 0].
+    if (lane < 16) {
+      float acc = 0.f;
+      for (int mm = 0; mm < 16; ++mm) acc += O[mm * mlp::kOutPad + r];
+      gb3 += acc;
+    }
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt) {
+      const mlp::bf16x4 A = mlp::pack4(H2[(4 * g) * mlp::kHid + 16 * mt + r], H2[(4 * g + 1) * mlp::kHid + 16 * mt + r],
+                                       H2[(4 * g + 2) * mlp::kHid + 16 * mt + r], H2[(4 * g + 3) * mlp::kHid + 16 * mt + r]);
+      const mlp::bf16x4 B = mlp::pack4(O[(4 * g) * mlp::kOutPad + r], O[(4 * g + 1) * mlp::kOutPad + r],
+                                       O[(4 * g + 2) * mlp::kOutPad + r], O[(4 * g + 3) * mlp::kOutPad + r]);
+      a3[mt] = mlp::mfma(A, B, a3[mt]);
+    }
+    {
+      const mlp::bf16x4 A = mlp::load_a(O, mlp::kOutPad, 0, lane);
+#pragma unroll
+      for (int nt = 0; nt < 4; ++nt) {
+        const uint16_t* p = w.w3 + (16 * nt + r) * mlp::kOutPad + 4 * g;
+        mlp::bf16x4 B;
+        B[0] = (short)p[0]; B[1] = (short)p[1]; B[2] = (short)p[2]; B[3] = (short)p[3];
+        const mlp::f32x4 d = mlp::mfma(A, B, mlp::f32x4{0.f, 0.f, 0.f, 0.f});
+        mlp::wave_sync();
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          float* h = &H2[(4 * g + j) * mlp::kHid + 16 * nt + r];
+          *h = *h > 0.f ? d[j] : 0.f;
+        }
+      }
+    }
+    mlp::wave_sync();
+    // db2, dW2 = H1^T dH2, dH1 = (dH2 W2^T) * [H1 > 0].
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      for (int mm = 0; mm < 4; ++mm) gb2[q] += H2[(4 * g + mm) * mlp::kHid + 16 * q + r];
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt) {
+      const mlp::bf16x4 A = mlp::pack4(H1[(4 * g) * mlp::kHid + 16 * mt + r], H1[(4 * g + 1) * mlp::kHid + 16 * mt + r],
+                                       H1[(4 * g + 2) * mlp::kHid + 16 * mt + r], H1[(4 * g + 3) * mlp::kHid + 16 * mt + r]);
+#pragma unroll
+      for (int nt = 0; nt < 4; ++nt) {
+        const mlp::bf16x4 B = mlp::pack4(H2[(4 * g) * mlp::kHid + 16 * nt + r], H2[(4 * g + 1) * mlp::kHid + 16 * nt + r],
+                                         H2[(4 * g + 2) * mlp::kHid + 16 * nt + r], H2[(4 * g + 3) * mlp::kHid + 16 * nt + r]);
+        a2[mt][nt] = mlp::mfma(A, B, a2[mt][nt]);
+      }
+    }
+    {
+      mlp::f32x4 d[4];
+#pragma unroll
+      for (int nt = 0; nt < 4; ++nt) {
+        d[nt] = mlp::f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int ks = 0; ks < 4; ++ks) {
+          const uint16_t* p = w.w2 + (16 * nt + r) * mlp::kHid + 16 * ks + 4 * g;
+          mlp::bf16x4 B;
+          B[0] = (short)p[0]; B[1] = (short)p[1]; B[2] = (short)p[2]; B[3] = (short)p[3];
+          d[nt] = mlp::mfma(mlp::load_a(H2, mlp::kHid, ks, lane), B, d[nt]);
+        }
+      }
+      mlp::wave_sync();
+#pragma unroll
+      for (int nt = 0; nt < 4; ++nt)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          float* h = &H1[(4 * g + j) * mlp::kHid + 16 * nt + r];
+          *h = *h > 0.f ? d[nt][j] : 0.f;
+        }
+    }
+    mlp::wave_sync();
+    // db1, dW1 = X^T dH1.
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+      for (int mm = 0; mm < 4; ++mm) gb1[q] += H1[(4 * g + mm) * mlp::kHid + 16 * q + r];
+#pragma unroll
+    for (int mt = 0; mt < 2; ++mt) {
+      const mlp::bf16x4 A = mlp::pack4(Xs[(4 * g) * mlp::kInPad + 16 * mt + r], Xs[(4 * g + 1) * mlp::kInPad + 16 * mt + r],
+                                       Xs[(4 * g + 2) * mlp::kInPad + 16 * mt + r], Xs[(4 * g + 3) * mlp::kInPad + 16 * mt + r]);
+#pragma unroll
+      for (int nt = 0; nt < 4; ++nt) {
+        const mlp::bf16x4 B = mlp::pack4(H1[(4 * g) * mlp::kHid + 16 * nt + r], H1[(4 * g + 1) * mlp::kHid + 16 * nt + r],
+                                         H1[(4 * g + 2) * mlp::kHid + 16 * nt + r], H1[(4 * g + 3) * mlp::kHid + 16 * nt + r]);
+        a1[mt][nt] = mlp::mfma(A, B, a1[mt][nt]);
+      }
+    }
+    mlp::wave_sync();
+  }
+  // Merge the wave accumulators in LDS, then into global memory.
+#pragma unroll
+  for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int in = 16 * mt + 4 * g + j, hid = 16 * nt + r;
+        if (in < mlp::kIn) atomicAdd(&red[mlp::kOffW1 + in * mlp::kHid + hid], a1[mt][nt][j]);
+      }
+#pragma unroll
+  for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+    for (int nt = 0; nt < 4; ++nt)
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        atomicAdd(&red[mlp::kOffW2 + (16 * mt + 4 * g + j) * mlp::kHid + 16 * nt + r], a2[mt][nt][j]);
+#pragma unroll
+  for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      if (r < mlp::kOut) atomicAdd(&red[mlp::kOffW3 + (16 * mt + 4 * g + j) * mlp::kOut + r], a3[mt][j]);
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    atomicAdd(&red[mlp::kOffB1 + 16 * q + r], gb1[q]);
+    atomicAdd(&red[mlp::kOffB2 + 16 * q + r], gb2[q]);
+  }
+  if (lane < mlp::kOut) atomicAdd(&red[mlp::kOffB3 + lane], gb3);
+  __syncthreads();
+  for (int i = threadIdx.x; i < mlp::kNumParams; i += kMlpBlock)
+    if (red[i] != 0.f) atomicAdd(&grad[i], red[i]);
+  // Loss: lanes hold partial sums of squared errors.
+  float l = lsum;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) l += __shfl_xor(l, off, 64);
+  if (lane == 0 && l != 0.f) atomicAdd(loss, l);
+}
+
+// Adam step on the FP32 parameters, then the BF16/padded copies the MFMA
+// kernels read.
+__global__ void k_mlp_adam(float* prm, float* mom, float* vel, const float* grad, int n, float lr, float beta1,
+                           float beta2, float eps, float c1, float c2) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float gr = grad[i];
+  const float mo = beta1 * mom[i] + (1.f - beta1) * gr;
+  const float ve = beta2 * vel[i] + (1.f - beta2) * gr * gr;
+  mom[i] = mo;
+  vel[i] = ve;
+  prm[i] -= lr * (mo / c1) / (sqrtf(ve / c2) + eps);
+}
+
+__global__ void k_mlp_pack(const float* prm, uint16_t* w1, uint16_t* w2, uint16_t* w3, float* b1, float* b2,
+                           float* b3) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < mlp::kInPad * mlp::kHid) {
+    const int in = i / mlp::kHid, h = i % mlp::kHid;
+    w1[i] = in < mlp::kIn ? mlp::to_bf16(prm[mlp::kOffW1 + in * mlp::kHid + h]) : (uint16_t)0;
+  }
+  if (i < mlp::kHid * mlp::kHid) w2[i] = mlp::to_bf16(prm[mlp::kOffW2 + i]);
+  if (i < mlp::kHid * mlp::kOutPad) {
+    const int h = i / mlp::kOutPad, o = i % mlp::kOutPad;
+    w3[i] = o < mlp::kOut ? mlp::to_bf16(prm[mlp::kOffW3 + h * mlp::kOut + o]) : (uint16_t)0;
+  }
+  if (i < mlp::kHid) {
+    b1[i] = prm[mlp::kOffB1 + i];
+    b2[i] = prm[mlp::kOffB2 + i];
+  }
+  if (i < mlp::kOutPad) b3[i] = i < mlp::kOut ? prm[mlp::kOffB3 + i] : 0.f;
+}
+
+// ---------------------------------------------------------------------------
 // Persistent path: one cooperative launch per timestep. Phases are separated
 // by grid barriers; the FIM active-list loop runs on the device, so a step
 // needs no host round trip.
@@ -881,6 +1346,9 @@ struct StepArgs {
   double leps;
   double dt;
   int mode;          // LS_MODE_ADVECT or LS_MODE_MCF
+  int warm;          // 0: plain; 1: warm start from |phi|; 2: from s.warm
+  double warm_margin;// warm value = prediction * (1 + warm_margin)
+  float res_tol;     // residual above which a warm start falls back
 };
 
 // Counters are read right after a grid barrier, whose acquire fence already
@@ -935,18 +1403,30 @@ __global__ __launch_bounds__(BS) void k_step(DevMesh<T> m, DevState<T> s, StepAr
   pcl.start(clk);
   unsigned long long tk[kNumTicks] = {0, 0, 0, 0, 0, 0, 0, 0};
 
+  // Phases 0-2 run once, or twice when a warm start fails its residual
+  // check: the second attempt is the plain method from the same phi.
+  for (int attempt = 0; attempt < 2; ++attempt) {
+    const bool warm = a.warm != 0 && !a.first && attempt == 0;
+
   // Phase 0: vertex reset and sign save; patch marks; counters.
   for (int64_t v = gtid; v < m.N; v += gsize) init_vertex(s, v);
   for (int64_t q = gtid; q < m.P; q += gsize) { s.inA[q] = 0; s.freshE[q] = 0; }
-  if (gtid < kNumSizes) s.sizes[gtid] = 0ull;
+  if (gtid < kNumSizes && attempt == 0) s.sizes[gtid] = 0ull;
   if (gtid == 0) {
     s.pc[kPcSeed] = 0;
     s.pc[kPcFresh] = s.pc[kPcFresh + 1] = 0;
     s.pc[kPcNext] = s.pc[kPcNext + 1] = 0;
     s.pc[kPcNbVerts] = 0;
-    s.pc[kPcIters] = s.pc[kPcVisits] = s.pc[kPcProbes] = 0;
+    if (attempt == 0) {
+      s.pc[kPcIters] = s.pc[kPcVisits] = s.pc[kPcProbes] = 0;
+      s.pc[kPcFallback] = 0;
+    }
     s.pc[kPcOverflow] = 0;
     s.pc[kPcNb + nb_out] = 0;
+    if (attempt == 0) {  // the first attempt's warm-start figures are reported
+      s.pc[kPcResid] = 0;
+      s.pc[kPcWarm] = 0;
+    }
   }
   grid_barrier(s.bar, G);
 
@@ -960,8 +1440,22 @@ __global__ __launch_bounds__(BS) void k_step(DevMesh<T> m, DevState<T> s, StepAr
     }
   }
   grid_barrier(s.bar, G);
+  if (warm) {
+    // Phase 1b: warm values over the previous narrowband; every patch that
+    // receives one joins the first active list.
+    for (int i = b; i < nscan; i += G) {
+      const int p = s.nb[a.nb_in][i];
+      if (warm_patch<T, BS>(m, s, p, a.warm, a.warm_margin) && tid == 0) {
+        atomicAdd(&s.pc[kPcWarm], 1);
+        if (s.inA[p] == 0) {
+          s.inA[p] = 1;
+          s.listA[atomicAdd(&s.pc[kPcSeed], 1)] = p;
+        }
+      }
+    }
+    grid_barrier(s.bar, G);
+  }
   pcl.lap(clk, &tk[kTkSeed]);
-
   // Phase 2: FIM (redistance3d.cu:85-224). Epoch E tags the members of the
   // current list (inA[p] == E) and the patches probed in this iteration
   // (freshE[q] == E).
@@ -1034,6 +1528,18 @@ __global__ __launch_bounds__(BS) void k_step(DevMesh<T> m, DevState<T> s, StepAr
     cur = nxt;
     nxt = t;
     ++it;
+  }
+
+
+  if (!warm) break;
+  // Phase 2b: fixed-point residual over the band; above the tolerance the
+  // warm start is rejected and the step re-runs plain.
+  for (int p = b; p < m.P; p += G) residual_patch<T, NC, BS>(m, s, p, band_of(m, p, a.bw), a.leps, smem, red);
+  grid_barrier(s.bar, G);
+  const float resid = __uint_as_float((unsigned int)ld_acquire(&s.pc[kPcResid]));
+  if (resid <= a.res_tol) break;
+  if (gtid == 0) s.pc[kPcFallback] = 1;
+  grid_barrier(s.bar, G);
   }
 
   // Phase 3: narrowband + sign restore over every patch.
@@ -1110,6 +1616,10 @@ class EngineImpl final : public Engine {
     P_ = L.P; N_ = L.N; F_ = L.F;
     vert_off_h_ = L.vert_off;
     vert_perm_h_ = L.vert_perm;
+    ele_off_h_ = L.ele_off;
+    max_ep_ = L.max_ep;
+    max_vp_ = L.max_vp;
+    max_hp_ = L.max_hp;
     int lds_max = 0;
     LS_HIP(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, device_));
     // Exact per-patch LDS needs.
@@ -1201,6 +1711,224 @@ class EngineImpl final : public Engine {
     return LS_OK;
   }
 
+  // -- learned warm start ------------------------------------------------
+  int mlp_alloc(std::string& err) {
+    if (d_mlp_prm_) return LS_OK;
+    LS_HIP(dev_alloc(&d_mlp_prm_, (size_t)mlp::kNumParams));
+    LS_HIP(dev_alloc(&d_mlp_m_, (size_t)mlp::kNumParams));
+    LS_HIP(dev_alloc(&d_mlp_v_, (size_t)mlp::kNumParams));
+    LS_HIP(dev_alloc(&d_mlp_grad_, (size_t)mlp::kNumParams + 1));  // + loss
+    LS_HIP(dev_alloc(&d_mlp_w1_, (size_t)mlp::kInPad * mlp::kHid));
+    LS_HIP(dev_alloc(&d_mlp_w2_, (size_t)mlp::kHid * mlp::kHid));
+    LS_HIP(dev_alloc(&d_mlp_w3_, (size_t)mlp::kHid * mlp::kOutPad));
+    LS_HIP(dev_alloc(&d_mlp_b_, (size_t)(2 * mlp::kHid + mlp::kOutPad)));
+    LS_HIP(hipMemset(d_mlp_m_, 0, mlp::kNumParams * sizeof(float)));
+    LS_HIP(hipMemset(d_mlp_v_, 0, mlp::kNumParams * sizeof(float)));
+    return LS_OK;
+  }
+  mlp::Weights mlp_weights() const {
+    return mlp::Weights{d_mlp_w1_, d_mlp_w2_, d_mlp_w3_, d_mlp_b_, d_mlp_b_ + mlp::kHid, d_mlp_b_ + 2 * mlp::kHid};
+  }
+  int mlp_pack(hipStream_t q, std::string& err) {
+    k_mlp_pack<<<(mlp::kHid * mlp::kHid + 255) / 256, 256, 0, q>>>(d_mlp_prm_, d_mlp_w1_, d_mlp_w2_, d_mlp_w3_, d_mlp_b_,
+                                                                 d_mlp_b_ + mlp::kHid, d_mlp_b_ + 2 * mlp::kHid);
+    LS_HIP(hipGetLastError());
+    return LS_OK;
+  }
+  int mlp_tet_only(std::string& err) const {
+    if (NC != 4) { err = "the learned warm start supports tetrahedral meshes"; return LS_ERR_ARG; }
+    return LS_OK;
+  }
+  int mlp_set_params(const float* prm, std::string& err) override {
+    if (int rc = mlp_tet_only(err)) return rc;
+    if (!uploaded_) { err = "layout not built"; return LS_ERR_STATE; }
+    if (int rc = mlp_alloc(err)) return rc;
+    LS_HIP(hipMemcpy(d_mlp_prm_, prm, mlp::kNumParams * sizeof(float), hipMemcpyHostToDevice));
+    LS_HIP(hipMemset(d_mlp_m_, 0, mlp::kNumParams * sizeof(float)));
+    LS_HIP(hipMemset(d_mlp_v_, 0, mlp::kNumParams * sizeof(float)));
+    adam_t_ = 0;
+    if (int rc = mlp_pack(stream_, err)) return rc;
+    LS_HIP(hipStreamSynchronize(stream_));
+    mlp_ready_ = true;
+    return LS_OK;
+  }
+  int mlp_get_params(float* prm, std::string& err) override {
+    if (!mlp_ready_) { err = "no network parameters set"; return LS_ERR_STATE; }
+    LS_HIP(hipMemcpy(prm, d_mlp_prm_, mlp::kNumParams * sizeof(float), hipMemcpyDeviceToHost));
+    return LS_OK;
+  }
+  size_t mlp_predict_lds() const {
+    return (size_t)(kMlpBlock / 64) * mlp::kWaveTile * sizeof(float) +
+           (size_t)(4 * max_ep_ + max_vp_ + max_hp_) * sizeof(float);
+  }
+  // Warm values for the next step from the current phi over the current
+  // narrowband (the next step's previous narrowband).
+  int mlp_predict(std::string& err) override {
+    if (int rc = mlp_tet_only(err)) return rc;
+    if (!mlp_ready_) { err = "no network parameters set"; return LS_ERR_STATE; }
+    if (nnb_ == 0) { err = "no narrowband yet: the first step has nothing to warm-start"; return LS_ERR_STATE; }
+    if (!d_warm_) {
+      LS_HIP(dev_alloc(&d_warm_, (size_t)N_));
+      std::vector<T> big(N_, (T)kLargeNum);
+      LS_HIP(hipMemcpy(d_warm_, big.data(), N_ * sizeof(T), hipMemcpyHostToDevice));
+    }
+    k_mlp_predict<T><<<nnb_, kMlpBlock, mlp_predict_lds(), stream_>>>(mesh(), state(), d_nb_cur(), mlp_weights(),
+                                                                      d_warm_);
+    LS_HIP(hipGetLastError());
+    LS_HIP(hipStreamSynchronize(stream_));
+    return LS_OK;
+  }
+  // Training pairs: phase 0 before a step records the inputs of every copy
+  // of the current narrowband; phase 1 after the step records their targets.
+  int mlp_collect(int phase, int64_t capacity, std::string& err) override {
+    if (int rc = mlp_tet_only(err)) return rc;
+    if (phase == 0) {
+      if (nnb_ == 0) { err = "no narrowband yet"; return LS_ERR_STATE; }
+      if (!d_ds_x_) {
+        if (capacity <= 0) { err = "dataset capacity must be > 0"; return LS_ERR_ARG; }
+        LS_HIP(dev_alloc(&d_ds_x_, (size_t)capacity * mlp::kInPad));
+        LS_HIP(dev_alloc(&d_ds_y_, (size_t)capacity * mlp::kOut));
+        ds_cap_ = capacity;
+        ds_n_ = 0;
+      }
+      std::vector<int32_t> nb(nnb_);
+      LS_HIP(hipMemcpy(nb.data(), d_nb_cur(), nnb_ * sizeof(int32_t), hipMemcpyDeviceToHost));
+      std::sort(nb.begin(), nb.end());
+      std::vector<int32_t> copies;
+      for (int32_t p : nb)
+        for (int32_t c = ele_off_h_[p]; c < ele_off_h_[p + 1]; ++c) copies.push_back(c);
+      const int64_t n = std::min<int64_t>((int64_t)copies.size(), ds_cap_ - ds_n_);
+      pend_n_ = n;
+      if (n <= 0) return LS_OK;
+      if (pend_cap_ < n) {
+        if (d_pend_) LS_HIP(hipFree(d_pend_));
+        d_pend_ = nullptr;
+        LS_HIP(dev_alloc(&d_pend_, (size_t)n));
+        pend_cap_ = n;
+      }
+      LS_HIP(hipMemcpy(d_pend_, copies.data(), n * sizeof(int32_t), hipMemcpyHostToDevice));
+      k_mlp_collect_x<T><<<(int)((n + kBlock - 1) / kBlock), kBlock, 0, stream_>>>(
+          mesh(), state(), d_pend_, n, d_ds_x_ + ds_n_ * mlp::kInPad);
+      LS_HIP(hipGetLastError());
+      LS_HIP(hipStreamSynchronize(stream_));
+      return LS_OK;
+    }
+    if (pend_n_ <= 0) return LS_OK;
+    k_mlp_collect_y<T><<<(int)((pend_n_ + kBlock - 1) / kBlock), kBlock, 0, stream_>>>(
+        mesh(), state(), d_pend_, pend_n_, d_ds_x_ + ds_n_ * mlp::kInPad, d_ds_y_ + ds_n_ * mlp::kOut);
+    LS_HIP(hipGetLastError());
+    LS_HIP(hipStreamSynchronize(stream_));
+    ds_n_ += pend_n_;
+    pend_n_ = 0;
+    return LS_OK;
+  }
+  int64_t mlp_dataset_size() const override { return ds_n_; }
+  int mlp_get_dataset(float* X, float* Y, std::string& err) override {
+    if (ds_n_ == 0) return LS_OK;
+    std::vector<float> xp((size_t)ds_n_ * mlp::kInPad);
+    LS_HIP(hipMemcpy(xp.data(), d_ds_x_, xp.size() * sizeof(float), hipMemcpyDeviceToHost));
+    if (X)
+      for (int64_t i = 0; i < ds_n_; ++i) std::memcpy(X + i * mlp::kIn, &xp[i * mlp::kInPad], mlp::kIn * sizeof(float));
+    if (Y) LS_HIP(hipMemcpy(Y, d_ds_y_, (size_t)ds_n_ * mlp::kOut * sizeof(float), hipMemcpyDeviceToHost));
+    return LS_OK;
+  }
+  size_t mlp_grad_lds() const {
+    return (size_t)(kMlpBlock / 64) * mlp::kWaveTile * sizeof(float) + (size_t)mlp::kNumParams * sizeof(float);
+  }
+  int mlp_grad_launch(const float* X, const float* Y, const int32_t* idx, int64_t n, float inv_total, hipStream_t q,
+                      std::string& err) {
+    const int64_t tiles = (n + 15) / 16;
+    const int grid = (int)std::max<int64_t>(1, std::min<int64_t>((tiles + 3) / 4, 1024));
+    k_mlp_grad<<<grid, kMlpBlock, mlp_grad_lds(), q>>>(X, Y, idx, n, inv_total, mlp_weights(), d_mlp_grad_,
+                                                       d_mlp_grad_ + mlp::kNumParams);
+    LS_HIP(hipGetLastError());
+    return LS_OK;
+  }
+  // Loss sum and the gradient of the mean loss over n caller samples.
+  int mlp_gradient(const float* X, const float* Y, int64_t n, float* grad, double* loss, std::string& err) override {
+    if (int rc = mlp_tet_only(err)) return rc;
+    if (!mlp_ready_) { err = "no network parameters set"; return LS_ERR_STATE; }
+    if (n <= 0) { err = "n must be > 0"; return LS_ERR_ARG; }
+    std::vector<float> xp((size_t)n * mlp::kInPad, 0.f);
+    for (int64_t i = 0; i < n; ++i) std::memcpy(&xp[i * mlp::kInPad], X + i * mlp::kIn, mlp::kIn * sizeof(float));
+    float *dx = nullptr, *dy = nullptr;
+    LS_HIP(dev_alloc(&dx, xp.size()));
+    hipError_t e = dev_alloc(&dy, (size_t)n * mlp::kOut);
+    if (e == hipSuccess) e = hipMemcpy(dx, xp.data(), xp.size() * sizeof(float), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dy, Y, (size_t)n * mlp::kOut * sizeof(float), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemset(d_mlp_grad_, 0, (mlp::kNumParams + 1) * sizeof(float));
+    int rc = LS_OK;
+    if (e == hipSuccess) rc = mlp_grad_launch(dx, dy, nullptr, n, 1.0f / (float)n, stream_, err);
+    if (e == hipSuccess && rc == LS_OK) e = hipStreamSynchronize(stream_);
+    std::vector<float> g(mlp::kNumParams + 1);
+    if (e == hipSuccess && rc == LS_OK)
+      e = hipMemcpy(g.data(), d_mlp_grad_, g.size() * sizeof(float), hipMemcpyDeviceToHost);
+    (void)hipFree(dx);
+    if (dy) (void)hipFree(dy);
+    if (rc != LS_OK) return rc;
+    LS_HIP(e);
+    std::memcpy(grad, g.data(), mlp::kNumParams * sizeof(float));
+    *loss = g[mlp::kNumParams];
+    return LS_OK;
+  }
+  // Adam over the collected pairs, shuffled every epoch; returns the mean
+  // squared error per output of the last epoch.
+  int mlp_train(int epochs, int batch, double lr, uint64_t seed, double* loss, std::string& err) override {
+    if (int rc = mlp_tet_only(err)) return rc;
+    if (!mlp_ready_) { err = "no network parameters set"; return LS_ERR_STATE; }
+    if (ds_n_ == 0) { err = "no training pairs collected"; return LS_ERR_STATE; }
+    if (epochs <= 0 || batch <= 0) { err = "epochs and batch must be > 0"; return LS_ERR_ARG; }
+    int32_t* didx = nullptr;
+    LS_HIP(dev_alloc(&didx, (size_t)ds_n_));
+    std::vector<int32_t> perm(ds_n_);
+    for (int64_t i = 0; i < ds_n_; ++i) perm[i] = (int32_t)i;
+    uint64_t st = seed ? seed : 0x9e3779b97f4a7c15ull;
+    auto rnd = [&st]() {  // splitmix64
+      uint64_t z = (st += 0x9e3779b97f4a7c15ull);
+      z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+      z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+      return z ^ (z >> 31);
+    };
+    const float b1 = 0.9f, b2 = 0.999f;
+    double last = 0.0;
+    hipError_t e = hipSuccess;
+    int rc = LS_OK;
+    for (int ep = 0; ep < epochs && e == hipSuccess && rc == LS_OK; ++ep) {
+      for (int64_t i = ds_n_ - 1; i > 0; --i) std::swap(perm[i], perm[(int64_t)(rnd() % (uint64_t)(i + 1))]);
+      e = hipMemcpy(didx, perm.data(), ds_n_ * sizeof(int32_t), hipMemcpyHostToDevice);
+      double sum = 0.0;
+      for (int64_t b0 = 0; b0 < ds_n_ && e == hipSuccess && rc == LS_OK; b0 += batch) {
+        const int64_t nb = std::min<int64_t>(batch, ds_n_ - b0);
+        e = hipMemsetAsync(d_mlp_grad_, 0, (mlp::kNumParams + 1) * sizeof(float), stream_);
+        if (e != hipSuccess) break;
+        rc = mlp_grad_launch(d_ds_x_, d_ds_y_, didx + b0, nb, 1.0f / (float)nb, stream_, err);
+        if (rc != LS_OK) break;
+        ++adam_t_;
+        const float c1 = 1.f - std::pow(b1, (float)adam_t_), c2 = 1.f - std::pow(b2, (float)adam_t_);
+        k_mlp_adam<<<(mlp::kNumParams + 255) / 256, 256, 0, stream_>>>(d_mlp_prm_, d_mlp_m_, d_mlp_v_, d_mlp_grad_,
+                                                                      mlp::kNumParams, (float)lr, b1, b2, 1e-8f, c1, c2);
+        rc = mlp_pack(stream_, err);
+        float l = 0.f;
+        if (rc == LS_OK) e = hipMemcpyAsync(&l, d_mlp_grad_ + mlp::kNumParams, sizeof(float), hipMemcpyDeviceToHost, stream_);
+        if (e == hipSuccess) e = hipStreamSynchronize(stream_);
+        sum += l;
+      }
+      last = sum / (double)(ds_n_ * mlp::kOut);
+    }
+    (void)hipFree(didx);
+    if (rc != LS_OK) return rc;
+    LS_HIP(e);
+    *loss = last;
+    return LS_OK;
+  }
+  // Unsigned distances after the last step's redistancing (caller order).
+  int get_distances(double* out, std::string& err) override {
+    std::vector<T> h(N_);
+    LS_HIP(hipMemcpy(h.data(), d_DT_, N_ * sizeof(T), hipMemcpyDeviceToHost));
+    for (int64_t v = 0; v < N_; ++v) out[vert_perm_h_[v]] = (double)h[v];
+    return LS_OK;
+  }
+
   int set_phi(const double* phi_user, std::string& err) override {
     if (!uploaded_) { err = "layout not built"; return LS_ERR_STATE; }
     std::vector<T> h(N_);
@@ -1277,6 +2005,14 @@ class EngineImpl final : public Engine {
     a.leps = prm.lambda_eps;
     a.dt = prm.dt;
     a.mode = prm.mode;
+    a.warm = prm.warm;
+    a.warm_margin = prm.warm_margin;
+    a.res_tol = (float)(prm.warm_res_tol > 0 ? prm.warm_res_tol : prm.tol);
+    if (a.warm == LS_WARM_LEARNED && !a.first && !d_warm_) {
+      err = "no learned warm-start values (run ls_mlp_predict)";
+      return LS_ERR_STATE;
+    }
+    if (a.warm < LS_WARM_NONE || a.warm > LS_WARM_LEARNED) { err = "bad warm mode"; return LS_ERR_ARG; }
     hipStream_t q = stream_;
     void* args[] = {&m, &s, &a};
     if (!pc_synced_) {  // the previous narrowband size, when the host-driven path ran last
@@ -1286,7 +2022,9 @@ class EngineImpl final : public Engine {
     }
     LS_HIP(hipEventRecord(ev_[0], q));
     if (exec_ == LS_EXEC_GRAPH) {
-      const int key[3] = {a.first, a.nb_in, use_patch_prm_ ? 2 : 1};
+      // The captured arguments hold device pointers: re-capture when one that
+      // may be null (per-patch parameters, warm values) appears.
+      const int key[3] = {a.first, a.nb_in, (use_patch_prm_ ? 2 : 1) + (d_warm_ ? 4 : 0)};
       const bool same = graph_exec_ && key[0] == graph_key_[0] && key[1] == graph_key_[1] &&
                         std::memcmp(&graph_prm_, &prm, sizeof(prm)) == 0;
       if (!same) {
@@ -1343,6 +2081,12 @@ class EngineImpl final : public Engine {
     st.ms_ph_seed = tk[kTkSeed] * to_ms; st.ms_ph_visit = tk[kTkVisit] * to_ms;
     st.ms_ph_copyback = tk[kTkCopyback] * to_ms; st.ms_ph_probe = tk[kTkProbe] * to_ms;
     st.ms_ph_next = tk[kTkNext] * to_ms; st.ms_ph_nb = tk[kTkNb] * to_ms; st.ms_ph_advect = tk[kTkAdvect] * to_ms;
+    unsigned int rb = (unsigned int)pc[kPcResid];
+    float rf;
+    std::memcpy(&rf, &rb, sizeof(rf));
+    st.warm_residual = rf;
+    st.warm_patches = pc[kPcWarm];
+    st.warm_fallback = pc[kPcFallback];
     return LS_OK;
   }
 
@@ -1359,6 +2103,10 @@ class EngineImpl final : public Engine {
     if (prm.niter <= 0 || prm.inside_niter < 0) { err = "niter must be > 0, inside_niter >= 0"; return LS_ERR_ARG; }
     if (prm.mode != LS_MODE_ADVECT && prm.mode != LS_MODE_MCF) {
       err = "mode must be LS_MODE_ADVECT or LS_MODE_MCF";
+      return LS_ERR_ARG;
+    }
+    if (prm.warm != LS_WARM_NONE) {
+      err = "warm starts run in the persistent step kernel (LS_EXEC_PERSISTENT or LS_EXEC_GRAPH)";
       return LS_ERR_ARG;
     }
     std::memset(&st, 0, sizeof(st));
@@ -1491,6 +2239,7 @@ class EngineImpl final : public Engine {
     s.pc = d_pc_;
     s.ticks = d_ticks_;
     s.bar = d_bar_;
+    s.warm = d_warm_;
     return s;
   }
   int32_t* d_nb_cur() const { return nb_sel_ ? d_lists_[3] : d_nb_; }
@@ -1502,7 +2251,8 @@ class EngineImpl final : public Engine {
                     d_DT_out_, d_label_, d_tsign_, d_bcon_, d_bmin_, d_act_, d_fresh_, d_seedflag_,
                     d_counters_, d_sizes_, d_nb_, d_scan_all_, d_lists_[0], d_lists_[1], d_lists_[2],
                     d_lists_[3], d_inA_, d_freshE_, d_listF_, d_pc_, d_ticks_, d_bar_,
-                    d_patch_dt_, d_patch_bw_};
+                    d_patch_dt_, d_patch_bw_, d_warm_, d_mlp_prm_, d_mlp_m_, d_mlp_v_, d_mlp_grad_,
+                    d_mlp_b_, d_mlp_w1_, d_mlp_w2_, d_mlp_w3_, d_ds_x_, d_ds_y_, d_pend_};
     if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
     if (graph_) (void)hipGraphDestroy(graph_);
     for (void* p : ptrs) if (p) (void)hipFree(p);
@@ -1527,6 +2277,16 @@ class EngineImpl final : public Engine {
   uint16_t* d_vslots_ = nullptr;
   int32_t *d_padj_off_ = nullptr, *d_padj_ = nullptr;
   T *d_coords_ = nullptr, *d_sigma_ = nullptr;
+  T* d_warm_ = nullptr;  // learned warm-start distances (N), written by the predictor
+  float *d_mlp_prm_ = nullptr, *d_mlp_m_ = nullptr, *d_mlp_v_ = nullptr, *d_mlp_grad_ = nullptr, *d_mlp_b_ = nullptr;
+  uint16_t *d_mlp_w1_ = nullptr, *d_mlp_w2_ = nullptr, *d_mlp_w3_ = nullptr;
+  bool mlp_ready_ = false;
+  int adam_t_ = 0;
+  float *d_ds_x_ = nullptr, *d_ds_y_ = nullptr;  // collected training pairs
+  int64_t ds_cap_ = 0, ds_n_ = 0, pend_n_ = 0, pend_cap_ = 0;
+  int32_t* d_pend_ = nullptr;  // copies awaiting their targets
+  std::vector<int32_t> ele_off_h_;
+  int max_ep_ = 0, max_vp_ = 0, max_hp_ = 0;
   double* d_patch_dt_ = nullptr;
   T* d_patch_bw_ = nullptr;
   bool use_patch_prm_ = false;

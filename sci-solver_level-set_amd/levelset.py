@@ -17,12 +17,14 @@ LIB_PATH = os.path.join(_HERE, "liblevelset_hip.so")
 LS_F32, LS_F64 = 0, 1
 LS_MODE_ADVECT, LS_MODE_MCF = 0, 1
 LS_EXEC_PERSISTENT, LS_EXEC_KERNELS, LS_EXEC_GRAPH = 0, 1, 2
+LS_WARM_NONE, LS_WARM_IDENTITY, LS_WARM_LEARNED = 0, 1, 2
 
 
 class StepParams(C.Structure):
     _fields_ = [("dt", C.c_double), ("bandwidth", C.c_double), ("tol", C.c_double),
                 ("lambda_eps", C.c_double), ("niter", C.c_int32), ("inside_niter", C.c_int32),
-                ("mode", C.c_int32), ("max_fim_iters", C.c_int32)]
+                ("mode", C.c_int32), ("max_fim_iters", C.c_int32), ("warm", C.c_int32),
+                ("reserved0", C.c_int32), ("warm_margin", C.c_double), ("warm_res_tol", C.c_double)]
 
 
 class Stats(C.Structure):
@@ -38,7 +40,8 @@ class Stats(C.Structure):
                 ("ms_step_kernel", C.c_double), ("exec_mode", C.c_int32), ("reserved", C.c_int32),
                 ("ms_ph_seed", C.c_double), ("ms_ph_visit", C.c_double), ("ms_ph_copyback", C.c_double),
                 ("ms_ph_probe", C.c_double), ("ms_ph_next", C.c_double), ("ms_ph_nb", C.c_double),
-                ("ms_ph_advect", C.c_double), ("elem_updates", C.c_int64), ("local_solves", C.c_int64)]
+                ("ms_ph_advect", C.c_double), ("elem_updates", C.c_int64), ("local_solves", C.c_int64),
+                ("warm_residual", C.c_double), ("warm_patches", C.c_int32), ("warm_fallback", C.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -54,7 +57,10 @@ class LayoutInfo(C.Structure):
 EXPORTS = ["ls_create", "ls_destroy", "ls_last_error", "ls_set_mesh", "ls_set_precision", "ls_set_execution", "ls_partition_grid",
            "ls_partition_rcb", "ls_partition_labels", "ls_get_layout_info", "ls_get_layout",
            "ls_set_phi", "ls_set_velocity", "ls_step", "ls_get_phi", "ls_get_narrowband",
-           "ls_set_mesh_batch", "ls_partition_batch", "ls_set_batch_params", "ls_get_batch_nb_counts"]
+           "ls_set_mesh_batch", "ls_partition_batch", "ls_set_batch_params", "ls_get_batch_nb_counts",
+           "ls_get_distances", "ls_mlp_set_params", "ls_mlp_get_params", "ls_mlp_predict", "ls_mlp_collect",
+           "ls_mlp_dataset", "ls_mlp_gradient", "ls_mlp_train"]
+MLP_PARAMS = 5892
 
 
 def build(verbose=False) -> str:
@@ -97,6 +103,14 @@ def lib():
         L.ls_partition_batch.argtypes = [vp, C.c_int]
         L.ls_set_batch_params.argtypes = [vp, vp, vp]
         L.ls_get_batch_nb_counts.argtypes = [vp, vp]
+        L.ls_get_distances.argtypes = [vp, vp]
+        L.ls_mlp_set_params.argtypes = [vp, vp, C.c_int]
+        L.ls_mlp_get_params.argtypes = [vp, vp, C.c_int]
+        L.ls_mlp_predict.argtypes = [vp]
+        L.ls_mlp_collect.argtypes = [vp, C.c_int, C.c_int64]
+        L.ls_mlp_dataset.argtypes = [vp, vp, vp, C.POINTER(C.c_int64)]
+        L.ls_mlp_gradient.argtypes = [vp, vp, vp, C.c_int64, vp, C.POINTER(C.c_double)]
+        L.ls_mlp_train.argtypes = [vp, C.c_int, C.c_int, C.c_double, C.c_uint64, C.POINTER(C.c_double)]
         _lib = L
     return _lib
 
@@ -106,8 +120,9 @@ class LevelSetError(RuntimeError):
 
 
 def step_params(dt, bandwidth, niter, tol=1e-5, lambda_eps=1e-5, inside_niter=1, mode=LS_MODE_ADVECT,
-                max_fim_iters=0):
-    return StepParams(dt, bandwidth, tol, lambda_eps, niter, inside_niter, mode, max_fim_iters)
+                max_fim_iters=0, warm=LS_WARM_NONE, warm_margin=0.0, warm_res_tol=0.0):
+    return StepParams(dt, bandwidth, tol, lambda_eps, niter, inside_niter, mode, max_fim_iters, warm, 0,
+                      warm_margin, warm_res_tol)
 
 
 class Solver:
@@ -226,6 +241,51 @@ class Solver:
         self._check(self.L.ls_get_batch_nb_counts(self.ctx, out.ctypes.data))
         return out
 
+    def distances(self):
+        out = np.empty(self.N, dtype=np.float64)
+        self._check(self.L.ls_get_distances(self.ctx, out.ctypes.data))
+        return out
+
+    # -- learned warm start ------------------------------------------------
+    def mlp_set_params(self, params):
+        a = np.ascontiguousarray(params, dtype=np.float32)
+        assert a.size == MLP_PARAMS
+        self._check(self.L.ls_mlp_set_params(self.ctx, a.ctypes.data, MLP_PARAMS))
+
+    def mlp_params(self):
+        out = np.empty(MLP_PARAMS, np.float32)
+        self._check(self.L.ls_mlp_get_params(self.ctx, out.ctypes.data, MLP_PARAMS))
+        return out
+
+    def mlp_predict(self):
+        self._check(self.L.ls_mlp_predict(self.ctx))
+
+    def mlp_collect(self, phase, capacity=0):
+        self._check(self.L.ls_mlp_collect(self.ctx, int(phase), int(capacity)))
+
+    def mlp_dataset(self):
+        n = C.c_int64(0)
+        self._check(self.L.ls_mlp_dataset(self.ctx, None, None, C.byref(n)))
+        X = np.empty((n.value, 22), np.float32)
+        Y = np.empty((n.value, 4), np.float32)
+        if n.value:
+            self._check(self.L.ls_mlp_dataset(self.ctx, X.ctypes.data, Y.ctypes.data, C.byref(n)))
+        return X, Y
+
+    def mlp_gradient(self, X, Y):
+        X = np.ascontiguousarray(X, np.float32)
+        Y = np.ascontiguousarray(Y, np.float32)
+        g = np.empty(MLP_PARAMS, np.float32)
+        loss = C.c_double(0.0)
+        self._check(self.L.ls_mlp_gradient(self.ctx, X.ctypes.data, Y.ctypes.data, X.shape[0], g.ctypes.data,
+                                           C.byref(loss)))
+        return g, loss.value
+
+    def mlp_train(self, epochs, batch, lr, seed=1):
+        loss = C.c_double(0.0)
+        self._check(self.L.ls_mlp_train(self.ctx, int(epochs), int(batch), float(lr), int(seed), C.byref(loss)))
+        return loss.value
+
     def split(self, values):
         """Per-mesh views of a concatenated per-vertex array."""
         return [values[self._voff[m]:self._voff[m + 1]] for m in range(self.n_meshes)]
@@ -277,7 +337,15 @@ def autotune_batch(batch, candidates, executions=(LS_EXEC_GRAPH, LS_EXEC_PERSIST
     for cand in candidates:
         for ex in executions:
             kw = {"patch_verts": cand} if isinstance(cand, int) else {"labels": (cand[1], cand[2])}
-            s = solver_for_batch(batch, device=device, execution=ex, **kw)
+            name = cand if isinstance(cand, int) else cand[0]
+            try:
+                s = solver_for_batch(batch, device=device, execution=ex, **kw)
+            except LevelSetError as e:  # a patch of this size exceeds the device's LDS
+                if "rc=-4" not in str(e):
+                    raise
+                if log:
+                    log(f"autotune: patches {name} skipped: {e}")
+                break
             prm = step_params(1.0, 1.0, batch.niter, mode=batch.mode)
             s.step(prm)  # first step: graph capture and warm caches
             ms = 0.0
@@ -285,7 +353,6 @@ def autotune_batch(batch, candidates, executions=(LS_EXEC_GRAPH, LS_EXEC_PERSIST
                 ms += s.step(prm).ms_total
             s.close()
             ms /= trial_steps
-            name = cand if isinstance(cand, int) else cand[0]
             table.append((name, ex, ms))
             if log:
                 log(f"autotune: patches {name} exec {ex}: {ms:.3f} ms/step")

@@ -40,3 +40,9 @@ def oracle_mod():
     import oracle
     oracle.lib()
     return oracle
+
+
+@pytest.fixture(scope="session")
+def mlp_oracle():
+    import mlp_oracle
+    return mlp_oracle

@@ -243,6 +243,50 @@ def mcf_workload(args, lsb, cases):
     return out, tot, ex == lsb.LS_EXEC_KERNELS, cpu
 
 
+def learned_run(args, lsb, case, exec_mode, plain_value):
+    """The learned warm start on the same workload: pairs collected from
+    plain steps, GPU training, then the timed steps with warm values from
+    the network (prediction time included). A warm start whose fixed-point
+    residual over the band exceeds 1e-5 re-runs the plain method."""
+    g = lsb.solver_for_case(case, execution=exec_mode)
+    g.mlp_set_params(lsb.mlp_init_params(5))
+    prm = lsb.step_params(case.dt, case.bandwidth, case.niter)
+    g.step(prm)
+    t0 = time.perf_counter()
+    for _ in range(args.train_steps):
+        g.mlp_collect(0, 12_000_000)
+        g.step(prm)
+        g.mlp_collect(1)
+    X, Y = g.mlp_dataset()
+    t1 = time.perf_counter()
+    loss = g.mlp_train(args.train_epochs, 4096, 2e-3, seed=7)
+    t2 = time.perf_counter()
+    held = slice(0, min(200_000, X.shape[0]))
+    _, sse = g.mlp_gradient(X[held], Y[held])  # the network's squared error on the GPU
+    trained_mse = sse / Y[held].size
+    identity_mse = float(np.mean(Y[held].astype(np.float64) ** 2))
+    wprm = lsb.step_params(case.dt, case.bandwidth, case.niter, warm=lsb.LS_WARM_LEARNED, warm_margin=2e-3)
+    g.set_phi(case.phi0)
+    verts, fallbacks, iters = 0, 0, 0
+    ts = time.perf_counter()
+    for k in range(args.steps):
+        if k > 0:
+            g.mlp_predict()
+        st = g.step(wprm)
+        verts += st.nb_verts
+        fallbacks += st.warm_fallback
+        iters += st.fim_iters
+    wall = time.perf_counter() - ts
+    g.close()
+    value = verts / wall
+    return {"value": value, "unit": "vertex-updates/s", "vs_plain_fim": value / plain_value,
+            "ms_per_step": 1e3 * wall / args.steps, "fim_iters_per_step": iters / args.steps,
+            "fallback_steps": fallbacks, "steps": args.steps, "train_pairs": int(X.shape[0]),
+            "collect_s": t1 - t0, "train_s": t2 - t1, "train_epochs": args.train_epochs,
+            "train_loss": loss, "trained_mse": trained_mse, "identity_mse": identity_mse,
+            "residual_tolerance": 1e-5, "warm_margin": 2e-3}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -253,10 +297,24 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--exec", default="graph", choices=["graph", "persistent", "kernels"],
                     help="graph: persistent step kernel replayed from a hipGraph (default)")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned warm-start run")
+    ap.add_argument("--train-steps", type=int, default=13, help="plain steps whose pairs train the network")
+    ap.add_argument("--train-epochs", type=int, default=2)
     ap.add_argument("--workload", default="torus", choices=["torus", "batch", "mcf"],
                     help="torus: 3D torus revolve, 64^3 (default); batch: 512 ellipsoid meshes; "
                          "mcf: 8-sphere mean-curvature flow, 192^3")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy 1 GiB device-to-device (a reference kernel for HBM byte counters)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        a = torch.empty(1 << 28, dtype=torch.float32, device="cuda").fill_(1.0)
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": a.numel() * 4, "copy_bytes_written": b.numel() * 4, "launches": 3}))
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU path is not built.",
               file=sys.stderr)
@@ -367,6 +425,9 @@ def main():
             "launches": launches,
         },
     }
+
+    if not args.no_learned:
+        out["learned"] = learned_run(args, lsb, case, exec_mode, value)
 
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))

@@ -33,6 +33,10 @@ enum { LS_MODE_ADVECT = 0, LS_MODE_MCF = 1 };
  * same kernel replayed from a captured hipGraph, or one launch per phase with
  * the FIM loop driven from the host. Results are identical. */
 enum { LS_EXEC_PERSISTENT = 0, LS_EXEC_KERNELS = 1, LS_EXEC_GRAPH = 2 };
+/* FIM sweep organisation (results are identical): one thread per element
+ * copy, or the (element, corner) solves left by the exact lower-bound skip
+ * spread one per thread (needs more LDS); AUTO takes the latter when it fits. */
+enum { LS_SWEEP_AUTO = 0, LS_SWEEP_ELEMENTS = 1, LS_SWEEP_TASKS = 2 };
 /* Warm start of the FIM (no reference counterpart): none, |phi| after the
  * previous step (identity predictor), or the learned predictor's distances
  * (ls_mlp_predict). Warm values apply to non-seed vertices of the previous
@@ -89,6 +93,8 @@ typedef struct ls_stats {
   double warm_residual;     /* warm start: max |DT - F(DT)| over non-seed band vertices */
   int32_t warm_patches;     /* warm start: patches that received warm values */
   int32_t warm_fallback;    /* warm start: 1 when the residual check failed and the step re-ran plain */
+  int32_t sweep_tasks;      /* 1: the FIM sweeps ran task-compacted */
+  int32_t reserved1;
 } ls_stats;
 
 typedef struct ls_layout_info {
@@ -117,6 +123,8 @@ int ls_set_precision(ls_ctx* ctx, int dtype);
 
 /* Selects the execution mode (LS_EXEC_*); may be called at any time. */
 int ls_set_execution(ls_ctx* ctx, int mode);
+/* Selects the FIM sweep organisation (LS_SWEEP_*); may be called at any time. */
+int ls_set_sweep(ls_ctx* ctx, int mode);
 
 /* Partition + patch layout (meshFIM3d::InitPatches/InitPatches2/
  * GenerateBlockNeighbors, src/core3d/cuda/meshFIM3d.cu:423-570).

@@ -21,6 +21,7 @@ struct ls_ctx {
   int patch_sigma = 0;
   int precision = LS_F32;  // tet storage precision
   int exec = LS_EXEC_PERSISTENT;
+  int sweep = LS_SWEEP_AUTO;
   bool have_vel = false;
   lsamd::Layout layout;
   bool have_layout = false;
@@ -128,6 +129,8 @@ int finish_layout(ls_ctx* c, const std::vector<int32_t>& labels, int nparts) {
   c->layout = std::move(L);
   c->engine = std::move(eng);
   c->engine->set_exec(c->exec);
+  rc = c->engine->set_sweep_mode(c->sweep, err);
+  if (rc != LS_OK) return fail(c, rc, err);
   c->have_layout = true;
   rc = apply_batch_params(c);
   if (rc != LS_OK) return rc;
@@ -254,6 +257,17 @@ int ls_set_precision(ls_ctx* c, int dtype) {
   if (c->have_layout) return fail(c, LS_ERR_STATE, "set the precision before partitioning");
   c->precision = dtype;
   return LS_OK;
+}
+
+int ls_set_sweep(ls_ctx* c, int mode) {
+  if (!c) return LS_ERR_ARG;
+  if (mode != LS_SWEEP_AUTO && mode != LS_SWEEP_ELEMENTS && mode != LS_SWEEP_TASKS)
+    return fail(c, LS_ERR_ARG, "mode must be LS_SWEEP_AUTO, LS_SWEEP_ELEMENTS or LS_SWEEP_TASKS");
+  c->sweep = mode;
+  if (!c->engine) return LS_OK;
+  std::string err;
+  const int rc = c->engine->set_sweep_mode(mode, err);
+  return rc == LS_OK ? rc : fail(c, rc, err);
 }
 
 int ls_set_execution(ls_ctx* c, int mode) {

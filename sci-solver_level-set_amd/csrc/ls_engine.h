@@ -38,6 +38,8 @@ class Engine {
   virtual int mlp_train(int epochs, int batch, double lr, uint64_t seed, double* loss, std::string& err) = 0;
   virtual int get_distances(double* out_user, std::string& err) = 0;
   virtual void set_exec(int mode) = 0;
+  virtual int set_sweep_mode(int mode, std::string& err) = 0;  // LS_SWEEP_*
+  virtual int sweep_tasks() const = 0;                         // 1: task-compacted sweeps in use
 };
 
 // dim 3 -> tet engine with FP32 (LS_F32) or FP64 (LS_F64) storage,

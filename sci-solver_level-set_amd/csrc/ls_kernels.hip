@@ -246,7 +246,9 @@ struct TetDots {
 template <typename T>
 struct ElemSolve<T, 4> {
   static constexpr int kCoords = 6;
-  __device__ static int update(const T* Lt, T* v, double leps) {
+  // Bit k set: corner k must be solved (its value is not below the exact
+  // lower bound min(other values) + height of corner k over its face).
+  __device__ static int need_mask(const T* Lt, const T* v) {
     double L[6];
 #pragma unroll
     for (int k = 0; k < 6; ++k) L[k] = (double)Lt[k];
@@ -262,23 +264,40 @@ struct ElemSolve<T, 4> {
     h[2] = d_abs(L[2] * L[5]) / d_sqrt(L[4] * L[4] + L[5] * L[5]);
     h[3] = d_abs(L[5]);
     const double scale = L[0] + d_abs(L[1]) + d_abs(L[2]) + d_abs(L[3]) + d_abs(L[4]) + d_abs(L[5]);
-    bool need[4];
+    int mask = 0;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       double mo = 1e300, Mo = 0.0;
 #pragma unroll
       for (int j = 0; j < 4; ++j)
         if (j != k) { mo = d_min(mo, Tv[j]); Mo = d_max(Mo, Tv[j]); }
-      need[k] = must_solve(Tv[k], mo, Mo, h[k], scale);
+      if (must_solve(Tv[k], mo, Mo, h[k], scale)) mask |= 1 << k;
     }
-    if (!(need[0] || need[1] || need[2] || need[3])) return 0;
+    return mask;
+  }
+  // Corner k's updated value min(v_k, solve_k) from the element's values v.
+  __device__ static T corner_update(const T* Lt, const T* v, int k, double leps) {
+    double L[6];
+#pragma unroll
+    for (int j = 0; j < 6; ++j) L[j] = (double)Lt[j];
+    const double Tv[4] = {(double)v[0], (double)v[1], (double)v[2], (double)v[3]};
+    const TetDots d(L);
+    return (T)d_min(Tv[k], d.solve(k, Tv, leps));
+  }
+  __device__ static int update(const T* Lt, T* v, double leps) {
+    const int mask = need_mask(Lt, v);
+    if (!mask) return 0;
+    double L[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) L[k] = (double)Lt[k];
+    const double Tv[4] = {(double)v[0], (double)v[1], (double)v[2], (double)v[3]};
     const TetDots d(L);
     const double TA = Tv[0], TB = Tv[1], TC = Tv[2], TD = Tv[3];
-    if (need[3]) v[3] = (T)d_min(TD, d.solve(3, Tv, leps));
-    if (need[0]) v[0] = (T)d_min(TA, d.solve(0, Tv, leps));
-    if (need[1]) v[1] = (T)d_min(TB, d.solve(1, Tv, leps));
-    if (need[2]) v[2] = (T)d_min(TC, d.solve(2, Tv, leps));
-    return (int)need[0] + (int)need[1] + (int)need[2] + (int)need[3];
+    if (mask & 8) v[3] = (T)d_min(TD, d.solve(3, Tv, leps));
+    if (mask & 1) v[0] = (T)d_min(TA, d.solve(0, Tv, leps));
+    if (mask & 2) v[1] = (T)d_min(TB, d.solve(1, Tv, leps));
+    if (mask & 4) v[2] = (T)d_min(TC, d.solve(2, Tv, leps));
+    return __popc(mask);
   }
   // The value the element gives corner k from the other three (no min with
   // corner k's own value, no skip): what update() stores when it solves k.
@@ -295,7 +314,7 @@ struct ElemSolve<T, 4> {
 template <typename T>
 struct ElemSolve<T, 3> {
   static constexpr int kCoords = 3;
-  __device__ static int update(const T* ct, T* v, double) {
+  __device__ static int need_mask(const T* ct, const T* v) {
     const double c[3] = {(double)ct[0], (double)ct[1], (double)ct[2]};
     const double LenAB = d_abs(c[0]);
     const double LenBC = d_sqrt(c[2] * c[2] + (c[1] - c[0]) * (c[1] - c[0]));
@@ -305,17 +324,29 @@ struct ElemSolve<T, 3> {
     const double a2 = d_abs(c[0] * c[2]);
     const double h[3] = {a2 / LenBC, a2 / LenCA, d_abs(c[2])};
     const double scale = LenAB + LenBC + LenCA;
-    bool need[3];
+    int mask = 0;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
       const double o1 = Tv[(k + 1) % 3], o2 = Tv[(k + 2) % 3];
-      need[k] = must_solve(Tv[k], d_min(o1, o2), d_max(o1, o2), h[k], scale);
+      if (must_solve(Tv[k], d_min(o1, o2), d_max(o1, o2), h[k], scale)) mask |= 1 << k;
     }
-    const double TA = Tv[0], TB = Tv[1], TC = Tv[2];
-    if (need[2]) v[2] = (T)d_min(TC, tri_solve(TA, TB, LenAB, LenBC, LenCA));
-    if (need[0]) v[0] = (T)d_min(TA, tri_solve(TB, TC, LenBC, LenCA, LenAB));
-    if (need[1]) v[1] = (T)d_min(TB, tri_solve(TC, TA, LenCA, LenAB, LenBC));
-    return (int)need[0] + (int)need[1] + (int)need[2];
+    return mask;
+  }
+  __device__ static T corner_update(const T* ct, const T* v, int k, double leps) {
+    return (T)d_min((double)v[k], (double)corner_value(ct, v, k, leps));
+  }
+  __device__ static int update(const T* ct, T* v, double leps) {
+    const int mask = need_mask(ct, v);
+    if (!mask) return 0;
+    const double c[3] = {(double)ct[0], (double)ct[1], (double)ct[2]};
+    const double LenAB = d_abs(c[0]);
+    const double LenBC = d_sqrt(c[2] * c[2] + (c[1] - c[0]) * (c[1] - c[0]));
+    const double LenCA = d_sqrt(c[1] * c[1] + c[2] * c[2]);
+    const double TA = (double)v[0], TB = (double)v[1], TC = (double)v[2];
+    if (mask & 4) v[2] = (T)d_min(TC, tri_solve(TA, TB, LenAB, LenBC, LenCA));
+    if (mask & 1) v[0] = (T)d_min(TA, tri_solve(TB, TC, LenBC, LenCA, LenAB));
+    if (mask & 2) v[1] = (T)d_min(TB, tri_solve(TC, TA, LenCA, LenAB, LenBC));
+    return __popc(mask);
   }
   __device__ static T corner_value(const T* ct, const T* v, int k, double) {
     const double c[3] = {(double)ct[0], (double)ct[1], (double)ct[2]};
@@ -440,9 +471,14 @@ __device__ __forceinline__ T block_min_n(T v, T* scratch) {
 // decides convergence per vertex (old - new < tol; seeds count as converged),
 // and the workgroup reduces it to the patch flag plus the patch min of the
 // written distances (run_reduction_bandwidth, redistance_kernels3d.h:57-97).
+//
+// With `tasks` set, each sweep first lists the (element, corner) solves the
+// exact lower-bound skip leaves, then spreads those solves over the
+// workgroup one per thread, so a wave no longer waits for its lane with the
+// most solves; results are bit-identical (same per-corner arithmetic).
 template <typename T, int NC, int BS>
 __device__ void sweep_patch(const DevMesh<T>& m, DevState<T>& s, int p, int niter, int write_out, T tol,
-                            double leps, unsigned char* smem, T* red, int* s_ndirty) {
+                            double leps, unsigned char* smem, T* red, int* s_ndirty, bool tasks) {
   __syncthreads();
   const int vs = m.vert_off[p], ve = m.vert_off[p + 1];
   const int es = m.ele_off[p], ee = m.ele_off[p + 1];
@@ -457,6 +493,9 @@ __device__ void sweep_patch(const DevMesh<T>& m, DevState<T>& s, int p, int nite
   uint16_t* s_vsl = s_voff + nv + 1;
   uint16_t* s_work = s_vsl + nslots;            // compacted dirty elements
   uint8_t* s_dirty = (uint8_t*)(s_work + ne);
+  // Task mode: (element, corner) solves and their results.
+  uint32_t* s_task = (uint32_t*)(((uintptr_t)(s_dirty + ne) + 7) & ~(uintptr_t)7);
+  T* s_new = (T*)(s_task + NC * ne + (NC * ne) % 2);
   const int tid = threadIdx.x;
 
   for (int i = tid; i < nv; i += BS) {
@@ -494,24 +533,85 @@ __device__ void sweep_patch(const DevMesh<T>& m, DevState<T>& s, int p, int nite
       off = __shfl(off, 0, 64);
       if (d) s_work[off + __popcll(mask & ((1ull << lane) - 1ull))] = (uint16_t)e;
     }
+    if (tid == 0) s_ndirty[1] = 0;
     __syncthreads();
     const int nd = *s_ndirty;
-    for (int k = tid; k < nd; k += BS) {
-      const int e = s_work[k];
-      T x[NC], y[NC], L[K];
+    if (tasks) {
+      // Need masks of the dirty elements, then their solves as tasks.
+      for (int base = 0; base < nd; base += BS) {
+        const int k = base + tid;
+        int e = 0, mask = 0;
+        if (k < nd) {
+          e = s_work[k];
+          T x[NC], L[K];
 #pragma unroll
-      for (int i = 0; i < NC; ++i) x[i] = y[i] = s_slot[e * NC + i];
-      const T* gc = m.coords + (int64_t)(es + e) * K;
+          for (int i = 0; i < NC; ++i) x[i] = s_slot[e * NC + i];
+          const T* gc = m.coords + (int64_t)(es + e) * K;
 #pragma unroll
-      for (int q = 0; q < K; ++q) L[q] = gc[q];
-      n_solve += ElemSolve<T, NC>::update(L, y, leps);
-      ++n_upd;
-      int changed = 0;
+          for (int q = 0; q < K; ++q) L[q] = gc[q];
+          mask = ElemSolve<T, NC>::need_mask(L, x);
+          s_dirty[e] = 0;
+          ++n_upd;
+        }
+        const int c = __popc(mask);
+        n_solve += c;
+        const int lane = tid & 63;
+        int incl = c;
 #pragma unroll
-      for (int i = 0; i < NC; ++i) {
-        if (y[i] != x[i]) { s_slot[e * NC + i] = y[i]; changed = 1; }
+        for (int off = 1; off < 64; off <<= 1) {
+          const int t = __shfl_up(incl, off, 64);
+          if (lane >= off) incl += t;
+        }
+        const int total = __shfl(incl, 63, 64);
+        int wbase = 0;
+        if (lane == 0 && total) wbase = atomicAdd(&s_ndirty[1], total);
+        wbase = __shfl(wbase, 0, 64);
+        int o = wbase + incl - c;
+#pragma unroll
+        for (int j = 0; j < NC; ++j)
+          if (mask & (1 << j)) s_task[o++] = (uint32_t)(e * 4 + j);
       }
-      s_dirty[e] = (uint8_t)changed;
+      __syncthreads();
+      const int nt = s_ndirty[1];
+      for (int t = tid; t < nt; t += BS) {
+        const uint32_t tk = s_task[t];
+        const int e = (int)(tk >> 2), j = (int)(tk & 3u);
+        T x[NC], L[K];
+#pragma unroll
+        for (int i = 0; i < NC; ++i) x[i] = s_slot[e * NC + i];
+        const T* gc = m.coords + (int64_t)(es + e) * K;
+#pragma unroll
+        for (int q = 0; q < K; ++q) L[q] = gc[q];
+        s_new[t] = ElemSolve<T, NC>::corner_update(L, x, j, leps);
+      }
+      __syncthreads();
+      for (int t = tid; t < nt; t += BS) {
+        const uint32_t tk = s_task[t];
+        const int e = (int)(tk >> 2), j = (int)(tk & 3u);
+        const T y = s_new[t];
+        if (y != s_slot[e * NC + j]) {
+          s_slot[e * NC + j] = y;
+          s_dirty[e] = 1;
+        }
+      }
+    } else {
+      for (int k = tid; k < nd; k += BS) {
+        const int e = s_work[k];
+        T x[NC], y[NC], L[K];
+#pragma unroll
+        for (int i = 0; i < NC; ++i) x[i] = y[i] = s_slot[e * NC + i];
+        const T* gc = m.coords + (int64_t)(es + e) * K;
+#pragma unroll
+        for (int q = 0; q < K; ++q) L[q] = gc[q];
+        n_solve += ElemSolve<T, NC>::update(L, y, leps);
+        ++n_upd;
+        int changed = 0;
+#pragma unroll
+        for (int i = 0; i < NC; ++i) {
+          if (y[i] != x[i]) { s_slot[e * NC + i] = y[i]; changed = 1; }
+        }
+        s_dirty[e] = (uint8_t)changed;
+      }
     }
     __syncthreads();
     const bool last = (it == niter - 1);
@@ -622,8 +722,9 @@ __device__ bool warm_patch(const DevMesh<T>& m, DevState<T>& s, int p, int mode,
 }
 
 // Fixed-point residual of one patch: for every owned, non-seed vertex with
-// DT <= band, |DT_v - F_v| where F_v is the least value any element copy of
-// v gives it from its other corners (no min with DT_v, no skip). At the
+// DT <= band and no seed among its elements' corners, |DT_v - F_v| where
+// F_v is the least value any element copy of v gives it from its other
+// corners (no min with DT_v, no skip). At the
 // scheme's fixed point F_v == DT_v; a warm value left below it (F_v > DT_v)
 // or above it shows up alike. The block maximum is merged into
 // pc[kPcResid] (non-negative floats order as their bit patterns).
@@ -642,18 +743,34 @@ __device__ void residual_patch(const DevMesh<T>& m, DevState<T>& s, int p, T bw,
   using Bits = typename OrderedBits<T>::type;
   T* s_vtx = (T*)smem;
   Bits* s_F = (Bits*)(s_vtx + nv + nh);
+  uint8_t* s_near = (uint8_t*)(s_F + nv);  // owned vertex shares an element with a seed
   const Bits big = OrderedBits<T>::of((T)kLargeNum);
   for (int i = threadIdx.x; i < nv; i += BS) {
     s_vtx[i] = s.DT[vs + i];
     s_F[i] = big;
+    s_near[i] = 0;
   }
   for (int i = threadIdx.x; i < nh; i += BS) s_vtx[nv + i] = s.DT[m.halo_ids[hs + i]];
   __syncthreads();
   for (int e = threadIdx.x; e < ne; e += BS) {
     const ushort4 c = m.copy_local[es + e];
     T val[NC];
+    bool seeded = false;
 #pragma unroll
-    for (int i = 0; i < NC; ++i) val[i] = s_vtx[corner(c, i)];
+    for (int i = 0; i < NC; ++i) {
+      const int lv = corner(c, i);
+      val[i] = s_vtx[lv];
+      seeded = seeded || s.label[lv < nv ? vs + lv : m.halo_ids[hs + lv - nv]] == kSeedPoint;
+    }
+    // Seeds keep |phi| while their slots drop inside a visit (Q12 of the
+    // reference's scheme), so their neighbours need not match the stored
+    // seed values: elements with a seed are left out.
+    if (seeded) {
+#pragma unroll
+      for (int i = 0; i < NC; ++i)
+        if (corner(c, i) < nv) s_near[corner(c, i)] = 1;
+      continue;
+    }
     const T* gc = m.coords + (int64_t)(es + e) * K;
 #pragma unroll
     for (int k = 0; k < NC; ++k) {
@@ -665,7 +782,7 @@ __device__ void residual_patch(const DevMesh<T>& m, DevState<T>& s, int p, T bw,
   __syncthreads();
   float r = 0.f;
   for (int v = threadIdx.x; v < nv; v += BS) {
-    if (!(s_vtx[v] <= bw) || s.label[vs + v] == kSeedPoint) continue;
+    if (!(s_vtx[v] <= bw) || s.label[vs + v] == kSeedPoint || s_near[v]) continue;
     r = fmaxf(r, (float)d_abs((double)s_vtx[v] - (double)OrderedBits<T>::value(s_F[v])));
   }
   r = -block_min_n<float, BS>(-r, (float*)red);
@@ -917,11 +1034,11 @@ __global__ __launch_bounds__(kBlock) void k_seeds(DevMesh<T> m, DevState<T> s, c
 
 template <typename T, int NC, int BS>
 __global__ __launch_bounds__(BS) void k_sweep(DevMesh<T> m, DevState<T> s, const int32_t* list,
-                                              int niter, int write_out, T tol, double leps) {
+                                              int niter, int write_out, T tol, double leps, int tasks) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ T red[BS / 64];
-  __shared__ int s_ndirty;
-  sweep_patch<T, NC, BS>(m, s, list[blockIdx.x], niter, write_out, tol, leps, smem, red, &s_ndirty);
+  __shared__ int s_ndirty[2];
+  sweep_patch<T, NC, BS>(m, s, list[blockIdx.x], niter, write_out, tol, leps, smem, red, s_ndirty, tasks != 0);
 }
 
 // CopyOutBack (redistance_kernels3d.h:772-783) plus the active-list label of
@@ -1346,6 +1463,7 @@ struct StepArgs {
   double leps;
   double dt;
   int mode;          // LS_MODE_ADVECT or LS_MODE_MCF
+  int tasks;         // FIM sweeps spread (element, corner) solves over the workgroup
   int warm;          // 0: plain; 1: warm start from |phi|; 2: from s.warm
   double warm_margin;// warm value = prediction * (1 + warm_margin)
   float res_tol;     // residual above which a warm start falls back
@@ -1394,7 +1512,7 @@ template <typename T, int NC, int BS>
 __global__ __launch_bounds__(BS) void k_step(DevMesh<T> m, DevState<T> s, StepArgs<T> a) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ T red[BS / 64];
-  __shared__ int s_ndirty;
+  __shared__ int s_ndirty[2];
   const int G = gridDim.x, b = blockIdx.x, tid = threadIdx.x;
   const int64_t gtid = (int64_t)b * BS + tid, gsize = (int64_t)G * BS;
   const int nb_out = 1 - a.nb_in;
@@ -1470,7 +1588,7 @@ __global__ __launch_bounds__(BS) void k_step(DevMesh<T> m, DevState<T> s, StepAr
     }
     const int E = it + 1, par = it & 1;
     for (int i = b; i < nA; i += G)
-      sweep_patch<T, NC, BS>(m, s, cur[i], a.niter, 1, a.tol, a.leps, smem, red, &s_ndirty);
+      sweep_patch<T, NC, BS>(m, s, cur[i], a.niter, 1, a.tol, a.leps, smem, red, s_ndirty, a.tasks != 0);
     grid_barrier(s.bar, G);
     pcl.lap(clk, &tk[kTkVisit]);
     // Every block has read this iteration's list size, so the counters the
@@ -1501,7 +1619,7 @@ __global__ __launch_bounds__(BS) void k_step(DevMesh<T> m, DevState<T> s, StepAr
     const int nF = ld_acquire(&s.pc[kPcFresh + par]);
     if (gtid == 0) s.pc[kPcProbes] += nF;
     for (int i = b; i < nF; i += G)
-      sweep_patch<T, NC, BS>(m, s, s.listF[i], 1, 0, a.tol, a.leps, smem, red, &s_ndirty);
+      sweep_patch<T, NC, BS>(m, s, s.listF[i], 1, 0, a.tol, a.leps, smem, red, s_ndirty, a.tasks != 0);
     grid_barrier(s.bar, G);
     pcl.lap(clk, &tk[kTkProbe]);
     // Next list: unconverged members not probed this iteration, then
@@ -1624,34 +1742,29 @@ class EngineImpl final : public Engine {
     LS_HIP(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, device_));
     // Exact per-patch LDS needs.
     fim_lds_ = adv_lds_ = seed_lds_ = 0;
+    size_t fim_task_lds = 0;
     for (int p = 0; p < P_; ++p) {
       const size_t ne = L.ele_off[p + 1] - L.ele_off[p], nv = L.vert_off[p + 1] - L.vert_off[p];
       const size_t nh = L.halo_off[p + 1] - L.halo_off[p];
       const size_t ns = L.vslot_off[L.vert_off[p + 1]] - L.vslot_off[L.vert_off[p]];
       const size_t fim = NC * ne * sizeof(T) + (2 * nv + nh) * sizeof(T) + 2 * (nv + 1 + ns) + 2 * ne + ne;
+      const size_t fim_tasks = ((fim + 7) & ~size_t(7)) + 4 * (NC * ne + (NC * ne) % 2) + NC * ne * sizeof(T);
+      fim_task_lds = std::max(fim_task_lds, fim_tasks);
       const size_t adv = 2 * NC * ne * sizeof(T) + (nv + nh + ne) * sizeof(T);
       fim_lds_ = std::max(fim_lds_, fim);
       adv_lds_ = std::max(adv_lds_, adv);
       seed_lds_ = std::max(seed_lds_, (nv + nh) * sizeof(T));
     }
-    fim_lds_ = (fim_lds_ + 15) & ~size_t(15);
-    step_lds_ = std::max(fim_lds_, std::max(adv_lds_, seed_lds_));
-    step_lds_ = (step_lds_ + 15) & ~size_t(15);
+    // The task-compacted sweep needs more LDS; patches too large for it use
+    // the per-element sweep.
+    fim_elem_lds_ = (fim_lds_ + 15) & ~size_t(15);
+    fim_task_lds_ = (fim_task_lds + 15) & ~size_t(15);
     adv_lds_ = (adv_lds_ + 15) & ~size_t(15);
     seed_lds_ = (seed_lds_ + 15) & ~size_t(15);
-    if (step_lds_ > (size_t)lds_max) {
-      err = "patch needs " + std::to_string(step_lds_) + " B of LDS, device allows " +
-            std::to_string(lds_max) + " B; use smaller patches";
-      return LS_ERR_LIMIT;
-    }
-
+    lds_max_ = lds_max;
     {
-      int cus = 0, per_cu = 0;
-      LS_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device_));
-      LS_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_step<T, NC, kFimBlock>,
-                                                          kFimBlock, step_lds_));
-      coop_grid_ = cus * per_cu;
-      if (coop_grid_ <= 0) { err = "persistent step kernel does not fit on a CU"; return LS_ERR_LIMIT; }
+      const int rc = size_kernels(err);
+      if (rc != LS_OK) return rc;
     }
     LS_HIP(dev_upload(&d_vert_off_, L.vert_off.data(), L.vert_off.size()));
     LS_HIP(dev_upload(&d_ele_off_, L.ele_off.data(), L.ele_off.size()));
@@ -1946,6 +2059,47 @@ class EngineImpl final : public Engine {
     return LS_OK;
   }
 
+  // LDS of the FIM sweep for the sweep mode, and the persistent grid size.
+  int size_kernels(std::string& err) {
+    const bool fits = fim_task_lds_ <= (size_t)lds_max_;
+    if (sweep_mode_ == LS_SWEEP_TASKS && !fits) {
+      err = "the task-compacted sweep needs " + std::to_string(fim_task_lds_) + " B of LDS, device allows " +
+            std::to_string(lds_max_) + " B";
+      return LS_ERR_LIMIT;
+    }
+    use_tasks_ = (sweep_mode_ != LS_SWEEP_ELEMENTS && fits) ? 1 : 0;
+    fim_lds_ = use_tasks_ ? fim_task_lds_ : fim_elem_lds_;
+    step_lds_ = std::max(fim_lds_, std::max(adv_lds_, seed_lds_));
+    step_lds_ = (step_lds_ + 15) & ~size_t(15);
+    if (step_lds_ > (size_t)lds_max_) {
+      err = "patch needs " + std::to_string(step_lds_) + " B of LDS, device allows " + std::to_string(lds_max_) +
+            " B; use smaller patches";
+      return LS_ERR_LIMIT;
+    }
+    int cus = 0, per_cu = 0;
+    LS_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device_));
+    LS_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_step<T, NC, kFimBlock>, kFimBlock,
+                                                        step_lds_));
+    coop_grid_ = cus * per_cu;
+    if (coop_grid_ <= 0) { err = "persistent step kernel does not fit on a CU"; return LS_ERR_LIMIT; }
+    if (graph_exec_) { LS_HIP(hipGraphExecDestroy(graph_exec_)); graph_exec_ = nullptr; }
+    if (graph_) { LS_HIP(hipGraphDestroy(graph_)); graph_ = nullptr; }
+    return LS_OK;
+  }
+  int set_sweep_mode(int mode, std::string& err) override {
+    const int old = sweep_mode_;
+    sweep_mode_ = mode;
+    if (!uploaded_) return LS_OK;
+    const int rc = size_kernels(err);
+    if (rc != LS_OK) {
+      sweep_mode_ = old;
+      std::string e2;
+      (void)size_kernels(e2);
+    }
+    return rc;
+  }
+  int sweep_tasks() const override { return use_tasks_; }
+
   int set_patch_params(const std::vector<double>& dt, const std::vector<double>& bw, std::string& err) override {
     if (!uploaded_) { err = "layout not built"; return LS_ERR_STATE; }
     if (dt.empty() && bw.empty()) {
@@ -1991,6 +2145,7 @@ class EngineImpl final : public Engine {
     }
     std::memset(&st, 0, sizeof(st));
     st.exec_mode = exec_;
+    st.sweep_tasks = use_tasks_;
     DevMesh<T> m = mesh();
     DevState<T> s = state();
     StepArgs<T> a;
@@ -2005,6 +2160,7 @@ class EngineImpl final : public Engine {
     a.leps = prm.lambda_eps;
     a.dt = prm.dt;
     a.mode = prm.mode;
+    a.tasks = use_tasks_;
     a.warm = prm.warm;
     a.warm_margin = prm.warm_margin;
     a.res_tol = (float)(prm.warm_res_tol > 0 ? prm.warm_res_tol : prm.tol);
@@ -2111,6 +2267,7 @@ class EngineImpl final : public Engine {
     }
     std::memset(&st, 0, sizeof(st));
     st.exec_mode = LS_EXEC_KERNELS;
+    st.sweep_tasks = use_tasks_;
     pc_synced_ = false;
     const DevMesh<T> m = mesh();
     const DevState<T> s = state();
@@ -2140,7 +2297,7 @@ class EngineImpl final : public Engine {
       st.fim_iters++;
       st.patch_visits += nact;
       int ke = kev_begin(0, q);
-      k_sweep<T, NC, kFimBlock><<<nact, kFimBlock, fim_lds_, q>>>(m, s, cur, prm.niter, 1, tol, leps);
+      k_sweep<T, NC, kFimBlock><<<nact, kFimBlock, fim_lds_, q>>>(m, s, cur, prm.niter, 1, tol, leps, use_tasks_);
       kev_end(ke, q);
       k_copyback<T><<<nact, kBlock, 0, q>>>(m, s, cur);
       k_set_counters<<<1, 64, 0, q>>>(d_counters_ + kCntFresh, 2, 0);
@@ -2152,7 +2309,7 @@ class EngineImpl final : public Engine {
       if (nf > 0) {
         st.probes += nf;
         int kp = kev_begin(0, q);
-        k_sweep<T, NC, kFimBlock><<<nf, kFimBlock, fim_lds_, q>>>(m, s, fresh_list, 1, 0, tol, leps);
+        k_sweep<T, NC, kFimBlock><<<nf, kFimBlock, fim_lds_, q>>>(m, s, fresh_list, 1, 0, tol, leps, use_tasks_);
         kev_end(kp, q);
       }
       k_finalize<T><<<grid_for(nact + nf), kBlock, 0, q>>>(s, cur, nact, fresh_list, nf, next);
@@ -2269,6 +2426,10 @@ class EngineImpl final : public Engine {
   int64_t N_ = 0, F_ = 0;
   size_t fim_lds_ = 0, adv_lds_ = 0, seed_lds_ = 0;
   bool coop_ok_ = true;  // cooperative launches accepted so far
+  int use_tasks_ = 0;     // task-compacted FIM sweeps in use
+  int sweep_mode_ = LS_SWEEP_AUTO;
+  size_t fim_elem_lds_ = 0, fim_task_lds_ = 0;
+  int lds_max_ = 0;
   std::vector<int32_t> vert_off_h_, vert_perm_h_;
   int nnb_ = 0, nb_sel_ = 0;
   int32_t *d_vert_off_ = nullptr, *d_ele_off_ = nullptr, *d_halo_off_ = nullptr, *d_halo_ids_ = nullptr;

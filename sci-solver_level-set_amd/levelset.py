@@ -18,6 +18,7 @@ LS_F32, LS_F64 = 0, 1
 LS_MODE_ADVECT, LS_MODE_MCF = 0, 1
 LS_EXEC_PERSISTENT, LS_EXEC_KERNELS, LS_EXEC_GRAPH = 0, 1, 2
 LS_WARM_NONE, LS_WARM_IDENTITY, LS_WARM_LEARNED = 0, 1, 2
+LS_SWEEP_AUTO, LS_SWEEP_ELEMENTS, LS_SWEEP_TASKS = 0, 1, 2
 
 
 class StepParams(C.Structure):
@@ -41,7 +42,8 @@ class Stats(C.Structure):
                 ("ms_ph_seed", C.c_double), ("ms_ph_visit", C.c_double), ("ms_ph_copyback", C.c_double),
                 ("ms_ph_probe", C.c_double), ("ms_ph_next", C.c_double), ("ms_ph_nb", C.c_double),
                 ("ms_ph_advect", C.c_double), ("elem_updates", C.c_int64), ("local_solves", C.c_int64),
-                ("warm_residual", C.c_double), ("warm_patches", C.c_int32), ("warm_fallback", C.c_int32)]
+                ("warm_residual", C.c_double), ("warm_patches", C.c_int32), ("warm_fallback", C.c_int32),
+                ("sweep_tasks", C.c_int32), ("reserved1", C.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -54,7 +56,8 @@ class LayoutInfo(C.Structure):
                 ("max_vertex_slots", C.c_int32), ("dim", C.c_int32)]
 
 
-EXPORTS = ["ls_create", "ls_destroy", "ls_last_error", "ls_set_mesh", "ls_set_precision", "ls_set_execution", "ls_partition_grid",
+EXPORTS = ["ls_create", "ls_destroy", "ls_last_error", "ls_set_mesh", "ls_set_precision", "ls_set_execution",
+           "ls_set_sweep", "ls_partition_grid",
            "ls_partition_rcb", "ls_partition_labels", "ls_get_layout_info", "ls_get_layout",
            "ls_set_phi", "ls_set_velocity", "ls_step", "ls_get_phi", "ls_get_narrowband",
            "ls_set_mesh_batch", "ls_partition_batch", "ls_set_batch_params", "ls_get_batch_nb_counts",
@@ -89,6 +92,7 @@ def lib():
         L.ls_set_mesh.argtypes = [vp, C.c_int, C.c_int64, vp, C.c_int64, vp]
         L.ls_set_precision.argtypes = [vp, C.c_int]
         L.ls_set_execution.argtypes = [vp, C.c_int]
+        L.ls_set_sweep.argtypes = [vp, C.c_int]
         L.ls_partition_grid.argtypes = [vp] + [C.c_int] * 6
         L.ls_partition_rcb.argtypes = [vp, C.c_int]
         L.ls_partition_labels.argtypes = [vp, vp, C.c_int]
@@ -160,6 +164,9 @@ class Solver:
 
     def set_execution(self, mode):
         self._check(self.L.ls_set_execution(self.ctx, int(mode)))
+
+    def set_sweep(self, mode):
+        self._check(self.L.ls_set_sweep(self.ctx, int(mode)))
 
     def set_precision(self, dtype):
         self._check(self.L.ls_set_precision(self.ctx, int(dtype)))
@@ -307,6 +314,18 @@ def solver_for_case(case, device=0, precision=LS_F32, execution=LS_EXEC_PERSISTE
     s.set_phi(case.phi0)
     s.set_velocity(case.vel)
     return s
+
+
+def mlp_init_params(seed=0):
+    """Initial network parameters (flat layout of ls_mlp_set_params):
+    He-normal W1 and W2, W3 scaled by 0.1 so the first corrections are
+    small, zero biases."""
+    rng = np.random.default_rng(seed)
+    w1 = rng.normal(0, np.sqrt(2.0 / 22), (22, 64))
+    w2 = rng.normal(0, np.sqrt(2.0 / 64), (64, 64))
+    w3 = rng.normal(0, 0.1 * np.sqrt(2.0 / 64), (64, 4))
+    return np.concatenate([w1.ravel(), np.zeros(64), w2.ravel(), np.zeros(64), w3.ravel(),
+                           np.zeros(4)]).astype(np.float32)
 
 
 def solver_for_batch(batch, device=0, execution=LS_EXEC_GRAPH, patch_verts=None, labels=None):

@@ -196,3 +196,30 @@ def test_unstructured_fp32_vs_fp64_oracle(oracle_mod):
         phi, nb = gphi, s.narrowband()
     s.close()
     o.close()
+
+
+@pytest.mark.parametrize("execution", [lsb.LS_EXEC_PERSISTENT, lsb.LS_EXEC_KERNELS])
+def test_sweep_modes_bit_exact(execution, oracle_mod):
+    # Per-element and task-compacted FIM sweeps give the same bits as the
+    # oracle (tasks only reorganise which thread runs which corner solve).
+    case = cases.small3d(n=32, shape="torus", block=(8, 8, 4), steps=4)
+    runs = {}
+    for mode in (lsb.LS_SWEEP_ELEMENTS, lsb.LS_SWEEP_TASKS):
+        g = lsb.solver_for_case(case, execution=execution)
+        g.set_sweep(mode)
+        out = []
+        for _ in range(4):
+            st = g.step(lsb.step_params(case.dt, case.bandwidth, case.niter))
+            assert st.sweep_tasks == (1 if mode == lsb.LS_SWEEP_TASKS else 0)
+            out.append(g.phi())
+        g.close()
+        runs[mode] = out
+    o = oracle_mod.Oracle(3, case.xyz, case.cells, case.labels, case.nparts, 32)
+    o.set_phi(case.phi0)
+    o.set_velocity(case.vel)
+    for k in range(4):
+        o.step(oracle_mod.make_params(case.dt, case.bandwidth, case.niter))
+        ref = o.phi().astype(np.float32)
+        for mode, out in runs.items():
+            np.testing.assert_array_equal(out[k].astype(np.float32), ref, err_msg=f"mode {mode} step {k}")
+    o.close()

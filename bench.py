@@ -243,12 +243,26 @@ def mcf_workload(args, lsb, cases):
     return out, tot, ex == lsb.LS_EXEC_KERNELS, cpu
 
 
+def autotune_exec(lsb, case, names, steps=6):
+    """Mean GPU ms/step of each execution mode over a few steps from phi0
+    (after one untimed step); the bench runs the fastest."""
+    out = {}
+    for name, mode in names.items():
+        g = lsb.solver_for_case(case, execution=mode)
+        prm = lsb.step_params(case.dt, case.bandwidth, case.niter)
+        g.step(prm)
+        out[name] = sum(g.step(prm).ms_total for _ in range(steps)) / steps
+        g.close()
+    return out
+
+
 def learned_run(args, lsb, case, exec_mode, plain_value):
     """The learned warm start on the same workload: pairs collected from
     plain steps, GPU training, then the timed steps with warm values from
     the network (prediction time included). A warm start whose fixed-point
     residual over the band exceeds 1e-5 re-runs the plain method."""
-    g = lsb.solver_for_case(case, execution=exec_mode)
+    # warm starts run in the persistent step kernel (here replayed from a graph)
+    g = lsb.solver_for_case(case, execution=lsb.LS_EXEC_GRAPH)
     g.mlp_set_params(lsb.mlp_init_params(5))
     prm = lsb.step_params(case.dt, case.bandwidth, case.niter)
     g.step(prm)
@@ -279,7 +293,7 @@ def learned_run(args, lsb, case, exec_mode, plain_value):
     wall = time.perf_counter() - ts
     g.close()
     value = verts / wall
-    return {"value": value, "unit": "vertex-updates/s", "vs_plain_fim": value / plain_value,
+    return {"value": value, "unit": "vertex-updates/s", "vs_plain_fim": value / plain_value, "execution": "graph",
             "ms_per_step": 1e3 * wall / args.steps, "fim_iters_per_step": iters / args.steps,
             "fallback_steps": fallbacks, "steps": args.steps, "train_pairs": int(X.shape[0]),
             "collect_s": t1 - t0, "train_s": t2 - t1, "train_epochs": args.train_epochs,
@@ -295,8 +309,11 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=8, help="timesteps of the CPU-baseline sample")
     ap.add_argument("--parity-steps", type=int, default=4, help="timesteps checked against the FP64 oracle")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--exec", default="graph", choices=["graph", "persistent", "kernels"],
-                    help="graph: persistent step kernel replayed from a hipGraph (default)")
+    ap.add_argument("--exec", default="auto", choices=["auto", "graph", "persistent", "kernels"],
+                    help="auto (default): the fastest of graph (persistent step kernel replayed from a "
+                         "hipGraph), persistent and kernels over a few trial steps")
+    ap.add_argument("--sweep", default="auto", choices=["auto", "elements", "tasks"],
+                    help="FIM sweep organisation (identical results)")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned warm-start run")
     ap.add_argument("--train-steps", type=int, default=13, help="plain steps whose pairs train the network")
     ap.add_argument("--train-epochs", type=int, default=2)
@@ -341,9 +358,15 @@ def main():
         return
 
     case = cases.config2(steps=args.steps)
-    exec_mode = {"graph": lsb.LS_EXEC_GRAPH, "persistent": lsb.LS_EXEC_PERSISTENT,
-                 "kernels": lsb.LS_EXEC_KERNELS}[args.exec]
+    names = {"graph": lsb.LS_EXEC_GRAPH, "persistent": lsb.LS_EXEC_PERSISTENT, "kernels": lsb.LS_EXEC_KERNELS}
+    tune = None
+    if args.exec == "auto":
+        tune = autotune_exec(lsb, case, names)
+        args.exec = min(tune, key=tune.get)
+    exec_mode = names[args.exec]
+    sweep = {"auto": lsb.LS_SWEEP_AUTO, "elements": lsb.LS_SWEEP_ELEMENTS, "tasks": lsb.LS_SWEEP_TASKS}[args.sweep]
     g = lsb.solver_for_case(case, execution=exec_mode)
+    g.set_sweep(sweep)
     info = g.layout_info()
     prm = lsb.step_params(case.dt, case.bandwidth, case.niter)
     for _ in range(args.warmup):
@@ -370,6 +393,7 @@ def main():
         tot["probes"] += st.probes
         tot["step_ms"] += st.ms_step_kernel
         tot["step_bytes"] += step_bytes(st)
+        tot["sweep_tasks"] = st.sweep_tasks
     wall = time.perf_counter() - t0
     g.close()
 
@@ -404,6 +428,8 @@ def main():
                         "256-vertex patches, band 6h, CFL 0.5, rigid rotation, 100 timesteps",
             "mode": "plain_fim",
             "execution": args.exec,
+            "execution_trial_ms_per_step": tune,
+            "sweep": "tasks" if tot.get("sweep_tasks") else "elements",
             "patches": info["n_patches"],
             "element_copies": info["n_copies"],
             "fim_iters_per_step": tot["fim_iters"] / args.steps,

@@ -35,7 +35,8 @@ enum { LS_MODE_ADVECT = 0, LS_MODE_MCF = 1 };
 enum { LS_EXEC_PERSISTENT = 0, LS_EXEC_KERNELS = 1, LS_EXEC_GRAPH = 2 };
 /* FIM sweep organisation (results are identical): one thread per element
  * copy, or the (element, corner) solves left by the exact lower-bound skip
- * spread one per thread (needs more LDS); AUTO takes the latter when it fits. */
+ * spread one per thread (needs more LDS). AUTO (default) is per element,
+ * the faster of the two on MI355X in measurements so far. */
 enum { LS_SWEEP_AUTO = 0, LS_SWEEP_ELEMENTS = 1, LS_SWEEP_TASKS = 2 };
 /* Warm start of the FIM (no reference counterpart): none, |phi| after the
  * previous step (identity predictor), or the learned predictor's distances

@@ -281,8 +281,10 @@ struct ElemSolve<T, 4> {
 #pragma unroll
     for (int j = 0; j < 6; ++j) L[j] = (double)Lt[j];
     const double Tv[4] = {(double)v[0], (double)v[1], (double)v[2], (double)v[3]};
+    // Selects instead of Tv[k]: a runtime index would put Tv in scratch memory.
+    const double Tk = k == 0 ? Tv[0] : (k == 1 ? Tv[1] : (k == 2 ? Tv[2] : Tv[3]));
     const TetDots d(L);
-    return (T)d_min(Tv[k], d.solve(k, Tv, leps));
+    return (T)d_min(Tk, d.solve(k, Tv, leps));
   }
   __device__ static int update(const T* Lt, T* v, double leps) {
     const int mask = need_mask(Lt, v);
@@ -333,7 +335,8 @@ struct ElemSolve<T, 3> {
     return mask;
   }
   __device__ static T corner_update(const T* ct, const T* v, int k, double leps) {
-    return (T)d_min((double)v[k], (double)corner_value(ct, v, k, leps));
+    const double vk = k == 0 ? (double)v[0] : (k == 1 ? (double)v[1] : (double)v[2]);
+    return (T)d_min(vk, (double)corner_value(ct, v, k, leps));
   }
   __device__ static int update(const T* ct, T* v, double leps) {
     const int mask = need_mask(ct, v);
@@ -1508,7 +1511,10 @@ struct PhaseClock {
   }
 };
 
-template <typename T, int NC, int BS>
+// TASKS selects the sweep organisation and WARM compiles the warm-start
+// phases in: each variant carries only the code it runs, which keeps the
+// register demand of the 1024-thread workgroup within its 128 VGPRs.
+template <typename T, int NC, int BS, bool TASKS, bool WARM>
 __global__ __launch_bounds__(BS) void k_step(DevMesh<T> m, DevState<T> s, StepArgs<T> a) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ T red[BS / 64];
@@ -1524,7 +1530,7 @@ __global__ __launch_bounds__(BS) void k_step(DevMesh<T> m, DevState<T> s, StepAr
   // Phases 0-2 run once, or twice when a warm start fails its residual
   // check: the second attempt is the plain method from the same phi.
   for (int attempt = 0; attempt < 2; ++attempt) {
-    const bool warm = a.warm != 0 && !a.first && attempt == 0;
+    const bool warm = WARM && a.warm != 0 && !a.first && attempt == 0;
 
   // Phase 0: vertex reset and sign save; patch marks; counters.
   for (int64_t v = gtid; v < m.N; v += gsize) init_vertex(s, v);
@@ -1558,7 +1564,7 @@ __global__ __launch_bounds__(BS) void k_step(DevMesh<T> m, DevState<T> s, StepAr
     }
   }
   grid_barrier(s.bar, G);
-  if (warm) {
+  if constexpr (WARM) if (warm) {
     // Phase 1b: warm values over the previous narrowband; every patch that
     // receives one joins the first active list.
     for (int i = b; i < nscan; i += G) {
@@ -1588,7 +1594,7 @@ __global__ __launch_bounds__(BS) void k_step(DevMesh<T> m, DevState<T> s, StepAr
     }
     const int E = it + 1, par = it & 1;
     for (int i = b; i < nA; i += G)
-      sweep_patch<T, NC, BS>(m, s, cur[i], a.niter, 1, a.tol, a.leps, smem, red, s_ndirty, a.tasks != 0);
+      sweep_patch<T, NC, BS>(m, s, cur[i], a.niter, 1, a.tol, a.leps, smem, red, s_ndirty, TASKS);
     grid_barrier(s.bar, G);
     pcl.lap(clk, &tk[kTkVisit]);
     // Every block has read this iteration's list size, so the counters the
@@ -1619,7 +1625,7 @@ __global__ __launch_bounds__(BS) void k_step(DevMesh<T> m, DevState<T> s, StepAr
     const int nF = ld_acquire(&s.pc[kPcFresh + par]);
     if (gtid == 0) s.pc[kPcProbes] += nF;
     for (int i = b; i < nF; i += G)
-      sweep_patch<T, NC, BS>(m, s, s.listF[i], 1, 0, a.tol, a.leps, smem, red, s_ndirty, a.tasks != 0);
+      sweep_patch<T, NC, BS>(m, s, s.listF[i], 1, 0, a.tol, a.leps, smem, red, s_ndirty, TASKS);
     grid_barrier(s.bar, G);
     pcl.lap(clk, &tk[kTkProbe]);
     // Next list: unconverged members not probed this iteration, then
@@ -1652,7 +1658,8 @@ __global__ __launch_bounds__(BS) void k_step(DevMesh<T> m, DevState<T> s, StepAr
   if (!warm) break;
   // Phase 2b: fixed-point residual over the band; above the tolerance the
   // warm start is rejected and the step re-runs plain.
-  for (int p = b; p < m.P; p += G) residual_patch<T, NC, BS>(m, s, p, band_of(m, p, a.bw), a.leps, smem, red);
+  if constexpr (WARM)
+    for (int p = b; p < m.P; p += G) residual_patch<T, NC, BS>(m, s, p, band_of(m, p, a.bw), a.leps, smem, red);
   grid_barrier(s.bar, G);
   const float resid = __uint_as_float((unsigned int)ld_acquire(&s.pc[kPcResid]));
   if (resid <= a.res_tol) break;
@@ -2067,7 +2074,11 @@ class EngineImpl final : public Engine {
             std::to_string(lds_max_) + " B";
       return LS_ERR_LIMIT;
     }
-    use_tasks_ = (sweep_mode_ != LS_SWEEP_ELEMENTS && fits) ? 1 : 0;
+    // AUTO runs per-element sweeps: measured faster on MI355X for 256-vertex
+    // patches (per-launch 184 us against 250 us for the task-compacted sweep
+    // on the 64^3 torus), whose extra compaction pass and per-task reloads of
+    // the element geometry outweigh the divergence they remove.
+    use_tasks_ = (sweep_mode_ == LS_SWEEP_TASKS && fits) ? 1 : 0;
     fim_lds_ = use_tasks_ ? fim_task_lds_ : fim_elem_lds_;
     step_lds_ = std::max(fim_lds_, std::max(adv_lds_, seed_lds_));
     step_lds_ = (step_lds_ + 15) & ~size_t(15);
@@ -2078,13 +2089,21 @@ class EngineImpl final : public Engine {
     }
     int cus = 0, per_cu = 0;
     LS_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device_));
-    LS_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_step<T, NC, kFimBlock>, kFimBlock,
-                                                        step_lds_));
+    per_cu = 1 << 30;
+    for (int v = 0; v < 4; ++v) {
+      int n = 0;
+      LS_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, step_kernel(v & 1, v >> 1), kFimBlock, step_lds_));
+      per_cu = std::min(per_cu, n);
+    }
     coop_grid_ = cus * per_cu;
     if (coop_grid_ <= 0) { err = "persistent step kernel does not fit on a CU"; return LS_ERR_LIMIT; }
     if (graph_exec_) { LS_HIP(hipGraphExecDestroy(graph_exec_)); graph_exec_ = nullptr; }
     if (graph_) { LS_HIP(hipGraphDestroy(graph_)); graph_ = nullptr; }
     return LS_OK;
+  }
+  static const void* step_kernel(int tasks, int warm) {
+    if (tasks) return warm ? (const void*)k_step<T, NC, kFimBlock, true, true> : (const void*)k_step<T, NC, kFimBlock, true, false>;
+    return warm ? (const void*)k_step<T, NC, kFimBlock, false, true> : (const void*)k_step<T, NC, kFimBlock, false, false>;
   }
   int set_sweep_mode(int mode, std::string& err) override {
     const int old = sweep_mode_;
@@ -2187,7 +2206,7 @@ class EngineImpl final : public Engine {
         if (graph_exec_) { LS_HIP(hipGraphExecDestroy(graph_exec_)); graph_exec_ = nullptr; }
         if (graph_) { LS_HIP(hipGraphDestroy(graph_)); graph_ = nullptr; }
         LS_HIP(hipStreamBeginCapture(q, hipStreamCaptureModeThreadLocal));
-        hipError_t le = hipLaunchCooperativeKernel((const void*)k_step<T, NC, kFimBlock>, dim3(coop_grid_),
+        hipError_t le = hipLaunchCooperativeKernel(step_kernel(use_tasks_, a.warm != 0), dim3(coop_grid_),
                                                    dim3(kFimBlock), args, step_lds_, q);
         hipError_t ce = hipStreamEndCapture(q, &graph_);
         if (le != hipSuccess || ce != hipSuccess) {
@@ -2203,7 +2222,7 @@ class EngineImpl final : public Engine {
       if (coop_refused(ge)) return step_kernels(prm, st, err);
       LS_HIP(ge);
     } else {
-      const hipError_t le = hipLaunchCooperativeKernel((const void*)k_step<T, NC, kFimBlock>, dim3(coop_grid_),
+      const hipError_t le = hipLaunchCooperativeKernel(step_kernel(use_tasks_, a.warm != 0), dim3(coop_grid_),
                                                        dim3(kFimBlock), args, step_lds_, q);
       if (coop_refused(le)) return step_kernels(prm, st, err);
       LS_HIP(le);

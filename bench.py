@@ -101,6 +101,20 @@ def lockstep(case, oracle_mod, lsb, precision, steps):
     return res
 
 
+def profiled_traffic(name):
+    """HBM bytes per launch that the committed rocprofv3 counter summary
+    (profiles/<name>.md, FETCH_SIZE and WRITE_SIZE scaled by a 1 GiB copy)
+    records, or None."""
+    path = os.path.join(ROOT, "profiles", name + ".md")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            if line.startswith("HBM traffic per launch (bytes):"):
+                return float(line.split(":")[1])
+    return None
+
+
 def timed_run(g, prm, steps):
     """Times `steps` timesteps; sums the per-step statistics."""
     tot = {"verts": 0, "sweep_ms": 0.0, "sweep_launches": 0, "sweep_bytes": 0, "adv_ms": 0.0,
@@ -409,6 +423,7 @@ def main():
         avg_ms = tot["step_ms"] / args.steps
         avg_bytes = tot["step_bytes"] / args.steps
     achieved = avg_bytes / (avg_ms * 1e-3) / 1e9
+    traffic = profiled_traffic("k_sweep_torus_revolve_64") if exec_mode == lsb.LS_EXEC_KERNELS else None
 
     out = {
         "metric": "narrowband vertex-updates/s, learned warm-start vs plain FIM @ Linf<=1e-5",
@@ -445,7 +460,9 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_source": "profiles/k_sweep_torus_revolve_64.md (rocprofv3 FETCH_SIZE + WRITE_SIZE, "
+                              "copy-calibrated)" if traffic else None,
             "avg_launch_ms": avg_ms,
             "avg_launch_algorithmic_bytes": avg_bytes,
             "launches": launches,

@@ -221,10 +221,16 @@ def mcf_workload(args, lsb, cases):
     """Mean-curvature flow of 8 spheres on a 192^3 Kuhn grid (41.8M tets)."""
     case = cases.config3(steps=args.steps)
     print(f"mcf: {case.xyz.shape[0]} vertices, {case.cells.shape[0]} tets", file=sys.stderr, flush=True)
-    ex = {"graph": lsb.LS_EXEC_GRAPH, "persistent": lsb.LS_EXEC_PERSISTENT, "kernels": lsb.LS_EXEC_KERNELS}[args.exec]
-    g = lsb.solver_for_case(case, execution=ex)
+    names = {"graph": lsb.LS_EXEC_GRAPH, "persistent": lsb.LS_EXEC_PERSISTENT, "kernels": lsb.LS_EXEC_KERNELS}
+    g = lsb.solver_for_case(case)
     info = g.layout_info()
     prm = lsb.step_params(case.dt, case.bandwidth, case.niter, mode=lsb.LS_MODE_MCF)
+    tune = None
+    if args.exec == "auto":
+        tune = autotune_exec(lsb, case, names, steps=3, g=g, prm=prm)
+        args.exec = min(tune, key=tune.get)
+    ex = names[args.exec]
+    g.set_execution(ex)
     for _ in range(args.warmup):
         g.step(prm)
     g.set_phi(case.phi0)
@@ -232,7 +238,8 @@ def mcf_workload(args, lsb, cases):
     g.close()
     out = {"workload": f"{case.name}: 192^3 Kuhn tets ({case.xyz.shape[0]} verts, {case.cells.shape[0]} tets), "
                        f"8 spheres, mean-curvature flow, dt = h^2/8, band 8h, {args.steps} timesteps",
-           "execution": args.exec, "patches": info["n_patches"], "element_copies": info["n_copies"]}
+           "execution": args.exec, "execution_trial_ms_per_step": tune, "patches": info["n_patches"],
+           "element_copies": info["n_copies"]}
     cpu = None
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -257,15 +264,20 @@ def mcf_workload(args, lsb, cases):
     return out, tot, ex == lsb.LS_EXEC_KERNELS, cpu
 
 
-def autotune_exec(lsb, case, names, steps=6):
+def autotune_exec(lsb, case, names, steps=6, g=None, prm=None):
     """Mean GPU ms/step of each execution mode over a few steps from phi0
-    (after one untimed step); the bench runs the fastest."""
+    (after one untimed step) on one solver; the bench runs the fastest."""
+    own = g is None
+    if own:
+        g = lsb.solver_for_case(case)
+    prm = prm or lsb.step_params(case.dt, case.bandwidth, case.niter, mode=case.mode)
     out = {}
     for name, mode in names.items():
-        g = lsb.solver_for_case(case, execution=mode)
-        prm = lsb.step_params(case.dt, case.bandwidth, case.niter)
+        g.set_execution(mode)
+        g.set_phi(case.phi0)
         g.step(prm)
         out[name] = sum(g.step(prm).ms_total for _ in range(steps)) / steps
+    if own:
         g.close()
     return out
 

@@ -113,3 +113,32 @@ def test_errors_are_returned_not_raised_across_abi():
     bad[0, 0] = 10 ** 6
     with pytest.raises(lsb.LevelSetError, match="out of range"):
         s.set_mesh(3, xyz, bad)
+
+
+def test_python_structs_match_c_header(tmp_path):
+    """ctypes mirrors of ls_step_params / ls_stats / ls_layout_info have the C
+    sizes and field offsets of include/levelset_hip.h."""
+    import ctypes as C
+    import os
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    structs = {"ls_step_params": lsb.StepParams, "ls_stats": lsb.Stats, "ls_layout_info": lsb.LayoutInfo}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "levelset_hip.h"', "int main(void) {"]
+    for cname, py in structs.items():
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for f, _ in py._fields_:
+            lines.append(f'printf("{cname} {f} %zu\\n", offsetof({cname}, {f}));')
+    lines.append("return 0; }")
+    src = tmp_path / "abi.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "abi"
+    subprocess.check_call(["gcc", "-I", os.path.join(root, "include"), str(src), "-o", str(exe)])
+    got = {}
+    for line in subprocess.check_output([str(exe)], text=True).splitlines():
+        cname, field, val = line.split()
+        got[(cname, field)] = int(val)
+    for cname, py in structs.items():
+        assert got[(cname, "size")] == C.sizeof(py), cname
+        for f, _ in py._fields_:
+            assert got[(cname, f)] == getattr(py, f).offset, (cname, f)
+    assert lsb.MLP_PARAMS == 5892

@@ -89,15 +89,17 @@ bool roots(T p, T q, T r, T& x1, T& x2) {
   if (delta < 0.0 && delta >= -kDiscRel * (dq * dq + std::fabs((D)4 * dp * dr))) delta = 0.0;
   if (!(delta >= 0.0)) return false;
   const D sq = std::sqrt(delta);
-  x1 = (T)((-dq + sq) / ((D)2 * dp));
-  x2 = (T)((-dq - sq) / ((D)2 * dp));
+  const D inv = (D)1 / ((D)2 * dp);  // one division for both roots
+  x1 = (T)((-dq + sq) * inv);
+  x2 = (T)((-dq - sq) * inv);
   return true;
 }
 
 // Tet Hopf-Lax local solve for D from A, B, C given six edge dot products
 // (src/core3d/include/redistance_kernels3d.h:124-285): three face (edge)
 // quadratics, three vertex candidates, and one interior quadratic whose second
-// barycentric coordinate is (s*l1 + t) / (h + leps).
+// barycentric coordinate is (s*l1 + t) / (h + leps), evaluated as a product
+// with the reciprocal.
 template <typename T>
 T tet_solve(T TA, T TB, T TC, T ACAC, T ACBC, T ACCD, T BCBC, T BCCD, T CDCD, T leps) {
   const T big = (T)kLarge;
@@ -184,9 +186,10 @@ T tet_solve(T TA, T TB, T TC, T ACAC, T ACBC, T ACCD, T BCBC, T BCCD, T CDCD, T 
     const T r = c2 * t * t + c5 * t * h + c6 * h * h;
     if (roots(p, q, r, x1, x2)) {
       const T cand[2] = {x1, x2};
+      const T ihd = (T)1 / (h + leps);
       for (int k = 0; k < 2; ++k) {
         const T l1 = cand[k];
-        const T l2 = (s * l1 + t) / (h + leps);
+        const T l2 = (s * l1 + t) * ihd;
         const T l3 = (T)1 - l1 - l2;
         if (l1 >= (T)0 && l1 <= (T)1 && l2 >= (T)0 && l2 <= (T)1 && l3 >= (T)0 && l3 <= (T)1) {
           const T TE = TA * l1 + TB * l2 + TC * l3;

@@ -55,8 +55,9 @@ __device__ __forceinline__ bool quad_roots(T p, T q, T r, T& x1, T& x2) {
   if (delta < 0.0 && delta >= -kDiscRel * (dq * dq + d_abs(4.0 * dp * dr))) delta = 0.0;
   if (!(delta >= 0.0)) return false;
   const double sq = sqrt(delta);
-  x1 = (T)((-dq + sq) / (2.0 * dp));
-  x2 = (T)((-dq - sq) / (2.0 * dp));
+  const double inv = 1.0 / (2.0 * dp);  // one division for both roots
+  x1 = (T)((-dq + sq) * inv);
+  x2 = (T)((-dq - sq) * inv);
   return true;
 }
 
@@ -142,11 +143,11 @@ __device__ T tet_solve(T TA, T TB, T TC, T ACAC, T ACBC, T ACCD, T BCBC, T BCCD,
     const T q = c2 * (T)2 * s * t + c3 * t * h + c4 * h * h + c5 * s * h;
     const T r = c2 * t * t + c5 * t * h + c6 * h * h;
     if (quad_roots(p, q, r, x1, x2)) {
-      const T hd = h + leps;
+      const T ihd = d_div((T)1, h + leps);
 #pragma unroll
       for (int k = 0; k < 2; ++k) {
         const T l1 = k == 0 ? x1 : x2;
-        const T l2 = d_div(s * l1 + t, hd);
+        const T l2 = (s * l1 + t) * ihd;
         const T l3 = (T)1 - l1 - l2;
         if (l1 >= (T)0 && l1 <= (T)1 && l2 >= (T)0 && l2 <= (T)1 && l3 >= (T)0 && l3 <= (T)1) {
           const T TE = TA * l1 + TB * l2 + TC * l3;

@@ -26,6 +26,7 @@ struct DevMesh {
   const uint16_t* vslots;      // patch-local slot ids per owned vertex
   const T* coords;             // F x (6 | 3), local coordinates
   const T* sigma;              // F x (3 | 2), velocity in the local frame
+  const T* hgt;                // F x 4 corner heights over the opposite face (tets), or null
   const int32_t* padj_off;     // P+1
   const int32_t* padj;
   // Per-patch timestep and narrowband width for batches of meshes with

@@ -247,23 +247,26 @@ struct TetDots {
 template <typename T>
 struct ElemSolve<T, 4> {
   static constexpr int kCoords = 6;
-  // Bit k set: corner k must be solved (its value is not below the exact
-  // lower bound min(other values) + height of corner k over its face).
-  __device__ static int need_mask(const T* Lt, const T* v) {
-    double L[6];
-#pragma unroll
-    for (int k = 0; k < 6; ++k) L[k] = (double)Lt[k];
-    const double Tv[4] = {(double)v[0], (double)v[1], (double)v[2], (double)v[3]};
-    // Heights of each corner over its opposite face, in the local frame
-    // A=(0,0,0), B=(L0,0,0), C=(L1,L2,0), D=(L3,L4,L5).
+  // Heights of each corner over its opposite face, in the local frame
+  // A=(0,0,0), B=(L0,0,0), C=(L1,L2,0), D=(L3,L4,L5).
+  __device__ static void heights(const double* L, double* h) {
     const double v6 = d_abs(L[0] * L[2] * L[5]);
     const double nAx = L[2] * L[5], nAy = -(L[1] - L[0]) * L[5], nAz = (L[1] - L[0]) * L[4] - L[2] * (L[3] - L[0]);
     const double nBx = L[2] * L[5], nBy = -L[1] * L[5], nBz = L[1] * L[4] - L[2] * L[3];
-    double h[4];
     h[0] = v6 / d_sqrt(nAx * nAx + nAy * nAy + nAz * nAz);
     h[1] = v6 / d_sqrt(nBx * nBx + nBy * nBy + nBz * nBz);
     h[2] = d_abs(L[2] * L[5]) / d_sqrt(L[4] * L[4] + L[5] * L[5]);
     h[3] = d_abs(L[5]);
+  }
+  // Bit k set: corner k must be solved (its value is not below the exact
+  // lower bound min(other values) + height of corner k over its face).
+  // `hs` holds the copy's heights rounded toward zero (a smaller height only
+  // solves more corners, so the skip stays exact).
+  __device__ static int need_mask(const T* Lt, const T* hs, const T* v) {
+    double L[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) L[k] = (double)Lt[k];
+    const double Tv[4] = {(double)v[0], (double)v[1], (double)v[2], (double)v[3]};
     const double scale = L[0] + d_abs(L[1]) + d_abs(L[2]) + d_abs(L[3]) + d_abs(L[4]) + d_abs(L[5]);
     int mask = 0;
 #pragma unroll
@@ -272,7 +275,7 @@ struct ElemSolve<T, 4> {
 #pragma unroll
       for (int j = 0; j < 4; ++j)
         if (j != k) { mo = d_min(mo, Tv[j]); Mo = d_max(Mo, Tv[j]); }
-      if (must_solve(Tv[k], mo, Mo, h[k], scale)) mask |= 1 << k;
+      if (must_solve(Tv[k], mo, Mo, (double)hs[k], scale)) mask |= 1 << k;
     }
     return mask;
   }
@@ -287,8 +290,8 @@ struct ElemSolve<T, 4> {
     const TetDots d(L);
     return (T)d_min(Tk, d.solve(k, Tv, leps));
   }
-  __device__ static int update(const T* Lt, T* v, double leps) {
-    const int mask = need_mask(Lt, v);
+  __device__ static int update(const T* Lt, const T* hs, T* v, double leps) {
+    const int mask = need_mask(Lt, hs, v);
     if (!mask) return 0;
     double L[6];
 #pragma unroll
@@ -317,7 +320,7 @@ struct ElemSolve<T, 4> {
 template <typename T>
 struct ElemSolve<T, 3> {
   static constexpr int kCoords = 3;
-  __device__ static int need_mask(const T* ct, const T* v) {
+  __device__ static int need_mask(const T* ct, const T*, const T* v) {
     const double c[3] = {(double)ct[0], (double)ct[1], (double)ct[2]};
     const double LenAB = d_abs(c[0]);
     const double LenBC = d_sqrt(c[2] * c[2] + (c[1] - c[0]) * (c[1] - c[0]));
@@ -339,8 +342,8 @@ struct ElemSolve<T, 3> {
     const double vk = k == 0 ? (double)v[0] : (k == 1 ? (double)v[1] : (double)v[2]);
     return (T)d_min(vk, (double)corner_value(ct, v, k, leps));
   }
-  __device__ static int update(const T* ct, T* v, double leps) {
-    const int mask = need_mask(ct, v);
+  __device__ static int update(const T* ct, const T*, T* v, double leps) {
+    const int mask = need_mask(ct, nullptr, v);
     if (!mask) return 0;
     const double c[3] = {(double)ct[0], (double)ct[1], (double)ct[2]};
     const double LenAB = d_abs(c[0]);
@@ -365,6 +368,26 @@ struct ElemSolve<T, 3> {
     return (T)d_min(kLargeNum, r);
   }
 };
+
+// Stored corner heights of a tet copy (null for triangles).
+template <typename T>
+__device__ __forceinline__ const T* copy_heights(const DevMesh<T>& m, int64_t c) {
+  return m.hgt ? m.hgt + c * 4 : nullptr;
+}
+
+// Corner heights of every tet copy, computed once at upload with the same
+// code the sweeps used to evaluate per visit; FP32 storage rounds toward zero.
+template <typename T>
+__global__ void k_heights(const T* coords, int64_t F, T* hgt) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= F) return;
+  double L[6], h[4];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) L[k] = (double)coords[c * 6 + k];
+  ElemSolve<T, 4>::heights(L, h);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) hgt[c * 4 + k] = sizeof(T) == 4 ? (T)__double2float_rz(h[k]) : (T)h[k];
+}
 
 template <typename T>
 __device__ __forceinline__ T wave_min(T v) {
@@ -553,7 +576,7 @@ __device__ void sweep_patch(const DevMesh<T>& m, DevState<T>& s, int p, int nite
           const T* gc = m.coords + (int64_t)(es + e) * K;
 #pragma unroll
           for (int q = 0; q < K; ++q) L[q] = gc[q];
-          mask = ElemSolve<T, NC>::need_mask(L, x);
+          mask = ElemSolve<T, NC>::need_mask(L, copy_heights(m, es + e), x);
           s_dirty[e] = 0;
           ++n_upd;
         }
@@ -607,7 +630,7 @@ __device__ void sweep_patch(const DevMesh<T>& m, DevState<T>& s, int p, int nite
         const T* gc = m.coords + (int64_t)(es + e) * K;
 #pragma unroll
         for (int q = 0; q < K; ++q) L[q] = gc[q];
-        n_solve += ElemSolve<T, NC>::update(L, y, leps);
+        n_solve += ElemSolve<T, NC>::update(L, copy_heights(m, es + e), y, leps);
         ++n_upd;
         int changed = 0;
 #pragma unroll
@@ -1787,6 +1810,12 @@ class EngineImpl final : public Engine {
       std::vector<T> c(L.local_coords.size());
       for (size_t i = 0; i < c.size(); ++i) c[i] = (T)L.local_coords[i];
       LS_HIP(dev_upload(&d_coords_, c.data(), c.size()));
+      if (NC == 4 && F_ > 0) {
+        LS_HIP(dev_alloc(&d_hgt_, (size_t)F_ * 4));
+        k_heights<T><<<(int)((F_ + 255) / 256), 256, 0, stream_>>>(d_coords_, F_, d_hgt_);
+        LS_HIP(hipGetLastError());
+        LS_HIP(hipStreamSynchronize(stream_));
+      }
     }
     LS_HIP(dev_alloc(&d_sigma_, (size_t)F_ * (NC - 1)));
     LS_HIP(hipMemset(d_sigma_, 0, (size_t)F_ * (NC - 1) * sizeof(T)));
@@ -2400,6 +2429,7 @@ class EngineImpl final : public Engine {
     m.vert_off = d_vert_off_; m.ele_off = d_ele_off_; m.halo_off = d_halo_off_; m.halo_ids = d_halo_ids_;
     m.copy_local = d_copy_local_; m.vslot_off = d_vslot_off_; m.vslots = d_vslots_;
     m.coords = d_coords_; m.sigma = d_sigma_; m.padj_off = d_padj_off_; m.padj = d_padj_;
+    m.hgt = d_hgt_;
     m.patch_dt = use_patch_prm_ ? d_patch_dt_ : nullptr;
     m.patch_bw = use_patch_prm_ ? d_patch_bw_ : nullptr;
     return m;
@@ -2428,7 +2458,7 @@ class EngineImpl final : public Engine {
                     d_DT_out_, d_label_, d_tsign_, d_bcon_, d_bmin_, d_act_, d_fresh_, d_seedflag_,
                     d_counters_, d_sizes_, d_nb_, d_scan_all_, d_lists_[0], d_lists_[1], d_lists_[2],
                     d_lists_[3], d_inA_, d_freshE_, d_listF_, d_pc_, d_ticks_, d_bar_,
-                    d_patch_dt_, d_patch_bw_, d_warm_, d_mlp_prm_, d_mlp_m_, d_mlp_v_, d_mlp_grad_,
+                    d_patch_dt_, d_patch_bw_, d_warm_, d_hgt_, d_mlp_prm_, d_mlp_m_, d_mlp_v_, d_mlp_grad_,
                     d_mlp_b_, d_mlp_w1_, d_mlp_w2_, d_mlp_w3_, d_ds_x_, d_ds_y_, d_pend_};
     if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
     if (graph_) (void)hipGraphDestroy(graph_);
@@ -2458,6 +2488,7 @@ class EngineImpl final : public Engine {
   uint16_t* d_vslots_ = nullptr;
   int32_t *d_padj_off_ = nullptr, *d_padj_ = nullptr;
   T *d_coords_ = nullptr, *d_sigma_ = nullptr;
+  T* d_hgt_ = nullptr;  // tet copies: corner heights (F x 4)
   T* d_warm_ = nullptr;  // learned warm-start distances (N), written by the predictor
   float *d_mlp_prm_ = nullptr, *d_mlp_m_ = nullptr, *d_mlp_v_ = nullptr, *d_mlp_grad_ = nullptr, *d_mlp_b_ = nullptr;
   uint16_t *d_mlp_w1_ = nullptr, *d_mlp_w2_ = nullptr, *d_mlp_w3_ = nullptr;

@@ -1527,11 +1527,23 @@ __device__ __forceinline__ void grid_barrier(unsigned int* bar, unsigned int nbl
 }
 
 // Block 0 accumulates barrier-to-barrier phase times.
+// Phase timer kept in LDS: tk[0..kNumTicks) accumulate, tk[kNumTicks] holds
+// the last stamp. Only the timing thread touches it, so no thread carries
+// timer registers through the launch.
 struct PhaseClock {
-  unsigned long long t;
-  __device__ void start(bool on) { if (on) t = wall_clock64(); }
-  __device__ void lap(bool on, unsigned long long* acc) {
-    if (on) { const unsigned long long n = wall_clock64(); *acc += n - t; t = n; }
+  unsigned long long* tk;
+  __device__ void start(bool on) {
+    if (on) {
+      for (int i = 0; i < kNumTicks; ++i) tk[i] = 0ull;
+      tk[kNumTicks] = wall_clock64();
+    }
+  }
+  __device__ void lap(bool on, int i) {
+    if (on) {
+      const unsigned long long n = wall_clock64();
+      tk[i] += n - tk[kNumTicks];
+      tk[kNumTicks] = n;
+    }
   }
 };
 
@@ -1547,9 +1559,9 @@ __global__ __launch_bounds__(BS) void k_step(DevMesh<T> m, DevState<T> s, StepAr
   const int64_t gtid = (int64_t)b * BS + tid, gsize = (int64_t)G * BS;
   const int nb_out = 1 - a.nb_in;
   const bool clk = (gtid == 0);
-  PhaseClock pcl;
+  __shared__ unsigned long long tk[kNumTicks + 1];
+  PhaseClock pcl{tk};
   pcl.start(clk);
-  unsigned long long tk[kNumTicks] = {0, 0, 0, 0, 0, 0, 0, 0};
 
   // Phases 0-2 run once, or twice when a warm start fails its residual
   // check: the second attempt is the plain method from the same phi.
@@ -1603,7 +1615,7 @@ __global__ __launch_bounds__(BS) void k_step(DevMesh<T> m, DevState<T> s, StepAr
     }
     grid_barrier(s.bar, G);
   }
-  pcl.lap(clk, &tk[kTkSeed]);
+  pcl.lap(clk, kTkSeed);
   // Phase 2: FIM (redistance3d.cu:85-224). Epoch E tags the members of the
   // current list (inA[p] == E) and the patches probed in this iteration
   // (freshE[q] == E).
@@ -1620,7 +1632,7 @@ __global__ __launch_bounds__(BS) void k_step(DevMesh<T> m, DevState<T> s, StepAr
     for (int i = b; i < nA; i += G)
       sweep_patch<T, NC, BS>(m, s, cur[i], a.niter, 1, a.tol, a.leps, smem, red, s_ndirty, TASKS);
     grid_barrier(s.bar, G);
-    pcl.lap(clk, &tk[kTkVisit]);
+    pcl.lap(clk, kTkVisit);
     // Every block has read this iteration's list size, so the counters the
     // next iteration appends to can be cleared.
     if (gtid == 0) {
@@ -1645,13 +1657,13 @@ __global__ __launch_bounds__(BS) void k_step(DevMesh<T> m, DevState<T> s, StepAr
       }
     }
     grid_barrier(s.bar, G);
-    pcl.lap(clk, &tk[kTkCopyback]);
+    pcl.lap(clk, kTkCopyback);
     const int nF = ld_acquire(&s.pc[kPcFresh + par]);
     if (gtid == 0) s.pc[kPcProbes] += nF;
     for (int i = b; i < nF; i += G)
       sweep_patch<T, NC, BS>(m, s, s.listF[i], 1, 0, a.tol, a.leps, smem, red, s_ndirty, TASKS);
     grid_barrier(s.bar, G);
-    pcl.lap(clk, &tk[kTkProbe]);
+    pcl.lap(clk, kTkProbe);
     // Next list: unconverged members not probed this iteration, then
     // unconverged probed patches.
     for (int64_t i = gtid; i < (int64_t)nA + nF; i += gsize) {
@@ -1670,7 +1682,7 @@ __global__ __launch_bounds__(BS) void k_step(DevMesh<T> m, DevState<T> s, StepAr
       }
     }
     grid_barrier(s.bar, G);
-    pcl.lap(clk, &tk[kTkNext]);
+    pcl.lap(clk, kTkNext);
     nA = ld_acquire(&s.pc[kPcNext + par]);
     int32_t* t = cur;
     cur = nxt;
@@ -1700,7 +1712,7 @@ __global__ __launch_bounds__(BS) void k_step(DevMesh<T> m, DevState<T> s, StepAr
     }
   }
   grid_barrier(s.bar, G);
-  pcl.lap(clk, &tk[kTkNb]);
+  pcl.lap(clk, kTkNb);
 
   // Phase 4: advection over the new narrowband, then CopyOutBack_levelset3d.
   const int nnb = ld_acquire(&s.pc[kPcNb + nb_out]);
@@ -1713,7 +1725,7 @@ __global__ __launch_bounds__(BS) void k_step(DevMesh<T> m, DevState<T> s, StepAr
     }
     grid_barrier(s.bar, G);
   }
-  pcl.lap(clk, &tk[kTkAdvect]);
+  pcl.lap(clk, kTkAdvect);
   if (clk)
     for (int i = 0; i < kNumTicks; ++i) s.ticks[i] = tk[i];
 }

@@ -223,3 +223,47 @@ def test_sweep_modes_bit_exact(execution, oracle_mod):
         for mode, out in runs.items():
             np.testing.assert_array_equal(out[k].astype(np.float32), ref, err_msg=f"mode {mode} step {k}")
     o.close()
+
+
+def test_patch_too_large_is_reported_and_recoverable(oracle_mod):
+    # One patch over a 12^3 Kuhn grid (7986 element copies) needs ~230 KB of
+    # LDS, more than a workgroup has: partitioning reports LS_ERR_LIMIT, and
+    # the context works after a finer partition.
+    case = cases.small3d(n=12, shape="sphere", block=(4, 4, 4), steps=1)
+    s = lsb.Solver(device=0)
+    s.set_mesh(3, case.xyz, case.cells)
+    with pytest.raises(lsb.LevelSetError, match="rc=-4"):
+        s.partition_labels(np.zeros(case.xyz.shape[0], np.int32), 1)
+    s.partition_labels(case.labels, case.nparts)
+    s.set_phi(case.phi0)
+    s.set_velocity(case.vel)
+    s.step(lsb.step_params(case.dt, case.bandwidth, case.niter))
+    o = oracle_mod.Oracle(3, case.xyz, case.cells, case.labels, case.nparts, 32)
+    o.set_phi(case.phi0)
+    o.set_velocity(case.vel)
+    o.step(oracle_mod.make_params(case.dt, case.bandwidth, case.niter))
+    np.testing.assert_array_equal(s.phi().astype(np.float32), o.phi().astype(np.float32))
+    s.close()
+    o.close()
+
+
+def test_step_errors_are_returned():
+    case = cases.small3d(n=8, shape="sphere", block=(4, 4, 4), steps=1)
+    s = lsb.Solver(device=0)
+    with pytest.raises(lsb.LevelSetError, match="partition"):
+        s.step(lsb.step_params(case.dt, case.bandwidth, case.niter))
+    s.set_mesh(3, case.xyz, case.cells)
+    s.partition_labels(case.labels, case.nparts)
+    s.set_phi(case.phi0)
+    s.set_velocity(case.vel)
+    with pytest.raises(lsb.LevelSetError, match="niter"):
+        s.step(lsb.step_params(case.dt, case.bandwidth, 0))
+    with pytest.raises(lsb.LevelSetError, match="mode"):
+        s.step(lsb.step_params(case.dt, case.bandwidth, 5, mode=7))
+    s.set_execution(lsb.LS_EXEC_KERNELS)
+    with pytest.raises(lsb.LevelSetError, match="warm"):
+        s.step(lsb.step_params(case.dt, case.bandwidth, 5, warm=lsb.LS_WARM_IDENTITY))
+    s.set_execution(lsb.LS_EXEC_PERSISTENT)
+    st = s.step(lsb.step_params(case.dt, case.bandwidth, 5))
+    assert st.nb_count > 0
+    s.close()

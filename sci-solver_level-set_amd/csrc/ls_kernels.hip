@@ -1,16 +1,21 @@
 // gfx950 kernels for the level-set per-timestep hot path.
 //
-// One workgroup (4 x wave64) owns one patch at a time: the patch's owned and
-// halo vertex values, its element-copy slots and its vertex slot lists are
-// staged in LDS, so HBM sees each patch's arrays once per visit and every
-// halo vertex once per patch (the reference re-reads halo values per element
-// copy, src/core3d/include/redistance_kernels3d.h:342-358).
+// One workgroup owns one patch at a time (1024 threads for FIM sweeps, 256
+// for the other phases): the patch's owned and halo vertex values, its
+// element-copy slots and its vertex slot lists are staged in LDS, so HBM sees
+// each patch's arrays once per visit and every halo vertex once per patch
+// (the reference re-reads halo values per element copy,
+// src/core3d/include/redistance_kernels3d.h:342-358).
 //
-// Arithmetic follows the reference scheme operation by operation in the
-// working type T (FP32 for tets, FP64 for triangles), with contraction off,
-// correctly rounded division and square root, and FP64 for the tet solver's
-// quadratic discriminant and roots.
-#include <hip/hip_cooperative_groups.h>
+// Values are stored in the working type T (FP32 or FP64 for tets, FP64 for
+// triangles); every element update (local solves, advection, curvature) is
+// evaluated in FP64 from the stored values and rounded to T, with
+// contraction off and correctly rounded division and square root.
+//
+// Two execution paths give identical results: one launch per phase with the
+// FIM loop on the host (k_seeds, k_sweep, k_copyback, k_activate,
+// k_finalize, k_narrowband, k_advect), and one persistent cooperative launch
+// per timestep (k_step) that runs the FIM loop on the device.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>

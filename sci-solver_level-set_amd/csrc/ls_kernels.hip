@@ -238,14 +238,24 @@ struct TetDots {
     BDBD = BCBD + CDBD;
     BCBC = BCBD - BCCD;
   }
+  // The solve for corner k as one body with its arguments chosen by selects,
+  // so a wave whose lanes solve different corners runs it once per round:
+  //   k=3: (TA, TB, TC; ACAC,  ACBC,  ACCD, BCBC,  BCCD, CDCD)
+  //   k=0: (TB, TD, TC; BCBC, -BCCD, -ACBC, CDCD,  ACCD, ACAC)
+  //   k=1: (TA, TD, TC; ACAC, -ACCD, -ACBC, CDCD,  BCCD, BCBC)
+  //   k=2: (TA, TB, TD; ADAD,  ADBD, -ADCD, BDBD, -CDBD, CDCD)
   __device__ double solve(int k, const double* Tv, double leps) const {
-    const double TA = Tv[0], TB = Tv[1], TC = Tv[2], TD = Tv[3];
-    switch (k) {
-      case 3: return tet_solve(TA, TB, TC, ACAC, ACBC, ACCD, BCBC, BCCD, CDCD, leps);
-      case 0: return tet_solve(TB, TD, TC, BCBC, -BCCD, -ACBC, CDCD, ACCD, ACAC, leps);
-      case 1: return tet_solve(TA, TD, TC, ACAC, -ACCD, -ACBC, CDCD, BCCD, BCBC, leps);
-      default: return tet_solve(TA, TB, TD, ADAD, ADBD, -ADCD, BDBD, -CDBD, CDCD, leps);
-    }
+    const bool k0 = k == 0, k1 = k == 1, k2 = k == 2, k01 = k0 || k1;
+    const double a = k0 ? Tv[1] : Tv[0];
+    const double b = k01 ? Tv[3] : Tv[1];
+    const double c = k2 ? Tv[3] : Tv[2];
+    const double d1 = k0 ? BCBC : (k2 ? ADAD : ACAC);
+    const double d2 = k0 ? -BCCD : (k1 ? -ACCD : (k2 ? ADBD : ACBC));
+    const double d3 = k01 ? -ACBC : (k2 ? -ADCD : ACCD);
+    const double d4 = k01 ? CDCD : (k2 ? BDBD : BCBC);
+    const double d5 = k0 ? ACCD : (k2 ? -CDBD : BCCD);
+    const double d6 = k0 ? ACAC : (k1 ? BCBC : CDCD);
+    return tet_solve(a, b, c, d1, d2, d3, d4, d5, d6, leps);
   }
 };
 
@@ -303,11 +313,18 @@ struct ElemSolve<T, 4> {
     for (int k = 0; k < 6; ++k) L[k] = (double)Lt[k];
     const double Tv[4] = {(double)v[0], (double)v[1], (double)v[2], (double)v[3]};
     const TetDots d(L);
-    const double TA = Tv[0], TB = Tv[1], TC = Tv[2], TD = Tv[3];
-    if (mask & 8) v[3] = (T)d_min(TD, d.solve(3, Tv, leps));
-    if (mask & 1) v[0] = (T)d_min(TA, d.solve(0, Tv, leps));
-    if (mask & 2) v[1] = (T)d_min(TB, d.solve(1, Tv, leps));
-    if (mask & 4) v[2] = (T)d_min(TC, d.solve(2, Tv, leps));
+    // Every solve reads the values from before the update, so the order of
+    // the corners does not matter; each lane walks only its own corners.
+#pragma nounroll
+    for (int m = mask; m; m &= m - 1) {
+      const int k = __ffs(m) - 1;
+      const double Tk = k == 0 ? Tv[0] : (k == 1 ? Tv[1] : (k == 2 ? Tv[2] : Tv[3]));
+      const T y = (T)d_min(Tk, d.solve(k, Tv, leps));
+      if (k == 0) v[0] = y;
+      else if (k == 1) v[1] = y;
+      else if (k == 2) v[2] = y;
+      else v[3] = y;
+    }
     return __popc(mask);
   }
   // The value the element gives corner k from the other three (no min with

@@ -305,9 +305,8 @@ struct ElemSolve<T, 4> {
     const TetDots d(L);
     return (T)d_min(Tk, d.solve(k, Tv, leps));
   }
-  __device__ static int update(const T* Lt, const T* hs, T* v, double leps) {
-    const int mask = need_mask(Lt, hs, v);
-    if (!mask) return 0;
+  // Solves the corners of `mask` (from need_mask) into v.
+  __device__ static void solve_mask(const T* Lt, int mask, T* v, double leps) {
     double L[6];
 #pragma unroll
     for (int k = 0; k < 6; ++k) L[k] = (double)Lt[k];
@@ -325,7 +324,6 @@ struct ElemSolve<T, 4> {
       else if (k == 2) v[2] = y;
       else v[3] = y;
     }
-    return __popc(mask);
   }
   // The value the element gives corner k from the other three (no min with
   // corner k's own value, no skip): what update() stores when it solves k.
@@ -364,9 +362,7 @@ struct ElemSolve<T, 3> {
     const double vk = k == 0 ? (double)v[0] : (k == 1 ? (double)v[1] : (double)v[2]);
     return (T)d_min(vk, (double)corner_value(ct, v, k, leps));
   }
-  __device__ static int update(const T* ct, const T*, T* v, double leps) {
-    const int mask = need_mask(ct, nullptr, v);
-    if (!mask) return 0;
+  __device__ static void solve_mask(const T* ct, int mask, T* v, double leps) {
     const double c[3] = {(double)ct[0], (double)ct[1], (double)ct[2]};
     const double LenAB = d_abs(c[0]);
     const double LenBC = d_sqrt(c[2] * c[2] + (c[1] - c[0]) * (c[1] - c[0]));
@@ -375,7 +371,6 @@ struct ElemSolve<T, 3> {
     if (mask & 4) v[2] = (T)d_min(TC, tri_solve(TA, TB, LenAB, LenBC, LenCA));
     if (mask & 1) v[0] = (T)d_min(TA, tri_solve(TB, TC, LenBC, LenCA, LenAB));
     if (mask & 2) v[1] = (T)d_min(TB, tri_solve(TC, TA, LenCA, LenAB, LenBC));
-    return __popc(mask);
   }
   __device__ static T corner_value(const T* ct, const T* v, int k, double) {
     const double c[3] = {(double)ct[0], (double)ct[1], (double)ct[2]};
@@ -527,7 +522,7 @@ __device__ __forceinline__ T block_min_n(T v, T* scratch) {
 // most solves; results are bit-identical (same per-corner arithmetic).
 template <typename T, int NC, int BS>
 __device__ void sweep_patch(const DevMesh<T>& m, DevState<T>& s, int p, int niter, int write_out, T tol,
-                            double leps, unsigned char* smem, T* red, int* s_ndirty, bool tasks) {
+                            double leps, unsigned char* smem, T* red, int* s_ndirty /*[10]*/, bool tasks) {
   __syncthreads();
   const int vs = m.vert_off[p], ve = m.vert_off[p + 1];
   const int es = m.ele_off[p], ee = m.ele_off[p + 1];
@@ -569,23 +564,23 @@ __device__ void sweep_patch(const DevMesh<T>& m, DevState<T>& s, int p, int nite
   T vmin = (T)kLargeNum;
   int n_upd = 0, n_solve = 0;
   for (int it = 0; it < niter; ++it) {
-    if (tid == 0) *s_ndirty = 0;
-    __syncthreads();
-    // Wave-aggregated compaction; the loop bound is uniform across the block.
-    for (int base = 0; base < ne; base += BS) {
-      const int e = base + tid;
-      const bool d = e < ne && s_dirty[e];
-      const unsigned long long mask = __ballot(d);
-      const int lane = tid & 63;
-      int off = 0;
-      if (lane == 0 && mask) off = atomicAdd(s_ndirty, __popcll(mask));
-      off = __shfl(off, 0, 64);
-      if (d) s_work[off + __popcll(mask & ((1ull << lane) - 1ull))] = (uint16_t)e;
-    }
-    if (tid == 0) s_ndirty[1] = 0;
-    __syncthreads();
-    const int nd = *s_ndirty;
     if (tasks) {
+      if (tid == 0) *s_ndirty = 0;
+      __syncthreads();
+      // Wave-aggregated compaction; the loop bound is uniform across the block.
+      for (int base = 0; base < ne; base += BS) {
+        const int e = base + tid;
+        const bool d = e < ne && s_dirty[e];
+        const unsigned long long mask = __ballot(d);
+        const int lane = tid & 63;
+        int off = 0;
+        if (lane == 0 && mask) off = atomicAdd(s_ndirty, __popcll(mask));
+        off = __shfl(off, 0, 64);
+        if (d) s_work[off + __popcll(mask & ((1ull << lane) - 1ull))] = (uint16_t)e;
+      }
+      if (tid == 0) s_ndirty[1] = 0;
+      __syncthreads();
+      const int nd = *s_ndirty;
       // Need masks of the dirty elements, then their solves as tasks.
       for (int base = 0; base < nd; base += BS) {
         const int k = base + tid;
@@ -644,6 +639,56 @@ __device__ void sweep_patch(const DevMesh<T>& m, DevState<T>& s, int p, int nite
         }
       }
     } else {
+      // Need masks of the dirty elements (kept in s_dirty bits 1..NC), then
+      // the elements with a corner to solve listed by their number of
+      // solves, so the lanes of a wave walk the same number of corners.
+      // Counts per solve number c = 1..NC in s_ndirty[1 + c], list cursors
+      // in s_ndirty[5 + c].
+      if (tid < 8) s_ndirty[2 + tid] = 0;
+      __syncthreads();
+      for (int base = 0; base < ne; base += BS) {
+        const int e = base + tid;
+        int c = 0;
+        if (e < ne && s_dirty[e]) {
+          T x[NC], L[K];
+#pragma unroll
+          for (int i = 0; i < NC; ++i) x[i] = s_slot[e * NC + i];
+          const T* gc = m.coords + (int64_t)(es + e) * K;
+#pragma unroll
+          for (int q = 0; q < K; ++q) L[q] = gc[q];
+          const int mask = ElemSolve<T, NC>::need_mask(L, copy_heights(m, es + e), x);
+          s_dirty[e] = (uint8_t)(mask << 1);
+          c = __popc(mask);
+          n_solve += c;
+          ++n_upd;
+        }
+#pragma unroll
+        for (int j = 1; j <= NC; ++j) {
+          const unsigned long long b = __ballot(c == j);
+          if ((tid & 63) == 0 && b) atomicAdd(&s_ndirty[1 + j], __popcll(b));
+        }
+      }
+      __syncthreads();
+      int start[NC + 1];
+      start[1] = 0;
+#pragma unroll
+      for (int j = 2; j <= NC; ++j) start[j] = start[j - 1] + s_ndirty[j];
+      const int nd = start[NC] + s_ndirty[1 + NC];
+      for (int base = 0; base < ne; base += BS) {
+        const int e = base + tid;
+        const int c = e < ne ? __popc(s_dirty[e] >> 1) : 0;
+        const int lane = tid & 63;
+#pragma unroll
+        for (int j = 1; j <= NC; ++j) {
+          const unsigned long long b = __ballot(c == j);
+          if (!b) continue;
+          int off = 0;
+          if (lane == 0) off = atomicAdd(&s_ndirty[5 + j], __popcll(b));
+          off = __shfl(off, 0, 64);
+          if (c == j) s_work[start[j] + off + __popcll(b & ((1ull << lane) - 1ull))] = (uint16_t)e;
+        }
+      }
+      __syncthreads();
       for (int k = tid; k < nd; k += BS) {
         const int e = s_work[k];
         T x[NC], y[NC], L[K];
@@ -652,8 +697,7 @@ __device__ void sweep_patch(const DevMesh<T>& m, DevState<T>& s, int p, int nite
         const T* gc = m.coords + (int64_t)(es + e) * K;
 #pragma unroll
         for (int q = 0; q < K; ++q) L[q] = gc[q];
-        n_solve += ElemSolve<T, NC>::update(L, copy_heights(m, es + e), y, leps);
-        ++n_upd;
+        ElemSolve<T, NC>::solve_mask(L, s_dirty[e] >> 1, y, leps);
         int changed = 0;
 #pragma unroll
         for (int i = 0; i < NC; ++i) {
@@ -1086,7 +1130,7 @@ __global__ __launch_bounds__(BS) void k_sweep(DevMesh<T> m, DevState<T> s, const
                                               int niter, int write_out, T tol, double leps, int tasks) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ T red[BS / 64];
-  __shared__ int s_ndirty[2];
+  __shared__ int s_ndirty[10];
   sweep_patch<T, NC, BS>(m, s, list[blockIdx.x], niter, write_out, tol, leps, smem, red, s_ndirty, tasks != 0);
 }
 
@@ -1576,7 +1620,7 @@ template <typename T, int NC, int BS, bool TASKS, bool WARM>
 __global__ __launch_bounds__(BS) void k_step(DevMesh<T> m, DevState<T> s, StepArgs<T> a) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ T red[BS / 64];
-  __shared__ int s_ndirty[2];
+  __shared__ int s_ndirty[10];
   const int G = gridDim.x, b = blockIdx.x, tid = threadIdx.x;
   const int64_t gtid = (int64_t)b * BS + tid, gsize = (int64_t)G * BS;
   const int nb_out = 1 - a.nb_in;

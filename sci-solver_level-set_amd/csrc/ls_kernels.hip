@@ -516,13 +516,32 @@ __device__ __forceinline__ T block_min_n(T v, T* scratch) {
 // and the workgroup reduces it to the patch flag plus the patch min of the
 // written distances (run_reduction_bandwidth, redistance_kernels3d.h:57-97).
 //
-// With `tasks` set, each sweep first lists the (element, corner) solves the
-// exact lower-bound skip leaves, then spreads those solves over the
-// workgroup one per thread, so a wave no longer waits for its lane with the
-// most solves; results are bit-identical (same per-corner arithmetic).
-template <typename T, int NC, int BS>
+// Copies n items into LDS with U independent loads in flight per thread.
+template <int BS, int U, typename V>
+__device__ __forceinline__ void copy_to_lds(V* dst, const V* src, int n) {
+  for (int base = threadIdx.x; base < n; base += U * BS) {
+    V r[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (base + u * BS < n) r[u] = src[base + u * BS];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (base + u * BS < n) dst[base + u * BS] = r[u];
+  }
+}
+
+// Sweep organisations (bit-identical, same per-corner arithmetic):
+// kOrgElements runs one thread per element with a corner to solve, the
+// elements listed by their number of solves; kOrgTasks lists the
+// (element, corner) solves the exact lower-bound skip leaves and spreads
+// them over the workgroup one per thread.
+constexpr int kOrgElements = 0, kOrgTasks = 1;
+
+template <typename T, int NC, int BS, int ORG>
 __device__ void sweep_patch(const DevMesh<T>& m, DevState<T>& s, int p, int niter, int write_out, T tol,
-                            double leps, unsigned char* smem, T* red, int* s_ndirty /*[10]*/, bool tasks) {
+                            double leps, unsigned char* smem, T* red, int* s_ndirty /*[10]*/) {
+  constexpr bool tasks = ORG == kOrgTasks;
+  constexpr int K = ElemSolve<T, NC>::kCoords;
   __syncthreads();
   const int vs = m.vert_off[p], ve = m.vert_off[p + 1];
   const int es = m.ele_off[p], ee = m.ele_off[p + 1];
@@ -550,21 +569,29 @@ __device__ void sweep_patch(const DevMesh<T>& m, DevState<T>& s, int p, int nite
   }
   if (tid == 0) s_voff[nv] = (uint16_t)nslots;
   for (int i = tid; i < nh; i += BS) s_vtx[nv + i] = s.DT[m.halo_ids[hs + i]];
-  for (int i = tid; i < nslots; i += BS) s_vsl[i] = m.vslots[sbase + i];
+  copy_to_lds<BS, 8>(s_vsl, m.vslots + sbase, nslots);
   __syncthreads();
-  for (int e = tid; e < ne; e += BS) {
-    const ushort4 c = m.copy_local[es + e];
+  for (int base = tid; base < ne; base += 4 * BS) {
+    ushort4 c[4];
 #pragma unroll
-    for (int i = 0; i < NC; ++i) s_slot[e * NC + i] = s_vtx[corner(c, i)];
-    s_dirty[e] = 1;
+    for (int u = 0; u < 4; ++u)
+      if (base + u * BS < ne) c[u] = m.copy_local[es + base + u * BS];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int e = base + u * BS;
+      if (e < ne) {
+#pragma unroll
+        for (int i = 0; i < NC; ++i) s_slot[e * NC + i] = s_vtx[corner(c[u], i)];
+        s_dirty[e] = 1;
+      }
+    }
   }
 
-  constexpr int K = ElemSolve<T, NC>::kCoords;
   int con_all = 1;
   T vmin = (T)kLargeNum;
   int n_upd = 0, n_solve = 0;
   for (int it = 0; it < niter; ++it) {
-    if (tasks) {
+    if constexpr (tasks) {
       if (tid == 0) *s_ndirty = 0;
       __syncthreads();
       // Wave-aggregated compaction; the loop bound is uniform across the block.
@@ -1125,13 +1152,13 @@ __global__ __launch_bounds__(kBlock) void k_seeds(DevMesh<T> m, DevState<T> s, c
   if (threadIdx.x == 0 && flag) active[atomicAdd(&counters[kCntSeed], 1)] = p;
 }
 
-template <typename T, int NC, int BS>
+template <typename T, int NC, int BS, int ORG>
 __global__ __launch_bounds__(BS) void k_sweep(DevMesh<T> m, DevState<T> s, const int32_t* list,
-                                              int niter, int write_out, T tol, double leps, int tasks) {
+                                              int niter, int write_out, T tol, double leps) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ T red[BS / 64];
   __shared__ int s_ndirty[10];
-  sweep_patch<T, NC, BS>(m, s, list[blockIdx.x], niter, write_out, tol, leps, smem, red, s_ndirty, tasks != 0);
+  sweep_patch<T, NC, BS, ORG>(m, s, list[blockIdx.x], niter, write_out, tol, leps, smem, red, s_ndirty);
 }
 
 // CopyOutBack (redistance_kernels3d.h:772-783) plus the active-list label of
@@ -1556,7 +1583,6 @@ struct StepArgs {
   double leps;
   double dt;
   int mode;          // LS_MODE_ADVECT or LS_MODE_MCF
-  int tasks;         // FIM sweeps spread (element, corner) solves over the workgroup
   int warm;          // 0: plain; 1: warm start from |phi|; 2: from s.warm
   double warm_margin;// warm value = prediction * (1 + warm_margin)
   float res_tol;     // residual above which a warm start falls back
@@ -1613,10 +1639,10 @@ struct PhaseClock {
   }
 };
 
-// TASKS selects the sweep organisation and WARM compiles the warm-start
-// phases in: each variant carries only the code it runs, which keeps the
-// register demand of the 1024-thread workgroup within its 128 VGPRs.
-template <typename T, int NC, int BS, bool TASKS, bool WARM>
+// ORG selects the sweep organisation (kOrg*) and WARM compiles the
+// warm-start phases in: each variant carries only the code it runs, which
+// keeps the register demand of the 1024-thread workgroup within its 128 VGPRs.
+template <typename T, int NC, int BS, int ORG, bool WARM>
 __global__ __launch_bounds__(BS) void k_step(DevMesh<T> m, DevState<T> s, StepArgs<T> a) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ T red[BS / 64];
@@ -1696,7 +1722,7 @@ __global__ __launch_bounds__(BS) void k_step(DevMesh<T> m, DevState<T> s, StepAr
     }
     const int E = it + 1, par = it & 1;
     for (int i = b; i < nA; i += G)
-      sweep_patch<T, NC, BS>(m, s, cur[i], a.niter, 1, a.tol, a.leps, smem, red, s_ndirty, TASKS);
+      sweep_patch<T, NC, BS, ORG>(m, s, cur[i], a.niter, 1, a.tol, a.leps, smem, red, s_ndirty);
     grid_barrier(s.bar, G);
     pcl.lap(clk, kTkVisit);
     // Every block has read this iteration's list size, so the counters the
@@ -1727,7 +1753,7 @@ __global__ __launch_bounds__(BS) void k_step(DevMesh<T> m, DevState<T> s, StepAr
     const int nF = ld_acquire(&s.pc[kPcFresh + par]);
     if (gtid == 0) s.pc[kPcProbes] += nF;
     for (int i = b; i < nF; i += G)
-      sweep_patch<T, NC, BS>(m, s, s.listF[i], 1, 0, a.tol, a.leps, smem, red, s_ndirty, TASKS);
+      sweep_patch<T, NC, BS, ORG>(m, s, s.listF[i], 1, 0, a.tol, a.leps, smem, red, s_ndirty);
     grid_barrier(s.bar, G);
     pcl.lap(clk, kTkProbe);
     // Next list: unconverged members not probed this iteration, then
@@ -1864,8 +1890,8 @@ class EngineImpl final : public Engine {
       adv_lds_ = std::max(adv_lds_, adv);
       seed_lds_ = std::max(seed_lds_, (nv + nh) * sizeof(T));
     }
-    // The task-compacted sweep needs more LDS; patches too large for it use
-    // the per-element sweep.
+    // The task-compacted sweep needs more LDS; size_kernels reports it when
+    // it does not fit.
     fim_elem_lds_ = (fim_lds_ + 15) & ~size_t(15);
     fim_task_lds_ = (fim_task_lds + 15) & ~size_t(15);
     adv_lds_ = (adv_lds_ + 15) & ~size_t(15);
@@ -2182,12 +2208,12 @@ class EngineImpl final : public Engine {
             std::to_string(lds_max_) + " B";
       return LS_ERR_LIMIT;
     }
-    // AUTO runs per-element sweeps: measured faster on MI355X for 256-vertex
-    // patches (per-launch 184 us against 250 us for the task-compacted sweep
-    // on the 64^3 torus), whose extra compaction pass and per-task reloads of
-    // the element geometry outweigh the divergence they remove.
-    use_tasks_ = (sweep_mode_ == LS_SWEEP_TASKS && fits) ? 1 : 0;
-    fim_lds_ = use_tasks_ ? fim_task_lds_ : fim_elem_lds_;
+    // AUTO runs per-element sweeps: measured faster on MI355X than the
+    // task-compacted sweep for 256-vertex patches, whose extra listing pass
+    // and per-task reloads of the element geometry outweigh the divergence
+    // they remove.
+    org_ = sweep_mode_ == LS_SWEEP_TASKS ? kOrgTasks : kOrgElements;
+    fim_lds_ = org_ == kOrgTasks ? fim_task_lds_ : fim_elem_lds_;
     step_lds_ = std::max(fim_lds_, std::max(adv_lds_, seed_lds_));
     step_lds_ = (step_lds_ + 15) & ~size_t(15);
     if (step_lds_ > (size_t)lds_max_) {
@@ -2198,9 +2224,9 @@ class EngineImpl final : public Engine {
     int cus = 0, per_cu = 0;
     LS_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device_));
     per_cu = 1 << 30;
-    for (int v = 0; v < 4; ++v) {
+    for (int warm = 0; warm < 2; ++warm) {
       int n = 0;
-      LS_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, step_kernel(v & 1, v >> 1), kFimBlock, step_lds_));
+      LS_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, step_kernel(org_, warm), kFimBlock, step_lds_));
       per_cu = std::min(per_cu, n);
     }
     coop_grid_ = cus * per_cu;
@@ -2209,9 +2235,12 @@ class EngineImpl final : public Engine {
     if (graph_) { LS_HIP(hipGraphDestroy(graph_)); graph_ = nullptr; }
     return LS_OK;
   }
-  static const void* step_kernel(int tasks, int warm) {
-    if (tasks) return warm ? (const void*)k_step<T, NC, kFimBlock, true, true> : (const void*)k_step<T, NC, kFimBlock, true, false>;
-    return warm ? (const void*)k_step<T, NC, kFimBlock, false, true> : (const void*)k_step<T, NC, kFimBlock, false, false>;
+  static const void* step_kernel(int org, int warm) {
+    if (org == kOrgTasks)
+      return warm ? (const void*)k_step<T, NC, kFimBlock, kOrgTasks, true>
+                  : (const void*)k_step<T, NC, kFimBlock, kOrgTasks, false>;
+    return warm ? (const void*)k_step<T, NC, kFimBlock, kOrgElements, true>
+                : (const void*)k_step<T, NC, kFimBlock, kOrgElements, false>;
   }
   int set_sweep_mode(int mode, std::string& err) override {
     const int old = sweep_mode_;
@@ -2225,7 +2254,14 @@ class EngineImpl final : public Engine {
     }
     return rc;
   }
-  int sweep_tasks() const override { return use_tasks_; }
+  int sweep_tasks() const override { return org_ == kOrgTasks ? 1 : 0; }
+  void launch_sweep(int n, hipStream_t q, const DevMesh<T>& m, const DevState<T>& s, const int32_t* list, int niter,
+                    int write_out, T tol, double leps) {
+    if (org_ == kOrgTasks)
+      k_sweep<T, NC, kFimBlock, kOrgTasks><<<n, kFimBlock, fim_lds_, q>>>(m, s, list, niter, write_out, tol, leps);
+    else
+      k_sweep<T, NC, kFimBlock, kOrgElements><<<n, kFimBlock, fim_lds_, q>>>(m, s, list, niter, write_out, tol, leps);
+  }
 
   int set_patch_params(const std::vector<double>& dt, const std::vector<double>& bw, std::string& err) override {
     if (!uploaded_) { err = "layout not built"; return LS_ERR_STATE; }
@@ -2272,7 +2308,7 @@ class EngineImpl final : public Engine {
     }
     std::memset(&st, 0, sizeof(st));
     st.exec_mode = exec_;
-    st.sweep_tasks = use_tasks_;
+    st.sweep_tasks = org_ == kOrgTasks ? 1 : 0;
     DevMesh<T> m = mesh();
     DevState<T> s = state();
     StepArgs<T> a;
@@ -2287,7 +2323,6 @@ class EngineImpl final : public Engine {
     a.leps = prm.lambda_eps;
     a.dt = prm.dt;
     a.mode = prm.mode;
-    a.tasks = use_tasks_;
     a.warm = prm.warm;
     a.warm_margin = prm.warm_margin;
     a.res_tol = (float)(prm.warm_res_tol > 0 ? prm.warm_res_tol : prm.tol);
@@ -2314,7 +2349,7 @@ class EngineImpl final : public Engine {
         if (graph_exec_) { LS_HIP(hipGraphExecDestroy(graph_exec_)); graph_exec_ = nullptr; }
         if (graph_) { LS_HIP(hipGraphDestroy(graph_)); graph_ = nullptr; }
         LS_HIP(hipStreamBeginCapture(q, hipStreamCaptureModeThreadLocal));
-        hipError_t le = hipLaunchCooperativeKernel(step_kernel(use_tasks_, a.warm != 0), dim3(coop_grid_),
+        hipError_t le = hipLaunchCooperativeKernel(step_kernel(org_, a.warm != 0), dim3(coop_grid_),
                                                    dim3(kFimBlock), args, step_lds_, q);
         hipError_t ce = hipStreamEndCapture(q, &graph_);
         if (le != hipSuccess || ce != hipSuccess) {
@@ -2330,7 +2365,7 @@ class EngineImpl final : public Engine {
       if (coop_refused(ge)) return step_kernels(prm, st, err);
       LS_HIP(ge);
     } else {
-      const hipError_t le = hipLaunchCooperativeKernel(step_kernel(use_tasks_, a.warm != 0), dim3(coop_grid_),
+      const hipError_t le = hipLaunchCooperativeKernel(step_kernel(org_, a.warm != 0), dim3(coop_grid_),
                                                        dim3(kFimBlock), args, step_lds_, q);
       if (coop_refused(le)) return step_kernels(prm, st, err);
       LS_HIP(le);
@@ -2394,7 +2429,7 @@ class EngineImpl final : public Engine {
     }
     std::memset(&st, 0, sizeof(st));
     st.exec_mode = LS_EXEC_KERNELS;
-    st.sweep_tasks = use_tasks_;
+    st.sweep_tasks = org_ == kOrgTasks ? 1 : 0;
     pc_synced_ = false;
     const DevMesh<T> m = mesh();
     const DevState<T> s = state();
@@ -2424,7 +2459,7 @@ class EngineImpl final : public Engine {
       st.fim_iters++;
       st.patch_visits += nact;
       int ke = kev_begin(0, q);
-      k_sweep<T, NC, kFimBlock><<<nact, kFimBlock, fim_lds_, q>>>(m, s, cur, prm.niter, 1, tol, leps, use_tasks_);
+      launch_sweep(nact, q, m, s, cur, prm.niter, 1, tol, leps);
       kev_end(ke, q);
       k_copyback<T><<<nact, kBlock, 0, q>>>(m, s, cur);
       k_set_counters<<<1, 64, 0, q>>>(d_counters_ + kCntFresh, 2, 0);
@@ -2436,7 +2471,7 @@ class EngineImpl final : public Engine {
       if (nf > 0) {
         st.probes += nf;
         int kp = kev_begin(0, q);
-        k_sweep<T, NC, kFimBlock><<<nf, kFimBlock, fim_lds_, q>>>(m, s, fresh_list, 1, 0, tol, leps, use_tasks_);
+        launch_sweep(nf, q, m, s, fresh_list, 1, 0, tol, leps);
         kev_end(kp, q);
       }
       k_finalize<T><<<grid_for(nact + nf), kBlock, 0, q>>>(s, cur, nact, fresh_list, nf, next);
@@ -2554,7 +2589,7 @@ class EngineImpl final : public Engine {
   int64_t N_ = 0, F_ = 0;
   size_t fim_lds_ = 0, adv_lds_ = 0, seed_lds_ = 0;
   bool coop_ok_ = true;  // cooperative launches accepted so far
-  int use_tasks_ = 0;     // task-compacted FIM sweeps in use
+  int org_ = kOrgElements;  // FIM sweep organisation in use (kOrg*)
   int sweep_mode_ = LS_SWEEP_AUTO;
   size_t fim_elem_lds_ = 0, fim_task_lds_ = 0;
   int lds_max_ = 0;

@@ -168,7 +168,9 @@ def batch_workload(args, lsb, cases):
     sub = cases.Batch(b.name + "_tune", 3, b.meshes[:64], patch_verts=b.patch_verts)
     cand, ex, ms, table = lsb.autotune_batch(sub, [64, 128, 256, 512], trial_steps=2,
                                              log=lambda m: print(m, file=sys.stderr, flush=True))
+    t0 = time.perf_counter()
     g = lsb.solver_for_batch(b, execution=ex, patch_verts=cand)
+    setup_s = time.perf_counter() - t0
     info = g.layout_info()
     prm = lsb.step_params(1.0, 1.0, 5)
     for _ in range(args.warmup):
@@ -181,7 +183,8 @@ def batch_workload(args, lsb, cases):
            "patch_verts": cand, "execution": {lsb.LS_EXEC_GRAPH: "graph", lsb.LS_EXEC_PERSISTENT: "persistent",
                                               lsb.LS_EXEC_KERNELS: "kernels"}[ex],
            "autotune_ms_per_step": {f"{c}/{e}": m for c, e, m in table},
-           "patches": info["n_patches"], "element_copies": info["n_copies"]}
+           "patches": info["n_patches"], "element_copies": info["n_copies"],
+           "setup_s": setup_s}
     cpu = None
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -222,7 +225,9 @@ def mcf_workload(args, lsb, cases):
     case = cases.config3(steps=args.steps)
     print(f"mcf: {case.xyz.shape[0]} vertices, {case.cells.shape[0]} tets", file=sys.stderr, flush=True)
     names = {"graph": lsb.LS_EXEC_GRAPH, "persistent": lsb.LS_EXEC_PERSISTENT, "kernels": lsb.LS_EXEC_KERNELS}
+    t0 = time.perf_counter()
     g = lsb.solver_for_case(case)
+    setup_s = time.perf_counter() - t0
     info = g.layout_info()
     prm = lsb.step_params(case.dt, case.bandwidth, case.niter, mode=lsb.LS_MODE_MCF)
     tune = None
@@ -239,7 +244,7 @@ def mcf_workload(args, lsb, cases):
     out = {"workload": f"{case.name}: 192^3 Kuhn tets ({case.xyz.shape[0]} verts, {case.cells.shape[0]} tets), "
                        f"8 spheres, mean-curvature flow, dt = h^2/8, band 8h, {args.steps} timesteps",
            "execution": args.exec, "execution_trial_ms_per_step": tune, "patches": info["n_patches"],
-           "element_copies": info["n_copies"]}
+           "element_copies": info["n_copies"], "setup_s": setup_s}
     cpu = None
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))

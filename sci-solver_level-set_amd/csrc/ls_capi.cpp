@@ -42,29 +42,13 @@ int fail(ls_ctx* c, int code, const std::string& msg) {
   return code;
 }
 
-// Projects the per-element velocity into every copy's local frame and uploads
-// it. patch_sigma reproduces the reference's use of the patch's first copy
-// velocity for all of the patch's elements (meshFIM_kernels.h:51-53).
+// Uploads the per-element velocity; the engine projects it into every copy's
+// local frame. patch_sigma reproduces the reference's use of the patch's first
+// copy velocity for all of the patch's elements (meshFIM_kernels.h:51-53).
 int apply_velocity(ls_ctx* c) {
   if (!c->have_layout || !c->engine) return LS_OK;
-  const lsamd::Layout& L = c->layout;
-  std::vector<double> sig;
-  if (c->patch_sigma) {
-    const int nf = L.dim == 3 ? 9 : 6, ns = L.dim == 3 ? 3 : 2;
-    sig.assign(L.F * ns, 0.0);
-    for (int p = 0; p < L.P; ++p) {
-      const double* s = &c->vel[3 * (int64_t)L.copy_src[L.ele_off[p]]];
-      for (int64_t k = L.ele_off[p]; k < L.ele_off[p + 1]; ++k)
-        for (int a = 0; a < ns; ++a) {
-          const double* ax = &L.frame[k * nf + 3 * a];
-          sig[k * ns + a] = s[0] * ax[0] + s[1] * ax[1] + s[2] * ax[2];
-        }
-    }
-  } else {
-    lsamd::project_velocity(L, c->vel.data(), sig);
-  }
   std::string err;
-  const int rc = c->engine->set_velocity(sig, err);
+  const int rc = c->engine->set_velocity(c->vel.data(), c->E, c->patch_sigma, err);
   return rc == LS_OK ? rc : fail(c, rc, err);
 }
 

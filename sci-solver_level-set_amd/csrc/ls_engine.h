@@ -17,7 +17,9 @@ class Engine {
   // Uploads the layout; geometry is rounded to the engine's working type.
   virtual int upload(const Layout& L, std::string& err) = 0;
   virtual int set_phi(const double* phi_user, std::string& err) = 0;
-  virtual int set_velocity(const std::vector<double>& sigma_local, std::string& err) = 0;
+  // Velocity per original element (E x 3); projected on the device into every
+  // copy's local frame (patch_sigma: the patch's first copy's velocity).
+  virtual int set_velocity(const double* vel, int64_t E, int patch_sigma, std::string& err) = 0;
   // Per-patch timestep and narrowband width (P each); empty vectors restore
   // the step's uniform values.
   virtual int set_patch_params(const std::vector<double>& dt, const std::vector<double>& bw,

@@ -392,6 +392,90 @@ __device__ __forceinline__ const T* copy_heights(const DevMesh<T>& m, int64_t c)
   return m.hgt ? m.hgt + c * 4 : nullptr;
 }
 
+// Local frame and coordinates of one element copy in FP64 (scheme of
+// kernel_compute_local_coords3d / 2d, src/core3d/include/meshFIM_kernels.h:448-500):
+// tets X along AB, Z along AB x AC, Y = Z x X, coordinates
+// (|AB|, AC.X, AC.Y, AD.X, AD.Y, AD.Z); triangles X along AB, Y along
+// (AB x AC) x AB, coordinates (|AB|, AC.AB / |AB|, AC.Y / |Y|). `fr` gets the
+// axes (3 or 2 of them, 3 components each) for velocity projection.
+__device__ inline void g_cross3(const double* a, const double* b, double* c) {
+  c[0] = a[1] * b[2] - a[2] * b[1];
+  c[1] = a[2] * b[0] - a[0] * b[2];
+  c[2] = a[0] * b[1] - a[1] * b[0];
+}
+__device__ inline double g_dot3(const double* a, const double* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+template <int NC>
+__device__ void copy_frame(const double* xyz, const int32_t* cv, double* lc, double* fr) {
+  const double* A = xyz + 3 * (int64_t)cv[0];
+  const double* B = xyz + 3 * (int64_t)cv[1];
+  const double* C = xyz + 3 * (int64_t)cv[2];
+  const double AB[3] = {B[0] - A[0], B[1] - A[1], B[2] - A[2]};
+  const double AC[3] = {C[0] - A[0], C[1] - A[1], C[2] - A[2]};
+  const double lenAB = sqrt(g_dot3(AB, AB));
+  double nrm[3];
+  g_cross3(AB, AC, nrm);
+  if constexpr (NC == 4) {
+    const double* D = xyz + 3 * (int64_t)cv[3];
+    const double AD[3] = {D[0] - A[0], D[1] - A[1], D[2] - A[2]};
+    const double lenN = sqrt(g_dot3(nrm, nrm));
+    const double Z[3] = {nrm[0] / lenN, nrm[1] / lenN, nrm[2] / lenN};
+    const double X[3] = {AB[0] / lenAB, AB[1] / lenAB, AB[2] / lenAB};
+    double Y[3];
+    g_cross3(Z, X, Y);
+    lc[0] = lenAB; lc[1] = g_dot3(AC, X); lc[2] = g_dot3(AC, Y);
+    lc[3] = g_dot3(AD, X); lc[4] = g_dot3(AD, Y); lc[5] = g_dot3(AD, Z);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) { fr[a] = X[a]; fr[3 + a] = Y[a]; fr[6 + a] = Z[a]; }
+  } else {
+    double Y[3];
+    g_cross3(nrm, AB, Y);
+    const double lenY = sqrt(g_dot3(Y, Y));
+    lc[0] = lenAB; lc[1] = g_dot3(AC, AB) / lenAB; lc[2] = g_dot3(AC, Y) / lenY;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) { fr[a] = AB[a] / lenAB; fr[3 + a] = Y[a] / lenY; }
+  }
+}
+
+// Local coordinates of every copy (cv: F x NC renumbered corner ids, xyz:
+// renumbered vertex coordinates), rounded to the storage type.
+template <typename T, int NC>
+__global__ void k_local_coords(const int32_t* cv, const double* xyz, int64_t F, T* coords) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= F) return;
+  constexpr int K = NC == 4 ? 6 : 3;
+  double lc[6], fr[9];
+  copy_frame<NC>(xyz, cv + c * NC, lc, fr);
+#pragma unroll
+  for (int k = 0; k < K; ++k) coords[c * K + k] = (T)lc[k];
+}
+
+// Per-copy velocity in the copy's local frame from one velocity per original
+// element (vel: E x 3). With patch_sigma every copy of a patch uses the
+// velocity of the patch's first copy, as the reference's advection reads it
+// (src/core3d/include/meshFIM_kernels.h:51-53).
+template <typename T, int NC>
+__global__ void k_project_velocity(const int32_t* cv, const double* xyz, const int32_t* copy_src,
+                                   const int32_t* ele_off, int P, int64_t F, const double* vel, int patch_sigma,
+                                   T* sigma) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= F) return;
+  int64_t src = copy_src[c];
+  if (patch_sigma) {
+    int lo = 0, hi = P;  // largest p with ele_off[p] <= c
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (ele_off[mid] <= c) lo = mid; else hi = mid;
+    }
+    src = copy_src[ele_off[lo]];
+  }
+  double lc[6], fr[9];
+  copy_frame<NC>(xyz, cv + c * NC, lc, fr);
+  const double v[3] = {vel[3 * src], vel[3 * src + 1], vel[3 * src + 2]};
+#pragma unroll
+  for (int a = 0; a < NC - 1; ++a) sigma[c * (NC - 1) + a] = (T)g_dot3(v, fr + 3 * a);
+}
+
 // Corner heights of every tet copy, computed once at upload with the same
 // code the sweeps used to evaluate per visit; FP32 storage rounds toward zero.
 template <typename T>
@@ -1910,10 +1994,17 @@ class EngineImpl final : public Engine {
     LS_HIP(dev_upload(&d_vslots_, L.vslots.data(), L.vslots.size()));
     LS_HIP(dev_upload(&d_padj_off_, L.padj_off.data(), L.padj_off.size()));
     LS_HIP(dev_upload(&d_padj_, L.padj.data(), L.padj.size()));
+    // Element geometry on the device: the renumbered coordinates and corner
+    // ids stay resident for velocity projection.
+    LS_HIP(dev_upload(&d_xyz_, L.xyz.data(), L.xyz.size()));
+    LS_HIP(dev_upload(&d_copy_verts_, L.copy_verts.data(), L.copy_verts.size()));
+    LS_HIP(dev_upload(&d_copy_src_, L.copy_src.data(), L.copy_src.size()));
     {
-      std::vector<T> c(L.local_coords.size());
-      for (size_t i = 0; i < c.size(); ++i) c[i] = (T)L.local_coords[i];
-      LS_HIP(dev_upload(&d_coords_, c.data(), c.size()));
+      constexpr int K = ElemSolve<T, NC>::kCoords;
+      LS_HIP(dev_alloc(&d_coords_, (size_t)F_ * K));
+      if (F_ > 0)
+        k_local_coords<T, NC><<<(int)((F_ + 255) / 256), 256, 0, stream_>>>(d_copy_verts_, d_xyz_, F_, d_coords_);
+      LS_HIP(hipGetLastError());
       if (NC == 4 && F_ > 0) {
         LS_HIP(dev_alloc(&d_hgt_, (size_t)F_ * 4));
         k_heights<T><<<(int)((F_ + 255) / 256), 256, 0, stream_>>>(d_coords_, F_, d_hgt_);
@@ -2192,11 +2283,18 @@ class EngineImpl final : public Engine {
     return LS_OK;
   }
 
-  int set_velocity(const std::vector<double>& sig, std::string& err) override {
+  int set_velocity(const double* vel, int64_t E, int patch_sigma, std::string& err) override {
     if (!uploaded_) { err = "layout not built"; return LS_ERR_STATE; }
-    std::vector<T> h(sig.size());
-    for (size_t i = 0; i < sig.size(); ++i) h[i] = (T)sig[i];
-    LS_HIP(hipMemcpy(d_sigma_, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+    if (F_ == 0) return LS_OK;
+    double* d_vel = nullptr;
+    LS_HIP(dev_upload(&d_vel, vel, (size_t)E * 3));
+    k_project_velocity<T, NC><<<(int)((F_ + 255) / 256), 256, 0, stream_>>>(
+        d_copy_verts_, d_xyz_, d_copy_src_, d_ele_off_, P_, F_, d_vel, patch_sigma, d_sigma_);
+    const hipError_t le = hipGetLastError();
+    const hipError_t se = hipStreamSynchronize(stream_);
+    (void)hipFree(d_vel);
+    LS_HIP(le);
+    LS_HIP(se);
     return LS_OK;
   }
 
@@ -2572,7 +2670,8 @@ class EngineImpl final : public Engine {
                     d_counters_, d_sizes_, d_nb_, d_scan_all_, d_lists_[0], d_lists_[1], d_lists_[2],
                     d_lists_[3], d_inA_, d_freshE_, d_listF_, d_pc_, d_ticks_, d_bar_,
                     d_patch_dt_, d_patch_bw_, d_warm_, d_hgt_, d_mlp_prm_, d_mlp_m_, d_mlp_v_, d_mlp_grad_,
-                    d_mlp_b_, d_mlp_w1_, d_mlp_w2_, d_mlp_w3_, d_ds_x_, d_ds_y_, d_pend_};
+                    d_mlp_b_, d_mlp_w1_, d_mlp_w2_, d_mlp_w3_, d_ds_x_, d_ds_y_, d_pend_, d_xyz_,
+                    d_copy_verts_, d_copy_src_};
     if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
     if (graph_) (void)hipGraphDestroy(graph_);
     for (void* p : ptrs) if (p) (void)hipFree(p);
@@ -2601,6 +2700,9 @@ class EngineImpl final : public Engine {
   uint16_t* d_vslots_ = nullptr;
   int32_t *d_padj_off_ = nullptr, *d_padj_ = nullptr;
   T *d_coords_ = nullptr, *d_sigma_ = nullptr;
+  double* d_xyz_ = nullptr;          // N x 3, renumbered vertex coordinates
+  int32_t* d_copy_verts_ = nullptr;  // F x NC, renumbered corner ids
+  int32_t* d_copy_src_ = nullptr;    // F, original element of each copy
   T* d_hgt_ = nullptr;  // tet copies: corner heights (F x 4)
   T* d_warm_ = nullptr;  // learned warm-start distances (N), written by the predictor
   float *d_mlp_prm_ = nullptr, *d_mlp_m_ = nullptr, *d_mlp_v_ = nullptr, *d_mlp_grad_ = nullptr, *d_mlp_b_ = nullptr;

@@ -64,15 +64,6 @@ int partition_rcb(int64_t N, const double* xyz, int nparts, std::vector<int32_t>
   return 0;
 }
 
-static inline void cross3(const double* a, const double* b, double* c) {
-  c[0] = a[1] * b[2] - a[2] * b[1];
-  c[1] = a[2] * b[0] - a[0] * b[2];
-  c[2] = a[0] * b[1] - a[1] * b[0];
-}
-static inline double dot3(const double* a, const double* b) {
-  return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
-}
-
 int build_layout(int dim, int64_t N, const double* xyz, int64_t E, const int32_t* cells,
                  const int32_t* labels, int nparts, Layout& L, std::string& err) {
   if (dim != 2 && dim != 3) { err = "dim must be 2 or 3"; return -1; }
@@ -207,44 +198,6 @@ int build_layout(int dim, int64_t N, const double* xyz, int64_t E, const int32_t
       }
   }
 
-  // Per-copy local frame and coordinates (FP64).
-  const int ncoord = dim == 3 ? 6 : 3, nframe = dim == 3 ? 9 : 6;
-  L.local_coords.assign(F * ncoord, 0.0);
-  L.frame.assign(F * nframe, 0.0);
-  #pragma omp parallel for schedule(static)
-  for (int64_t c = 0; c < F; ++c) {
-    const double* A = &L.xyz[3 * (int64_t)L.copy_verts[c * nc + 0]];
-    const double* B = &L.xyz[3 * (int64_t)L.copy_verts[c * nc + 1]];
-    const double* C = &L.xyz[3 * (int64_t)L.copy_verts[c * nc + 2]];
-    double AB[3] = {B[0] - A[0], B[1] - A[1], B[2] - A[2]};
-    double AC[3] = {C[0] - A[0], C[1] - A[1], C[2] - A[2]};
-    const double lenAB = std::sqrt(dot3(AB, AB));
-    double nrm[3];
-    cross3(AB, AC, nrm);
-    if (dim == 3) {
-      const double* D = &L.xyz[3 * (int64_t)L.copy_verts[c * nc + 3]];
-      double AD[3] = {D[0] - A[0], D[1] - A[1], D[2] - A[2]};
-      const double lenN = std::sqrt(dot3(nrm, nrm));
-      double Z[3] = {nrm[0] / lenN, nrm[1] / lenN, nrm[2] / lenN};
-      double X[3] = {AB[0] / lenAB, AB[1] / lenAB, AB[2] / lenAB};
-      double Y[3];
-      cross3(Z, X, Y);
-      double* lc = &L.local_coords[c * 6];
-      lc[0] = lenAB; lc[1] = dot3(AC, X); lc[2] = dot3(AC, Y);
-      lc[3] = dot3(AD, X); lc[4] = dot3(AD, Y); lc[5] = dot3(AD, Z);
-      double* fr = &L.frame[c * 9];
-      for (int a = 0; a < 3; ++a) { fr[a] = X[a]; fr[3 + a] = Y[a]; fr[6 + a] = Z[a]; }
-    } else {
-      double Y[3];
-      cross3(nrm, AB, Y);
-      const double lenY = std::sqrt(dot3(Y, Y));
-      double* lc = &L.local_coords[c * 3];
-      lc[0] = lenAB; lc[1] = dot3(AC, AB) / lenAB; lc[2] = dot3(AC, Y) / lenY;
-      double* fr = &L.frame[c * 6];
-      for (int a = 0; a < 3; ++a) { fr[a] = AB[a] / lenAB; fr[3 + a] = Y[a] / lenY; }
-    }
-  }
-
   // Patch adjacency: patches owning a halo vertex of the patch.
   L.padj_off.assign(nparts + 1, 0);
   std::vector<std::vector<int32_t>> adj(nparts);
@@ -266,17 +219,6 @@ int build_layout(int dim, int64_t N, const double* xyz, int64_t E, const int32_t
   }
   for (int64_t v = 0; v < N; ++v) L.max_vslots = std::max(L.max_vslots, vcount[v]);
   return 0;
-}
-
-void project_velocity(const Layout& L, const double* vel, std::vector<double>& out) {
-  const int nout = L.dim == 3 ? 3 : 2, nframe = L.dim == 3 ? 9 : 6;
-  out.assign(L.F * nout, 0.0);
-  #pragma omp parallel for schedule(static)
-  for (int64_t c = 0; c < L.F; ++c) {
-    const double* s = &vel[3 * (int64_t)L.copy_src[c]];
-    const double* fr = &L.frame[c * nframe];
-    for (int a = 0; a < nout; ++a) out[c * nout + a] = dot3(s, fr + 3 * a);
-  }
 }
 
 }  // namespace lsamd

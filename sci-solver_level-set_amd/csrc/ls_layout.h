@@ -43,8 +43,6 @@ struct Layout {
   std::vector<int32_t> vslot_off;   // N+1, by renumbered vertex (global offsets)
   std::vector<uint16_t> vslots;     // patch-local slot = local_copy*nc + corner
   std::vector<double> xyz;          // N*3, renumbered order
-  std::vector<double> local_coords; // F*6 (3D: |AB|, AC.X, AC.Y, AD.X, AD.Y, AD.Z) or F*3 (2D)
-  std::vector<double> frame;        // F*9 (3D: X,Y,Z axes) or F*6 (2D: X,Y axes) for velocity projection
   std::vector<int32_t> padj_off;    // P+1 patch adjacency (sorted, unique)
   std::vector<int32_t> padj;
   int max_ep = 0, max_vp = 0, max_hp = 0, max_vslots = 0;
@@ -63,8 +61,5 @@ int partition_rcb(int64_t N, const double* xyz, int nparts, std::vector<int32_t>
 int build_layout(int dim, int64_t N, const double* xyz, int64_t E, const int32_t* cells,
                  const int32_t* labels, int nparts, Layout& L, std::string& err);
 
-// Per-copy velocity in the copy's local frame (3D: 3 values, 2D: 2 values),
-// from one velocity per ORIGINAL element (E x 3).
-void project_velocity(const Layout& L, const double* vel_per_element, std::vector<double>& out);
 
 }  // namespace lsamd

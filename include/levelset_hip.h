@@ -38,6 +38,10 @@ enum { LS_EXEC_PERSISTENT = 0, LS_EXEC_KERNELS = 1, LS_EXEC_GRAPH = 2 };
  * spread one per thread (needs more LDS). AUTO (default) is per element,
  * the faster of the two on MI355X in measurements so far. */
 enum { LS_SWEEP_AUTO = 0, LS_SWEEP_ELEMENTS = 1, LS_SWEEP_TASKS = 2 };
+/* Where the patch layout is built: on the context's GPU (default; radix
+ * sorts, identical arrays) or on the host. Host-only contexts always build
+ * on the host. */
+enum { LS_LAYOUT_DEVICE = 0, LS_LAYOUT_HOST = 1 };
 /* Warm start of the FIM (no reference counterpart): none, |phi| after the
  * previous step (identity predictor), or the learned predictor's distances
  * (ls_mlp_predict). Warm values apply to non-seed vertices of the previous
@@ -102,6 +106,7 @@ typedef struct ls_layout_info {
   int64_t n_verts, n_cells, n_copies;
   int32_t n_patches, max_patch_copies, max_patch_verts, max_patch_halo;
   int32_t max_vertex_slots, dim;
+  int64_t n_halo, n_slots, n_adjacent;  /* lengths of halo_ids, vslots, padj */
 } ls_layout_info;
 
 /* Context lifetime. Replaces LevelSet's device selection and reset
@@ -126,6 +131,8 @@ int ls_set_precision(ls_ctx* ctx, int dtype);
 int ls_set_execution(ls_ctx* ctx, int mode);
 /* Selects the FIM sweep organisation (LS_SWEEP_*); may be called at any time. */
 int ls_set_sweep(ls_ctx* ctx, int mode);
+/* Selects where the next partition call builds the layout (LS_LAYOUT_*). */
+int ls_set_layout_builder(ls_ctx* ctx, int builder);
 
 /* Partition + patch layout (meshFIM3d::InitPatches/InitPatches2/
  * GenerateBlockNeighbors, src/core3d/cuda/meshFIM3d.cu:423-570).
@@ -142,6 +149,15 @@ int ls_get_layout_info(ls_ctx* ctx, ls_layout_info* info);
  * copy_verts: F x (dim+1) renumbered vertex ids. */
 int ls_get_layout(ls_ctx* ctx, int32_t* vert_perm, int32_t* copy_src, int32_t* vert_off,
                   int32_t* ele_off, int32_t* copy_verts);
+/* Per-patch lists of the layout (no reference counterpart; for inspection
+ * and tests). Any pointer may be NULL. halo_off/padj_off: P+1; halo_ids:
+ * n_halo renumbered ids, sorted per patch; copy_local: F x 4 patch-local
+ * corner ids (owned vertices first, then the halo in order); vslot_off: N+1;
+ * vslots: n_slots patch-local slots (copy * (dim+1) + corner) per renumbered
+ * owned vertex, ordered by (corner, copy); padj: n_adjacent neighbour
+ * patches per patch, sorted. */
+int ls_get_patch_lists(ls_ctx* ctx, int32_t* halo_off, int32_t* halo_ids, uint16_t* copy_local,
+                       int32_t* vslot_off, uint16_t* vslots, int32_t* padj_off, int32_t* padj);
 
 /* Initial level set, one value per vertex in the caller's order
  * (LevelSet::initializeVertices, src/LevelSet.cu:52-80). dtype LS_F32/LS_F64. */

@@ -22,6 +22,7 @@ struct ls_ctx {
   int precision = LS_F32;  // tet storage precision
   int exec = LS_EXEC_PERSISTENT;
   int sweep = LS_SWEEP_AUTO;
+  int layout_builder = LS_LAYOUT_DEVICE;
   bool have_vel = false;
   lsamd::Layout layout;
   bool have_layout = false;
@@ -96,9 +97,17 @@ int finish_layout(ls_ctx* c, const std::vector<int32_t>& labels, int nparts) {
     if (l < 0 || l >= nparts) return fail(c, LS_ERR_ARG, "patch label out of range");
   if (map_patches(c, labels, nparts) != LS_OK) return LS_ERR_ARG;
   lsamd::Layout L;
-  if (lsamd::build_layout(c->dim, c->N, c->xyz.data(), c->E, c->cells.data(), labels.data(), nparts,
-                          L, err) != 0)
-    return fail(c, LS_ERR_ARG, err);
+  const bool on_device = c->device >= 0 && c->layout_builder == LS_LAYOUT_DEVICE;
+  struct Held {
+    lsamd::DeviceLayout d;
+    ~Held() { d.release(); }
+  } held;
+  const int brc = on_device ? lsamd::build_layout_device(c->dim, c->N, c->xyz.data(), c->E, c->cells.data(),
+                                                         labels.data(), nparts, c->device, L, held.d, err)
+                            : lsamd::build_layout(c->dim, c->N, c->xyz.data(), c->E, c->cells.data(),
+                                                  labels.data(), nparts, L, err);
+  if (brc == -2) return fail(c, LS_ERR_HIP, err);
+  if (brc != 0) return fail(c, LS_ERR_ARG, err);
   if (c->device < 0) {  // host-only context: layout queries, no device state
     c->layout = std::move(L);
     c->engine.reset();
@@ -108,7 +117,7 @@ int finish_layout(ls_ctx* c, const std::vector<int32_t>& labels, int nparts) {
   std::unique_ptr<lsamd::Engine> eng(
       lsamd::make_engine(c->dim, c->dim == 2 ? LS_F64 : c->precision, c->device, err));
   if (!eng) return fail(c, LS_ERR_ARG, err);
-  int rc = eng->upload(L, err);
+  int rc = eng->upload(L, on_device ? &held.d : nullptr, err);
   if (rc != LS_OK) return fail(c, rc, err);
   c->layout = std::move(L);
   c->engine = std::move(eng);
@@ -243,6 +252,14 @@ int ls_set_precision(ls_ctx* c, int dtype) {
   return LS_OK;
 }
 
+int ls_set_layout_builder(ls_ctx* c, int builder) {
+  if (!c) return LS_ERR_ARG;
+  if (builder != LS_LAYOUT_DEVICE && builder != LS_LAYOUT_HOST)
+    return fail(c, LS_ERR_ARG, "builder must be LS_LAYOUT_DEVICE or LS_LAYOUT_HOST");
+  c->layout_builder = builder;
+  return LS_OK;
+}
+
 int ls_set_sweep(ls_ctx* c, int mode) {
   if (!c) return LS_ERR_ARG;
   if (mode != LS_SWEEP_AUTO && mode != LS_SWEEP_ELEMENTS && mode != LS_SWEEP_TASKS)
@@ -304,6 +321,24 @@ int ls_get_layout_info(ls_ctx* c, ls_layout_info* info) {
   info->max_patch_halo = L.max_hp;
   info->max_vertex_slots = L.max_vslots;
   info->dim = L.dim;
+  info->n_halo = (int64_t)L.halo_ids.size();
+  info->n_slots = (int64_t)L.vslots.size();
+  info->n_adjacent = (int64_t)L.padj.size();
+  return LS_OK;
+}
+
+int ls_get_patch_lists(ls_ctx* c, int32_t* halo_off, int32_t* halo_ids, uint16_t* copy_local, int32_t* vslot_off,
+                       uint16_t* vslots, int32_t* padj_off, int32_t* padj) {
+  if (!c) return LS_ERR_ARG;
+  if (!c->have_layout) return fail(c, LS_ERR_STATE, "layout not built");
+  const lsamd::Layout& L = c->layout;
+  if (halo_off) std::copy(L.halo_off.begin(), L.halo_off.end(), halo_off);
+  if (halo_ids) std::copy(L.halo_ids.begin(), L.halo_ids.end(), halo_ids);
+  if (copy_local) std::copy(L.copy_local.begin(), L.copy_local.end(), copy_local);
+  if (vslot_off) std::copy(L.vslot_off.begin(), L.vslot_off.end(), vslot_off);
+  if (vslots) std::copy(L.vslots.begin(), L.vslots.end(), vslots);
+  if (padj_off) std::copy(L.padj_off.begin(), L.padj_off.end(), padj_off);
+  if (padj) std::copy(L.padj.begin(), L.padj.end(), padj);
   return LS_OK;
 }
 

@@ -15,7 +15,9 @@ class Engine {
  public:
   virtual ~Engine() {}
   // Uploads the layout; geometry is rounded to the engine's working type.
-  virtual int upload(const Layout& L, std::string& err) = 0;
+  // Device arrays from the layout: adopted from D when given (D's pointers
+  // are taken over and cleared), uploaded from L otherwise.
+  virtual int upload(const Layout& L, DeviceLayout* D, std::string& err) = 0;
   virtual int set_phi(const double* phi_user, std::string& err) = 0;
   // Velocity per original element (E x 3); projected on the device into every
   // copy's local frame (patch_sigma: the patch's first copy's velocity).

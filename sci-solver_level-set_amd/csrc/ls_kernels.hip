@@ -1931,6 +1931,13 @@ static hipError_t dev_upload(U** dptr, const U* host, size_t n) {
   if (n) e = hipMemcpy(*dptr, host, n * sizeof(U), hipMemcpyHostToDevice);
   return e;
 }
+// Takes over a device allocation (freeing what *dptr held) and clears the source.
+template <typename U>
+static void adopt(U** dptr, U*& src) {
+  if (*dptr) (void)hipFree(*dptr);
+  *dptr = src;
+  src = nullptr;
+}
 template <typename U>
 static hipError_t dev_alloc(U** dptr, size_t n) {
   if (*dptr) { (void)hipFree(*dptr); *dptr = nullptr; }
@@ -1946,7 +1953,7 @@ class EngineImpl final : public Engine {
   int working_bytes() const override { return (int)sizeof(T); }
   void set_exec(int mode) override { exec_ = mode; }
 
-  int upload(const Layout& L, std::string& err) override {
+  int upload(const Layout& L, DeviceLayout* D, std::string& err) override {
     LS_HIP(hipSetDevice(device_));
     if (!stream_) LS_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
     if (!ev_[0]) for (auto& e : ev_) LS_HIP(hipEventCreate(&e));
@@ -1985,20 +1992,35 @@ class EngineImpl final : public Engine {
       const int rc = size_kernels(err);
       if (rc != LS_OK) return rc;
     }
-    LS_HIP(dev_upload(&d_vert_off_, L.vert_off.data(), L.vert_off.size()));
-    LS_HIP(dev_upload(&d_ele_off_, L.ele_off.data(), L.ele_off.size()));
-    LS_HIP(dev_upload(&d_halo_off_, L.halo_off.data(), L.halo_off.size()));
-    LS_HIP(dev_upload(&d_halo_ids_, L.halo_ids.data(), L.halo_ids.size()));
-    LS_HIP(dev_upload(&d_copy_local_, reinterpret_cast<const ushort4*>(L.copy_local.data()), (size_t)F_));
-    LS_HIP(dev_upload(&d_vslot_off_, L.vslot_off.data(), L.vslot_off.size()));
-    LS_HIP(dev_upload(&d_vslots_, L.vslots.data(), L.vslots.size()));
-    LS_HIP(dev_upload(&d_padj_off_, L.padj_off.data(), L.padj_off.size()));
-    LS_HIP(dev_upload(&d_padj_, L.padj.data(), L.padj.size()));
-    // Element geometry on the device: the renumbered coordinates and corner
-    // ids stay resident for velocity projection.
-    LS_HIP(dev_upload(&d_xyz_, L.xyz.data(), L.xyz.size()));
-    LS_HIP(dev_upload(&d_copy_verts_, L.copy_verts.data(), L.copy_verts.size()));
-    LS_HIP(dev_upload(&d_copy_src_, L.copy_src.data(), L.copy_src.size()));
+    // The renumbered coordinates and corner ids stay resident for the local
+    // coordinates and velocity projection.
+    if (D) {
+      adopt(&d_vert_off_, D->vert_off);
+      adopt(&d_ele_off_, D->ele_off);
+      adopt(&d_halo_off_, D->halo_off);
+      adopt(&d_halo_ids_, D->halo_ids);
+      adopt(&d_copy_local_, reinterpret_cast<ushort4*&>(D->copy_local));
+      adopt(&d_vslot_off_, D->vslot_off);
+      adopt(&d_vslots_, D->vslots);
+      adopt(&d_padj_off_, D->padj_off);
+      adopt(&d_padj_, D->padj);
+      adopt(&d_xyz_, D->xyz);
+      adopt(&d_copy_verts_, D->copy_verts);
+      adopt(&d_copy_src_, D->copy_src);
+    } else {
+      LS_HIP(dev_upload(&d_vert_off_, L.vert_off.data(), L.vert_off.size()));
+      LS_HIP(dev_upload(&d_ele_off_, L.ele_off.data(), L.ele_off.size()));
+      LS_HIP(dev_upload(&d_halo_off_, L.halo_off.data(), L.halo_off.size()));
+      LS_HIP(dev_upload(&d_halo_ids_, L.halo_ids.data(), L.halo_ids.size()));
+      LS_HIP(dev_upload(&d_copy_local_, reinterpret_cast<const ushort4*>(L.copy_local.data()), (size_t)F_));
+      LS_HIP(dev_upload(&d_vslot_off_, L.vslot_off.data(), L.vslot_off.size()));
+      LS_HIP(dev_upload(&d_vslots_, L.vslots.data(), L.vslots.size()));
+      LS_HIP(dev_upload(&d_padj_off_, L.padj_off.data(), L.padj_off.size()));
+      LS_HIP(dev_upload(&d_padj_, L.padj.data(), L.padj.size()));
+      LS_HIP(dev_upload(&d_xyz_, L.xyz.data(), L.xyz.size()));
+      LS_HIP(dev_upload(&d_copy_verts_, L.copy_verts.data(), L.copy_verts.size()));
+      LS_HIP(dev_upload(&d_copy_src_, L.copy_src.data(), L.copy_src.size()));
+    }
     {
       constexpr int K = ElemSolve<T, NC>::kCoords;
       LS_HIP(dev_alloc(&d_coords_, (size_t)F_ * K));

@@ -12,8 +12,9 @@
 //     (src/core3d/cuda/meshFIM3d.cu:537-542, filtered per patch at
 //     src/core3d/include/redistance_kernels3d.h:311-329).
 // Unlike the reference, each patch also gets a sorted halo list so a kernel
-// fetches every outside vertex once per patch, element corners are stored as
-// 16-bit patch-local indices, and per-copy geometry is precomputed in FP64.
+// fetches every outside vertex once per patch and element corners are stored
+// as 16-bit patch-local indices. Per-copy local coordinates are computed on
+// the device from the renumbered coordinates and corner ids.
 #pragma once
 #include <cstdint>
 #include <string>
@@ -60,6 +61,23 @@ int partition_rcb(int64_t N, const double* xyz, int nparts, std::vector<int32_t>
 
 int build_layout(int dim, int64_t N, const double* xyz, int64_t E, const int32_t* cells,
                  const int32_t* labels, int nparts, Layout& L, std::string& err);
+
+// Device copies of the layout arrays (hipMalloc'd), as build_layout_device
+// leaves them for the engine to adopt; release() frees what is still held.
+struct DeviceLayout {
+  int32_t *vert_off = nullptr, *ele_off = nullptr, *halo_off = nullptr, *halo_ids = nullptr;
+  int32_t *vslot_off = nullptr, *padj_off = nullptr, *padj = nullptr, *copy_verts = nullptr, *copy_src = nullptr;
+  uint16_t *copy_local = nullptr, *vslots = nullptr;
+  double* xyz = nullptr;
+  void release();
+};
+
+// The same layout built on GPU `device` (ls_layout_gpu.hip): identical
+// arrays and orders to build_layout, also copied into L. Returns 0, -1 for
+// invalid input (err says why) or -2 for a HIP failure.
+int build_layout_device(int dim, int64_t N, const double* xyz, int64_t E, const int32_t* cells,
+                        const int32_t* labels, int nparts, int device, Layout& L, DeviceLayout& D,
+                        std::string& err);
 
 
 }  // namespace lsamd

@@ -19,6 +19,7 @@ LS_MODE_ADVECT, LS_MODE_MCF = 0, 1
 LS_EXEC_PERSISTENT, LS_EXEC_KERNELS, LS_EXEC_GRAPH = 0, 1, 2
 LS_WARM_NONE, LS_WARM_IDENTITY, LS_WARM_LEARNED = 0, 1, 2
 LS_SWEEP_AUTO, LS_SWEEP_ELEMENTS, LS_SWEEP_TASKS = 0, 1, 2
+LS_LAYOUT_DEVICE, LS_LAYOUT_HOST = 0, 1
 
 
 class StepParams(C.Structure):
@@ -53,12 +54,14 @@ class LayoutInfo(C.Structure):
     _fields_ = [("n_verts", C.c_int64), ("n_cells", C.c_int64), ("n_copies", C.c_int64),
                 ("n_patches", C.c_int32), ("max_patch_copies", C.c_int32),
                 ("max_patch_verts", C.c_int32), ("max_patch_halo", C.c_int32),
-                ("max_vertex_slots", C.c_int32), ("dim", C.c_int32)]
+                ("max_vertex_slots", C.c_int32), ("dim", C.c_int32),
+                ("n_halo", C.c_int64), ("n_slots", C.c_int64), ("n_adjacent", C.c_int64)]
 
 
 EXPORTS = ["ls_create", "ls_destroy", "ls_last_error", "ls_set_mesh", "ls_set_precision", "ls_set_execution",
            "ls_set_sweep", "ls_partition_grid",
-           "ls_partition_rcb", "ls_partition_labels", "ls_get_layout_info", "ls_get_layout",
+           "ls_partition_rcb", "ls_partition_labels", "ls_get_layout_info", "ls_get_layout", "ls_get_patch_lists",
+           "ls_set_layout_builder",
            "ls_set_phi", "ls_set_velocity", "ls_step", "ls_get_phi", "ls_get_narrowband",
            "ls_set_mesh_batch", "ls_partition_batch", "ls_set_batch_params", "ls_get_batch_nb_counts",
            "ls_get_distances", "ls_mlp_set_params", "ls_mlp_get_params", "ls_mlp_predict", "ls_mlp_collect",
@@ -93,6 +96,8 @@ def lib():
         L.ls_set_precision.argtypes = [vp, C.c_int]
         L.ls_set_execution.argtypes = [vp, C.c_int]
         L.ls_set_sweep.argtypes = [vp, C.c_int]
+        L.ls_set_layout_builder.argtypes = [vp, C.c_int]
+        L.ls_get_patch_lists.argtypes = [vp] + [vp] * 7
         L.ls_partition_grid.argtypes = [vp] + [C.c_int] * 6
         L.ls_partition_rcb.argtypes = [vp, C.c_int]
         L.ls_partition_labels.argtypes = [vp, vp, C.c_int]
@@ -168,6 +173,9 @@ class Solver:
     def set_sweep(self, mode):
         self._check(self.L.ls_set_sweep(self.ctx, int(mode)))
 
+    def set_layout_builder(self, builder):
+        self._check(self.L.ls_set_layout_builder(self.ctx, int(builder)))
+
     def set_precision(self, dtype):
         self._check(self.L.ls_set_precision(self.ctx, int(dtype)))
 
@@ -198,6 +206,18 @@ class Solver:
                                          eo.ctypes.data, cv.ctypes.data))
         return {"vert_perm": vp, "copy_src": cs, "vert_off": vo, "ele_off": eo,
                 "copy_verts": cv.reshape(F, nc)}
+
+    def patch_lists(self):
+        info = self.layout_info()
+        N, F, P = info["n_verts"], info["n_copies"], info["n_patches"]
+        out = {"halo_off": np.empty(P + 1, np.int32), "halo_ids": np.empty(info["n_halo"], np.int32),
+               "copy_local": np.empty((F, 4), np.uint16), "vslot_off": np.empty(N + 1, np.int32),
+               "vslots": np.empty(info["n_slots"], np.uint16), "padj_off": np.empty(P + 1, np.int32),
+               "padj": np.empty(info["n_adjacent"], np.int32)}
+        self._check(self.L.ls_get_patch_lists(self.ctx, *(out[k].ctypes.data for k in
+                                                          ("halo_off", "halo_ids", "copy_local", "vslot_off",
+                                                           "vslots", "padj_off", "padj"))))
+        return out
 
     def set_phi(self, phi):
         a = np.ascontiguousarray(phi, dtype=np.float64)

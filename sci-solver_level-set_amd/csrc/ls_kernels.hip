@@ -1834,31 +1834,30 @@ __global__ __launch_bounds__(BS) void k_step(DevMesh<T> m, DevState<T> s, StepAr
     }
     grid_barrier(s.bar, G);
     pcl.lap(clk, kTkCopyback);
+    // Probes, and the next list in the same phase: unconverged members not
+    // probed this iteration (their flags are final since the visit phase),
+    // and each probed patch that stays unconverged, appended by the block
+    // that probed it. The list order is immaterial (patches of one
+    // iteration never read each other's outputs).
     const int nF = ld_acquire(&s.pc[kPcFresh + par]);
     if (gtid == 0) s.pc[kPcProbes] += nF;
-    for (int i = b; i < nF; i += G)
-      sweep_patch<T, NC, BS, ORG>(m, s, s.listF[i], 1, 0, a.tol, a.leps, smem, red, s_ndirty);
-    grid_barrier(s.bar, G);
-    pcl.lap(clk, kTkProbe);
-    // Next list: unconverged members not probed this iteration, then
-    // unconverged probed patches.
-    for (int64_t i = gtid; i < (int64_t)nA + nF; i += gsize) {
-      int p;
-      bool keep;
-      if (i < nA) {
-        p = cur[i];
-        keep = s.freshE[p] != E && !s.bcon[p];
-      } else {
-        p = s.listF[i - nA];
-        keep = !s.bcon[p];
-      }
-      if (keep) {
+    for (int64_t i = gtid; i < (int64_t)nA; i += gsize) {
+      const int p = cur[i];
+      if (s.freshE[p] != E && !s.bcon[p]) {
         nxt[atomicAdd(&s.pc[kPcNext + par], 1)] = p;
         s.inA[p] = E + 1;
       }
     }
+    for (int i = b; i < nF; i += G) {
+      const int q = s.listF[i];
+      sweep_patch<T, NC, BS, ORG>(m, s, q, 1, 0, a.tol, a.leps, smem, red, s_ndirty);
+      if (tid == 0 && !s.bcon[q]) {  // written by this thread in sweep_patch
+        nxt[atomicAdd(&s.pc[kPcNext + par], 1)] = q;
+        s.inA[q] = E + 1;
+      }
+    }
     grid_barrier(s.bar, G);
-    pcl.lap(clk, kTkNext);
+    pcl.lap(clk, kTkProbe);
     nA = ld_acquire(&s.pc[kPcNext + par]);
     int32_t* t = cur;
     cur = nxt;

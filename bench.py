@@ -440,7 +440,8 @@ def main():
         avg_ms = tot["step_ms"] / args.steps
         avg_bytes = tot["step_bytes"] / args.steps
     achieved = avg_bytes / (avg_ms * 1e-3) / 1e9
-    traffic = profiled_traffic("k_sweep_torus_revolve_64") if exec_mode == lsb.LS_EXEC_KERNELS else None
+    prof_name = "k_sweep_torus_revolve_64" if exec_mode == lsb.LS_EXEC_KERNELS else "k_step_torus_revolve_64_graph"
+    traffic = profiled_traffic(prof_name)
 
     out = {
         "metric": "narrowband vertex-updates/s, learned warm-start vs plain FIM @ Linf<=1e-5",
@@ -478,7 +479,7 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
-            "traffic_source": "profiles/k_sweep_torus_revolve_64.md (rocprofv3 FETCH_SIZE + WRITE_SIZE, "
+            "traffic_source": f"profiles/{prof_name}.md (rocprofv3 FETCH_SIZE + WRITE_SIZE, "
                               "copy-calibrated)" if traffic else None,
             "avg_launch_ms": avg_ms,
             "avg_launch_algorithmic_bytes": avg_bytes,

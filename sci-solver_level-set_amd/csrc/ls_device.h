@@ -72,6 +72,16 @@ struct DevState {
   unsigned long long* ticks;  // kNumTicks phase times (device wall clock)
   unsigned int* bar;          // grid barrier: [0] arrivals, [1] generation
   const T* warm;              // N warm-start distances (learned predictor), or null
+  // A probe's state after its sweep (slot values, vertex values, dirty
+  // flags, at the patch's ele_off / vert_off positions), kept when the
+  // probe leaves the patch unconverged: the patch's visit in the next FIM
+  // iteration reads the same distances, so its first sweep would repeat the
+  // probe and it resumes from here instead. rs_epoch[p] names the iteration
+  // whose visit may resume (0: none).
+  T* rs_slot;                 // F x NC
+  T* rs_vval;                 // N
+  uint8_t* rs_dirty;          // F
+  int32_t* rs_epoch;          // P
 };
 
 enum TickIndex : int { kTkSeed = 0, kTkVisit, kTkCopyback, kTkProbe, kTkNext, kTkNb, kTkAdvect, kNumTicks = 8 };

@@ -621,9 +621,14 @@ __device__ __forceinline__ void copy_to_lds(V* dst, const V* src, int n) {
 // them over the workgroup one per thread.
 constexpr int kOrgElements = 0, kOrgTasks = 1;
 
+// epoch_in > 0 (a visit): resume after the first sweep from the state a
+// probe saved for this iteration, when there is one. epoch_out > 0 (a probe):
+// save the state after the sweep for the next iteration's visit when the
+// patch stays unconverged. The resumed visit computes the same bits.
 template <typename T, int NC, int BS, int ORG>
 __device__ void sweep_patch(const DevMesh<T>& m, DevState<T>& s, int p, int niter, int write_out, T tol,
-                            double leps, unsigned char* smem, T* red, int* s_ndirty /*[10]*/) {
+                            double leps, unsigned char* smem, T* red, int* s_ndirty /*[10]*/, int epoch_in,
+                            int epoch_out) {
   constexpr bool tasks = ORG == kOrgTasks;
   constexpr int K = ElemSolve<T, NC>::kCoords;
   __syncthreads();
@@ -654,7 +659,14 @@ __device__ void sweep_patch(const DevMesh<T>& m, DevState<T>& s, int p, int nite
   if (tid == 0) s_voff[nv] = (uint16_t)nslots;
   for (int i = tid; i < nh; i += BS) s_vtx[nv + i] = s.DT[m.halo_ids[hs + i]];
   copy_to_lds<BS, 8>(s_vsl, m.vslots + sbase, nslots);
+  const bool resume = epoch_in > 0 && s.rs_epoch[p] == epoch_in;
+  if (resume) {
+    copy_to_lds<BS, 4>(s_slot, s.rs_slot + (int64_t)es * NC, NC * ne);
+    copy_to_lds<BS, 4>(s_dirty, s.rs_dirty + es, ne);
+    copy_to_lds<BS, 4>(s_vval, s.rs_vval + vs, nv);
+  }
   __syncthreads();
+  if (!resume)
   for (int base = tid; base < ne; base += 4 * BS) {
     ushort4 c[4];
 #pragma unroll
@@ -674,7 +686,7 @@ __device__ void sweep_patch(const DevMesh<T>& m, DevState<T>& s, int p, int nite
   int con_all = 1;
   T vmin = (T)kLargeNum;
   int n_upd = 0, n_solve = 0;
-  for (int it = 0; it < niter; ++it) {
+  for (int it = resume ? 1 : 0; it < niter; ++it) {
     if constexpr (tasks) {
       if (tid == 0) *s_ndirty = 0;
       __syncthreads();
@@ -752,9 +764,10 @@ __device__ void sweep_patch(const DevMesh<T>& m, DevState<T>& s, int p, int nite
     } else {
       // Need masks of the dirty elements (kept in s_dirty bits 1..NC), then
       // the elements with a corner to solve listed by their number of
-      // solves, so the lanes of a wave walk the same number of corners.
-      // Counts per solve number c = 1..NC in s_ndirty[1 + c], list cursors
-      // in s_ndirty[5 + c].
+      // solves, most first, so the lanes of a wave walk the same number of
+      // corners and the chunks that wrap around to the first waves are the
+      // cheapest. Counts per solve number c = 1..NC in s_ndirty[1 + c], list
+      // cursors in s_ndirty[5 + c].
       if (tid < 8) s_ndirty[2 + tid] = 0;
       __syncthreads();
       for (int base = 0; base < ne; base += BS) {
@@ -780,11 +793,11 @@ __device__ void sweep_patch(const DevMesh<T>& m, DevState<T>& s, int p, int nite
         }
       }
       __syncthreads();
-      int start[NC + 1];
-      start[1] = 0;
+      int start[NC + 2];
+      start[NC] = 0;
 #pragma unroll
-      for (int j = 2; j <= NC; ++j) start[j] = start[j - 1] + s_ndirty[j];
-      const int nd = start[NC] + s_ndirty[1 + NC];
+      for (int j = NC - 1; j >= 1; --j) start[j] = start[j + 1] + s_ndirty[2 + j];
+      const int nd = start[1] + s_ndirty[2];
       for (int base = 0; base < ne; base += BS) {
         const int e = base + tid;
         const int c = e < ne ? __popc(s_dirty[e] >> 1) : 0;
@@ -849,6 +862,12 @@ __device__ void sweep_patch(const DevMesh<T>& m, DevState<T>& s, int p, int nite
     }
   }
   con_all = __syncthreads_and(con_all);
+  if (epoch_out > 0 && !con_all) {
+    for (int i = tid; i < NC * ne; i += BS) s.rs_slot[(int64_t)es * NC + i] = s_slot[i];
+    for (int i = tid; i < ne; i += BS) s.rs_dirty[es + i] = s_dirty[i];
+    for (int i = tid; i < nv; i += BS) s.rs_vval[vs + i] = s_vval[i];
+    if (tid == 0) s.rs_epoch[p] = epoch_out;
+  }
   const T bm = block_min_n<T, BS>(vmin, red);
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
@@ -1223,6 +1242,11 @@ __global__ void k_set_counters(int32_t* c, int n, int32_t val) {
   if ((int)threadIdx.x < n) c[threadIdx.x] = val;
 }
 
+__global__ void k_fill(int32_t* a, int64_t n, int32_t val) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = val;
+}
+
 __global__ void k_clear_sizes(unsigned long long* c) {
   if ((int)threadIdx.x < kNumSizes) c[threadIdx.x] = 0ull;
 }
@@ -1238,11 +1262,13 @@ __global__ __launch_bounds__(kBlock) void k_seeds(DevMesh<T> m, DevState<T> s, c
 
 template <typename T, int NC, int BS, int ORG>
 __global__ __launch_bounds__(BS) void k_sweep(DevMesh<T> m, DevState<T> s, const int32_t* list,
-                                              int niter, int write_out, T tol, double leps) {
+                                              int niter, int write_out, T tol, double leps, int epoch_in,
+                                              int epoch_out) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ T red[BS / 64];
   __shared__ int s_ndirty[10];
-  sweep_patch<T, NC, BS, ORG>(m, s, list[blockIdx.x], niter, write_out, tol, leps, smem, red, s_ndirty);
+  sweep_patch<T, NC, BS, ORG>(m, s, list[blockIdx.x], niter, write_out, tol, leps, smem, red, s_ndirty, epoch_in,
+                              epoch_out);
 }
 
 // CopyOutBack (redistance_kernels3d.h:772-783) plus the active-list label of
@@ -1746,7 +1772,7 @@ __global__ __launch_bounds__(BS) void k_step(DevMesh<T> m, DevState<T> s, StepAr
 
   // Phase 0: vertex reset and sign save; patch marks; counters.
   for (int64_t v = gtid; v < m.N; v += gsize) init_vertex(s, v);
-  for (int64_t q = gtid; q < m.P; q += gsize) { s.inA[q] = 0; s.freshE[q] = 0; }
+  for (int64_t q = gtid; q < m.P; q += gsize) { s.inA[q] = 0; s.freshE[q] = 0; s.rs_epoch[q] = 0; }
   if (gtid < kNumSizes && attempt == 0) s.sizes[gtid] = 0ull;
   if (gtid == 0) {
     s.pc[kPcSeed] = 0;
@@ -1806,7 +1832,7 @@ __global__ __launch_bounds__(BS) void k_step(DevMesh<T> m, DevState<T> s, StepAr
     }
     const int E = it + 1, par = it & 1;
     for (int i = b; i < nA; i += G)
-      sweep_patch<T, NC, BS, ORG>(m, s, cur[i], a.niter, 1, a.tol, a.leps, smem, red, s_ndirty);
+      sweep_patch<T, NC, BS, ORG>(m, s, cur[i], a.niter, 1, a.tol, a.leps, smem, red, s_ndirty, E, 0);
     grid_barrier(s.bar, G);
     pcl.lap(clk, kTkVisit);
     // Every block has read this iteration's list size, so the counters the
@@ -1850,7 +1876,7 @@ __global__ __launch_bounds__(BS) void k_step(DevMesh<T> m, DevState<T> s, StepAr
     }
     for (int i = b; i < nF; i += G) {
       const int q = s.listF[i];
-      sweep_patch<T, NC, BS, ORG>(m, s, q, 1, 0, a.tol, a.leps, smem, red, s_ndirty);
+      sweep_patch<T, NC, BS, ORG>(m, s, q, 1, 0, a.tol, a.leps, smem, red, s_ndirty, 0, E + 1);
       if (tid == 0 && !s.bcon[q]) {  // written by this thread in sweep_patch
         nxt[atomicAdd(&s.pc[kPcNext + par], 1)] = q;
         s.inA[q] = E + 1;
@@ -2033,6 +2059,11 @@ class EngineImpl final : public Engine {
         LS_HIP(hipStreamSynchronize(stream_));
       }
     }
+    LS_HIP(dev_alloc(&d_rs_slot_, (size_t)F_ * NC));
+    LS_HIP(dev_alloc(&d_rs_vval_, (size_t)N_));
+    LS_HIP(dev_alloc(&d_rs_dirty_, (size_t)F_));
+    LS_HIP(dev_alloc(&d_rs_epoch_, (size_t)P_));
+    LS_HIP(hipMemset(d_rs_epoch_, 0, (size_t)P_ * sizeof(int32_t)));
     LS_HIP(dev_alloc(&d_sigma_, (size_t)F_ * (NC - 1)));
     LS_HIP(hipMemset(d_sigma_, 0, (size_t)F_ * (NC - 1) * sizeof(T)));
     LS_HIP(dev_alloc(&d_phi_, N_));
@@ -2375,11 +2406,13 @@ class EngineImpl final : public Engine {
   }
   int sweep_tasks() const override { return org_ == kOrgTasks ? 1 : 0; }
   void launch_sweep(int n, hipStream_t q, const DevMesh<T>& m, const DevState<T>& s, const int32_t* list, int niter,
-                    int write_out, T tol, double leps) {
+                    int write_out, T tol, double leps, int epoch_in, int epoch_out) {
     if (org_ == kOrgTasks)
-      k_sweep<T, NC, kFimBlock, kOrgTasks><<<n, kFimBlock, fim_lds_, q>>>(m, s, list, niter, write_out, tol, leps);
+      k_sweep<T, NC, kFimBlock, kOrgTasks><<<n, kFimBlock, fim_lds_, q>>>(m, s, list, niter, write_out, tol, leps,
+                                                                         epoch_in, epoch_out);
     else
-      k_sweep<T, NC, kFimBlock, kOrgElements><<<n, kFimBlock, fim_lds_, q>>>(m, s, list, niter, write_out, tol, leps);
+      k_sweep<T, NC, kFimBlock, kOrgElements><<<n, kFimBlock, fim_lds_, q>>>(m, s, list, niter, write_out, tol, leps,
+                                                                            epoch_in, epoch_out);
   }
 
   int set_patch_params(const std::vector<double>& dt, const std::vector<double>& bw, std::string& err) override {
@@ -2559,6 +2592,7 @@ class EngineImpl final : public Engine {
     // Seeds over every patch on the first step, else over the previous narrowband.
     kev_used_ = 0;
     k_init<T><<<grid_for(N_), kBlock, 0, q>>>(N_, s);
+    k_fill<<<grid_for(P_), kBlock, 0, q>>>(d_rs_epoch_, P_, 0);
     k_set_counters<<<1, 64, 0, q>>>(d_counters_, kNumCounters, 0);
     k_clear_sizes<<<1, 64, 0, q>>>(d_sizes_);
     const int32_t* scan = nnb_ > 0 ? d_nb_cur() : d_scan_all_;
@@ -2578,7 +2612,7 @@ class EngineImpl final : public Engine {
       st.fim_iters++;
       st.patch_visits += nact;
       int ke = kev_begin(0, q);
-      launch_sweep(nact, q, m, s, cur, prm.niter, 1, tol, leps);
+      launch_sweep(nact, q, m, s, cur, prm.niter, 1, tol, leps, st.fim_iters, 0);
       kev_end(ke, q);
       k_copyback<T><<<nact, kBlock, 0, q>>>(m, s, cur);
       k_set_counters<<<1, 64, 0, q>>>(d_counters_ + kCntFresh, 2, 0);
@@ -2590,7 +2624,7 @@ class EngineImpl final : public Engine {
       if (nf > 0) {
         st.probes += nf;
         int kp = kev_begin(0, q);
-        launch_sweep(nf, q, m, s, fresh_list, 1, 0, tol, leps);
+        launch_sweep(nf, q, m, s, fresh_list, 1, 0, tol, leps, 0, st.fim_iters + 1);
         kev_end(kp, q);
       }
       k_finalize<T><<<grid_for(nact + nf), kBlock, 0, q>>>(s, cur, nact, fresh_list, nf, next);
@@ -2679,6 +2713,10 @@ class EngineImpl final : public Engine {
     s.ticks = d_ticks_;
     s.bar = d_bar_;
     s.warm = d_warm_;
+    s.rs_slot = d_rs_slot_;
+    s.rs_vval = d_rs_vval_;
+    s.rs_dirty = d_rs_dirty_;
+    s.rs_epoch = d_rs_epoch_;
     return s;
   }
   int32_t* d_nb_cur() const { return nb_sel_ ? d_lists_[3] : d_nb_; }
@@ -2692,7 +2730,7 @@ class EngineImpl final : public Engine {
                     d_lists_[3], d_inA_, d_freshE_, d_listF_, d_pc_, d_ticks_, d_bar_,
                     d_patch_dt_, d_patch_bw_, d_warm_, d_hgt_, d_mlp_prm_, d_mlp_m_, d_mlp_v_, d_mlp_grad_,
                     d_mlp_b_, d_mlp_w1_, d_mlp_w2_, d_mlp_w3_, d_ds_x_, d_ds_y_, d_pend_, d_xyz_,
-                    d_copy_verts_, d_copy_src_};
+                    d_copy_verts_, d_copy_src_, d_rs_slot_, d_rs_vval_, d_rs_dirty_, d_rs_epoch_};
     if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
     if (graph_) (void)hipGraphDestroy(graph_);
     for (void* p : ptrs) if (p) (void)hipFree(p);
@@ -2722,6 +2760,9 @@ class EngineImpl final : public Engine {
   int32_t *d_padj_off_ = nullptr, *d_padj_ = nullptr;
   T *d_coords_ = nullptr, *d_sigma_ = nullptr;
   double* d_xyz_ = nullptr;          // N x 3, renumbered vertex coordinates
+  T *d_rs_slot_ = nullptr, *d_rs_vval_ = nullptr;  // probe states for resumed visits
+  uint8_t* d_rs_dirty_ = nullptr;
+  int32_t* d_rs_epoch_ = nullptr;
   int32_t* d_copy_verts_ = nullptr;  // F x NC, renumbered corner ids
   int32_t* d_copy_src_ = nullptr;    // F, original element of each copy
   T* d_hgt_ = nullptr;  // tet copies: corner heights (F x 4)

@@ -68,11 +68,6 @@ __global__ void k_iota(int32_t* a, int64_t n) {
   if (i < n) a[i] = (int32_t)i;
 }
 
-__global__ void k_histogram(const int32_t* key, int64_t n, int32_t* cnt) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) atomicAdd(&cnt[key[i]], 1);
-}
-
 // Vertex renumbering products: inverse permutation and renumbered coordinates.
 __global__ void k_renumber(const int32_t* perm, const double* xyz, int64_t N, int32_t* iperm, double* xyz_new) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -166,6 +161,18 @@ __global__ void k_segment_offsets(const uint64_t* key, int64_t n, int P, int32_t
   while (lo < hi) {
     const int64_t mid = (lo + hi) >> 1;
     if ((int64_t)(key[mid] >> 32) < p) lo = mid + 1; else hi = mid;
+  }
+  off[p] = (int32_t)lo;
+}
+
+// off[p] = first index k with key[k] >= p, for p = 0..P (key sorted).
+__global__ void k_sorted_offsets(const int32_t* key, int64_t n, int P, int32_t* off) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p > P) return;
+  int64_t lo = 0, hi = n;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (key[mid] < p) lo = mid + 1; else hi = mid;
   }
   off[p] = (int32_t)lo;
 }
@@ -341,26 +348,23 @@ int build_layout_device(int dim, int64_t N, const double* xyz, int64_t E, const 
 
   // Vertex renumbering: labels sorted stably with the vertex ids as values.
   const int pbits = bits_for((uint64_t)P);
-  DevBuf<int32_t> d_iota, d_sorted_lab, d_perm, d_iperm, d_vcnt, d_vert_off;
+  DevBuf<int32_t> d_iota, d_sorted_lab, d_perm, d_iperm, d_vert_off;
   DevBuf<double> d_xyz;
   LG_HIP(d_iota.alloc((size_t)N));
   LG_HIP(d_sorted_lab.alloc((size_t)N));
   LG_HIP(d_perm.alloc((size_t)N));
   LG_HIP(d_iperm.alloc((size_t)N));
   LG_HIP(d_xyz.alloc((size_t)N * 3));
-  LG_HIP(d_vcnt.alloc((size_t)P));
   LG_HIP(d_vert_off.alloc((size_t)P + 1));
   k_iota<<<grid(N), kB, 0, q>>>(d_iota.p, N);
   LG_RUN(sort_pairs(tmp, d_labels.p, d_sorted_lab.p, d_iota.p, d_perm.p, N, pbits, q, err));
   k_renumber<<<grid(N), kB, 0, q>>>(d_perm.p, d_xyz_in.p, N, d_iperm.p, d_xyz.p);
-  LG_HIP(hipMemsetAsync(d_vcnt.p, 0, (size_t)P * sizeof(int32_t), q));
-  k_histogram<<<grid(N), kB, 0, q>>>(d_labels.p, N, d_vcnt.p);
-  LG_RUN(scan32(tmp, d_vcnt.p, d_vert_off.p, P, q, err));
+  k_sorted_offsets<<<grid(P + 1), kB, 0, q>>>(d_sorted_lab.p, N, P, d_vert_off.p);
   LG_HIP(hipGetLastError());
   d_xyz_in.reset();
 
   // Element copies: per element its distinct patches, then a stable sort by patch.
-  DevBuf<int32_t> d_ecnt, d_ckey, d_cval, d_cpatch, d_copy_src, d_pcnt, d_ele_off;
+  DevBuf<int32_t> d_ecnt, d_ckey, d_cval, d_cpatch, d_copy_src, d_ele_off;
   DevBuf<int64_t> d_eoff;
   LG_HIP(d_ecnt.alloc((size_t)E));
   LG_HIP(d_eoff.alloc((size_t)E + 1));
@@ -380,11 +384,8 @@ int build_layout_device(int dim, int64_t N, const double* xyz, int64_t E, const 
   d_cval.reset();
   d_ecnt.reset();
   d_eoff.reset();
-  LG_HIP(d_pcnt.alloc((size_t)P));
   LG_HIP(d_ele_off.alloc((size_t)P + 1));
-  LG_HIP(hipMemsetAsync(d_pcnt.p, 0, (size_t)P * sizeof(int32_t), q));
-  k_histogram<<<grid(F), kB, 0, q>>>(d_cpatch.p, F, d_pcnt.p);
-  LG_RUN(scan32(tmp, d_pcnt.p, d_ele_off.p, P, q, err));
+  k_sorted_offsets<<<grid(P + 1), kB, 0, q>>>(d_cpatch.p, F, P, d_ele_off.p);
 
   DevBuf<int32_t> d_cv;
   LG_HIP(d_cv.alloc((size_t)F * nc));

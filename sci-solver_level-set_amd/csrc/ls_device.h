@@ -29,6 +29,7 @@ struct DevMesh {
   const T* hgt;                // F x 4 corner heights over the opposite face (tets), or null
   const int32_t* padj_off;     // P+1
   const int32_t* padj;
+  const int32_t* vpatch;       // N, patch of each renumbered vertex
   // Per-patch timestep and narrowband width for batches of meshes with
   // their own parameters; null: the step's uniform values.
   const double* patch_dt;      // P or null
@@ -82,6 +83,7 @@ struct DevState {
   T* rs_vval;                 // N
   uint8_t* rs_dirty;          // F
   int32_t* rs_epoch;          // P
+  int32_t* visitE;            // P, FIM iteration in which the patch was last visited
 };
 
 enum TickIndex : int { kTkSeed = 0, kTkVisit, kTkCopyback, kTkProbe, kTkNext, kTkNb, kTkAdvect, kNumTicks = 8 };

@@ -625,10 +625,12 @@ constexpr int kOrgElements = 0, kOrgTasks = 1;
 // probe saved for this iteration, when there is one. epoch_out > 0 (a probe):
 // save the state after the sweep for the next iteration's visit when the
 // patch stays unconverged. The resumed visit computes the same bits.
+// members > 0: vertices of patches visited in FIM iteration `members` are
+// read from DT_out (their copy into DT runs concurrently), all others from DT.
 template <typename T, int NC, int BS, int ORG>
 __device__ void sweep_patch(const DevMesh<T>& m, DevState<T>& s, int p, int niter, int write_out, T tol,
                             double leps, unsigned char* smem, T* red, int* s_ndirty /*[10]*/, int epoch_in,
-                            int epoch_out) {
+                            int epoch_out, int members = 0) {
   constexpr bool tasks = ORG == kOrgTasks;
   constexpr int K = ElemSolve<T, NC>::kCoords;
   __syncthreads();
@@ -650,14 +652,18 @@ __device__ void sweep_patch(const DevMesh<T>& m, DevState<T>& s, int p, int nite
   T* s_new = (T*)(s_task + NC * ne + (NC * ne) % 2);
   const int tid = threadIdx.x;
 
+  const T* own = (members > 0 && s.visitE[p] == members) ? s.DT_out : s.DT;
   for (int i = tid; i < nv; i += BS) {
-    const T v = s.DT[vs + i];
+    const T v = own[vs + i];
     s_vtx[i] = v;
     s_vval[i] = v;
     s_voff[i] = (uint16_t)(m.vslot_off[vs + i] - sbase);
   }
   if (tid == 0) s_voff[nv] = (uint16_t)nslots;
-  for (int i = tid; i < nh; i += BS) s_vtx[nv + i] = s.DT[m.halo_ids[hs + i]];
+  for (int i = tid; i < nh; i += BS) {
+    const int h = m.halo_ids[hs + i];
+    s_vtx[nv + i] = (members > 0 && s.visitE[m.vpatch[h]] == members) ? s.DT_out[h] : s.DT[h];
+  }
   copy_to_lds<BS, 8>(s_vsl, m.vslots + sbase, nslots);
   const bool resume = epoch_in > 0 && s.rs_epoch[p] == epoch_in;
   if (resume) {
@@ -1247,6 +1253,13 @@ __global__ void k_fill(int32_t* a, int64_t n, int32_t val) {
   if (i < n) a[i] = val;
 }
 
+// vpatch[v] = p for the renumbered vertices v of patch p.
+__global__ void k_vertex_patch(const int32_t* vert_off, int P, int32_t* vpatch) {
+  const int p = blockIdx.x;
+  if (p >= P) return;
+  for (int v = vert_off[p] + threadIdx.x; v < vert_off[p + 1]; v += blockDim.x) vpatch[v] = p;
+}
+
 __global__ void k_clear_sizes(unsigned long long* c) {
   if ((int)threadIdx.x < kNumSizes) c[threadIdx.x] = 0ull;
 }
@@ -1772,7 +1785,12 @@ __global__ __launch_bounds__(BS) void k_step(DevMesh<T> m, DevState<T> s, StepAr
 
   // Phase 0: vertex reset and sign save; patch marks; counters.
   for (int64_t v = gtid; v < m.N; v += gsize) init_vertex(s, v);
-  for (int64_t q = gtid; q < m.P; q += gsize) { s.inA[q] = 0; s.freshE[q] = 0; s.rs_epoch[q] = 0; }
+  for (int64_t q = gtid; q < m.P; q += gsize) {
+    s.inA[q] = 0;
+    s.freshE[q] = 0;
+    s.rs_epoch[q] = 0;
+    s.visitE[q] = 0;
+  }
   if (gtid < kNumSizes && attempt == 0) s.sizes[gtid] = 0ull;
   if (gtid == 0) {
     s.pc[kPcSeed] = 0;
@@ -1831,8 +1849,22 @@ __global__ __launch_bounds__(BS) void k_step(DevMesh<T> m, DevState<T> s, StepAr
       break;
     }
     const int E = it + 1, par = it & 1;
-    for (int i = b; i < nA; i += G)
-      sweep_patch<T, NC, BS, ORG>(m, s, cur[i], a.niter, 1, a.tol, a.leps, smem, red, s_ndirty, E, 0);
+    // Visits. Each visiting block then marks, for a probe, every neighbour
+    // of its patch when the patch converged below the band (its own flags
+    // are final once its sweep ends); whether a marked patch is an
+    // unconverged member, which is not probed, is decided in the probe phase.
+    for (int i = b; i < nA; i += G) {
+      const int p = cur[i];
+      sweep_patch<T, NC, BS, ORG>(m, s, p, a.niter, 1, a.tol, a.leps, smem, red, s_ndirty, E, 0);
+      __syncthreads();  // bcon/bmin of p, written by thread 0
+      if (s.bcon[p] && s.bmin[p] < band_of(m, p, a.bw)) {
+        for (int k = m.padj_off[p] + tid; k < m.padj_off[p + 1]; k += BS) {
+          const int q = m.padj[k];
+          if (atomicMax(&s.freshE[q], E) < E) s.listF[atomicAdd(&s.pc[kPcFresh + par], 1)] = q;
+        }
+      }
+      if (tid == 0) s.visitE[p] = E;
+    }
     grid_barrier(s.bar, G);
     pcl.lap(clk, kTkVisit);
     // Every block has read this iteration's list size, so the counters the
@@ -1843,40 +1875,34 @@ __global__ __launch_bounds__(BS) void k_step(DevMesh<T> m, DevState<T> s, StepAr
       s.pc[kPcFresh + (1 - par)] = 0;
       s.pc[kPcNext + (1 - par)] = 0;
     }
-    // CopyOutBack and neighbour activation: a converged patch below the band
-    // marks every neighbour that is not an unconverged member of the list.
+    // CopyOutBack of the members, the probes and the next list in one
+    // phase. Probes read the members' vertices from DT_out while the copy
+    // runs. Next list: unconverged members that were not marked, marked
+    // unconverged members (kept without a probe), and each probed patch
+    // that stays unconverged, appended by the block that handled it. The
+    // list order is immaterial (patches of one iteration never read each
+    // other's outputs).
+    const int nF = ld_acquire(&s.pc[kPcFresh + par]);
     for (int i = b; i < nA; i += G) {
       const int p = cur[i];
       const int vs = m.vert_off[p], ve = m.vert_off[p + 1];
       for (int v = vs + tid; v < ve; v += BS) s.DT[v] = s.DT_out[v];
-      if (s.bcon[p] && s.bmin[p] < band_of(m, p, a.bw)) {
-        for (int k = m.padj_off[p] + tid; k < m.padj_off[p + 1]; k += BS) {
-          const int q = m.padj[k];
-          const bool active_unconv = (s.inA[q] == E) && !s.bcon[q];
-          if (!active_unconv && atomicMax(&s.freshE[q], E) < E)
-            s.listF[atomicAdd(&s.pc[kPcFresh + par], 1)] = q;
-        }
-      }
-    }
-    grid_barrier(s.bar, G);
-    pcl.lap(clk, kTkCopyback);
-    // Probes, and the next list in the same phase: unconverged members not
-    // probed this iteration (their flags are final since the visit phase),
-    // and each probed patch that stays unconverged, appended by the block
-    // that probed it. The list order is immaterial (patches of one
-    // iteration never read each other's outputs).
-    const int nF = ld_acquire(&s.pc[kPcFresh + par]);
-    if (gtid == 0) s.pc[kPcProbes] += nF;
-    for (int64_t i = gtid; i < (int64_t)nA; i += gsize) {
-      const int p = cur[i];
-      if (s.freshE[p] != E && !s.bcon[p]) {
+      if (tid == 0 && s.freshE[p] != E && !s.bcon[p]) {
         nxt[atomicAdd(&s.pc[kPcNext + par], 1)] = p;
         s.inA[p] = E + 1;
       }
     }
     for (int i = b; i < nF; i += G) {
       const int q = s.listF[i];
-      sweep_patch<T, NC, BS, ORG>(m, s, q, 1, 0, a.tol, a.leps, smem, red, s_ndirty, 0, E + 1);
+      if (s.visitE[q] == E && !s.bcon[q]) {  // unconverged member: not probed
+        if (tid == 0) {
+          nxt[atomicAdd(&s.pc[kPcNext + par], 1)] = q;
+          s.inA[q] = E + 1;
+        }
+        continue;
+      }
+      if (tid == 0) atomicAdd(&s.pc[kPcProbes], 1);
+      sweep_patch<T, NC, BS, ORG>(m, s, q, 1, 0, a.tol, a.leps, smem, red, s_ndirty, 0, E + 1, E);
       if (tid == 0 && !s.bcon[q]) {  // written by this thread in sweep_patch
         nxt[atomicAdd(&s.pc[kPcNext + par], 1)] = q;
         s.inA[q] = E + 1;
@@ -2064,6 +2090,12 @@ class EngineImpl final : public Engine {
     LS_HIP(dev_alloc(&d_rs_dirty_, (size_t)F_));
     LS_HIP(dev_alloc(&d_rs_epoch_, (size_t)P_));
     LS_HIP(hipMemset(d_rs_epoch_, 0, (size_t)P_ * sizeof(int32_t)));
+    LS_HIP(dev_alloc(&d_visitE_, (size_t)P_));
+    LS_HIP(hipMemset(d_visitE_, 0, (size_t)P_ * sizeof(int32_t)));
+    LS_HIP(dev_alloc(&d_vpatch_, (size_t)N_));
+    if (P_ > 0) k_vertex_patch<<<P_, kBlock, 0, stream_>>>(d_vert_off_, P_, d_vpatch_);
+    LS_HIP(hipGetLastError());
+    LS_HIP(hipStreamSynchronize(stream_));
     LS_HIP(dev_alloc(&d_sigma_, (size_t)F_ * (NC - 1)));
     LS_HIP(hipMemset(d_sigma_, 0, (size_t)F_ * (NC - 1) * sizeof(T)));
     LS_HIP(dev_alloc(&d_phi_, N_));
@@ -2695,6 +2727,7 @@ class EngineImpl final : public Engine {
     m.vert_off = d_vert_off_; m.ele_off = d_ele_off_; m.halo_off = d_halo_off_; m.halo_ids = d_halo_ids_;
     m.copy_local = d_copy_local_; m.vslot_off = d_vslot_off_; m.vslots = d_vslots_;
     m.coords = d_coords_; m.sigma = d_sigma_; m.padj_off = d_padj_off_; m.padj = d_padj_;
+    m.vpatch = d_vpatch_;
     m.hgt = d_hgt_;
     m.patch_dt = use_patch_prm_ ? d_patch_dt_ : nullptr;
     m.patch_bw = use_patch_prm_ ? d_patch_bw_ : nullptr;
@@ -2717,6 +2750,7 @@ class EngineImpl final : public Engine {
     s.rs_vval = d_rs_vval_;
     s.rs_dirty = d_rs_dirty_;
     s.rs_epoch = d_rs_epoch_;
+    s.visitE = d_visitE_;
     return s;
   }
   int32_t* d_nb_cur() const { return nb_sel_ ? d_lists_[3] : d_nb_; }
@@ -2730,7 +2764,8 @@ class EngineImpl final : public Engine {
                     d_lists_[3], d_inA_, d_freshE_, d_listF_, d_pc_, d_ticks_, d_bar_,
                     d_patch_dt_, d_patch_bw_, d_warm_, d_hgt_, d_mlp_prm_, d_mlp_m_, d_mlp_v_, d_mlp_grad_,
                     d_mlp_b_, d_mlp_w1_, d_mlp_w2_, d_mlp_w3_, d_ds_x_, d_ds_y_, d_pend_, d_xyz_,
-                    d_copy_verts_, d_copy_src_, d_rs_slot_, d_rs_vval_, d_rs_dirty_, d_rs_epoch_};
+                    d_copy_verts_, d_copy_src_, d_rs_slot_, d_rs_vval_, d_rs_dirty_, d_rs_epoch_, d_visitE_,
+                    d_vpatch_};
     if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
     if (graph_) (void)hipGraphDestroy(graph_);
     for (void* p : ptrs) if (p) (void)hipFree(p);
@@ -2763,6 +2798,8 @@ class EngineImpl final : public Engine {
   T *d_rs_slot_ = nullptr, *d_rs_vval_ = nullptr;  // probe states for resumed visits
   uint8_t* d_rs_dirty_ = nullptr;
   int32_t* d_rs_epoch_ = nullptr;
+  int32_t* d_visitE_ = nullptr;  // P, FIM iteration of the last visit (persistent step)
+  int32_t* d_vpatch_ = nullptr;  // N, patch of each renumbered vertex
   int32_t* d_copy_verts_ = nullptr;  // F x NC, renumbered corner ids
   int32_t* d_copy_src_ = nullptr;    // F, original element of each copy
   T* d_hgt_ = nullptr;  // tet copies: corner heights (F x 4)

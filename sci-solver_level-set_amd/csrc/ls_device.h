@@ -102,7 +102,9 @@ enum PcIndex : int {
   kPcResid = 12,   // warm start: max fixed-point residual over band vertices (float bits)
   kPcFallback = 13,// warm start: the residual check failed and the step re-ran plain
   kPcWarm = 14,    // warm start: patches that received warm values
-  kNumPc = 16
+  kPcWorkV = 15,   // [15],[16]: next list entry to visit, by iteration parity
+  kPcWorkP = 17,   // [17],[18]: next probe-list entry, by iteration parity
+  kNumPc = 20
 };
 
 // Patch-size sums (element copies, owned, halo) per launch kind.

@@ -600,6 +600,15 @@ __device__ __forceinline__ T block_min_n(T v, T* scratch) {
 // and the workgroup reduces it to the patch flag plus the patch min of the
 // written distances (run_reduction_bandwidth, redistance_kernels3d.h:57-97).
 //
+// The next work item of a persistent loop for the whole workgroup: thread 0
+// draws it from a device counter and shares it through LDS.
+__device__ __forceinline__ int next_work(int32_t* counter, int& s_item) {
+  __syncthreads();  // every thread has read the previous item
+  if (threadIdx.x == 0) s_item = atomicAdd(counter, 1);
+  __syncthreads();
+  return s_item;
+}
+
 // Copies n items into LDS with U independent loads in flight per thread.
 template <int BS, int U, typename V>
 __device__ __forceinline__ void copy_to_lds(V* dst, const V* src, int n) {
@@ -1770,6 +1779,7 @@ __global__ __launch_bounds__(BS) void k_step(DevMesh<T> m, DevState<T> s, StepAr
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ T red[BS / 64];
   __shared__ int s_ndirty[10];
+  __shared__ int s_work;
   const int G = gridDim.x, b = blockIdx.x, tid = threadIdx.x;
   const int64_t gtid = (int64_t)b * BS + tid, gsize = (int64_t)G * BS;
   const int nb_out = 1 - a.nb_in;
@@ -1796,6 +1806,7 @@ __global__ __launch_bounds__(BS) void k_step(DevMesh<T> m, DevState<T> s, StepAr
     s.pc[kPcSeed] = 0;
     s.pc[kPcFresh] = s.pc[kPcFresh + 1] = 0;
     s.pc[kPcNext] = s.pc[kPcNext + 1] = 0;
+    s.pc[kPcWorkV] = s.pc[kPcWorkV + 1] = s.pc[kPcWorkP] = s.pc[kPcWorkP + 1] = 0;
     s.pc[kPcNbVerts] = 0;
     if (attempt == 0) {
       s.pc[kPcIters] = s.pc[kPcVisits] = s.pc[kPcProbes] = 0;
@@ -1853,7 +1864,11 @@ __global__ __launch_bounds__(BS) void k_step(DevMesh<T> m, DevState<T> s, StepAr
     // of its patch when the patch converged below the band (its own flags
     // are final once its sweep ends); whether a marked patch is an
     // unconverged member, which is not probed, is decided in the probe phase.
-    for (int i = b; i < nA; i += G) {
+    // Patches are taken from a work counter, so a block that drew a cheap
+    // patch takes the next one.
+    for (;;) {
+      const int i = next_work(&s.pc[kPcWorkV + par], s_work);
+      if (i >= nA) break;
       const int p = cur[i];
       sweep_patch<T, NC, BS, ORG>(m, s, p, a.niter, 1, a.tol, a.leps, smem, red, s_ndirty, E, 0);
       __syncthreads();  // bcon/bmin of p, written by thread 0
@@ -1874,6 +1889,8 @@ __global__ __launch_bounds__(BS) void k_step(DevMesh<T> m, DevState<T> s, StepAr
       s.pc[kPcVisits] += nA;
       s.pc[kPcFresh + (1 - par)] = 0;
       s.pc[kPcNext + (1 - par)] = 0;
+      s.pc[kPcWorkV + (1 - par)] = 0;
+      s.pc[kPcWorkP + (1 - par)] = 0;
     }
     // CopyOutBack of the members, the probes and the next list in one
     // phase. Probes read the members' vertices from DT_out while the copy
@@ -1892,7 +1909,9 @@ __global__ __launch_bounds__(BS) void k_step(DevMesh<T> m, DevState<T> s, StepAr
         s.inA[p] = E + 1;
       }
     }
-    for (int i = b; i < nF; i += G) {
+    for (;;) {
+      const int i = next_work(&s.pc[kPcWorkP + par], s_work);
+      if (i >= nF) break;
       const int q = s.listF[i];
       if (s.visitE[q] == E && !s.bcon[q]) {  // unconverged member: not probed
         if (tid == 0) {

@@ -104,7 +104,9 @@ enum PcIndex : int {
   kPcWarm = 14,    // warm start: patches that received warm values
   kPcWorkV = 15,   // [15],[16]: next list entry to visit, by iteration parity
   kPcWorkP = 17,   // [17],[18]: next probe-list entry, by iteration parity
-  kNumPc = 20
+  kPcWorkS = 19,   // next patch to scan for seeds
+  kPcWorkA = 20,   // next narrowband patch to advect (per pass)
+  kNumPc = 24
 };
 
 // Patch-size sums (element copies, owned, halo) per launch kind.

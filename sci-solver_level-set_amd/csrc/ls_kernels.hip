@@ -1807,6 +1807,7 @@ __global__ __launch_bounds__(BS) void k_step(DevMesh<T> m, DevState<T> s, StepAr
     s.pc[kPcFresh] = s.pc[kPcFresh + 1] = 0;
     s.pc[kPcNext] = s.pc[kPcNext + 1] = 0;
     s.pc[kPcWorkV] = s.pc[kPcWorkV + 1] = s.pc[kPcWorkP] = s.pc[kPcWorkP + 1] = 0;
+    s.pc[kPcWorkS] = s.pc[kPcWorkA] = 0;
     s.pc[kPcNbVerts] = 0;
     if (attempt == 0) {
       s.pc[kPcIters] = s.pc[kPcVisits] = s.pc[kPcProbes] = 0;
@@ -1823,7 +1824,9 @@ __global__ __launch_bounds__(BS) void k_step(DevMesh<T> m, DevState<T> s, StepAr
 
   // Phase 1: seeds over every patch (first step) or the previous narrowband.
   const int nscan = a.first ? m.P : ld_acquire(&s.pc[kPcNb + a.nb_in]);
-  for (int i = b; i < nscan; i += G) {
+  for (;;) {
+    const int i = next_work(&s.pc[kPcWorkS], s_work);
+    if (i >= nscan) break;
     const int p = a.first ? i : s.nb[a.nb_in][i];
     if (seeds_patch<T, NC, BS>(m, s, p, smem) && tid == 0) {
       s.listA[atomicAdd(&s.pc[kPcSeed], 1)] = p;
@@ -1963,8 +1966,13 @@ __global__ __launch_bounds__(BS) void k_step(DevMesh<T> m, DevState<T> s, StepAr
   // Phase 4: advection over the new narrowband, then CopyOutBack_levelset3d.
   const int nnb = ld_acquire(&s.pc[kPcNb + nb_out]);
   for (int pass = 0; pass < a.inside_niter && nnb > 0; ++pass) {
-    for (int i = b; i < nnb; i += G) update_patch<T, NC, BS>(m, s, s.nb[nb_out][i], a.dt, a.mode, smem);
+    for (;;) {
+      const int i = next_work(&s.pc[kPcWorkA], s_work);
+      if (i >= nnb) break;
+      update_patch<T, NC, BS>(m, s, s.nb[nb_out][i], a.dt, a.mode, smem);
+    }
     grid_barrier(s.bar, G);
+    if (gtid == 0) s.pc[kPcWorkA] = 0;  // every block has left the loop above
     for (int i = b; i < nnb; i += G) {
       const int p = s.nb[nb_out][i];
       for (int v = m.vert_off[p] + tid; v < m.vert_off[p + 1]; v += BS) s.phi[v] = s.phi_out[v];

@@ -149,7 +149,7 @@ def roofline(tot, steps, kernels_mode):
         avg_ms = tot["sweep_ms"] / max(launches, 1)
         avg_bytes = tot["sweep_bytes"] / max(launches, 1)
     else:
-        kname = "k_step (persistent timestep: seeds, FIM, narrowband, advection)"
+        kname = "k_step + k_advect_dev + k_copy_phi_dev (one timestep: persistent seeds/FIM/narrowband, then advection)"
         launches = steps
         avg_ms = tot["step_ms"] / steps
         avg_bytes = tot["step_bytes"] / steps
@@ -435,7 +435,7 @@ def main():
         avg_ms = tot["sweep_ms"] / max(launches, 1)
         avg_bytes = tot["sweep_bytes"] / max(launches, 1)
     else:
-        kname = "k_step (persistent timestep: seeds, FIM, narrowband, advection)"
+        kname = "k_step + k_advect_dev + k_copy_phi_dev (one timestep: persistent seeds/FIM/narrowband, then advection)"
         launches = args.steps
         avg_ms = tot["step_ms"] / args.steps
         avg_bytes = tot["step_bytes"] / args.steps

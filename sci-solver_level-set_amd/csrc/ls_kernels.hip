@@ -1,6 +1,6 @@
 // gfx950 kernels for the level-set per-timestep hot path.
 //
-// One workgroup owns one patch at a time (1024 threads for FIM sweeps, 256
+// One workgroup owns one patch at a time (512 threads for FIM sweeps, 256
 // for the other phases): the patch's owned and halo vertex values, its
 // element-copy slots and its vertex slot lists are staged in LDS, so HBM sees
 // each patch's arrays once per visit and every halo vertex once per patch
@@ -15,7 +15,9 @@
 // Two execution paths give identical results: one launch per phase with the
 // FIM loop on the host (k_seeds, k_sweep, k_copyback, k_activate,
 // k_finalize, k_narrowband, k_advect), and one persistent cooperative launch
-// per timestep (k_step) that runs the FIM loop on the device.
+// per timestep (k_step) that runs seeds, the FIM loop and the narrowband on
+// the device, followed by the advection launches (k_advect_dev,
+// k_copy_phi_dev).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>

@@ -56,6 +56,21 @@ def test_tet_fp32_bit_exact(shape, n, block, oracle_mod):
                                       err_msg=f"phi step {k}")
 
 
+@pytest.mark.parametrize("execution", [lsb.LS_EXEC_PERSISTENT, lsb.LS_EXEC_GRAPH, lsb.LS_EXEC_KERNELS])
+def test_tet_fp32_bit_exact_many_patches(execution, oracle_mod):
+    # 52^3 Kuhn grid in 4x4x4 blocks: 2197 patches, more than 8 per CU on
+    # MI355X (256 CUs), so the persistent step runs two workgroups per CU and
+    # most FIM iterations have more active patches than workgroups.
+    case = cases.small3d(n=52, shape="torus", block=(4, 4, 4), steps=3)
+    print(f"patches {case.nparts}")
+    for k, gs, os_, gphi, ophi, gnb, onb in _lockstep(case, 3, oracle_mod, 32, execution=execution):
+        print(f"step {k}: iters {gs.fim_iters} visits {gs.patch_visits} probes {gs.probes} nb {gs.nb_count}")
+        assert gs.exec_mode == execution
+        np.testing.assert_array_equal(gnb, onb, err_msg=f"narrowband step {k}")
+        assert _same_schedule(gs, os_), f"schedule step {k}"
+        np.testing.assert_array_equal(gphi.astype(np.float32), ophi.astype(np.float32), err_msg=f"phi step {k}")
+
+
 @pytest.mark.parametrize("execution", [lsb.LS_EXEC_KERNELS, lsb.LS_EXEC_GRAPH])
 def test_tet_fp32_bit_exact_other_executions(execution, oracle_mod):
     case = cases.small3d(n=20, shape="torus", block=(8, 8, 4), steps=4)

@@ -246,9 +246,10 @@ class Solver:
         for m, (x, c) in enumerate(meshes):
             voff[m + 1] = voff[m] + x.shape[0]
             coff[m + 1] = coff[m] + c.shape[0]
+        if voff[-1] >= 2 ** 31:
+            raise LevelSetError("batch too large for int32 vertex ids")
         self._xyz = np.ascontiguousarray(np.concatenate([x for x, _ in meshes]), dtype=np.float64)
-        self._cells = np.ascontiguousarray(
-            np.concatenate([np.asarray(c, np.int64) + voff[m] for m, (_, c) in enumerate(meshes)]), dtype=np.int32)
+        self._cells = np.concatenate([np.asarray(c, np.int32) + np.int32(voff[m]) for m, (_, c) in enumerate(meshes)])
         self._voff, self._coff = voff, coff
         self.dim, self.N, self.E = dim, int(voff[-1]), int(coff[-1])
         self.n_meshes = len(meshes)

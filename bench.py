@@ -225,11 +225,31 @@ def mcf_workload(args, lsb, cases):
     case = cases.config3(steps=args.steps)
     print(f"mcf: {case.xyz.shape[0]} vertices, {case.cells.shape[0]} tets", file=sys.stderr, flush=True)
     names = {"graph": lsb.LS_EXEC_GRAPH, "persistent": lsb.LS_EXEC_PERSISTENT, "kernels": lsb.LS_EXEC_KERNELS}
+    prm = lsb.step_params(case.dt, case.bandwidth, case.niter, mode=lsb.LS_MODE_MCF)
+    # Patch size: block partitions timed over a few steps in the persistent
+    # mode (ms per step: the vertex-update count grows with the patch size).
+    patch_tune = {}
+    if args.mcf_block == "auto":
+        for blk in [(4, 4, 4), (8, 4, 4), (8, 8, 4), (8, 8, 8)]:
+            case.labels, case.nparts = cases.partition_blocks(192, blk, 3)
+            try:
+                gt = lsb.solver_for_case(case, execution=lsb.LS_EXEC_PERSISTENT)
+            except lsb.LevelSetError as e:
+                print(f"mcf block {blk} skipped: {e}", file=sys.stderr, flush=True)
+                continue
+            gt.step(prm)
+            patch_tune["x".join(map(str, blk))] = sum(gt.step(prm).ms_total for _ in range(3)) / 3
+            gt.close()
+            print(f"mcf block {blk}: {patch_tune['x'.join(map(str, blk))]:.2f} ms/step", file=sys.stderr, flush=True)
+        best = min(patch_tune, key=patch_tune.get)
+        block = tuple(int(x) for x in best.split("x"))
+    else:
+        block = tuple(int(x) for x in args.mcf_block.split("x"))
+    case.labels, case.nparts = cases.partition_blocks(192, block, 3)
     t0 = time.perf_counter()
     g = lsb.solver_for_case(case)
     setup_s = time.perf_counter() - t0
     info = g.layout_info()
-    prm = lsb.step_params(case.dt, case.bandwidth, case.niter, mode=lsb.LS_MODE_MCF)
     tune = None
     if args.exec == "auto":
         tune = autotune_exec(lsb, case, names, steps=3, g=g, prm=prm)
@@ -243,7 +263,8 @@ def mcf_workload(args, lsb, cases):
     g.close()
     out = {"workload": f"{case.name}: 192^3 Kuhn tets ({case.xyz.shape[0]} verts, {case.cells.shape[0]} tets), "
                        f"8 spheres, mean-curvature flow, dt = h^2/8, band 8h, {args.steps} timesteps",
-           "execution": args.exec, "execution_trial_ms_per_step": tune, "patches": info["n_patches"],
+           "execution": args.exec, "execution_trial_ms_per_step": tune, "patch_block": "x".join(map(str, block)),
+           "patch_trial_ms_per_step": patch_tune or None, "patches": info["n_patches"],
            "element_copies": info["n_copies"], "setup_s": setup_s}
     cpu = None
     if not args.no_cpu:
@@ -343,6 +364,8 @@ def main():
     ap.add_argument("--exec", default="auto", choices=["auto", "graph", "persistent", "kernels"],
                     help="auto (default): the fastest of graph (persistent step kernel replayed from a "
                          "hipGraph), persistent and kernels over a few trial steps")
+    ap.add_argument("--mcf-block", default="auto",
+                    help="MCF patch block LxWxH (e.g. 8x8x4), or auto: timed over 4x4x4..8x8x8")
     ap.add_argument("--sweep", default="auto", choices=["auto", "elements", "tasks"],
                     help="FIM sweep organisation (identical results)")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned warm-start run")

@@ -92,7 +92,8 @@ typedef struct ls_stats {
   /* Persistent kernel phase times (device wall clock, barrier to barrier):
    * reset+seeds, FIM visits with neighbour activation, (unused: 0),
    * copyback with the probes and the next list, (unused: 0), narrowband,
-   * advection. */
+   * (unused: 0; advection runs as its own launches after the kernel and is
+   * part of ms_total). */
   double ms_ph_seed, ms_ph_visit, ms_ph_copyback, ms_ph_probe, ms_ph_next, ms_ph_nb, ms_ph_advect;
   int64_t elem_updates;     /* element updates executed by FIM visits and probes */
   int64_t local_solves;     /* local solves executed (the rest were skipped exactly) */

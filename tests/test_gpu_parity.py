@@ -282,3 +282,50 @@ def test_step_errors_are_returned():
     st = s.step(lsb.step_params(case.dt, case.bandwidth, 5))
     assert st.nb_count > 0
     s.close()
+
+
+def _custom_case(n, jitter, phi_fn, block=(4, 4, 4), seed=7):
+    xyz, cells, h = cases.kuhn_grid(n)
+    if jitter > 0:
+        xyz = cases.jitter_interior(xyz, h, jitter, seed, 3)
+    cells = cases.reorient_tets(xyz, cells)
+    phi0 = phi_fn(xyz)
+    vel = cases.velocity_rotation_z(xyz, cells)
+    vmax = float(np.max(np.linalg.norm(vel, axis=1)))
+    labels, P = cases.partition_blocks(n, block, 3)
+    return cases.Case(f"3d_custom_{n}", 3, xyz, cells, phi0, vel, dt=0.5 * h / vmax, steps=3,
+                      bandwidth=6 * h, h=h, labels=labels, nparts=P, niter=5)
+
+
+def test_exact_zero_level_values_bit_exact(oracle_mod):
+    # Unjittered grid, plane x = 0.5 through a grid layer: every vertex of that
+    # layer has phi == 0 exactly and is a seed by the zero rule
+    # (redistance_kernels3d.h:623-770); equal values along edges make the
+    # quadratics' discriminants exactly zero.
+    case = _custom_case(17, 0.0, lambda x: x[:, 0] - 0.5)
+    assert np.count_nonzero(case.phi0 == 0.0) == 17 * 17
+    for k, gs, os_, gphi, ophi, gnb, onb in _lockstep(case, 3, oracle_mod, 32):
+        print(f"step {k}: iters {gs.fim_iters} nb {gs.nb_count} zeros {np.count_nonzero(gphi == 0)}")
+        assert gs.nb_count > 0
+        np.testing.assert_array_equal(gnb, onb, err_msg=f"narrowband step {k}")
+        assert _same_schedule(gs, os_), f"schedule step {k}"
+        np.testing.assert_array_equal(gphi.astype(np.float32), ophi.astype(np.float32), err_msg=f"phi step {k}")
+
+
+@pytest.mark.parametrize("engine,prec", [(lsb.LS_F32, 32), (lsb.LS_F64, 64)])
+def test_near_degenerate_tets_bit_exact(engine, prec, oracle_mod):
+    # Interior vertices moved by up to 0.45h per axis: slivers and very flat
+    # faces (smallest height / longest edge far below the undistorted grid's).
+    case = _custom_case(16, 0.45, lambda x: cases.sdf_sphere(x, r=0.3), seed=11)
+    a, b, c, d = (case.xyz[case.cells[:, i]] for i in range(4))
+    vol = np.abs(np.einsum("ij,ij->i", np.cross(b - a, c - a), d - a)) / 6
+    print(f"min volume / h^3 = {vol.min() / case.h ** 3:.2e}")
+    assert vol.min() < 1e-2 * case.h ** 3
+    for k, gs, os_, gphi, ophi, gnb, onb in _lockstep(case, 3, oracle_mod, prec, engine=engine):
+        print(f"step {k}: iters {gs.fim_iters} nb {gs.nb_count}")
+        np.testing.assert_array_equal(gnb, onb, err_msg=f"narrowband step {k}")
+        assert _same_schedule(gs, os_), f"schedule step {k}"
+        if prec == 32:
+            np.testing.assert_array_equal(gphi.astype(np.float32), ophi.astype(np.float32), err_msg=f"step {k}")
+        else:
+            np.testing.assert_array_equal(gphi, ophi, err_msg=f"step {k}")

@@ -602,6 +602,32 @@ __device__ __forceinline__ T block_min_n(T v, T* scratch) {
 // and the workgroup reduces it to the patch flag plus the patch min of the
 // written distances (run_reduction_bandwidth, redistance_kernels3d.h:57-97).
 //
+// Keys whose unsigned order is the order of the (non-NaN) floating-point
+// values: the sign bit is flipped for positive values and every bit for
+// negative ones, so an integer atomicMin over keys is a floating-point min.
+template <typename T>
+struct OrderedBits;
+template <>
+struct OrderedBits<float> {
+  using type = unsigned int;
+  __device__ static type of(float x) {
+    const type b = __float_as_uint(x);
+    return b ^ ((b >> 31) ? 0xffffffffu : 0x80000000u);
+  }
+  __device__ static float value(type k) { return __uint_as_float(k ^ ((k >> 31) ? 0x80000000u : 0xffffffffu)); }
+};
+template <>
+struct OrderedBits<double> {
+  using type = unsigned long long;
+  __device__ static type of(double x) {
+    const type b = (type)__double_as_longlong(x);
+    return b ^ ((b >> 63) ? ~0ull : (1ull << 63));
+  }
+  __device__ static double value(type k) {
+    return __longlong_as_double((long long)(k ^ ((k >> 63) ? (1ull << 63) : ~0ull)));
+  }
+};
+
 // The next work item of a persistent loop for the whole workgroup: thread 0
 // draws it from a device counter and shares it through LDS.
 __device__ __forceinline__ int next_work(int32_t* counter, int& s_item) {
@@ -658,8 +684,12 @@ __device__ void sweep_patch(const DevMesh<T>& m, DevState<T>& s, int p, int nite
   uint16_t* s_vsl = s_voff + nv + 1;
   uint16_t* s_work = s_vsl + nslots;            // compacted dirty elements
   uint8_t* s_dirty = (uint8_t*)(s_work + ne);
+  // Running minimum of each owned vertex's slots, as ordered bits (values are
+  // non-negative distances).
+  using Bits = typename OrderedBits<T>::type;
+  Bits* s_vmin = (Bits*)(((uintptr_t)(s_dirty + ne) + 7) & ~(uintptr_t)7);
   // Task mode: (element, corner) solves and their results.
-  uint32_t* s_task = (uint32_t*)(((uintptr_t)(s_dirty + ne) + 7) & ~(uintptr_t)7);
+  uint32_t* s_task = (uint32_t*)(((uintptr_t)(s_vmin + nv) + 7) & ~(uintptr_t)7);
   T* s_new = (T*)(s_task + NC * ne + (NC * ne) % 2);
   const int tid = threadIdx.x;
 
@@ -700,6 +730,7 @@ __device__ void sweep_patch(const DevMesh<T>& m, DevState<T>& s, int p, int nite
     }
   }
 
+  for (int i = tid; i < nv; i += BS) s_vmin[i] = OrderedBits<T>::of(s_vval[i]);
   int con_all = 1;
   T vmin = (T)kLargeNum;
   int n_upd = 0, n_solve = 0;
@@ -776,6 +807,8 @@ __device__ void sweep_patch(const DevMesh<T>& m, DevState<T>& s, int p, int nite
         if (y != s_slot[e * NC + j]) {
           s_slot[e * NC + j] = y;
           s_dirty[e] = 1;
+          const int lv = corner(m.copy_local[es + e], j);
+          if (lv < nv) atomicMin(&s_vmin[lv], OrderedBits<T>::of(y));
         }
       }
     } else {
@@ -840,22 +873,30 @@ __device__ void sweep_patch(const DevMesh<T>& m, DevState<T>& s, int p, int nite
         for (int q = 0; q < K; ++q) L[q] = gc[q];
         ElemSolve<T, NC>::solve_mask(L, s_dirty[e] >> 1, y, leps);
         int changed = 0;
+        const ushort4 cl = m.copy_local[es + e];
 #pragma unroll
         for (int i = 0; i < NC; ++i) {
-          if (y[i] != x[i]) { s_slot[e * NC + i] = y[i]; changed = 1; }
+          if (y[i] != x[i]) {
+            s_slot[e * NC + i] = y[i];
+            changed = 1;
+            const int lv = corner(cl, i);
+            if (lv < nv) atomicMin(&s_vmin[lv], OrderedBits<T>::of(y[i]));
+          }
         }
         s_dirty[e] = (uint8_t)changed;
       }
     }
     __syncthreads();
     const bool last = (it == niter - 1);
+    // Every slot of an owned vertex equals the vertex value after this pass
+    // and slots only decrease, so the minimum over the vertex's slots is the
+    // running minimum the solves above left in s_vmin; only a vertex whose
+    // minimum dropped lowers its slots.
     for (int v = tid; v < nv; v += BS) {
       const int o0 = s_voff[v], o1 = s_voff[v + 1];
       const T old = s_vval[v];
-      T mn = old;
-      if (o1 > o0) {
-        mn = s_slot[s_vsl[o0]];
-        for (int q = o0 + 1; q < o1; ++q) mn = d_min(s_slot[s_vsl[q]], mn);
+      const T mn = OrderedBits<T>::value(s_vmin[v]);
+      if (mn < old) {
         for (int q = o0; q < o1; ++q) {
           const int sl = s_vsl[q];
           if (mn < s_slot[sl]) { s_slot[sl] = mn; s_dirty[sl / NC] = 1; }
@@ -921,23 +962,6 @@ __device__ T narrowband_patch(const DevMesh<T>& m, DevState<T>& s, int p, T* red
   }
   return block_min_n<T, BS>(mn, red);
 }
-
-// Bit patterns of non-negative floating-point values order as the values,
-// so an integer atomicMin over them is a floating-point min.
-template <typename T>
-struct OrderedBits;
-template <>
-struct OrderedBits<float> {
-  using type = unsigned int;
-  __device__ static type of(float x) { return __float_as_uint(x); }
-  __device__ static float value(type b) { return __uint_as_float(b); }
-};
-template <>
-struct OrderedBits<double> {
-  using type = unsigned long long;
-  __device__ static type of(double x) { return (type)__double_as_longlong(x); }
-  __device__ static double value(type b) { return __longlong_as_double((long long)b); }
-};
 
 // Warm start of one patch of the previous narrowband: every non-seed vertex
 // whose warm value is below LARGENUM/2 starts the FIM there instead of at
@@ -2063,7 +2087,8 @@ class EngineImpl final : public Engine {
       const size_t ne = L.ele_off[p + 1] - L.ele_off[p], nv = L.vert_off[p + 1] - L.vert_off[p];
       const size_t nh = L.halo_off[p + 1] - L.halo_off[p];
       const size_t ns = L.vslot_off[L.vert_off[p + 1]] - L.vslot_off[L.vert_off[p]];
-      const size_t fim = NC * ne * sizeof(T) + (2 * nv + nh) * sizeof(T) + 2 * (nv + 1 + ns) + 2 * ne + ne;
+      const size_t fim = ((NC * ne * sizeof(T) + (2 * nv + nh) * sizeof(T) + 2 * (nv + 1 + ns) + 2 * ne + ne + 7) &
+                          ~size_t(7)) + nv * sizeof(T);  // + running vertex minima
       const size_t fim_tasks = ((fim + 7) & ~size_t(7)) + 4 * (NC * ne + (NC * ne) % 2) + NC * ne * sizeof(T);
       fim_task_lds = std::max(fim_task_lds, fim_tasks);
       const size_t adv = 2 * NC * ne * sizeof(T) + (nv + nh + ne) * sizeof(T);

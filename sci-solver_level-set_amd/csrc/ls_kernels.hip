@@ -731,6 +731,8 @@ __device__ void sweep_patch(const DevMesh<T>& m, DevState<T>& s, int p, int nite
   }
 
   for (int i = tid; i < nv; i += BS) s_vmin[i] = OrderedBits<T>::of(s_vval[i]);
+  int vsh = 0;  // log2 of the lanes per vertex in the vertex pass
+  while (vsh < 3 && (nv << (vsh + 1)) <= BS) ++vsh;
   int con_all = 1;
   T vmin = (T)kLargeNum;
   int n_upd = 0, n_solve = 0;
@@ -891,19 +893,22 @@ __device__ void sweep_patch(const DevMesh<T>& m, DevState<T>& s, int p, int nite
     // Every slot of an owned vertex equals the vertex value after this pass
     // and slots only decrease, so the minimum over the vertex's slots is the
     // running minimum the solves above left in s_vmin; only a vertex whose
-    // minimum dropped lowers its slots.
-    for (int v = tid; v < nv; v += BS) {
+    // minimum dropped lowers its slots. A vertex's slots are split over the
+    // 2^vsh adjacent lanes of one wave (as many as the workgroup holds, up to
+    // 8); all of them read s_vval[v] before the first lane rewrites it.
+    for (int w = tid; w < (nv << vsh); w += BS) {
+      const int v = w >> vsh, h = w & ((1 << vsh) - 1);
       const int o0 = s_voff[v], o1 = s_voff[v + 1];
       const T old = s_vval[v];
       const T mn = OrderedBits<T>::value(s_vmin[v]);
       if (mn < old) {
-        for (int q = o0; q < o1; ++q) {
+        for (int q = o0 + h; q < o1; q += 1 << vsh) {
           const int sl = s_vsl[q];
           if (mn < s_slot[sl]) { s_slot[sl] = mn; s_dirty[sl / NC] = 1; }
         }
-        s_vval[v] = mn;
+        if (h == 0) s_vval[v] = mn;
       }
-      if (last) {
+      if (last && h == 0) {
         T outv;
         int c;
         if (s.label[vs + v] != kSeedPoint) {

@@ -597,7 +597,8 @@ __device__ __forceinline__ T block_min_n(T v, T* scratch) {
 // values, then every owned vertex takes the min over its slots and writes it
 // back to them. An element whose four inputs are unchanged since its last
 // update would reproduce its values exactly, so each sweep only visits the
-// elements marked dirty, compacted into an LDS work list. The last sweep
+// elements marked dirty, compacted into an LDS work list; within one, a
+// corner whose only changed input is its own value is not solved again. The last sweep
 // decides convergence per vertex (old - new < tol; seeds count as converged),
 // and the workgroup reduces it to the patch flag plus the patch min of the
 // written distances (run_reduction_bandwidth, redistance_kernels3d.h:57-97).
@@ -651,6 +652,24 @@ __device__ __forceinline__ void copy_to_lds(V* dst, const V* src, int n) {
   }
 }
 
+// Marks corner i of element e changed (s_dirty is 4-byte aligned in LDS;
+// lanes updating other elements of the same word do so atomically too).
+__device__ __forceinline__ void mark_changed(uint8_t* s_dirty, int e, int i) {
+  atomicOr((unsigned int*)(s_dirty + (e & ~3)), 1u << (8 * (e & 3) + i));
+}
+
+// Corners whose solve can still lower their value, from the corners changed
+// since the element's last need-mask check: corner k's solve reads only the
+// other corners, so if none of them changed it returns what it returned (or
+// would have returned, when the lower-bound skip left it) at that check,
+// which is not below corner k's value then or now.
+template <int NC>
+__device__ __forceinline__ int unchanged_input_filter(int chg) {
+  constexpr int full = (1 << NC) - 1;
+  chg &= full;
+  return __popc(chg) >= 2 ? full : (full & ~chg);
+}
+
 // Sweep organisations (bit-identical, same per-corner arithmetic):
 // kOrgElements runs one thread per element with a corner to solve, the
 // elements listed by their number of solves; kOrgTasks lists the
@@ -683,13 +702,19 @@ __device__ void sweep_patch(const DevMesh<T>& m, DevState<T>& s, int p, int nite
   uint16_t* s_voff = (uint16_t*)(s_vval + nv);
   uint16_t* s_vsl = s_voff + nv + 1;
   uint16_t* s_work = s_vsl + nslots;            // compacted dirty elements
-  uint8_t* s_dirty = (uint8_t*)(s_work + ne);
-  // Running minimum of each owned vertex's slots, as ordered bits (values are
-  // non-negative distances).
+  // Aligned regions are carved by byte offsets from smem so that every
+  // access stays an LDS access. s_dirty[e]: bits 0..NC-1 mark the corner
+  // slots of element e changed since its last need-mask check; bits 4..7
+  // hold the need mask between that check and the solves (elements
+  // organisation). Whole words are updated with atomicOr.
+  const int o_dirty = ((int)((unsigned char*)(s_work + ne) - smem) + 3) & ~3;
+  uint8_t* s_dirty = smem + o_dirty;
+  // Running minimum of each owned vertex's slots, as ordered keys.
   using Bits = typename OrderedBits<T>::type;
-  Bits* s_vmin = (Bits*)(((uintptr_t)(s_dirty + ne) + 7) & ~(uintptr_t)7);
+  const int o_vmin = (o_dirty + ne + 7) & ~7;
+  Bits* s_vmin = (Bits*)(smem + o_vmin);
   // Task mode: (element, corner) solves and their results.
-  uint32_t* s_task = (uint32_t*)(((uintptr_t)(s_vmin + nv) + 7) & ~(uintptr_t)7);
+  uint32_t* s_task = (uint32_t*)(smem + ((o_vmin + nv * (int)sizeof(Bits) + 7) & ~7));
   T* s_new = (T*)(s_task + NC * ne + (NC * ne) % 2);
   const int tid = threadIdx.x;
 
@@ -725,7 +750,7 @@ __device__ void sweep_patch(const DevMesh<T>& m, DevState<T>& s, int p, int nite
       if (e < ne) {
 #pragma unroll
         for (int i = 0; i < NC; ++i) s_slot[e * NC + i] = s_vtx[corner(c[u], i)];
-        s_dirty[e] = 1;
+        s_dirty[e] = (uint8_t)((1 << NC) - 1);
       }
     }
   }
@@ -766,7 +791,7 @@ __device__ void sweep_patch(const DevMesh<T>& m, DevState<T>& s, int p, int nite
           const T* gc = m.coords + (int64_t)(es + e) * K;
 #pragma unroll
           for (int q = 0; q < K; ++q) L[q] = gc[q];
-          mask = ElemSolve<T, NC>::need_mask(L, copy_heights(m, es + e), x);
+          mask = ElemSolve<T, NC>::need_mask(L, copy_heights(m, es + e), x) & unchanged_input_filter<NC>(s_dirty[e]);
           s_dirty[e] = 0;
           ++n_upd;
         }
@@ -808,13 +833,13 @@ __device__ void sweep_patch(const DevMesh<T>& m, DevState<T>& s, int p, int nite
         const T y = s_new[t];
         if (y != s_slot[e * NC + j]) {
           s_slot[e * NC + j] = y;
-          s_dirty[e] = 1;
+          mark_changed(s_dirty, e, j);
           const int lv = corner(m.copy_local[es + e], j);
           if (lv < nv) atomicMin(&s_vmin[lv], OrderedBits<T>::of(y));
         }
       }
     } else {
-      // Need masks of the dirty elements (kept in s_dirty bits 1..NC), then
+      // Need masks of the dirty elements (kept in s_dirty bits 4..7), then
       // the elements with a corner to solve listed by their number of
       // solves, most first, so the lanes of a wave walk the same number of
       // corners and the chunks that wrap around to the first waves are the
@@ -832,8 +857,9 @@ __device__ void sweep_patch(const DevMesh<T>& m, DevState<T>& s, int p, int nite
           const T* gc = m.coords + (int64_t)(es + e) * K;
 #pragma unroll
           for (int q = 0; q < K; ++q) L[q] = gc[q];
-          const int mask = ElemSolve<T, NC>::need_mask(L, copy_heights(m, es + e), x);
-          s_dirty[e] = (uint8_t)(mask << 1);
+          const int mask =
+              ElemSolve<T, NC>::need_mask(L, copy_heights(m, es + e), x) & unchanged_input_filter<NC>(s_dirty[e]);
+          s_dirty[e] = (uint8_t)(mask << 4);
           c = __popc(mask);
           n_solve += c;
           ++n_upd;
@@ -852,7 +878,7 @@ __device__ void sweep_patch(const DevMesh<T>& m, DevState<T>& s, int p, int nite
       const int nd = start[1] + s_ndirty[2];
       for (int base = 0; base < ne; base += BS) {
         const int e = base + tid;
-        const int c = e < ne ? __popc(s_dirty[e] >> 1) : 0;
+        const int c = e < ne ? __popc(s_dirty[e] >> 4) : 0;
         const int lane = tid & 63;
 #pragma unroll
         for (int j = 1; j <= NC; ++j) {
@@ -873,14 +899,14 @@ __device__ void sweep_patch(const DevMesh<T>& m, DevState<T>& s, int p, int nite
         const T* gc = m.coords + (int64_t)(es + e) * K;
 #pragma unroll
         for (int q = 0; q < K; ++q) L[q] = gc[q];
-        ElemSolve<T, NC>::solve_mask(L, s_dirty[e] >> 1, y, leps);
+        ElemSolve<T, NC>::solve_mask(L, s_dirty[e] >> 4, y, leps);
         int changed = 0;
         const ushort4 cl = m.copy_local[es + e];
 #pragma unroll
         for (int i = 0; i < NC; ++i) {
           if (y[i] != x[i]) {
             s_slot[e * NC + i] = y[i];
-            changed = 1;
+            changed |= 1 << i;
             const int lv = corner(cl, i);
             if (lv < nv) atomicMin(&s_vmin[lv], OrderedBits<T>::of(y[i]));
           }
@@ -904,7 +930,10 @@ __device__ void sweep_patch(const DevMesh<T>& m, DevState<T>& s, int p, int nite
       if (mn < old) {
         for (int q = o0 + h; q < o1; q += 1 << vsh) {
           const int sl = s_vsl[q];
-          if (mn < s_slot[sl]) { s_slot[sl] = mn; s_dirty[sl / NC] = 1; }
+          if (mn < s_slot[sl]) {
+            s_slot[sl] = mn;
+            mark_changed(s_dirty, sl / NC, sl % NC);
+          }
         }
         if (h == 0) s_vval[v] = mn;
       }
@@ -2092,7 +2121,7 @@ class EngineImpl final : public Engine {
       const size_t ne = L.ele_off[p + 1] - L.ele_off[p], nv = L.vert_off[p + 1] - L.vert_off[p];
       const size_t nh = L.halo_off[p + 1] - L.halo_off[p];
       const size_t ns = L.vslot_off[L.vert_off[p + 1]] - L.vslot_off[L.vert_off[p]];
-      const size_t fim = ((NC * ne * sizeof(T) + (2 * nv + nh) * sizeof(T) + 2 * (nv + 1 + ns) + 2 * ne + ne + 7) &
+      const size_t fim = ((NC * ne * sizeof(T) + (2 * nv + nh) * sizeof(T) + 2 * (nv + 1 + ns) + 2 * ne + 3 + ne + 7) &
                           ~size_t(7)) + nv * sizeof(T);  // + running vertex minima
       const size_t fim_tasks = ((fim + 7) & ~size_t(7)) + 4 * (NC * ne + (NC * ne) % 2) + NC * ne * sizeof(T);
       fim_task_lds = std::max(fim_task_lds, fim_tasks);

@@ -84,6 +84,13 @@ struct DevState {
   uint8_t* rs_dirty;          // F
   int32_t* rs_epoch;          // P
   int32_t* visitE;            // P, FIM iteration in which the patch was last visited
+  // After a visit, ve_ok[copy] = 1 for each element copy that ended the
+  // visit clean (no corner changed since its last check: a fixed point of
+  // its solves) with every corner owned and holding the value the patch
+  // writes out. The next fresh load of the patch in the same FIM run
+  // (ve_tag[p] != 0) starts such elements clean instead of dirty.
+  uint8_t* ve_ok;             // F
+  int32_t* ve_tag;            // P
 };
 
 enum TickIndex : int { kTkSeed = 0, kTkVisit, kTkCopyback, kTkProbe, kTkNext, kTkNb, kTkAdvect, kNumTicks = 8 };

@@ -738,6 +738,7 @@ __device__ void sweep_patch(const DevMesh<T>& m, DevState<T>& s, int p, int nite
     copy_to_lds<BS, 4>(s_vval, s.rs_vval + vs, nv);
   }
   __syncthreads();
+  const bool saved = !resume && s.ve_tag[p] != 0;
   if (!resume)
   for (int base = tid; base < ne; base += 4 * BS) {
     ushort4 c[4];
@@ -750,12 +751,16 @@ __device__ void sweep_patch(const DevMesh<T>& m, DevState<T>& s, int p, int nite
       if (e < ne) {
 #pragma unroll
         for (int i = 0; i < NC; ++i) s_slot[e * NC + i] = s_vtx[corner(c[u], i)];
-        s_dirty[e] = (uint8_t)((1 << NC) - 1);
+        s_dirty[e] = (saved && s.ve_ok[es + e]) ? (uint8_t)0 : (uint8_t)((1 << NC) - 1);
       }
     }
   }
 
   for (int i = tid; i < nv; i += BS) s_vmin[i] = OrderedBits<T>::of(s_vval[i]);
+  // A visit records which elements end it clean (see DevState::ve_ok); the
+  // per-vertex flags of its last vertex pass live in the listing area.
+  const bool save_clean = write_out && nv <= 2 * ne;
+  uint8_t* s_same = (uint8_t*)s_work;
   int vsh = 0;  // log2 of the lanes per vertex in the vertex pass
   while (vsh < 3 && (nv << (vsh + 1)) <= BS) ++vsh;
   int con_all = 1;
@@ -947,6 +952,9 @@ __device__ void sweep_patch(const DevMesh<T>& m, DevState<T>& s, int p, int nite
           outv = s_vtx[v];
           c = 1;
         }
+        // The patch's slots of v all hold s_vval[v] now; a fresh load gives
+        // them outv.
+        if (save_clean) s_same[v] = (uint8_t)(d_min(old, mn) == outv);
         if (write_out) s.DT_out[vs + v] = outv;
         con_all &= c;
         vmin = d_min(vmin, outv);
@@ -954,6 +962,19 @@ __device__ void sweep_patch(const DevMesh<T>& m, DevState<T>& s, int p, int nite
     }
   }
   con_all = __syncthreads_and(con_all);
+  if (save_clean) {
+    for (int e = tid; e < ne; e += BS) {
+      const ushort4 cl = m.copy_local[es + e];
+      int ok = s_dirty[e] == 0;
+#pragma unroll
+      for (int i = 0; i < NC; ++i) {
+        const int lv = corner(cl, i);
+        ok = ok && lv < nv && s_same[lv];
+      }
+      s.ve_ok[es + e] = (uint8_t)ok;
+    }
+    if (tid == 0) s.ve_tag[p] = 1;
+  }
   if (epoch_out > 0 && !con_all) {
     for (int i = tid; i < NC * ne; i += BS) s.rs_slot[(int64_t)es * NC + i] = s_slot[i];
     for (int i = tid; i < ne; i += BS) s.rs_dirty[es + i] = s_dirty[i];
@@ -1884,6 +1905,7 @@ __global__ __launch_bounds__(BS, 4) void k_step(DevMesh<T> m, DevState<T> s, Ste
     s.freshE[q] = 0;
     s.rs_epoch[q] = 0;
     s.visitE[q] = 0;
+    s.ve_tag[q] = 0;
   }
   if (gtid < kNumSizes && attempt == 0) s.sizes[gtid] = 0ull;
   if (gtid == 0) {
@@ -2190,6 +2212,9 @@ class EngineImpl final : public Engine {
     LS_HIP(hipMemset(d_rs_epoch_, 0, (size_t)P_ * sizeof(int32_t)));
     LS_HIP(dev_alloc(&d_visitE_, (size_t)P_));
     LS_HIP(hipMemset(d_visitE_, 0, (size_t)P_ * sizeof(int32_t)));
+    LS_HIP(dev_alloc(&d_ve_ok_, (size_t)F_));
+    LS_HIP(dev_alloc(&d_ve_tag_, (size_t)P_));
+    LS_HIP(hipMemset(d_ve_tag_, 0, (size_t)P_ * sizeof(int32_t)));
     LS_HIP(dev_alloc(&d_vpatch_, (size_t)N_));
     if (P_ > 0) k_vertex_patch<<<P_, kBlock, 0, stream_>>>(d_vert_off_, P_, d_vpatch_);
     LS_HIP(hipGetLastError());
@@ -2745,6 +2770,7 @@ class EngineImpl final : public Engine {
     kev_used_ = 0;
     k_init<T><<<grid_for(N_), kBlock, 0, q>>>(N_, s);
     k_fill<<<grid_for(P_), kBlock, 0, q>>>(d_rs_epoch_, P_, 0);
+    k_fill<<<grid_for(P_), kBlock, 0, q>>>(d_ve_tag_, P_, 0);
     k_set_counters<<<1, 64, 0, q>>>(d_counters_, kNumCounters, 0);
     k_clear_sizes<<<1, 64, 0, q>>>(d_sizes_);
     const int32_t* scan = nnb_ > 0 ? d_nb_cur() : d_scan_all_;
@@ -2871,6 +2897,8 @@ class EngineImpl final : public Engine {
     s.rs_dirty = d_rs_dirty_;
     s.rs_epoch = d_rs_epoch_;
     s.visitE = d_visitE_;
+    s.ve_ok = d_ve_ok_;
+    s.ve_tag = d_ve_tag_;
     return s;
   }
   int32_t* d_nb_cur() const { return nb_sel_ ? d_lists_[3] : d_nb_; }
@@ -2885,7 +2913,7 @@ class EngineImpl final : public Engine {
                     d_patch_dt_, d_patch_bw_, d_warm_, d_hgt_, d_mlp_prm_, d_mlp_m_, d_mlp_v_, d_mlp_grad_,
                     d_mlp_b_, d_mlp_w1_, d_mlp_w2_, d_mlp_w3_, d_ds_x_, d_ds_y_, d_pend_, d_xyz_,
                     d_copy_verts_, d_copy_src_, d_rs_slot_, d_rs_vval_, d_rs_dirty_, d_rs_epoch_, d_visitE_,
-                    d_vpatch_};
+                    d_vpatch_, d_ve_ok_, d_ve_tag_};
     if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
     if (graph_) (void)hipGraphDestroy(graph_);
     for (void* p : ptrs) if (p) (void)hipFree(p);
@@ -2920,6 +2948,8 @@ class EngineImpl final : public Engine {
   uint8_t* d_rs_dirty_ = nullptr;
   int32_t* d_rs_epoch_ = nullptr;
   int32_t* d_visitE_ = nullptr;  // P, FIM iteration of the last visit (persistent step)
+  uint8_t* d_ve_ok_ = nullptr;   // F, element copies a visit left clean
+  int32_t* d_ve_tag_ = nullptr;  // P, the patch has ve_ok from this FIM run
   int32_t* d_vpatch_ = nullptr;  // N, patch of each renumbered vertex
   int32_t* d_copy_verts_ = nullptr;  // F x NC, renumbered corner ids
   int32_t* d_copy_src_ = nullptr;    // F, original element of each copy

@@ -1328,7 +1328,8 @@ __device__ void advect_patch(const DevMesh<T>& m, DevState<T>& s, int p, double 
 // 8 waves per patch: two patches share a CU (LDS permitting), so one
 // patch's barrier and FP64 latency stalls overlap the other's work. Measured
 // against 1024 threads on MI355X: 64^3 torus 3.72 vs 3.84 ms per step,
-// 192^3 MCF 15.1 vs 21.8 ms (per-phase path).
+// 192^3 MCF 15.1 vs 21.8 ms (per-phase path); with the persistent kernel,
+// torus 3.36 vs 3.38 ms and the 512-mesh batch 166 vs 257 ms per step.
 constexpr int kFimBlock = 512;
 constexpr int kFimIterCap = 100000;  // FIM iteration cap when none is requested
 
@@ -1878,7 +1879,8 @@ struct PhaseClock {
 
 // ORG selects the sweep organisation (kOrg*) and WARM compiles the
 // warm-start phases in: each variant carries only the code it runs, which
-// keeps the register demand of the 1024-thread workgroup within its 128 VGPRs.
+// keeps its register demand within the 128 VGPRs that let two 512-thread
+// workgroups share a CU.
 template <typename T, int NC, int BS, int ORG, bool WARM>
 __global__ __launch_bounds__(BS, 4) void k_step(DevMesh<T> m, DevState<T> s, StepArgs<T> a) {
   extern __shared__ __align__(16) unsigned char smem[];

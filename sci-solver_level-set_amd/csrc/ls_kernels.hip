@@ -638,6 +638,17 @@ __device__ __forceinline__ int next_work(int32_t* counter, int& s_item) {
   return s_item;
 }
 
+// Work item of round k of a phase with n items: with no more items than
+// workgroups every workgroup takes the item of its own index once (all G
+// indices are taken, so every item is), which keeps the counter's atomic
+// round trips off the phase's critical path; otherwise items are drawn from
+// the counter, so a workgroup done early takes the next one. Returns n when
+// the workgroup has no further item.
+__device__ __forceinline__ int phase_item(int k, int n, int32_t* counter, int& s_item) {
+  if (n <= (int)gridDim.x) return k == 0 && (int)blockIdx.x < n ? (int)blockIdx.x : n;
+  return next_work(counter, s_item);
+}
+
 // Copies n items into LDS with U independent loads in flight per thread.
 template <int BS, int U, typename V>
 __device__ __forceinline__ void copy_to_lds(V* dst, const V* src, int n) {
@@ -651,6 +662,14 @@ __device__ __forceinline__ void copy_to_lds(V* dst, const V* src, int n) {
       if (base + u * BS < n) dst[base + u * BS] = r[u];
   }
 }
+
+// The patch's convergence flag and min distance, as sweep_patch stores them in
+// bcon / bmin, for every thread of the workgroup.
+template <typename T>
+struct SweepResult {
+  int con;
+  T bmin;
+};
 
 // Marks corner i of element e changed (s_dirty is 4-byte aligned in LDS;
 // lanes updating other elements of the same word do so atomically too).
@@ -684,9 +703,9 @@ constexpr int kOrgElements = 0, kOrgTasks = 1;
 // members > 0: vertices of patches visited in FIM iteration `members` are
 // read from DT_out (their copy into DT runs concurrently), all others from DT.
 template <typename T, int NC, int BS, int ORG>
-__device__ void sweep_patch(const DevMesh<T>& m, DevState<T>& s, int p, int niter, int write_out, T tol,
-                            double leps, unsigned char* smem, T* red, int* s_ndirty /*[10]*/, int epoch_in,
-                            int epoch_out, int members = 0) {
+__device__ SweepResult<T> sweep_patch(const DevMesh<T>& m, DevState<T>& s, int p, int niter, int write_out,
+                                      T tol, double leps, unsigned char* smem, T* red, int* s_ndirty /*[10]*/,
+                                      int epoch_in, int epoch_out, int members = 0) {
   constexpr bool tasks = ORG == kOrgTasks;
   constexpr int K = ElemSolve<T, NC>::kCoords;
   __syncthreads();
@@ -999,6 +1018,7 @@ __device__ void sweep_patch(const DevMesh<T>& m, DevState<T>& s, int p, int nite
     atomicAdd(sz + 1, (unsigned long long)nv);
     atomicAdd(sz + 2, (unsigned long long)nh);
   }
+  return {con_all, bm};
 }
 
 // Narrowband test of one patch (kernel_compute_new_narrowband,
@@ -1932,8 +1952,8 @@ __global__ __launch_bounds__(BS, 4) void k_step(DevMesh<T> m, DevState<T> s, Ste
 
   // Phase 1: seeds over every patch (first step) or the previous narrowband.
   const int nscan = a.first ? m.P : ld_acquire(&s.pc[kPcNb + a.nb_in]);
-  for (;;) {
-    const int i = next_work(&s.pc[kPcWorkS], s_work);
+  for (int ks = 0;;) {
+    const int i = phase_item(ks++, nscan, &s.pc[kPcWorkS], s_work);
     if (i >= nscan) break;
     const int p = a.first ? i : s.nb[a.nb_in][i];
     if (seeds_patch<T, NC, BS>(m, s, p, smem) && tid == 0) {
@@ -1977,13 +1997,13 @@ __global__ __launch_bounds__(BS, 4) void k_step(DevMesh<T> m, DevState<T> s, Ste
     // unconverged member, which is not probed, is decided in the probe phase.
     // Patches are taken from a work counter, so a block that drew a cheap
     // patch takes the next one.
-    for (;;) {
-      const int i = next_work(&s.pc[kPcWorkV + par], s_work);
+    for (int kv = 0;;) {
+      const int i = phase_item(kv++, nA, &s.pc[kPcWorkV + par], s_work);
       if (i >= nA) break;
       const int p = cur[i];
-      sweep_patch<T, NC, BS, ORG>(m, s, p, a.niter, 1, a.tol, a.leps, smem, red, s_ndirty, E, 0);
-      __syncthreads();  // bcon/bmin of p, written by thread 0
-      if (s.bcon[p] && s.bmin[p] < band_of(m, p, a.bw)) {
+      const SweepResult<T> r =
+          sweep_patch<T, NC, BS, ORG>(m, s, p, a.niter, 1, a.tol, a.leps, smem, red, s_ndirty, E, 0);
+      if (r.con && r.bmin < band_of(m, p, a.bw)) {
         for (int k = m.padj_off[p] + tid; k < m.padj_off[p + 1]; k += BS) {
           const int q = m.padj[k];
           if (atomicMax(&s.freshE[q], E) < E) s.listF[atomicAdd(&s.pc[kPcFresh + par], 1)] = q;
@@ -2020,8 +2040,8 @@ __global__ __launch_bounds__(BS, 4) void k_step(DevMesh<T> m, DevState<T> s, Ste
         s.inA[p] = E + 1;
       }
     }
-    for (;;) {
-      const int i = next_work(&s.pc[kPcWorkP + par], s_work);
+    for (int kp = 0;;) {
+      const int i = phase_item(kp++, nF, &s.pc[kPcWorkP + par], s_work);
       if (i >= nF) break;
       const int q = s.listF[i];
       if (s.visitE[q] == E && !s.bcon[q]) {  // unconverged member: not probed
@@ -2032,8 +2052,9 @@ __global__ __launch_bounds__(BS, 4) void k_step(DevMesh<T> m, DevState<T> s, Ste
         continue;
       }
       if (tid == 0) atomicAdd(&s.pc[kPcProbes], 1);
-      sweep_patch<T, NC, BS, ORG>(m, s, q, 1, 0, a.tol, a.leps, smem, red, s_ndirty, 0, E + 1, E);
-      if (tid == 0 && !s.bcon[q]) {  // written by this thread in sweep_patch
+      const SweepResult<T> r =
+          sweep_patch<T, NC, BS, ORG>(m, s, q, 1, 0, a.tol, a.leps, smem, red, s_ndirty, 0, E + 1, E);
+      if (tid == 0 && !r.con) {
         nxt[atomicAdd(&s.pc[kPcNext + par], 1)] = q;
         s.inA[q] = E + 1;
       }

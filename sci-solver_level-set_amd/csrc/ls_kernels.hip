@@ -638,15 +638,16 @@ __device__ __forceinline__ int next_work(int32_t* counter, int& s_item) {
   return s_item;
 }
 
-// Work item of round k of a phase with n items: with no more items than
-// workgroups every workgroup takes the item of its own index once (all G
-// indices are taken, so every item is), which keeps the counter's atomic
-// round trips off the phase's critical path; otherwise items are drawn from
-// the counter, so a workgroup done early takes the next one. Returns n when
-// the workgroup has no further item.
-__device__ __forceinline__ int phase_item(int k, int n, int32_t* counter, int& s_item) {
-  if (n <= (int)gridDim.x) return k == 0 && (int)blockIdx.x < n ? (int)blockIdx.x : n;
-  return next_work(counter, s_item);
+// Work items of a phase with n items: with no more items than workgroups
+// every workgroup takes the item of its own index once (all G indices are
+// taken, so every item is), which keeps the counter's atomic round trips off
+// the phase's critical path; otherwise items are drawn from the counter, so a
+// workgroup done early takes the next one. n means no further item.
+__device__ __forceinline__ int first_item(int n, int32_t* counter, int& s_item) {
+  return n <= (int)gridDim.x ? ((int)blockIdx.x < n ? (int)blockIdx.x : n) : next_work(counter, s_item);
+}
+__device__ __forceinline__ int later_item(int n, int32_t* counter, int& s_item) {
+  return n <= (int)gridDim.x ? n : next_work(counter, s_item);
 }
 
 // Copies n items into LDS with U independent loads in flight per thread.
@@ -1860,7 +1861,9 @@ __device__ __forceinline__ int ld_acquire(const int32_t* p) {
 __device__ __forceinline__ void grid_barrier(unsigned int* bar, unsigned int nblocks) {
   __syncthreads();
   if (threadIdx.x == 0) {
-    __threadfence();  // release this workgroup's writes
+    // Release this workgroup's writes (L2 write-back; a full fence would
+    // also invalidate L2 here, which the acquire below repeats).
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     const unsigned int gen = __hip_atomic_load(&bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const unsigned int arrived =
         __hip_atomic_fetch_add(&bar[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1u;
@@ -1871,7 +1874,7 @@ __device__ __forceinline__ void grid_barrier(unsigned int* bar, unsigned int nbl
       while (__hip_atomic_load(&bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen)
         __builtin_amdgcn_s_sleep(1);
     }
-    __threadfence();  // acquire the other workgroups' writes
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // the other workgroups' writes
   }
   __syncthreads();
 }
@@ -1952,9 +1955,8 @@ __global__ __launch_bounds__(BS, 4) void k_step(DevMesh<T> m, DevState<T> s, Ste
 
   // Phase 1: seeds over every patch (first step) or the previous narrowband.
   const int nscan = a.first ? m.P : ld_acquire(&s.pc[kPcNb + a.nb_in]);
-  for (int ks = 0;;) {
-    const int i = phase_item(ks++, nscan, &s.pc[kPcWorkS], s_work);
-    if (i >= nscan) break;
+  for (int i = first_item(nscan, &s.pc[kPcWorkS], s_work); i < nscan;
+       i = later_item(nscan, &s.pc[kPcWorkS], s_work)) {
     const int p = a.first ? i : s.nb[a.nb_in][i];
     if (seeds_patch<T, NC, BS>(m, s, p, smem) && tid == 0) {
       s.listA[atomicAdd(&s.pc[kPcSeed], 1)] = p;
@@ -1997,9 +1999,8 @@ __global__ __launch_bounds__(BS, 4) void k_step(DevMesh<T> m, DevState<T> s, Ste
     // unconverged member, which is not probed, is decided in the probe phase.
     // Patches are taken from a work counter, so a block that drew a cheap
     // patch takes the next one.
-    for (int kv = 0;;) {
-      const int i = phase_item(kv++, nA, &s.pc[kPcWorkV + par], s_work);
-      if (i >= nA) break;
+    for (int i = first_item(nA, &s.pc[kPcWorkV + par], s_work); i < nA;
+         i = later_item(nA, &s.pc[kPcWorkV + par], s_work)) {
       const int p = cur[i];
       const SweepResult<T> r =
           sweep_patch<T, NC, BS, ORG>(m, s, p, a.niter, 1, a.tol, a.leps, smem, red, s_ndirty, E, 0);
@@ -2040,9 +2041,8 @@ __global__ __launch_bounds__(BS, 4) void k_step(DevMesh<T> m, DevState<T> s, Ste
         s.inA[p] = E + 1;
       }
     }
-    for (int kp = 0;;) {
-      const int i = phase_item(kp++, nF, &s.pc[kPcWorkP + par], s_work);
-      if (i >= nF) break;
+    for (int i = first_item(nF, &s.pc[kPcWorkP + par], s_work); i < nF;
+         i = later_item(nF, &s.pc[kPcWorkP + par], s_work)) {
       const int q = s.listF[i];
       if (s.visitE[q] == E && !s.bcon[q]) {  // unconverged member: not probed
         if (tid == 0) {

@@ -1901,11 +1901,13 @@ struct PhaseClock {
 };
 
 // ORG selects the sweep organisation (kOrg*) and WARM compiles the
-// warm-start phases in: each variant carries only the code it runs, which
-// keeps its register demand within the 128 VGPRs that let two 512-thread
-// workgroups share a CU.
-template <typename T, int NC, int BS, int ORG, bool WARM>
-__global__ __launch_bounds__(BS, 4) void k_step(DevMesh<T> m, DevState<T> s, StepArgs<T> a) {
+// warm-start phases in: each variant carries only the code it runs. WPE is
+// the waves per SIMD the variant is compiled for: 4 keeps the register
+// demand within the 128 VGPRs that let two 512-thread workgroups share a CU;
+// 2 (one workgroup per CU) leaves 256, so nothing is spilled to scratch,
+// whose dirty lines every grid barrier would otherwise write back.
+template <typename T, int NC, int BS, int ORG, bool WARM, int WPE>
+__global__ __launch_bounds__(BS, WPE) void k_step(DevMesh<T> m, DevState<T> s, StepArgs<T> a) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ T red[BS / 64];
   __shared__ int s_ndirty[10];
@@ -2553,6 +2555,7 @@ class EngineImpl final : public Engine {
     int cus = 0, per_cu = 0;
     LS_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device_));
     per_cu = 1 << 30;
+    step_one_per_cu_ = false;
     for (int warm = 0; warm < 2; ++warm) {
       int n = 0;
       LS_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, step_kernel(org_, warm), kFimBlock, step_lds_));
@@ -2560,20 +2563,35 @@ class EngineImpl final : public Engine {
     }
     // Two workgroups per CU only pay when patches outnumber CUs several times
     // over; with fewer, workgroups sharing a CU would take patches while other
-    // CUs idle (64^3 torus, 1024 patches: 5.0 ms per step at two per CU).
-    coop_grid_ = cus * (P_ >= 8 * cus ? per_cu : 1);
+    // CUs idle (64^3 torus, 1024 patches: 5.0 ms per step at two per CU). One
+    // per CU runs the variant compiled for it.
+    step_one_per_cu_ = !(P_ >= 8 * cus && per_cu > 1);
+    if (step_one_per_cu_) {
+      for (int warm = 0; warm < 2; ++warm) {
+        int n = 0;
+        LS_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, step_kernel(org_, warm), kFimBlock, step_lds_));
+        if (n < 1) per_cu = 0;
+      }
+    }
+    coop_grid_ = cus * (step_one_per_cu_ ? std::min(per_cu, 1) : per_cu);
     adv_grid_ = 4 * cus;
     if (coop_grid_ <= 0) { err = "persistent step kernel does not fit on a CU"; return LS_ERR_LIMIT; }
     if (graph_exec_) { LS_HIP(hipGraphExecDestroy(graph_exec_)); graph_exec_ = nullptr; }
     if (graph_) { LS_HIP(hipGraphDestroy(graph_)); graph_ = nullptr; }
     return LS_OK;
   }
-  static const void* step_kernel(int org, int warm) {
+  // The persistent kernel for the sweep organisation, warm-start support and
+  // workgroups per CU chosen by size_kernels (the tasks organisation has
+  // only the two-per-CU build).
+  const void* step_kernel(int org, int warm) const {
     if (org == kOrgTasks)
-      return warm ? (const void*)k_step<T, NC, kFimBlock, kOrgTasks, true>
-                  : (const void*)k_step<T, NC, kFimBlock, kOrgTasks, false>;
-    return warm ? (const void*)k_step<T, NC, kFimBlock, kOrgElements, true>
-                : (const void*)k_step<T, NC, kFimBlock, kOrgElements, false>;
+      return warm ? (const void*)k_step<T, NC, kFimBlock, kOrgTasks, true, 4>
+                  : (const void*)k_step<T, NC, kFimBlock, kOrgTasks, false, 4>;
+    if (step_one_per_cu_)
+      return warm ? (const void*)k_step<T, NC, kFimBlock, kOrgElements, true, 2>
+                  : (const void*)k_step<T, NC, kFimBlock, kOrgElements, false, 2>;
+    return warm ? (const void*)k_step<T, NC, kFimBlock, kOrgElements, true, 4>
+                : (const void*)k_step<T, NC, kFimBlock, kOrgElements, false, 4>;
   }
   int set_sweep_mode(int mode, std::string& err) override {
     const int old = sweep_mode_;
@@ -3026,6 +3044,7 @@ class EngineImpl final : public Engine {
   bool pc_synced_ = true;
   int32_t pc_nb_host_ = 0;
   int coop_grid_ = 0;
+  bool step_one_per_cu_ = false;  // persistent grid of one workgroup per CU
   size_t step_lds_ = 0;
   hipGraph_t graph_ = nullptr;
   hipGraphExec_t graph_exec_ = nullptr;

@@ -448,6 +448,7 @@ def main():
         tot["step_ms"] += st.ms_step_kernel
         tot["step_bytes"] += step_bytes(st)
         tot["sweep_tasks"] = st.sweep_tasks
+        tot["step_wg_per_cu"] = st.step_wg_per_cu
     wall = time.perf_counter() - t0
     g.close()
 
@@ -486,6 +487,7 @@ def main():
             "execution": args.exec,
             "execution_trial_ms_per_step": tune,
             "sweep": "tasks" if tot.get("sweep_tasks") else "elements",
+            "step_wg_per_cu": tot.get("step_wg_per_cu", 0),
             "patches": info["n_patches"],
             "element_copies": info["n_copies"],
             "fim_iters_per_step": tot["fim_iters"] / args.steps,

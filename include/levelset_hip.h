@@ -101,7 +101,8 @@ typedef struct ls_stats {
   int32_t warm_patches;     /* warm start: patches that received warm values */
   int32_t warm_fallback;    /* warm start: 1 when the residual check failed and the step re-ran plain */
   int32_t sweep_tasks;      /* 1: the FIM sweeps ran task-compacted */
-  int32_t reserved1;
+  int32_t step_wg_per_cu;   /* persistent step: workgroups per CU of its grid (the kernel build
+                               compiled for it); 0 when the step ran one launch per phase */
 } ls_stats;
 
 typedef struct ls_layout_info {

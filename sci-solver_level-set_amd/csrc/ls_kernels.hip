@@ -2574,6 +2574,7 @@ class EngineImpl final : public Engine {
       }
     }
     coop_grid_ = cus * (step_one_per_cu_ ? std::min(per_cu, 1) : per_cu);
+    cus_ = cus;
     adv_grid_ = 4 * cus;
     if (coop_grid_ <= 0) { err = "persistent step kernel does not fit on a CU"; return LS_ERR_LIMIT; }
     if (graph_exec_) { LS_HIP(hipGraphExecDestroy(graph_exec_)); graph_exec_ = nullptr; }
@@ -2677,6 +2678,7 @@ class EngineImpl final : public Engine {
     std::memset(&st, 0, sizeof(st));
     st.exec_mode = exec_;
     st.sweep_tasks = org_ == kOrgTasks ? 1 : 0;
+    st.step_wg_per_cu = step_one_per_cu_ ? 1 : coop_grid_ / std::max(1, cus_);
     DevMesh<T> m = mesh();
     DevState<T> s = state();
     StepArgs<T> a;
@@ -2800,6 +2802,7 @@ class EngineImpl final : public Engine {
     std::memset(&st, 0, sizeof(st));
     st.exec_mode = LS_EXEC_KERNELS;
     st.sweep_tasks = org_ == kOrgTasks ? 1 : 0;
+    st.step_wg_per_cu = 0;
     pc_synced_ = false;
     const DevMesh<T> m = mesh();
     const DevState<T> s = state();
@@ -3045,6 +3048,7 @@ class EngineImpl final : public Engine {
   int32_t pc_nb_host_ = 0;
   int coop_grid_ = 0;
   bool step_one_per_cu_ = false;  // persistent grid of one workgroup per CU
+  int cus_ = 1;                   // compute units of the device
   size_t step_lds_ = 0;
   hipGraph_t graph_ = nullptr;
   hipGraphExec_t graph_exec_ = nullptr;

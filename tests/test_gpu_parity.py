@@ -66,6 +66,8 @@ def test_tet_fp32_bit_exact_many_patches(execution, oracle_mod):
     for k, gs, os_, gphi, ophi, gnb, onb in _lockstep(case, 3, oracle_mod, 32, execution=execution):
         print(f"step {k}: iters {gs.fim_iters} visits {gs.patch_visits} probes {gs.probes} nb {gs.nb_count}")
         assert gs.exec_mode == execution
+        # the two-workgroups-per-CU build of the persistent kernel
+        assert gs.step_wg_per_cu == (0 if execution == lsb.LS_EXEC_KERNELS else 2)
         np.testing.assert_array_equal(gnb, onb, err_msg=f"narrowband step {k}")
         assert _same_schedule(gs, os_), f"schedule step {k}"
         np.testing.assert_array_equal(gphi.astype(np.float32), ophi.astype(np.float32), err_msg=f"phi step {k}")
@@ -76,6 +78,8 @@ def test_tet_fp32_bit_exact_other_executions(execution, oracle_mod):
     case = cases.small3d(n=20, shape="torus", block=(8, 8, 4), steps=4)
     for k, gs, os_, gphi, ophi, gnb, onb in _lockstep(case, 4, oracle_mod, 32, execution=execution):
         assert gs.exec_mode == execution
+        # few patches: the one-workgroup-per-CU build
+        assert gs.step_wg_per_cu == (0 if execution == lsb.LS_EXEC_KERNELS else 1)
         np.testing.assert_array_equal(gnb, onb, err_msg=f"narrowband step {k}")
         assert _same_schedule(gs, os_), f"schedule step {k}"
         np.testing.assert_array_equal(gphi.astype(np.float32), ophi.astype(np.float32))

@@ -119,7 +119,8 @@ def timed_run(g, prm, steps):
     """Times `steps` timesteps; sums the per-step statistics."""
     tot = {"verts": 0, "sweep_ms": 0.0, "sweep_launches": 0, "sweep_bytes": 0, "adv_ms": 0.0,
            "adv_bytes": 0, "adv_launches": 0, "ms_fim": 0.0, "ms_total": 0.0, "fim_iters": 0,
-           "visits": 0, "probes": 0, "step_ms": 0.0, "step_bytes": 0, "exec_modes": set()}
+           "visits": 0, "probes": 0, "step_ms": 0.0, "step_bytes": 0, "exec_modes": set(),
+           "step_wg_per_cu": set()}
     t0 = time.perf_counter()
     for _ in range(steps):
         st = g.step(prm)
@@ -138,6 +139,7 @@ def timed_run(g, prm, steps):
         tot["step_ms"] += st.ms_step_kernel
         tot["step_bytes"] += step_bytes(st)
         tot["exec_modes"].add(st.exec_mode)
+        tot["step_wg_per_cu"].add(st.step_wg_per_cu)
     tot["wall"] = time.perf_counter() - t0
     return tot
 
@@ -404,7 +406,8 @@ def main():
                "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
                "config": dict(cfg, mode="plain_fim", gpu_event_ms_per_step=tot["ms_total"] / args.steps,
                               fim_iters_per_step=tot["fim_iters"] / args.steps,
-                              exec_modes_used=sorted(tot["exec_modes"])),
+                              exec_modes_used=sorted(tot["exec_modes"]),
+                              step_wg_per_cu=sorted(tot["step_wg_per_cu"])),
                "roofline": roofline(tot, args.steps, kmode)}
         if cpu:
             out["cpu_baseline"], out["parity"] = cpu

@@ -150,6 +150,22 @@ def test_tri_fp64_bit_exact_config1(oracle_mod):
         np.testing.assert_array_equal(gphi, ophi, err_msg=f"phi step {k}")
 
 
+def test_fp64_and_tri_two_per_cu_builds_bit_exact(oracle_mod):
+    # Meshes with at least eight patches per CU (256 CUs on MI355X) run the
+    # two-workgroups-per-CU build of the persistent kernel: cover it for the
+    # FP64 tet engine and the triangle engine, against the FP64 oracle.
+    import dataclasses
+    tri = cases.config1(n=200)
+    tri = dataclasses.replace(tri, labels=cases.partition_rcb(tri.xyz, 2500), nparts=2500)
+    for case, steps in ((cases.small3d(n=52, shape="torus", block=(4, 4, 4), steps=2), 2), (tri, 3)):
+        for k, gs, os_, gphi, ophi, gnb, onb in _lockstep(case, steps, oracle_mod, 64, engine=lsb.LS_F64):
+            print(f"{case.name} P={case.nparts} step {k}: iters {gs.fim_iters} wg/cu {gs.step_wg_per_cu}")
+            assert gs.step_wg_per_cu == 2
+            np.testing.assert_array_equal(gnb, onb, err_msg=f"{case.name} narrowband step {k}")
+            assert _same_schedule(gs, os_), f"{case.name} schedule step {k}"
+            np.testing.assert_array_equal(gphi, ophi, err_msg=f"{case.name} phi step {k}")
+
+
 def _golden(name):
     import os
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", f"{name}.npz")

@@ -167,9 +167,13 @@ def batch_workload(args, lsb, cases):
     subset. CPU baseline and parity on a sample of the meshes."""
     b = cases.config4(steps=args.steps)
     print(f"batch: {len(b.meshes)} meshes, {b.n_verts} vertices, {b.n_cells} tets", file=sys.stderr, flush=True)
-    sub = cases.Batch(b.name + "_tune", 3, b.meshes[:64], patch_verts=b.patch_verts)
-    cand, ex, ms, table = lsb.autotune_batch(sub, [64, 128, 256, 512], trial_steps=2,
-                                             log=lambda m: print(m, file=sys.stderr, flush=True))
+    if args.batch_patch != "auto" and args.exec != "auto":
+        names = {"graph": lsb.LS_EXEC_GRAPH, "persistent": lsb.LS_EXEC_PERSISTENT, "kernels": lsb.LS_EXEC_KERNELS}
+        cand, ex, table = int(args.batch_patch), names[args.exec], []
+    else:
+        sub = cases.Batch(b.name + "_tune", 3, b.meshes[:64], patch_verts=b.patch_verts)
+        cand, ex, ms, table = lsb.autotune_batch(sub, [64, 128, 256, 512], trial_steps=2,
+                                                 log=lambda m: print(m, file=sys.stderr, flush=True))
     t0 = time.perf_counter()
     g = lsb.solver_for_batch(b, execution=ex, patch_verts=cand)
     setup_s = time.perf_counter() - t0
@@ -184,7 +188,7 @@ def batch_workload(args, lsb, cases):
                        f"ellipsoids, affine velocity, CFL 0.5, band 6h, {args.steps} timesteps",
            "patch_verts": cand, "execution": {lsb.LS_EXEC_GRAPH: "graph", lsb.LS_EXEC_PERSISTENT: "persistent",
                                               lsb.LS_EXEC_KERNELS: "kernels"}[ex],
-           "autotune_ms_per_step": {f"{c}/{e}": m for c, e, m in table},
+           "autotune_ms_per_step": {f"{c}/{e}": m for c, e, m in table} or None,
            "patches": info["n_patches"], "element_copies": info["n_copies"],
            "setup_s": setup_s}
     cpu = None
@@ -366,6 +370,8 @@ def main():
     ap.add_argument("--exec", default="auto", choices=["auto", "graph", "persistent", "kernels"],
                     help="auto (default): the fastest of graph (persistent step kernel replayed from a "
                          "hipGraph), persistent and kernels over a few trial steps")
+    ap.add_argument("--batch-patch", default="auto",
+                    help="batch patch size in vertices (with --exec other than auto: no autotuning), or auto")
     ap.add_argument("--mcf-block", default="auto",
                     help="MCF patch block LxWxH (e.g. 8x8x4), or auto: timed over 4x4x4..8x8x8")
     ap.add_argument("--sweep", default="auto", choices=["auto", "elements", "tasks"],
@@ -409,6 +415,13 @@ def main():
                               exec_modes_used=sorted(tot["exec_modes"]),
                               step_wg_per_cu=sorted(tot["step_wg_per_cu"])),
                "roofline": roofline(tot, args.steps, kmode)}
+        if not kmode:
+            prof_name = "k_step_batch_512" if args.workload == "batch" else "k_step_mcf_spheres_192"
+            traffic = profiled_traffic(prof_name)
+            if traffic is not None:
+                out["roofline"]["traffic"] = traffic
+                out["roofline"]["traffic_source"] = (f"profiles/{prof_name}.md (rocprofv3 FETCH_SIZE + WRITE_SIZE, "
+                                                     "copy-calibrated)")
         if cpu:
             out["cpu_baseline"], out["parity"] = cpu
         print(json.dumps(out))

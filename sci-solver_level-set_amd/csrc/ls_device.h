@@ -1,5 +1,6 @@
 // Device-side data views and kernel launchers shared by the C-ABI layer.
 #pragma once
+#include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -26,7 +27,7 @@ struct DevMesh {
   const uint16_t* vslots;      // patch-local slot ids per owned vertex
   const T* coords;             // F x (6 | 3), local coordinates
   const T* sigma;              // F x (3 | 2), velocity in the local frame
-  const T* hgt;                // F x 4 corner heights over the opposite face (tets), or null
+  const __half* hgt;           // F x 4 corner heights over the opposite face, rounded toward zero (tets), or null
   const int32_t* padj_off;     // P+1
   const int32_t* padj;
   const int32_t* vpatch;       // N, patch of each renumbered vertex

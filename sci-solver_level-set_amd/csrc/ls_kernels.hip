@@ -277,9 +277,9 @@ struct ElemSolve<T, 4> {
   }
   // Bit k set: corner k must be solved (its value is not below the exact
   // lower bound min(other values) + height of corner k over its face).
-  // `hs` holds the copy's heights rounded toward zero (a smaller height only
-  // solves more corners, so the skip stays exact).
-  __device__ static int need_mask(const T* Lt, const T* hs, const T* v) {
+  // `hs` holds the copy's heights in FP16 rounded toward zero (a smaller
+  // height only solves more corners, so the skip stays exact).
+  __device__ static int need_mask(const T* Lt, const __half* hs, const T* v) {
     double L[6];
 #pragma unroll
     for (int k = 0; k < 6; ++k) L[k] = (double)Lt[k];
@@ -292,7 +292,7 @@ struct ElemSolve<T, 4> {
 #pragma unroll
       for (int j = 0; j < 4; ++j)
         if (j != k) { mo = d_min(mo, Tv[j]); Mo = d_max(Mo, Tv[j]); }
-      if (must_solve(Tv[k], mo, Mo, (double)hs[k], scale)) mask |= 1 << k;
+      if (must_solve(Tv[k], mo, Mo, (double)__half2float(hs[k]), scale)) mask |= 1 << k;
     }
     return mask;
   }
@@ -342,7 +342,7 @@ struct ElemSolve<T, 4> {
 template <typename T>
 struct ElemSolve<T, 3> {
   static constexpr int kCoords = 3;
-  __device__ static int need_mask(const T* ct, const T*, const T* v) {
+  __device__ static int need_mask(const T* ct, const __half*, const T* v) {
     const double c[3] = {(double)ct[0], (double)ct[1], (double)ct[2]};
     const double LenAB = d_abs(c[0]);
     const double LenBC = d_sqrt(c[2] * c[2] + (c[1] - c[0]) * (c[1] - c[0]));
@@ -390,7 +390,7 @@ struct ElemSolve<T, 3> {
 
 // Stored corner heights of a tet copy (null for triangles).
 template <typename T>
-__device__ __forceinline__ const T* copy_heights(const DevMesh<T>& m, int64_t c) {
+__device__ __forceinline__ const __half* copy_heights(const DevMesh<T>& m, int64_t c) {
   return m.hgt ? m.hgt + c * 4 : nullptr;
 }
 
@@ -481,7 +481,7 @@ __global__ void k_project_velocity(const int32_t* cv, const double* xyz, const i
 // Corner heights of every tet copy, computed once at upload with the same
 // code the sweeps used to evaluate per visit; FP32 storage rounds toward zero.
 template <typename T>
-__global__ void k_heights(const T* coords, int64_t F, T* hgt) {
+__global__ void k_heights(const T* coords, int64_t F, __half* hgt) {
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= F) return;
   double L[6], h[4];
@@ -489,7 +489,7 @@ __global__ void k_heights(const T* coords, int64_t F, T* hgt) {
   for (int k = 0; k < 6; ++k) L[k] = (double)coords[c * 6 + k];
   ElemSolve<T, 4>::heights(L, h);
 #pragma unroll
-  for (int k = 0; k < 4; ++k) hgt[c * 4 + k] = sizeof(T) == 4 ? (T)__double2float_rz(h[k]) : (T)h[k];
+  for (int k = 0; k < 4; ++k) hgt[c * 4 + k] = __float2half_rz(__double2float_rz(h[k]));
 }
 
 template <typename T>
@@ -2997,7 +2997,7 @@ class EngineImpl final : public Engine {
   int32_t* d_vpatch_ = nullptr;  // N, patch of each renumbered vertex
   int32_t* d_copy_verts_ = nullptr;  // F x NC, renumbered corner ids
   int32_t* d_copy_src_ = nullptr;    // F, original element of each copy
-  T* d_hgt_ = nullptr;  // tet copies: corner heights (F x 4)
+  __half* d_hgt_ = nullptr;  // tet copies: corner heights (F x 4), FP16 rounded toward zero
   T* d_warm_ = nullptr;  // learned warm-start distances (N), written by the predictor
   float *d_mlp_prm_ = nullptr, *d_mlp_m_ = nullptr, *d_mlp_v_ = nullptr, *d_mlp_grad_ = nullptr, *d_mlp_b_ = nullptr;
   uint16_t *d_mlp_w1_ = nullptr, *d_mlp_w2_ = nullptr, *d_mlp_w3_ = nullptr;

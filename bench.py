@@ -56,6 +56,23 @@ def adv_bytes(st):
     return 28 * st.adv_copies + 16 * (st.adv_verts + st.adv_halo) + 4 * st.adv_verts
 
 
+def fp64_error(gphi, ophi, amb, band):
+    """L-inf of the GPU's phi against the FP64 oracle's over finite values,
+    without and with the vertices whose advection update hinges on a
+    threshold decision inside the FP32-storage window (counted), and over the
+    vertices inside the band (|phi| <= band; beyond it a patch's FIM stops
+    when a sweep changes its values by less than the tolerance, a test that
+    FP32 and FP64 storage can decide a sweep apart)."""
+    fin = (np.abs(ophi) < 1e8) & (np.abs(gphi) < 1e8)
+    d = np.abs(gphi - ophi)
+    ok = fin & ~amb
+    inb = ok & (np.abs(ophi) <= band)
+    return {"linf_vs_fp64": float(np.max(d[ok])) if ok.any() else 0.0,
+            "linf_vs_fp64_in_band": float(np.max(d[inb])) if inb.any() else 0.0,
+            "linf_vs_fp64_incl_threshold_ambiguous": float(np.max(d[fin])) if fin.any() else 0.0,
+            "threshold_ambiguous_verts": int(np.count_nonzero(fin & amb))}
+
+
 def lockstep(case, oracle_mod, lsb, precision, steps):
     """FP32 oracle: runs beside the GPU from phi0 (bit-exact check, timed as the
     CPU baseline). FP64 oracle: restarted every step from the GPU's phi and
@@ -68,7 +85,7 @@ def lockstep(case, oracle_mod, lsb, precision, steps):
     o.set_velocity(case.vel)
     gp = lsb.step_params(case.dt, case.bandwidth, case.niter)
     op = oracle_mod.make_params(case.dt, case.bandwidth, case.niter)
-    res = {"steps": 0, "linf": 0.0, "linf_all": 0.0, "ambiguous": 0, "nb_diff": 0, "bit_mismatch": 0,
+    res = {"steps": 0, "linf": 0.0, "linf_band": 0.0, "linf_all": 0.0, "ambiguous": 0, "nb_diff": 0, "bit_mismatch": 0,
            "cpu_s": 0.0, "cpu_verts": 0}
     gphi, gnb = case.phi0, None
     for _ in range(steps):
@@ -92,6 +109,8 @@ def lockstep(case, oracle_mod, lsb, precision, steps):
             res["ambiguous"] = max(res["ambiguous"], int(amb.sum()))
             ok = fin & ~amb
             res["linf"] = max(res["linf"], float(np.max(d[ok])) if ok.any() else 0.0)
+            inb = ok & (np.abs(ophi) <= case.bandwidth)
+            res["linf_band"] = max(res["linf_band"], float(np.max(d[inb])) if inb.any() else 0.0)
         res["bit_mismatch"] += int(np.count_nonzero(gphi.astype(np.float32) != ophi.astype(np.float32)))
         res["steps"] += 1
         if ost.nb_count == 0:
@@ -217,12 +236,30 @@ def batch_workload(args, lsb, cases):
                 nbd += abs(int(counts[k]) - ost.nb_count)
         for o in orc:
             o.close()
+        # One more step of every sample mesh checked against the FP64 oracle
+        # from the GPU's state (the oracle scans every patch for seeds).
+        o64s = []
+        for m, phi in zip(sample.meshes, gs.split(gs.phi())):
+            nparts = -(-m.xyz.shape[0] // cand)
+            o = O.Oracle(3, m.xyz, m.cells, cases.partition_rcb(m.xyz, nparts), nparts, 64)
+            o.set_velocity(m.vel)
+            o.set_phi(phi)
+            o64s.append(o)
+        gs.step(prm)
+        l64 = {"linf_vs_fp64": 0.0, "linf_vs_fp64_in_band": 0.0, "linf_vs_fp64_incl_threshold_ambiguous": 0.0,
+               "threshold_ambiguous_verts": 0}
+        for k, (o, phi) in enumerate(zip(o64s, gs.split(gs.phi()))):
+            o.step(O.make_params(sample.meshes[k].dt, sample.meshes[k].bandwidth, 5))
+            e = fp64_error(phi, o.phi(), o.ambiguous(), sample.meshes[k].bandwidth)
+            l64 = {key: max(l64[key], e[key]) if key != "threshold_ambiguous_verts" else l64[key] + e[key]
+                   for key in l64}
+            o.close()
         gs.close()
         cpu = ({"value": cpu_v / cpu_s, "unit": "vertex-updates/s", "cores": O.max_threads(), "kind": "port",
                 "sample": f"first {args.cpu_steps} timesteps of the first 24 meshes, oracle FP32 instance, "
                           "OpenMP over patches"},
-               {"fp32_oracle_meshes": 24, "fp32_oracle_steps": args.cpu_steps, "fp32_bit_mismatches": mism,
-                "nb_count_diff": nbd, "tolerance": 1e-5})
+               dict({"fp32_oracle_meshes": 24, "fp32_oracle_steps": args.cpu_steps, "fp32_bit_mismatches": mism,
+                     "nb_count_diff": nbd, "fp64_oracle_steps": 1}, **l64, tolerance=1e-5))
     return out, tot, ex == lsb.LS_EXEC_KERNELS, cpu
 
 
@@ -289,10 +326,20 @@ def mcf_workload(args, lsb, cases):
             cpu_v += ost.nb_verts
             mism += int(np.count_nonzero(g.phi().astype(np.float32) != o.phi().astype(np.float32)))
         o.close()
+        # One more step checked against the FP64 oracle from the GPU's state.
+        o64 = O.Oracle(3, case.xyz, case.cells, case.labels, case.nparts, 64)
+        o64.set_velocity(case.vel)
+        o64.set_phi(g.phi())
+        o64.set_narrowband(g.narrowband())
+        g.step(prm)
+        o64.step(O.make_params(case.dt, case.bandwidth, case.niter, mode=1))
+        l64 = fp64_error(g.phi(), o64.phi(), o64.ambiguous(), case.bandwidth)
+        o64.close()
         g.close()
         cpu = ({"value": cpu_v / cpu_s, "unit": "vertex-updates/s", "cores": O.max_threads(), "kind": "port",
                 "sample": f"first {args.cpu_steps} timesteps of the same workload, oracle FP32 instance"},
-               {"fp32_oracle_steps": args.cpu_steps, "fp32_bit_mismatches": mism, "tolerance": 1e-5})
+               dict({"fp32_oracle_steps": args.cpu_steps, "fp32_bit_mismatches": mism, "fp64_oracle_steps": 1},
+                    **l64, tolerance=1e-5))
     return out, tot, ex == lsb.LS_EXEC_KERNELS, cpu
 
 
@@ -550,6 +597,7 @@ def main():
             "fp32_nb_membership_diff": fp32["nb_diff"],
             "fp64_oracle_steps": fp64["steps"],
             "linf_vs_fp64": fp64["linf"],
+            "linf_vs_fp64_in_band": fp64["linf_band"],
             "linf_vs_fp64_incl_threshold_ambiguous": fp64["linf_all"],
             "threshold_ambiguous_verts_max_per_step": fp64["ambiguous"],
             "fp64_nb_membership_diff": fp64["nb_diff"],

@@ -65,6 +65,37 @@ def test_batch_matches_oracle_per_mesh(oracle_mod):
     s.close()
 
 
+def test_batch_fp32_vs_fp64_oracle_in_band_per_step(oracle_mod):
+    # Each step of each mesh from the GPU's own state against the FP64
+    # oracle: within 1e-5 on every vertex inside the band that the oracle
+    # does not flag as hinging on an advection threshold (beyond the band
+    # the FIM's stopping test can settle a sweep apart in FP32 and FP64).
+    b = cases.small_batch(n_meshes=4, seed=11)
+    s = lsb.solver_for_batch(b)
+    prm = lsb.step_params(1.0, 1.0, 5)
+    parts = []
+    for m in b.meshes:
+        nparts = -(-m.xyz.shape[0] // b.patch_verts)
+        parts.append((cases.partition_rcb(m.xyz, nparts), nparts))
+    for k in range(4):
+        before = s.split(s.phi()) if k else [m.phi0 for m in b.meshes]
+        s.step(prm)
+        for m, (mesh, phi) in enumerate(zip(b.meshes, s.split(s.phi()))):
+            o = oracle_mod.Oracle(3, mesh.xyz, mesh.cells, parts[m][0], parts[m][1], 64)
+            o.set_velocity(mesh.vel)
+            o.set_phi(before[m])
+            o.step(oracle_mod.make_params(mesh.dt, mesh.bandwidth, 5))
+            ophi, amb = o.phi(), o.ambiguous()
+            o.close()
+            fin = (np.abs(ophi) < 1e8) & (np.abs(phi) < 1e8)
+            np.testing.assert_array_equal(np.abs(phi) < 1e8, np.abs(ophi) < 1e8)
+            inb = fin & ~amb & (np.abs(ophi) <= mesh.bandwidth)
+            err = float(np.max(np.abs(phi - ophi)[inb]))
+            print(f"step {k} mesh {m}: in-band L-inf vs FP64 {err:.2e} ({int(inb.sum())} vertices)")
+            assert err <= 1e-5
+    s.close()
+
+
 def test_autotune_picks_a_candidate_with_identical_results():
     b = cases.small_batch(n_meshes=4, seed=9)
     cand, ex, ms, table = lsb.autotune_batch(b, [64, 128], trial_steps=2, log=print)

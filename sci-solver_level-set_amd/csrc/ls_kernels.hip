@@ -751,7 +751,8 @@ __device__ SweepResult<T> sweep_patch(const DevMesh<T>& m, DevState<T>& s, int p
     s_vtx[nv + i] = (members > 0 && s.visitE[m.vpatch[h]] == members) ? s.DT_out[h] : s.DT[h];
   }
   copy_to_lds<BS, 8>(s_vsl, m.vslots + sbase, nslots);
-  const bool resume = epoch_in > 0 && s.rs_epoch[p] == epoch_in;
+  // (A single-sweep visit recomputes the sweep: resuming would leave it none.)
+  const bool resume = niter > 1 && epoch_in > 0 && s.rs_epoch[p] == epoch_in;
   if (resume) {
     copy_to_lds<BS, 4>(s_slot, s.rs_slot + (int64_t)es * NC, NC * ne);
     copy_to_lds<BS, 4>(s_dirty, s.rs_dirty + es, ne);

@@ -85,6 +85,19 @@ def test_tet_fp32_bit_exact_other_executions(execution, oracle_mod):
         np.testing.assert_array_equal(gphi.astype(np.float32), ophi.astype(np.float32))
 
 
+@pytest.mark.parametrize("execution", [lsb.LS_EXEC_PERSISTENT, lsb.LS_EXEC_KERNELS])
+def test_single_sweep_visits_bit_exact(execution, oracle_mod):
+    # niter = 1: a visit is one sweep, so it cannot resume from a probe's
+    # saved state and recomputes it.
+    import dataclasses
+    case = dataclasses.replace(cases.small3d(n=24, shape="torus", block=(4, 4, 4), steps=4), niter=1)
+    for k, gs, os_, gphi, ophi, gnb, onb in _lockstep(case, 4, oracle_mod, 32, execution=execution):
+        print(f"step {k}: iters {gs.fim_iters} visits {gs.patch_visits} probes {gs.probes}")
+        np.testing.assert_array_equal(gnb, onb, err_msg=f"narrowband step {k}")
+        assert _same_schedule(gs, os_), f"schedule step {k}"
+        np.testing.assert_array_equal(gphi.astype(np.float32), ophi.astype(np.float32), err_msg=f"phi step {k}")
+
+
 def test_execution_modes_switch_mid_run(oracle_mod):
     case = cases.small3d(n=16, shape="sphere", block=(4, 4, 4), steps=6)
     modes = [lsb.LS_EXEC_PERSISTENT, lsb.LS_EXEC_KERNELS, lsb.LS_EXEC_GRAPH, lsb.LS_EXEC_PERSISTENT,

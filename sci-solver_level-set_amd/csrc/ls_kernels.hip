@@ -629,13 +629,28 @@ struct OrderedBits<double> {
   }
 };
 
-// The next work item of a persistent loop for the whole workgroup: thread 0
-// draws it from a device counter and shares it through LDS.
-__device__ __forceinline__ int next_work(int32_t* counter, int& s_item) {
+// A draw from a phase counter for the whole workgroup. s_draw[0] is the item
+// drawn last, s_draw[1] a second item reserved with it (-1: none). While at
+// least 16 draws per workgroup remain a draw reserves two consecutive items,
+// halving the counter round trips; near the end of the phase it reserves one,
+// so a workgroup done early still finds work. Every counter value is one
+// item, taken once. A phase's loop ends only on a draw past n, which finds no
+// reserved item, so the pair state starts every phase empty.
+__device__ __forceinline__ int draw_items(int n, int32_t* counter, int* s_draw) {
   __syncthreads();  // every thread has read the previous item
-  if (threadIdx.x == 0) s_item = atomicAdd(counter, 1);
+  if (threadIdx.x == 0) {
+    if (s_draw[1] >= 0) {
+      s_draw[0] = s_draw[1];
+      s_draw[1] = -1;
+    } else {
+      const bool pair = s_draw[0] + 16 * (int)gridDim.x < n;
+      const int i = atomicAdd(counter, pair ? 2 : 1);
+      s_draw[0] = i;
+      s_draw[1] = pair && i + 1 < n ? i + 1 : -1;
+    }
+  }
   __syncthreads();
-  return s_item;
+  return s_draw[0];
 }
 
 // Work items of a phase with n items: with no more items than workgroups
@@ -643,11 +658,11 @@ __device__ __forceinline__ int next_work(int32_t* counter, int& s_item) {
 // taken, so every item is), which keeps the counter's atomic round trips off
 // the phase's critical path; otherwise items are drawn from the counter, so a
 // workgroup done early takes the next one. n means no further item.
-__device__ __forceinline__ int first_item(int n, int32_t* counter, int& s_item) {
-  return n <= (int)gridDim.x ? ((int)blockIdx.x < n ? (int)blockIdx.x : n) : next_work(counter, s_item);
+__device__ __forceinline__ int first_item(int n, int32_t* counter, int* s_draw) {
+  return n <= (int)gridDim.x ? ((int)blockIdx.x < n ? (int)blockIdx.x : n) : draw_items(n, counter, s_draw);
 }
-__device__ __forceinline__ int later_item(int n, int32_t* counter, int& s_item) {
-  return n <= (int)gridDim.x ? n : next_work(counter, s_item);
+__device__ __forceinline__ int later_item(int n, int32_t* counter, int* s_draw) {
+  return n <= (int)gridDim.x ? n : draw_items(n, counter, s_draw);
 }
 
 // Copies n items into LDS with U independent loads in flight per thread.
@@ -1912,7 +1927,7 @@ __global__ __launch_bounds__(BS, WPE) void k_step(DevMesh<T> m, DevState<T> s, S
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ T red[BS / 64];
   __shared__ int s_ndirty[10];
-  __shared__ int s_work;
+  __shared__ int s_work[2];  // work draws: item, reserved second item
   const int G = gridDim.x, b = blockIdx.x, tid = threadIdx.x;
   const int64_t gtid = (int64_t)b * BS + tid, gsize = (int64_t)G * BS;
   const int nb_out = 1 - a.nb_in;
@@ -1920,6 +1935,7 @@ __global__ __launch_bounds__(BS, WPE) void k_step(DevMesh<T> m, DevState<T> s, S
   __shared__ unsigned long long tk[kNumTicks + 1];
   PhaseClock pcl{tk};
   pcl.start(clk);
+  if (tid == 0) s_work[0] = s_work[1] = -1;  // visible after the first grid barrier
 
   // Phases 0-2 run once, or twice when a warm start fails its residual
   // check: the second attempt is the plain method from the same phi.

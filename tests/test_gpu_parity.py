@@ -85,6 +85,19 @@ def test_tet_fp32_bit_exact_other_executions(execution, oracle_mod):
         np.testing.assert_array_equal(gphi.astype(np.float32), ophi.astype(np.float32))
 
 
+def test_tet_fp32_bit_exact_paired_draws(oracle_mod):
+    # 128^3 Kuhn grid in 4x4x4 blocks: 32768 patches, so the seed scan of the
+    # first step and the large visit phases hold more than 16 work draws per
+    # workgroup and the persistent kernel reserves items two at a time.
+    case = cases.small3d(n=128, shape="torus", block=(4, 4, 4), steps=2)
+    for k, gs, os_, gphi, ophi, gnb, onb in _lockstep(case, 2, oracle_mod, 32):
+        print(f"P={case.nparts} step {k}: iters {gs.fim_iters} visits {gs.patch_visits} probes {gs.probes}")
+        assert gs.step_wg_per_cu == 2
+        np.testing.assert_array_equal(gnb, onb, err_msg=f"narrowband step {k}")
+        assert _same_schedule(gs, os_), f"schedule step {k}"
+        np.testing.assert_array_equal(gphi.astype(np.float32), ophi.astype(np.float32), err_msg=f"phi step {k}")
+
+
 @pytest.mark.parametrize("execution", [lsb.LS_EXEC_PERSISTENT, lsb.LS_EXEC_KERNELS])
 def test_single_sweep_visits_bit_exact(execution, oracle_mod):
     # niter = 1: a visit is one sweep, so it cannot resume from a probe's

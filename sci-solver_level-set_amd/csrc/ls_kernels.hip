@@ -1202,10 +1202,12 @@ __device__ void curvature_patch(const DevMesh<T>& m, DevState<T>& s, int p, doub
   const int es = m.ele_off[p], ee = m.ele_off[p + 1];
   const int hs = m.halo_off[p], he = m.halo_off[p + 1];
   const int nv = ve - vs, ne = ee - es, nh = he - hs;
+  // Per slot: vol gradN_i . n_e; per element: vol and vol |grad phi_e| (both
+  // the same for every corner of the element).
   T* s_cv = (T*)smem;
   T* s_vol = s_cv + NC * ne;
-  T* s_vtx = s_vol + NC * ne;
-  T* s_eik = s_vtx + nv + nh;
+  T* s_eik = s_vol + ne;
+  T* s_vtx = s_eik + ne;
   for (int i = threadIdx.x; i < nv; i += BS) s_vtx[i] = s.phi[vs + i];
   for (int i = threadIdx.x; i < nh; i += BS) s_vtx[nv + i] = s.phi[m.halo_ids[hs + i]];
   if (threadIdx.x == 0) {
@@ -1238,8 +1240,8 @@ __device__ void curvature_patch(const DevMesh<T>& m, DevState<T>& s, int p, doub
     for (int i = 0; i < NC; ++i) {
       const double d = (g[i][0] * gp[0] + g[i][1] * gp[1]) + g[i][2] * gp[2];
       s_cv[e * NC + i] = (T)(use ? vol * d / gn : 0.0);
-      s_vol[e * NC + i] = (T)(use ? vol : 0.0);
     }
+    s_vol[e] = (T)(use ? vol : 0.0);
     s_eik[e] = (T)(use ? vol * gn : 0.0);
   }
   __syncthreads();
@@ -1247,7 +1249,7 @@ __device__ void curvature_patch(const DevMesh<T>& m, DevState<T>& s, int p, doub
     const int o0 = m.vslot_off[vs + v], o1 = m.vslot_off[vs + v + 1];
     double cv = 0.0, sv = 0.0, se = 0.0;
     for (int k = o0; k < o1; ++k) cv += (double)s_cv[m.vslots[k]];
-    for (int k = o0; k < o1; ++k) sv += (double)s_vol[m.vslots[k]];
+    for (int k = o0; k < o1; ++k) sv += (double)s_vol[m.vslots[k] / NC];
     for (int k = o0; k < o1; ++k) se += (double)s_eik[m.vslots[k] / NC];
     const double old = (double)s_vtx[v];
     if (sv > 0.0) {
@@ -1278,19 +1280,25 @@ __device__ __forceinline__ void update_patch(const DevMesh<T>& m, DevState<T>& s
 // K_i = vol sigma.gradN_i, H = sum K_i phi_i, distribution coefficients
 // alpha~_i; per owned vertex phi -= dt * sum(alpha~ H) / sum(alpha~ vol).
 // Evaluated in FP64 from the stored values; per-slot products and the result
-// are stored as T.
+// are stored as T. With FP64 storage the LDS holds H, vol and the NC
+// coefficients per element instead of the 2 NC per-slot products (75 % of
+// the bytes, so 256-vertex tet patches fit): the vertex pass forms the same
+// FP64 products alpha~_i H and alpha~_i vol, so the bits are unchanged.
 template <typename T, int NC, int BS>
 __device__ void advect_patch(const DevMesh<T>& m, DevState<T>& s, int p, double dt, unsigned char* smem) {
   __syncthreads();
   constexpr int K = ElemSolve<T, NC>::kCoords;
   constexpr int D = NC - 1;
+  constexpr bool per_elem = sizeof(T) == 8;
+  constexpr int EW = NC + 2;  // per-element record: H, vol, alpha~_0..NC-1
   const int vs = m.vert_off[p], ve = m.vert_off[p + 1];
   const int es = m.ele_off[p], ee = m.ele_off[p + 1];
   const int hs = m.halo_off[p], he = m.halo_off[p + 1];
   const int nv = ve - vs, ne = ee - es, nh = he - hs;
   T* s_up = (T*)smem;
   T* s_dn = s_up + NC * ne;
-  T* s_vtx = s_dn + NC * ne;
+  double* s_el = (double*)smem;
+  T* s_vtx = per_elem ? (T*)(s_el + EW * ne) : s_dn + NC * ne;
   for (int i = threadIdx.x; i < nv; i += BS) s_vtx[i] = s.phi[vs + i];
   for (int i = threadIdx.x; i < nh; i += BS) s_vtx[nv + i] = s.phi[m.halo_ids[hs + i]];
   if (threadIdx.x == 0) {
@@ -1342,18 +1350,36 @@ __device__ void advect_patch(const DevMesh<T>& m, DevState<T>& s, int p, double 
 #pragma unroll
       for (int i = 0; i < NC; ++i) at[i] = d_max(alpha[i], 0.0) / theta;
     }
+    if constexpr (per_elem) {
+      s_el[e * EW + 0] = H;
+      s_el[e * EW + 1] = vol;
 #pragma unroll
-    for (int i = 0; i < NC; ++i) {
-      s_up[e * NC + i] = (T)(at[i] * H);
-      s_dn[e * NC + i] = (T)(at[i] * vol);
+      for (int i = 0; i < NC; ++i) s_el[e * EW + 2 + i] = at[i];
+    } else {
+#pragma unroll
+      for (int i = 0; i < NC; ++i) {
+        s_up[e * NC + i] = (T)(at[i] * H);
+        s_dn[e * NC + i] = (T)(at[i] * vol);
+      }
     }
   }
   __syncthreads();
   for (int v = threadIdx.x; v < nv; v += BS) {
     const int o0 = m.vslot_off[vs + v], o1 = m.vslot_off[vs + v + 1];
     double up = 0.0, dn = 0.0;
-    for (int k = o0; k < o1; ++k) up += (double)s_up[m.vslots[k]];
-    for (int k = o0; k < o1; ++k) dn += (double)s_dn[m.vslots[k]];
+    if constexpr (per_elem) {
+      for (int k = o0; k < o1; ++k) {
+        const int sl = m.vslots[k], e = sl / NC;
+        up += s_el[e * EW + 2 + sl % NC] * s_el[e * EW];
+      }
+      for (int k = o0; k < o1; ++k) {
+        const int sl = m.vslots[k], e = sl / NC;
+        dn += s_el[e * EW + 2 + sl % NC] * s_el[e * EW + 1];
+      }
+    } else {
+      for (int k = o0; k < o1; ++k) up += (double)s_up[m.vslots[k]];
+      for (int k = o0; k < o1; ++k) dn += (double)s_dn[m.vslots[k]];
+    }
     const double old = (double)s_vtx[v];
     s.phi_out[vs + v] = (T)((d_abs(dn) > 1e-8) ? old - dt * (up / dn) : old);
   }
@@ -2189,7 +2215,9 @@ class EngineImpl final : public Engine {
                           ~size_t(7)) + nv * sizeof(T);  // + running vertex minima
       const size_t fim_tasks = ((fim + 7) & ~size_t(7)) + 4 * (NC * ne + (NC * ne) % 2) + NC * ne * sizeof(T);
       fim_task_lds = std::max(fim_task_lds, fim_tasks);
-      const size_t adv = 2 * NC * ne * sizeof(T) + (nv + nh + ne) * sizeof(T);
+      // Advection: per-slot products (FP32) or per-element records (FP64);
+      // curvature: per-slot terms plus two per-element values.
+      const size_t adv = ((sizeof(T) == 8 ? (NC + 2) * ne : 2 * NC * ne) + nv + nh) * sizeof(T);
       fim_lds_ = std::max(fim_lds_, fim);
       adv_lds_ = std::max(adv_lds_, adv);
       seed_lds_ = std::max(seed_lds_, (nv + nh) * sizeof(T));

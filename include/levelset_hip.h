@@ -43,10 +43,16 @@ enum { LS_SWEEP_AUTO = 0, LS_SWEEP_ELEMENTS = 1, LS_SWEEP_TASKS = 2 };
  * on the host. */
 enum { LS_LAYOUT_DEVICE = 0, LS_LAYOUT_HOST = 1 };
 /* Warm start of the FIM (no reference counterpart): none, |phi| after the
- * previous step (identity predictor), or the learned predictor's distances
- * (ls_mlp_predict). Warm values apply to non-seed vertices of the previous
- * narrowband; a step whose fixed-point residual over the band exceeds
- * warm_res_tol re-runs plain in the same launch. */
+ * previous step (identity predictor), or the predictor distances held by the
+ * context (ls_mlp_predict, or caller values from ls_set_warm). Non-seed
+ * vertices of the previous narrowband start the FIM at
+ * prediction * (1 + warm_margin) + warm_abs_margin instead of LARGENUM. The
+ * FIM only lowers values and every value it lowers comes from an element
+ * solve, so the only unsupported values at its end are warm values still
+ * held: a step in which such a vertex inside the band lies more than
+ * warm_res_tol below what its elements give it from the final distances
+ * (an undercut) re-runs the plain method from the same phi in the same
+ * launch, and returns the plain method's bits. */
 enum { LS_WARM_NONE = 0, LS_WARM_IDENTITY = 1, LS_WARM_LEARNED = 2 };
 
 /* Per-step parameters. Replaces the argument list of
@@ -65,8 +71,9 @@ typedef struct ls_step_params {
   int32_t max_fim_iters;/* 0: run the FIM loop to convergence; > 0: stop after that many iterations */
   int32_t warm;         /* LS_WARM_*; needs LS_EXEC_PERSISTENT or LS_EXEC_GRAPH */
   int32_t reserved0;
-  double warm_margin;   /* warm value = predicted distance * (1 + warm_margin) */
-  double warm_res_tol;  /* residual above which a warm step re-runs plain (0: tol) */
+  double warm_margin;   /* warm value = predicted distance * (1 + warm_margin) + warm_abs_margin */
+  double warm_res_tol;  /* undercut above which a warm step re-runs plain (0: tol) */
+  double warm_abs_margin;
 } ls_step_params;
 
 typedef struct ls_stats {
@@ -97,7 +104,8 @@ typedef struct ls_stats {
   double ms_ph_seed, ms_ph_visit, ms_ph_copyback, ms_ph_probe, ms_ph_next, ms_ph_nb, ms_ph_advect;
   int64_t elem_updates;     /* element updates executed by FIM visits and probes */
   int64_t local_solves;     /* local solves executed (the rest were skipped exactly) */
-  double warm_residual;     /* warm start: max |DT - F(DT)| over non-seed band vertices */
+  double warm_residual;     /* warm start: largest undercut, max(F_v - DT_v) over vertices inside the band
+                               still holding their warm value (0: none) */
   int32_t warm_patches;     /* warm start: patches that received warm values */
   int32_t warm_fallback;    /* warm start: 1 when the residual check failed and the step re-ran plain */
   int32_t sweep_tasks;      /* 1: the FIM sweeps ran task-compacted */
@@ -211,6 +219,15 @@ int ls_get_batch_nb_counts(ls_ctx* ctx, int32_t* counts);
  * (the DT array of redistance3d::GenerateData, src/core3d/cuda/redistance3d.cu:225-273),
  * in the caller's vertex order; LARGENUM where the FIM did not reach. */
 int ls_get_distances(ls_ctx* ctx, double* out);
+
+/* Predictor distances for the next ls_step with warm = LS_WARM_LEARNED, one
+ * per vertex in the caller's order (no reference counterpart): a warm start
+ * from any external predictor, checked and finished by the FIM exactly as the
+ * network's values are. Values at or above LARGENUM/2 leave a vertex cold. */
+int ls_set_warm(ls_ctx* ctx, const double* warm);
+/* The predictor distances the context holds (from ls_mlp_predict or
+ * ls_set_warm, rounded to the working type), in the caller's order. */
+int ls_get_warm(ls_ctx* ctx, double* out);
 
 /* Learned warm start (no reference counterpart; tetrahedral meshes). A
  * 22-64-64-4 ReLU MLP per element copy: inputs are the four corner phi values

@@ -296,8 +296,9 @@ struct lso_params {
   int nthreads;         // OpenMP threads (0 = runtime default)
   int max_fim_iters;    // 0: FIM to convergence; > 0: stop after that many iterations
   int warm;             // 0: plain; 1: warm start from |phi|; 2: from the values given by set_warm
-  double warm_margin;   // warm value = prediction * (1 + warm_margin)
-  double warm_res_tol;  // residual above which a warm step re-runs plain (0: tol)
+  double warm_margin;   // warm value = prediction * (1 + warm_margin) + warm_abs_margin
+  double warm_res_tol;  // acceptance threshold of a warm step (0: tol); above it the step re-runs plain
+  double warm_abs_margin;
 };
 struct lso_stats {
   int fim_iters;
@@ -330,6 +331,7 @@ struct Base {
   virtual void advect_all(double dt) = 0;
   virtual int nparts() const = 0;
   virtual void set_warm(const double*) = 0;
+  virtual void get_resid(float*) const = 0;
 };
 
 template <typename T>
@@ -890,27 +892,44 @@ struct Solver : Base {
   }
 
   // Warm start (no reference counterpart): over the previous narrowband,
-  // every non-seed vertex whose warm value prediction * (1 + margin) is below
-  // LARGENUM/2 starts the FIM there instead of at LARGENUM, and every patch
-  // receiving one joins the first active list. The prediction is |phi|
-  // (warm 1) or the values given by set_warm (warm 2).
-  std::vector<double> warm;  // renumbered order, for warm 2
+  // every vertex sharing no element with a seed whose warm value
+  // prediction * (1 + margin) + abs margin is below LARGENUM/2 starts the
+  // FIM there instead of at LARGENUM,
+  // and every patch receiving one joins the first active list. The
+  // prediction is |phi| (warm 1) or the values given by set_warm (warm 2).
+  std::vector<double> warm;     // renumbered order, for warm 2
+  std::vector<T> warm_applied;  // the warm value each vertex started from (LARGENUM: none)
   void set_warm(const double* w) override {
     if (!w) { warm.clear(); return; }
     warm.resize(N);
     for (int64_t v = 0; v < N; ++v) warm[vert_ipermute[v]] = w[v];
   }
+  // Vertices sharing an element with a seed keep LARGENUM: the FIM solves
+  // them from seed slots it lowers inside a visit and never writes back
+  // (redistance_kernels3d.h:440-449), which no check from stored distances
+  // can reproduce, so they are finished exactly as in the plain method.
   int apply_warm(const std::vector<int32_t>& prev_nb, const lso_params& prm, std::vector<int32_t>& act) {
     std::vector<char> in(P, 0);
     for (int32_t p : act) in[p] = 1;
+    warm_applied.assign(N, (T)kLarge);
     int npatch = 0;
     for (int32_t p : prev_nb) {
       bool any = false;
+      std::vector<char> near(vert_offsets[p + 1] - vert_offsets[p], 0);
+      for (int32_t c = ele_offsets[p]; c < ele_offsets[p + 1]; ++c) {
+        bool seeded = false;
+        for (int i = 0; i < nc; ++i) seeded = seeded || label[ele[i * F + c]] == SeedPoint;
+        if (!seeded) continue;
+        for (int i = 0; i < nc; ++i) {
+          const int32_t v = ele[i * F + c];
+          if (v >= vert_offsets[p] && v < vert_offsets[p + 1]) near[v - vert_offsets[p]] = 1;
+        }
+      }
       for (int32_t v = vert_offsets[p]; v < vert_offsets[p + 1]; ++v) {
-        if (label[v] == SeedPoint) continue;
+        if (label[v] == SeedPoint || near[v - vert_offsets[p]]) continue;
         const double base = prm.warm == 2 ? (double)(T)warm[v] : std::fabs((double)phi[v]);
-        const double w = base * (1.0 + prm.warm_margin);
-        if (w < 0.5 * kLarge) { DT[v] = (T)w; any = true; }
+        const double w = base * (1.0 + prm.warm_margin) + prm.warm_abs_margin;
+        if (w < 0.5 * kLarge) { DT[v] = (T)w; warm_applied[v] = (T)w; any = true; }
       }
       if (any) { npatch++; in[p] = 1; }
     }
@@ -919,50 +938,47 @@ struct Solver : Base {
     return npatch;
   }
 
-  // Fixed-point residual over the band: max over non-seed vertices with
-  // DT <= band of |DT_v - F_v|, F_v the least value any element copy gives v
-  // from its other corners.
+  // Acceptance of a warm start. The FIM only lowers values, and a value it
+  // lowers is an element solve from corner values no lower than their final
+  // ones, so (the solve being monotone) it is no lower than what that element
+  // gives from the final values. The only values no solve produced are warm
+  // values still held at the end: a vertex holding one inside the band must
+  // not lie below F_v, the least value any of its element copies gives it
+  // from the other corners' final values (no seed is among them). An undercut
+  // region of warm values has such a vertex at its minimum (nothing lower
+  // around it can support it), so every undercut deeper than the threshold
+  // per element layer is caught; the plain method's own output has no warm
+  // values and always passes. Returns max(F_v - DT_v) over those vertices
+  // (0 when there are none).
+  std::vector<float> resid_v;  // per vertex: F_v - DT_v where checked, else 0
   float residual(const lso_params& prm) {
     const T bw = (T)prm.bandwidth;
     float r = 0.f;
+    resid_v.assign(N, 0.f);
     for (int p = 0; p < P; ++p) {
-      const int32_t vs = vert_offsets[p], ve = vert_offsets[p + 1];
-      T mn = (T)kLarge;
-      for (int32_t v = vs; v < ve; ++v) mn = std::fmin(mn, DT[v]);
-      if (!(mn <= bw)) continue;
-      std::vector<T> F(ve - vs, (T)kLarge);
-      std::vector<char> near_seed(ve - vs, 0);
-      for (int32_t c = ele_offsets[p]; c < ele_offsets[p + 1]; ++c) {
-        T val[4], L[6];
-        int32_t vid[4];
-        bool seeded = false;
-        for (int i = 0; i < nc; ++i) {
-          vid[i] = ele[i * F_of() + c];
-          val[i] = DT[vid[i]];
-          seeded = seeded || label[vid[i]] == SeedPoint;
-        }
-        if (seeded) {
-          for (int i = 0; i < nc; ++i)
-            if (vid[i] >= vs && vid[i] < ve) near_seed[vid[i] - vs] = 1;
-          continue;
-        }
-        for (int k = 0; k < (dim == 3 ? 6 : 3); ++k) L[k] = coords[k * F_of() + c];
-        for (int k = 0; k < nc; ++k) {
-          const int32_t v = vid[k];
-          if (v < vs || v >= ve || !(val[k] <= bw) || label[v] == SeedPoint) continue;
-          const T f = dim == 3 ? TetGeom(L).corner_value(val, k, prm.lambda_eps)
+      for (int32_t v = vert_offsets[p]; v < vert_offsets[p + 1]; ++v) {
+        if (label[v] == SeedPoint || !(DT[v] <= bw) || DT[v] != warm_applied[v]) continue;
+        T f = (T)kLarge;
+        for (int32_t l : lmem[v]) {  // v's slots in its own patch: copy * nc + corner
+          const int64_t c = ele_offsets[p] + l / nc;
+          const int k = l % nc;
+          T val[4], L[6];
+          for (int i = 0; i < nc; ++i) val[i] = DT[ele[i * F_of() + c]];
+          for (int q = 0; q < (dim == 3 ? 6 : 3); ++q) L[q] = coords[q * F_of() + c];
+          const T g = dim == 3 ? TetGeom(L).corner_value(val, k, prm.lambda_eps)
                                : TriGeom(L).corner_value(val, k, prm.lambda_eps);
-          F[v - vs] = std::fmin(F[v - vs], f);
+          f = std::fmin(f, g);
         }
-      }
-      for (int32_t v = vs; v < ve; ++v) {
-        if (!(DT[v] <= bw) || label[v] == SeedPoint || near_seed[v - vs]) continue;
-        r = std::fmax(r, (float)std::fabs((double)DT[v] - (double)F[v - vs]));
+        resid_v[v] = (float)((double)f - (double)DT[v]);
+        r = std::fmax(r, resid_v[v]);
       }
     }
     return r;
   }
   int64_t F_of() const { return F; }
+  void get_resid(float* out) const override {
+    for (int64_t v = 0; v < N; ++v) out[vert_permute[v]] = resid_v.empty() ? 0.f : resid_v[v];
+  }
 
   // One timestep (meshFIM3d::GenerateData loop body, meshFIM3d.cu:639-678).
   int step(const lso_params& prm, lso_stats& st) override {
@@ -1026,6 +1042,8 @@ void lso_get_ambiguous(void* h, uint8_t* out) { static_cast<Base*>(h)->get_amb(o
 void lso_advect_all(void* h, double dt) { static_cast<Base*>(h)->advect_all(dt); }
 // Warm-start distances for the next step only (caller order; >= LARGENUM/2: none).
 void lso_set_warm(void* h, const double* warm) { static_cast<Base*>(h)->set_warm(warm); }
+// Per-vertex acceptance residuals of the last warm step (caller order).
+void lso_get_warm_residuals(void* h, float* out) { static_cast<Base*>(h)->get_resid(out); }
 // Per-copy local coordinates in the storage type, copy-major.
 void lso_get_coords(void* h, double* out) { static_cast<Base*>(h)->get_coords(out); }
 void lso_get_layout(void* h, int32_t* vert_perm, int32_t* ele_perm, int32_t* vert_off,

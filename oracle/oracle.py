@@ -19,7 +19,7 @@ class Params(C.Structure):
     _fields_ = [("dt", C.c_double), ("bandwidth", C.c_double), ("tol", C.c_double),
                 ("lambda_eps", C.c_double), ("niter", C.c_int), ("inside_niter", C.c_int),
                 ("mode", C.c_int), ("nthreads", C.c_int), ("max_fim_iters", C.c_int), ("warm", C.c_int),
-                ("warm_margin", C.c_double), ("warm_res_tol", C.c_double)]
+                ("warm_margin", C.c_double), ("warm_res_tol", C.c_double), ("warm_abs_margin", C.c_double)]
 
 
 class Stats(C.Structure):
@@ -68,14 +68,15 @@ def lib():
         L.lso_get_coords.argtypes = [vp, C.c_void_p]
         L.lso_advect_all.argtypes = [vp, C.c_double]
         L.lso_set_warm.argtypes = [vp, C.c_void_p]
+        L.lso_get_warm_residuals.argtypes = [vp, C.c_void_p]
         _lib = L
     return _lib
 
 
 def make_params(dt, bandwidth, niter, tol=1e-5, lambda_eps=1e-5, inside_niter=1, mode=0, nthreads=0,
-                max_fim_iters=0, warm=0, warm_margin=0.0, warm_res_tol=0.0):
+                max_fim_iters=0, warm=0, warm_margin=0.0, warm_res_tol=0.0, warm_abs_margin=0.0):
     return Params(dt, bandwidth, tol, lambda_eps, niter, inside_niter, mode, nthreads, max_fim_iters, warm,
-                  warm_margin, warm_res_tol)
+                  warm_margin, warm_res_tol, warm_abs_margin)
 
 
 class Oracle:
@@ -137,6 +138,12 @@ class Oracle:
         """Warm-start distances (caller order) for the next step only."""
         self._warm = np.ascontiguousarray(warm, dtype=np.float64)
         self.L.lso_set_warm(self.h, self._warm.ctypes.data)
+
+    def warm_residuals(self):
+        """Per-vertex acceptance residuals F_v - DT_v of the last warm step."""
+        out = np.zeros(self.N, dtype=np.float32)
+        self.L.lso_get_warm_residuals(self.h, out.ctypes.data)
+        return out
 
     def advect_all(self, dt):
         self.L.lso_advect_all(self.h, float(dt))

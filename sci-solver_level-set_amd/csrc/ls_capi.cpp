@@ -501,4 +501,20 @@ int ls_get_distances(ls_ctx* c, double* out) {
   return rc == LS_OK ? rc : fail(c, rc, err);
 }
 
+int ls_set_warm(ls_ctx* c, const double* warm) {
+  if (!c || !warm) return LS_ERR_ARG;
+  if (int rc = engine_ready(c)) return rc;
+  std::string err;
+  const int rc = c->engine->set_warm(warm, err);
+  return rc == LS_OK ? rc : fail(c, rc, err);
+}
+
+int ls_get_warm(ls_ctx* c, double* out) {
+  if (!c || !out) return LS_ERR_ARG;
+  if (int rc = engine_ready(c)) return rc;
+  std::string err;
+  const int rc = c->engine->get_warm(out, err);
+  return rc == LS_OK ? rc : fail(c, rc, err);
+}
+
 }  // extern "C"

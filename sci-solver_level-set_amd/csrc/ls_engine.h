@@ -41,6 +41,8 @@ class Engine {
                            std::string& err) = 0;
   virtual int mlp_train(int epochs, int batch, double lr, uint64_t seed, double* loss, std::string& err) = 0;
   virtual int get_distances(double* out_user, std::string& err) = 0;
+  virtual int set_warm(const double* warm_user, std::string& err) = 0;  // caller order
+  virtual int get_warm(double* out_user, std::string& err) = 0;         // caller order
   virtual void set_exec(int mode) = 0;
   virtual int set_sweep_mode(int mode, std::string& err) = 0;  // LS_SWEEP_*
   virtual int sweep_tasks() const = 0;                         // 1: task-compacted sweeps in use

@@ -1055,20 +1055,60 @@ __device__ T narrowband_patch(const DevMesh<T>& m, DevState<T>& s, int p, T* red
   return block_min_n<T, BS>(mn, red);
 }
 
-// Warm start of one patch of the previous narrowband: every non-seed vertex
-// whose warm value is below LARGENUM/2 starts the FIM there instead of at
-// LARGENUM. The warm value is the predictor's distance times (1 + margin):
-// the learned predictor's values (s.warm) or |phi| (identity predictor).
-// Returns (to every thread) whether the patch received any warm value.
-template <typename T, int BS>
-__device__ bool warm_patch(const DevMesh<T>& m, DevState<T>& s, int p, int mode, double margin) {
+// Warm value of vertex v: the predictor's distance times (1 + margin) plus
+// an absolute margin; the prediction is the learned predictor's (s.warm) or
+// |phi| (identity predictor).
+template <typename T>
+__device__ __forceinline__ double warm_value(const DevState<T>& s, int64_t v, int mode, double margin,
+                                             double abs_margin) {
+  const double base = mode == 2 ? (double)s.warm[v] : d_abs((double)s.phi[v]);
+  return base * (1.0 + margin) + abs_margin;
+}
+
+// Warm start of one patch of the previous narrowband: every vertex sharing
+// no element with a seed whose warm value is below LARGENUM/2 starts the FIM
+// there instead of at LARGENUM. Vertices next to a seed stay cold: the FIM
+// solves them from seed slots it lowers inside a visit and never writes back
+// (redistance_kernels3d.h:440-449), which no check from stored distances can
+// reproduce, so they are finished exactly as in the plain method. Returns (to
+// every thread) whether the patch received any warm value.
+// s_near[i] = 1 for each owned vertex i of patch p that shares an element
+// with a seed (every element holding it has a copy in p). Starts and ends
+// with a workgroup barrier.
+template <typename T, int NC, int BS>
+__device__ void mark_seed_adjacent(const DevMesh<T>& m, const DevState<T>& s, int p, uint8_t* s_near) {
+  const int vs = m.vert_off[p], nv = m.vert_off[p + 1] - vs;
+  const int es = m.ele_off[p], ee = m.ele_off[p + 1];
+  const int hs = m.halo_off[p];
   __syncthreads();
+  for (int i = threadIdx.x; i < nv; i += BS) s_near[i] = 0;
+  __syncthreads();
+  for (int e = es + threadIdx.x; e < ee; e += BS) {
+    const ushort4 c = m.copy_local[e];
+    bool seeded = false;
+#pragma unroll
+    for (int i = 0; i < NC; ++i) {
+      const int l = corner(c, i);
+      seeded = seeded || s.label[l < nv ? vs + l : m.halo_ids[hs + l - nv]] == kSeedPoint;
+    }
+    if (seeded)
+#pragma unroll
+      for (int i = 0; i < NC; ++i)
+        if (corner(c, i) < nv) s_near[corner(c, i)] = 1;
+  }
+  __syncthreads();
+}
+
+template <typename T, int NC, int BS>
+__device__ bool warm_patch(const DevMesh<T>& m, DevState<T>& s, int p, int mode, double margin, double abs_margin,
+                           unsigned char* smem) {
   const int vs = m.vert_off[p], ve = m.vert_off[p + 1];
+  uint8_t* s_near = smem;
+  mark_seed_adjacent<T, NC, BS>(m, s, p, s_near);
   int any = 0;
   for (int v = vs + threadIdx.x; v < ve; v += BS) {
-    if (s.label[v] == kSeedPoint) continue;
-    const double base = mode == 2 ? (double)s.warm[v] : d_abs((double)s.phi[v]);
-    const double w = base * (1.0 + margin);
+    if (s.label[v] == kSeedPoint || s_near[v - vs]) continue;
+    const double w = warm_value(s, v, mode, margin, abs_margin);
     if (w < 0.5 * kLargeNum) {
       s.DT[v] = (T)w;
       any = 1;
@@ -1077,69 +1117,50 @@ __device__ bool warm_patch(const DevMesh<T>& m, DevState<T>& s, int p, int mode,
   return __syncthreads_or(any) != 0;
 }
 
-// Fixed-point residual of one patch: for every owned, non-seed vertex with
-// DT <= band and no seed among its elements' corners, |DT_v - F_v| where
-// F_v is the least value any element copy of v gives it from its other
-// corners (no min with DT_v, no skip). At the
-// scheme's fixed point F_v == DT_v; a warm value left below it (F_v > DT_v)
-// or above it shows up alike. The block maximum is merged into
-// pc[kPcResid] (non-negative floats order as their bit patterns).
+// Acceptance of a warm start, over one patch that received warm values. The
+// FIM only lowers values, and a value it lowers is an element solve from
+// corner values no lower than their final ones, so it is no lower than what
+// that element gives from the final values. The only values no solve
+// produced are warm values still held at the end: an owned non-seed vertex
+// holding one inside the band must not lie below F_v, the least value any of
+// its element copies gives it from the other corners' final distances (seeds
+// at |phi|). An undercut region of warm values has such a vertex at its
+// minimum, since nothing lower around it can support it; the plain method's
+// output holds no warm values and always passes. The patch maximum of
+// F_v - DT_v is merged into pc[kPcResid] (non-negative floats order as their
+// bit patterns).
 template <typename T, int NC, int BS>
-__device__ void residual_patch(const DevMesh<T>& m, DevState<T>& s, int p, T bw, double leps, unsigned char* smem,
-                               T* red) {
-  __syncthreads();
+__device__ void residual_patch(const DevMesh<T>& m, DevState<T>& s, int p, T bw, double leps, int mode,
+                               double margin, double abs_margin, unsigned char* smem, T* red) {
   constexpr int K = ElemSolve<T, NC>::kCoords;
   const int vs = m.vert_off[p], ve = m.vert_off[p + 1];
-  const int es = m.ele_off[p], ee = m.ele_off[p + 1];
-  const int hs = m.halo_off[p], he = m.halo_off[p + 1];
-  const int nv = ve - vs, ne = ee - es, nh = he - hs;
-  T mn = (T)kLargeNum;
-  for (int v = vs + threadIdx.x; v < ve; v += BS) mn = d_min(mn, s.DT[v]);
-  if (!(block_min_n<T, BS>(mn, red) <= bw)) return;  // no band vertex here
-  using Bits = typename OrderedBits<T>::type;
-  T* s_vtx = (T*)smem;
-  Bits* s_F = (Bits*)(s_vtx + nv + nh);
-  uint8_t* s_near = (uint8_t*)(s_F + nv);  // owned vertex shares an element with a seed
-  const Bits big = OrderedBits<T>::of((T)kLargeNum);
-  for (int i = threadIdx.x; i < nv; i += BS) {
-    s_vtx[i] = s.DT[vs + i];
-    s_F[i] = big;
-    s_near[i] = 0;
-  }
-  for (int i = threadIdx.x; i < nh; i += BS) s_vtx[nv + i] = s.DT[m.halo_ids[hs + i]];
-  __syncthreads();
-  for (int e = threadIdx.x; e < ne; e += BS) {
-    const ushort4 c = m.copy_local[es + e];
-    T val[NC];
-    bool seeded = false;
-#pragma unroll
-    for (int i = 0; i < NC; ++i) {
-      const int lv = corner(c, i);
-      val[i] = s_vtx[lv];
-      seeded = seeded || s.label[lv < nv ? vs + lv : m.halo_ids[hs + lv - nv]] == kSeedPoint;
-    }
-    // Seeds keep |phi| while their slots drop inside a visit (Q12 of the
-    // reference's scheme), so their neighbours need not match the stored
-    // seed values: elements with a seed are left out.
-    if (seeded) {
-#pragma unroll
-      for (int i = 0; i < NC; ++i)
-        if (corner(c, i) < nv) s_near[corner(c, i)] = 1;
-      continue;
-    }
-    const T* gc = m.coords + (int64_t)(es + e) * K;
-#pragma unroll
-    for (int k = 0; k < NC; ++k) {
-      const int lv = corner(c, k);
-      if (lv >= nv || !(val[k] <= bw) || s.label[vs + lv] == kSeedPoint) continue;
-      atomicMin(&s_F[lv], OrderedBits<T>::of(ElemSolve<T, NC>::corner_value(gc, val, k, leps)));
-    }
-  }
-  __syncthreads();
+  const int es = m.ele_off[p], hs = m.halo_off[p];
+  const int nv = ve - vs;
+  uint8_t* s_near = smem;  // vertices next to a seed received no warm value
+  mark_seed_adjacent<T, NC, BS>(m, s, p, s_near);
   float r = 0.f;
-  for (int v = threadIdx.x; v < nv; v += BS) {
-    if (!(s_vtx[v] <= bw) || s.label[vs + v] == kSeedPoint || s_near[v]) continue;
-    r = fmaxf(r, (float)d_abs((double)s_vtx[v] - (double)OrderedBits<T>::value(s_F[v])));
+  for (int v = vs + threadIdx.x; v < ve; v += BS) {
+    const T dv = s.DT[v];
+    if (s.label[v] == kSeedPoint || s_near[v - vs] || !(dv <= bw) ||
+        dv != (T)warm_value(s, v, mode, margin, abs_margin))
+      continue;
+    double f = kLargeNum;
+    for (int k = m.vslot_off[v]; k < m.vslot_off[v + 1]; ++k) {
+      const int sl = m.vslots[k];
+      const int64_t c = es + sl / NC;
+      const ushort4 cl = m.copy_local[c];
+      T val[NC], L[K];
+#pragma unroll
+      for (int j = 0; j < NC; ++j) {
+        const int l = corner(cl, j);
+        val[j] = s.DT[l < nv ? vs + l : m.halo_ids[hs + l - nv]];
+      }
+      const T* gc = m.coords + c * K;
+#pragma unroll
+      for (int q = 0; q < K; ++q) L[q] = gc[q];
+      f = d_min(f, (double)ElemSolve<T, NC>::corner_value(L, val, sl % NC, leps));
+    }
+    r = fmaxf(r, (float)(f - (double)dv));
   }
   r = -block_min_n<float, BS>(-r, (float*)red);
   if (threadIdx.x == 0 && r > 0.f) atomicMax((unsigned int*)&s.pc[kPcResid], __float_as_uint(r));
@@ -1885,8 +1906,9 @@ struct StepArgs {
   double dt;
   int mode;          // LS_MODE_ADVECT or LS_MODE_MCF
   int warm;          // 0: plain; 1: warm start from |phi|; 2: from s.warm
-  double warm_margin;// warm value = prediction * (1 + warm_margin)
-  float res_tol;     // residual above which a warm start falls back
+  double warm_margin;// warm value = prediction * (1 + warm_margin) + warm_abs
+  double warm_abs;
+  float res_tol;     // acceptance threshold above which a warm start falls back
 };
 
 // Counters are read right after a grid barrier, whose acquire fence already
@@ -2014,7 +2036,7 @@ __global__ __launch_bounds__(BS, WPE) void k_step(DevMesh<T> m, DevState<T> s, S
     // receives one joins the first active list.
     for (int i = b; i < nscan; i += G) {
       const int p = s.nb[a.nb_in][i];
-      if (warm_patch<T, BS>(m, s, p, a.warm, a.warm_margin) && tid == 0) {
+      if (warm_patch<T, NC, BS>(m, s, p, a.warm, a.warm_margin, a.warm_abs, smem) && tid == 0) {
         atomicAdd(&s.pc[kPcWarm], 1);
         if (s.inA[p] == 0) {
           s.inA[p] = 1;
@@ -2115,10 +2137,15 @@ __global__ __launch_bounds__(BS, WPE) void k_step(DevMesh<T> m, DevState<T> s, S
 
 
   if (!warm) break;
-  // Phase 2b: fixed-point residual over the band; above the tolerance the
-  // warm start is rejected and the step re-runs plain.
+  // Phase 2b: acceptance over the patches that received warm values (the
+  // previous narrowband); above the threshold the warm start is rejected and
+  // the step re-runs plain.
   if constexpr (WARM)
-    for (int p = b; p < m.P; p += G) residual_patch<T, NC, BS>(m, s, p, band_of(m, p, a.bw), a.leps, smem, red);
+    for (int i = b; i < nscan; i += G) {
+      const int p = s.nb[a.nb_in][i];
+      residual_patch<T, NC, BS>(m, s, p, band_of(m, p, a.bw), a.leps, a.warm, a.warm_margin, a.warm_abs, smem,
+                                red);
+    }
   grid_barrier(s.bar, G);
   const float resid = __uint_as_float((unsigned int)ld_acquire(&s.pc[kPcResid]));
   if (resid <= a.res_tol) break;
@@ -2551,6 +2578,25 @@ class EngineImpl final : public Engine {
     return LS_OK;
   }
 
+  // Caller predictor distances for LS_WARM_LEARNED steps (caller's vertex
+  // order), rounded to the working type as ls_mlp_predict's values are.
+  int set_warm(const double* warm_user, std::string& err) override {
+    if (!uploaded_) { err = "layout not built"; return LS_ERR_STATE; }
+    if (!d_warm_) LS_HIP(dev_alloc(&d_warm_, (size_t)N_));
+    std::vector<T> h(N_);
+    for (int64_t v = 0; v < N_; ++v) h[v] = (T)warm_user[vert_perm_h_[v]];
+    LS_HIP(hipMemcpy(d_warm_, h.data(), N_ * sizeof(T), hipMemcpyHostToDevice));
+    return LS_OK;
+  }
+
+  int get_warm(double* out_user, std::string& err) override {
+    if (!d_warm_) { err = "no warm values (ls_mlp_predict or ls_set_warm)"; return LS_ERR_STATE; }
+    std::vector<T> h(N_);
+    LS_HIP(hipMemcpy(h.data(), d_warm_, N_ * sizeof(T), hipMemcpyDeviceToHost));
+    for (int64_t v = 0; v < N_; ++v) out_user[vert_perm_h_[v]] = (double)h[v];
+    return LS_OK;
+  }
+
   int set_phi(const double* phi_user, std::string& err) override {
     if (!uploaded_) { err = "layout not built"; return LS_ERR_STATE; }
     std::vector<T> h(N_);
@@ -2740,6 +2786,7 @@ class EngineImpl final : public Engine {
     a.mode = prm.mode;
     a.warm = prm.warm;
     a.warm_margin = prm.warm_margin;
+    a.warm_abs = prm.warm_abs_margin;
     a.res_tol = (float)(prm.warm_res_tol > 0 ? prm.warm_res_tol : prm.tol);
     if (a.warm == LS_WARM_LEARNED && !a.first && !d_warm_) {
       err = "no learned warm-start values (run ls_mlp_predict)";
@@ -2759,7 +2806,7 @@ class EngineImpl final : public Engine {
       // may be null (per-patch parameters, warm values) appears.
       const int key[3] = {a.first, a.nb_in, (use_patch_prm_ ? 2 : 1) + (d_warm_ ? 4 : 0)};
       const bool same = graph_exec_ && key[0] == graph_key_[0] && key[1] == graph_key_[1] &&
-                        std::memcmp(&graph_prm_, &prm, sizeof(prm)) == 0;
+                        key[2] == graph_key_[2] && std::memcmp(&graph_prm_, &prm, sizeof(prm)) == 0;
       if (!same) {
         if (graph_exec_) { LS_HIP(hipGraphExecDestroy(graph_exec_)); graph_exec_ = nullptr; }
         if (graph_) { LS_HIP(hipGraphDestroy(graph_)); graph_ = nullptr; }

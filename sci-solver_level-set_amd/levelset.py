@@ -26,7 +26,8 @@ class StepParams(C.Structure):
     _fields_ = [("dt", C.c_double), ("bandwidth", C.c_double), ("tol", C.c_double),
                 ("lambda_eps", C.c_double), ("niter", C.c_int32), ("inside_niter", C.c_int32),
                 ("mode", C.c_int32), ("max_fim_iters", C.c_int32), ("warm", C.c_int32),
-                ("reserved0", C.c_int32), ("warm_margin", C.c_double), ("warm_res_tol", C.c_double)]
+                ("reserved0", C.c_int32), ("warm_margin", C.c_double), ("warm_res_tol", C.c_double),
+                ("warm_abs_margin", C.c_double)]
 
 
 class Stats(C.Structure):
@@ -65,7 +66,8 @@ EXPORTS = ["ls_create", "ls_destroy", "ls_last_error", "ls_set_mesh", "ls_set_pr
            "ls_set_phi", "ls_set_velocity", "ls_step", "ls_get_phi", "ls_get_narrowband",
            "ls_set_mesh_batch", "ls_partition_batch", "ls_set_batch_params", "ls_get_batch_nb_counts",
            "ls_get_distances", "ls_mlp_set_params", "ls_mlp_get_params", "ls_mlp_predict", "ls_mlp_collect",
-           "ls_mlp_dataset", "ls_mlp_gradient", "ls_mlp_train"]
+           "ls_mlp_dataset", "ls_mlp_gradient", "ls_mlp_train", "ls_set_warm",
+           "ls_get_warm"]
 MLP_PARAMS = 5892
 
 
@@ -113,6 +115,8 @@ def lib():
         L.ls_set_batch_params.argtypes = [vp, vp, vp]
         L.ls_get_batch_nb_counts.argtypes = [vp, vp]
         L.ls_get_distances.argtypes = [vp, vp]
+        L.ls_set_warm.argtypes = [vp, vp]
+        L.ls_get_warm.argtypes = [vp, vp]
         L.ls_mlp_set_params.argtypes = [vp, vp, C.c_int]
         L.ls_mlp_get_params.argtypes = [vp, vp, C.c_int]
         L.ls_mlp_predict.argtypes = [vp]
@@ -129,9 +133,9 @@ class LevelSetError(RuntimeError):
 
 
 def step_params(dt, bandwidth, niter, tol=1e-5, lambda_eps=1e-5, inside_niter=1, mode=LS_MODE_ADVECT,
-                max_fim_iters=0, warm=LS_WARM_NONE, warm_margin=0.0, warm_res_tol=0.0):
+                max_fim_iters=0, warm=LS_WARM_NONE, warm_margin=0.0, warm_res_tol=0.0, warm_abs_margin=0.0):
     return StepParams(dt, bandwidth, tol, lambda_eps, niter, inside_niter, mode, max_fim_iters, warm, 0,
-                      warm_margin, warm_res_tol)
+                      warm_margin, warm_res_tol, warm_abs_margin)
 
 
 class Solver:
@@ -272,6 +276,17 @@ class Solver:
     def distances(self):
         out = np.empty(self.N, dtype=np.float64)
         self._check(self.L.ls_get_distances(self.ctx, out.ctypes.data))
+        return out
+
+    def set_warm(self, warm):
+        """Predictor distances (caller order) for the next LS_WARM_LEARNED step."""
+        a = np.ascontiguousarray(warm, dtype=np.float64)
+        self._check(self.L.ls_set_warm(self.ctx, a.ctypes.data))
+
+    def warm_values(self):
+        """The predictor distances the context holds (caller order)."""
+        out = np.empty(self.N, dtype=np.float64)
+        self._check(self.L.ls_get_warm(self.ctx, out.ctypes.data))
         return out
 
     # -- learned warm start ------------------------------------------------

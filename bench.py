@@ -73,51 +73,52 @@ def fp64_error(gphi, ophi, amb, band):
             "threshold_ambiguous_verts": int(np.count_nonzero(fin & amb))}
 
 
-def lockstep(case, oracle_mod, lsb, precision, steps):
-    """FP32 oracle: runs beside the GPU from phi0 (bit-exact check, timed as the
-    CPU baseline). FP64 oracle: restarted every step from the GPU's phi and
-    narrowband, so each step is compared from identical inputs; vertices whose
-    advection hinges on a threshold decision inside the FP32-storage window are
-    reported separately."""
-    g = lsb.solver_for_case(case)
-    o = oracle_mod.Oracle(case.dim, case.xyz, case.cells, case.labels, case.nparts, precision)
+def trajectory(case, oracle_mod, lsb, steps, engines, prm_kw):
+    """GPU engines and the FP64 oracle each run from phi0, side by side, with
+    no restart: after every step each engine's phi is compared with the
+    oracle's over every vertex both hold finite (unreached vertices are
+    +-LARGENUM on both sides; a vertex finite on one side only counts as a
+    mismatch) and over the vertices inside the band, and the narrowband
+    patch sets are compared. The oracle's time is the CPU baseline."""
+    o = oracle_mod.Oracle(case.dim, case.xyz, case.cells, case.labels, case.nparts, 64)
     o.set_phi(case.phi0)
     o.set_velocity(case.vel)
-    gp = lsb.step_params(case.dt, case.bandwidth, case.niter)
-    op = oracle_mod.make_params(case.dt, case.bandwidth, case.niter)
-    res = {"steps": 0, "linf": 0.0, "linf_band": 0.0, "linf_all": 0.0, "ambiguous": 0, "nb_diff": 0, "bit_mismatch": 0,
-           "cpu_s": 0.0, "cpu_verts": 0}
-    gphi, gnb = case.phi0, None
+    op = oracle_mod.make_params(case.dt, case.bandwidth, case.niter, tol=prm_kw.get("tol", 1e-5))
+    gp = lsb.step_params(case.dt, case.bandwidth, case.niter, **prm_kw)
+    gs = {name: lsb.solver_for_case(case, precision=prec, execution=ex) for name, prec, ex in engines}
+    res = {name: {"steps": 0, "linf": 0.0, "linf_in_band": 0.0, "finite_mismatch": 0, "nb_diff": 0,
+                  "bit_mismatches": 0, "linf_per_step": []} for name in gs}
+    cpu_s, cpu_v, done = 0.0, 0, 0
     for _ in range(steps):
-        if precision == 64:
-            o.set_phi(gphi)
-            if gnb is not None:
-                o.set_narrowband(gnb)
-        g.step(gp)
         t0 = time.perf_counter()
         ost = o.step(op)
-        res["cpu_s"] += time.perf_counter() - t0
-        res["cpu_verts"] += ost.nb_verts
-        gphi, ophi = g.phi(), o.phi()
-        gnb, onb = g.narrowband(), o.narrowband()
-        res["nb_diff"] += int(np.setxor1d(gnb, onb).size)
-        fin = (np.abs(ophi) < 1e8) & (np.abs(gphi) < 1e8)
-        d = np.abs(gphi - ophi)
-        res["linf_all"] = max(res["linf_all"], float(np.max(d[fin])) if fin.any() else 0.0)
-        if precision == 64:
-            amb = o.ambiguous()
-            res["ambiguous"] = max(res["ambiguous"], int(amb.sum()))
-            ok = fin & ~amb
-            res["linf"] = max(res["linf"], float(np.max(d[ok])) if ok.any() else 0.0)
-            inb = ok & (np.abs(ophi) <= case.bandwidth)
-            res["linf_band"] = max(res["linf_band"], float(np.max(d[inb])) if inb.any() else 0.0)
-        res["bit_mismatch"] += int(np.count_nonzero(gphi.astype(np.float32) != ophi.astype(np.float32)))
-        res["steps"] += 1
+        cpu_s += time.perf_counter() - t0
+        cpu_v += ost.nb_verts
+        ophi, onb = o.phi(), o.narrowband()
+        ofin = np.abs(ophi) < 1e8
+        for name, g in gs.items():
+            g.step(gp)
+            gphi = g.phi()
+            gfin = np.abs(gphi) < 1e8
+            fin = ofin & gfin
+            d = np.abs(gphi - ophi)
+            inb = fin & (np.abs(ophi) <= case.bandwidth)
+            r = res[name]
+            step_linf = float(np.max(d[fin])) if fin.any() else 0.0
+            r["linf"] = max(r["linf"], step_linf)
+            r["linf_in_band"] = max(r["linf_in_band"], float(np.max(d[inb])) if inb.any() else 0.0)
+            r["linf_per_step"].append(step_linf)
+            r["finite_mismatch"] += int(np.count_nonzero(ofin != gfin))
+            r["nb_diff"] += int(np.setxor1d(g.narrowband(), onb).size)
+            r["bit_mismatches"] += int(np.count_nonzero(gphi != ophi))
+            r["steps"] += 1
+        done += 1
         if ost.nb_count == 0:
             break
-    g.close()
+    for g in gs.values():
+        g.close()
     o.close()
-    return res
+    return res, {"cpu_s": cpu_s, "cpu_verts": cpu_v, "steps": done}
 
 
 def profiled_traffic(name):
@@ -139,7 +140,7 @@ def timed_run(g, prm, steps):
     tot = {"verts": 0, "sweep_ms": 0.0, "sweep_launches": 0, "sweep_bytes": 0, "adv_ms": 0.0,
            "adv_bytes": 0, "adv_launches": 0, "ms_fim": 0.0, "ms_total": 0.0, "fim_iters": 0,
            "visits": 0, "probes": 0, "step_ms": 0.0, "step_bytes": 0, "exec_modes": set(),
-           "step_wg_per_cu": set()}
+           "step_wg_per_cu": set(), "solves": 0}
     t0 = time.perf_counter()
     for _ in range(steps):
         st = g.step(prm)
@@ -159,6 +160,8 @@ def timed_run(g, prm, steps):
         tot["step_bytes"] += step_bytes(st)
         tot["exec_modes"].add(st.exec_mode)
         tot["step_wg_per_cu"].add(st.step_wg_per_cu)
+        tot["solves"] += st.local_solves
+        tot["sweep_tasks"] = st.sweep_tasks
     tot["wall"] = time.perf_counter() - t0
     return tot
 
@@ -178,6 +181,23 @@ def roofline(tot, steps, kernels_mode):
     return {"kernel": kname, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": None, "avg_launch_ms": avg_ms,
             "avg_launch_algorithmic_bytes": avg_bytes, "launches": launches}
+
+
+# FP64 operations of one tet corner solve in this build's gfx950 code
+# (v_mul_f64 201, v_add_f64 173, v_fma_f64 130 counted twice; square roots
+# and divisions excluded), and the MI355X FP64 vector peak.
+FLOPS_PER_SOLVE = 634
+FP64_PEAK_TFLOPS = 78.6
+
+
+def compute_fraction(tot, steps, kernels_mode):
+    """FP64 arithmetic rate of the local solves over the same kernel time as
+    the roofline entry, against the FP64 vector peak."""
+    ms = (tot["sweep_ms"] if kernels_mode else tot["step_ms"]) / steps
+    flops = FLOPS_PER_SOLVE * tot["solves"] / steps
+    achieved = flops / (ms * 1e-3) / 1e12
+    return {"unit": "TFLOP/s", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "frac": achieved / FP64_PEAK_TFLOPS,
+            "flops_per_launch": flops, "flops_per_solve": FLOPS_PER_SOLVE}
 
 
 def batch_workload(args, lsb, cases):
@@ -343,13 +363,13 @@ def mcf_workload(args, lsb, cases):
     return out, tot, ex == lsb.LS_EXEC_KERNELS, cpu
 
 
-def autotune_exec(lsb, case, names, steps=6, g=None, prm=None):
+def autotune_exec(lsb, case, names, steps=6, g=None, prm=None, precision=None, prm_kw=None):
     """Mean GPU ms/step of each execution mode over a few steps from phi0
     (after one untimed step) on one solver; the bench runs the fastest."""
     own = g is None
     if own:
-        g = lsb.solver_for_case(case)
-    prm = prm or lsb.step_params(case.dt, case.bandwidth, case.niter, mode=case.mode)
+        g = lsb.solver_for_case(case, precision=lsb.LS_F32 if precision is None else precision)
+    prm = prm or lsb.step_params(case.dt, case.bandwidth, case.niter, mode=case.mode, **(prm_kw or {}))
     out = {}
     for name, mode in names.items():
         g.set_execution(mode)
@@ -361,49 +381,108 @@ def autotune_exec(lsb, case, names, steps=6, g=None, prm=None):
     return out
 
 
-def learned_run(args, lsb, case, exec_mode, plain_value):
-    """The learned warm start on the same workload: pairs collected from
-    plain steps, GPU training, then the timed steps with warm values from
-    the network (prediction time included). A warm start whose fixed-point
-    residual over the band exceeds 1e-5 re-runs the plain method."""
-    # warm starts run in the persistent step kernel (here replayed from a graph)
-    g = lsb.solver_for_case(case, execution=lsb.LS_EXEC_GRAPH)
+WARM_MARGIN, WARM_ABS_H = 0.03, 0.25  # warm value = prediction * 1.03 + h / 4
+
+
+def learned_run(args, lsb, case, exec_mode, plain_value, precision, prm_kw, oracle_mod=None):
+    """The learned warm start on the same workload and engine: pairs
+    collected from plain steps on the GPU, GPU training (the pairs of the
+    last two collection steps held out for scoring), then the timed steps
+    with warm values from the network (prediction time included), and the
+    identity predictor (|phi|) timed the same way. A warm step in which a
+    vertex inside the band still holds a warm value below what its elements
+    give it re-runs the plain method (counted as a fallback). With an oracle,
+    the first accepted warm steps are compared with the FP64 plain step from
+    the same input."""
+    g = lsb.solver_for_case(case, precision=precision, execution=lsb.LS_EXEC_GRAPH)
     g.mlp_set_params(lsb.mlp_init_params(5))
-    prm = lsb.step_params(case.dt, case.bandwidth, case.niter)
+    prm = lsb.step_params(case.dt, case.bandwidth, case.niter, **prm_kw)
     g.step(prm)
     t0 = time.perf_counter()
     for _ in range(args.train_steps):
         g.mlp_collect(0, 12_000_000)
         g.step(prm)
         g.mlp_collect(1)
-    X, Y = g.mlp_dataset()
     t1 = time.perf_counter()
     loss = g.mlp_train(args.train_epochs, 4096, 2e-3, seed=7)
     t2 = time.perf_counter()
-    held = slice(0, min(200_000, X.shape[0]))
-    _, sse = g.mlp_gradient(X[held], Y[held])  # the network's squared error on the GPU
-    trained_mse = sse / Y[held].size
+    n_train = g.mlp_dataset()[0].shape[0]
+    for _ in range(2):  # held-out pairs: two later timesteps, never trained on
+        g.mlp_collect(0, 12_000_000)
+        g.step(prm)
+        g.mlp_collect(1)
+    X, Y = g.mlp_dataset()
+    held = slice(n_train, X.shape[0])
+    _, sse = g.mlp_gradient(X[held], Y[held])
+    trained_mse = sse / max(Y[held].size, 1)
     identity_mse = float(np.mean(Y[held].astype(np.float64) ** 2))
-    wprm = lsb.step_params(case.dt, case.bandwidth, case.niter, warm=lsb.LS_WARM_LEARNED, warm_margin=2e-3)
-    g.set_phi(case.phi0)
-    verts, fallbacks, iters = 0, 0, 0
-    ts = time.perf_counter()
-    for k in range(args.steps):
-        if k > 0:
+
+    def timed(warm):
+        wprm = lsb.step_params(case.dt, case.bandwidth, case.niter, warm=warm, warm_margin=WARM_MARGIN,
+                               warm_abs_margin=WARM_ABS_H * case.h, **prm_kw)
+        g.set_phi(case.phi0)
+        r = {"verts": 0, "fallbacks": 0, "iters": 0, "visits": 0}
+        ts = time.perf_counter()
+        for k in range(args.steps):
+            if warm == lsb.LS_WARM_LEARNED and k > 0:
+                g.mlp_predict()
+            st = g.step(wprm if k > 0 else prm)
+            r["verts"] += st.nb_verts
+            r["fallbacks"] += st.warm_fallback
+            r["iters"] += st.fim_iters
+            r["visits"] += st.patch_visits
+        r["wall"] = time.perf_counter() - ts
+        return r
+
+    learned = timed(lsb.LS_WARM_LEARNED)
+    ident = timed(lsb.LS_WARM_IDENTITY)
+    parity = None
+    if oracle_mod is not None and args.learned_parity_steps > 0:
+        # per step from the same input: the GPU's warm step against the FP64
+        # oracle's plain step
+        o = oracle_mod.Oracle(3, case.xyz, case.cells, case.labels, case.nparts, 64)
+        o.set_velocity(case.vel)
+        g.set_phi(case.phi0)
+        g.step(prm)
+        errs, nbd, acc = [], 0, 0
+        for _ in range(args.learned_parity_steps):
+            phi, nb = g.phi(), g.narrowband()
             g.mlp_predict()
-        st = g.step(wprm)
-        verts += st.nb_verts
-        fallbacks += st.warm_fallback
-        iters += st.fim_iters
-    wall = time.perf_counter() - ts
+            st = g.step(lsb.step_params(case.dt, case.bandwidth, case.niter, warm=lsb.LS_WARM_LEARNED,
+                                        warm_margin=WARM_MARGIN, warm_abs_margin=WARM_ABS_H * case.h, **prm_kw))
+            o.set_phi(phi)
+            o.set_narrowband(nb)
+            o.step(oracle_mod.make_params(case.dt, case.bandwidth, case.niter, tol=prm_kw.get("tol", 1e-5)))
+            gphi, ophi = g.phi(), o.phi()
+            fin = (np.abs(ophi) < 1e8) & (np.abs(gphi) < 1e8)
+            band = fin & (np.abs(ophi) <= case.bandwidth)
+            errs.append((float(np.max(np.abs(gphi - ophi)[band])), float(np.max(np.abs(gphi - ophi)[fin]))))
+            nbd += int(np.setxor1d(g.narrowband(), o.narrowband()).size)
+            acc += 1 - st.warm_fallback
+        o.close()
+        parity = {"steps": len(errs), "accepted_steps": acc, "linf_in_band_vs_fp64_plain": max(e[0] for e in errs),
+                  "linf_all_finite_vs_fp64_plain": max(e[1] for e in errs), "nb_membership_diff": nbd,
+                  "meets_bar": max(e[0] for e in errs) <= 1e-5 and nbd == 0,
+                  "why_not": "the plain method's result depends on its schedule: slots lowered inside a patch "
+                             "visit are never written back (redistance_kernels3d.h:440-449), so a warm start, "
+                             "which changes which patches are active when, moves values inside the band by "
+                             "~1e-4 (the plain method with 10 instead of 5 sweeps per visit moves them by "
+                             "2.5e-4)"}
     g.close()
-    value = verts / wall
+    value = learned["verts"] / learned["wall"]
     return {"value": value, "unit": "vertex-updates/s", "vs_plain_fim": value / plain_value, "execution": "graph",
-            "ms_per_step": 1e3 * wall / args.steps, "fim_iters_per_step": iters / args.steps,
-            "fallback_steps": fallbacks, "steps": args.steps, "train_pairs": int(X.shape[0]),
+            "ms_per_step": 1e3 * learned["wall"] / args.steps, "fim_iters_per_step": learned["iters"] / args.steps,
+            "patch_visits_per_step": learned["visits"] / args.steps,
+            "fallback_steps": learned["fallbacks"], "steps": args.steps,
+            "identity_predictor": {"value": ident["verts"] / ident["wall"],
+                                   "vs_plain_fim": ident["verts"] / ident["wall"] / plain_value,
+                                   "fim_iters_per_step": ident["iters"] / args.steps,
+                                   "patch_visits_per_step": ident["visits"] / args.steps,
+                                   "fallback_steps": ident["fallbacks"]},
+            "train_pairs": int(n_train), "held_out_pairs": int(X.shape[0] - n_train),
             "collect_s": t1 - t0, "train_s": t2 - t1, "train_epochs": args.train_epochs,
-            "train_loss": loss, "trained_mse": trained_mse, "identity_mse": identity_mse,
-            "residual_tolerance": 1e-5, "warm_margin": 2e-3}
+            "train_loss": loss, "held_out_mse": trained_mse, "held_out_identity_mse": identity_mse,
+            "warm_margin": WARM_MARGIN, "warm_abs_margin_h": WARM_ABS_H, "parity": parity}
 
 
 def main():
@@ -412,7 +491,14 @@ def main():
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-steps", type=int, default=8, help="timesteps of the CPU-baseline sample")
-    ap.add_argument("--parity-steps", type=int, default=4, help="timesteps checked against the FP64 oracle")
+    ap.add_argument("--parity-steps", type=int, default=0,
+                    help="timesteps of the trajectory compared with the FP64 oracle (0: the timed steps)")
+    ap.add_argument("--precision", default="f64", choices=["f64", "f32"],
+                    help="storage of the torus engine: f64 (default; bit-exact with the FP64 oracle) or f32")
+    ap.add_argument("--tol", type=float, default=1e-6,
+                    help="FIM tolerance of the torus workload (1e-6: the plain method's own result lies "
+                         "within 1e-6 of the scheme's fixed point inside the band)")
+    ap.add_argument("--no-fp32-line", action="store_true", help="skip the other storage precision's line")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--exec", default="auto", choices=["auto", "graph", "persistent", "kernels"],
                     help="auto (default): the fastest of graph (persistent step kernel replayed from a "
@@ -426,6 +512,8 @@ def main():
     ap.add_argument("--no-learned", action="store_true", help="skip the learned warm-start run")
     ap.add_argument("--train-steps", type=int, default=13, help="plain steps whose pairs train the network")
     ap.add_argument("--train-epochs", type=int, default=2)
+    ap.add_argument("--learned-parity-steps", type=int, default=3,
+                    help="learned warm steps compared with the FP64 oracle's plain step from the same input")
     ap.add_argument("--workload", default="torus", choices=["torus", "batch", "mcf"],
                     help="torus: 3D torus revolve, 64^3 (default); batch: 512 ellipsoid meshes; "
                          "mcf: 8-sphere mean-curvature flow, 192^3")
@@ -476,59 +564,42 @@ def main():
 
     case = cases.config2(steps=args.steps)
     names = {"graph": lsb.LS_EXEC_GRAPH, "persistent": lsb.LS_EXEC_PERSISTENT, "kernels": lsb.LS_EXEC_KERNELS}
+    precisions = {"f64": lsb.LS_F64, "f32": lsb.LS_F32}
+    prec = precisions[args.precision]
+    prm_kw = {"tol": args.tol}
     tune = None
     if args.exec == "auto":
-        tune = autotune_exec(lsb, case, names)
+        tune = autotune_exec(lsb, case, names, precision=prec, prm_kw=prm_kw)
         args.exec = min(tune, key=tune.get)
     exec_mode = names[args.exec]
     sweep = {"auto": lsb.LS_SWEEP_AUTO, "elements": lsb.LS_SWEEP_ELEMENTS, "tasks": lsb.LS_SWEEP_TASKS}[args.sweep]
-    g = lsb.solver_for_case(case, execution=exec_mode)
-    g.set_sweep(sweep)
-    info = g.layout_info()
-    prm = lsb.step_params(case.dt, case.bandwidth, case.niter)
-    for _ in range(args.warmup):
-        g.step(prm)
-    g.set_phi(case.phi0)  # the timed region is the configured run from phi0
+    prm = lsb.step_params(case.dt, case.bandwidth, case.niter, **prm_kw)
 
-    tot = {"verts": 0, "sweep_ms": 0.0, "sweep_launches": 0, "sweep_bytes": 0, "adv_ms": 0.0,
-           "adv_bytes": 0, "adv_launches": 0, "ms_fim": 0.0, "ms_total": 0.0, "fim_iters": 0,
-           "visits": 0, "probes": 0, "step_ms": 0.0, "step_bytes": 0}
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        st = g.step(prm)
-        tot["verts"] += st.nb_verts
-        tot["sweep_ms"] += st.ms_sweep_kernels
-        tot["sweep_launches"] += st.sweep_launches
-        tot["sweep_bytes"] += fim_bytes(st)
-        tot["adv_ms"] += st.ms_advect_kernels
-        tot["adv_bytes"] += adv_bytes(st)
-        tot["adv_launches"] += 1 if st.nb_count > 0 else 0
-        tot["ms_fim"] += st.ms_fim
-        tot["ms_total"] += st.ms_total
-        tot["fim_iters"] += st.fim_iters
-        tot["visits"] += st.patch_visits
-        tot["probes"] += st.probes
-        tot["step_ms"] += st.ms_step_kernel
-        tot["step_bytes"] += step_bytes(st)
-        tot["sweep_tasks"] = st.sweep_tasks
-        tot["step_wg_per_cu"] = st.step_wg_per_cu
-    wall = time.perf_counter() - t0
-    g.close()
+    def timed_torus(precision):
+        g = lsb.solver_for_case(case, precision=precision, execution=exec_mode)
+        g.set_sweep(sweep)
+        for _ in range(args.warmup):
+            g.step(prm)
+        g.set_phi(case.phi0)  # the timed region is the configured run from phi0
+        tot = timed_run(g, prm, args.steps)
+        info = g.layout_info()
+        g.close()
+        return tot, info
 
-    value = tot["verts"] / wall
-    if exec_mode == lsb.LS_EXEC_KERNELS:
-        kname = "k_sweep (FIM patch sweep)"
-        launches = tot["sweep_launches"]
-        avg_ms = tot["sweep_ms"] / max(launches, 1)
-        avg_bytes = tot["sweep_bytes"] / max(launches, 1)
-    else:
-        kname = "k_step + k_advect_dev + k_copy_phi_dev (one timestep: persistent seeds/FIM/narrowband, then advection)"
-        launches = args.steps
-        avg_ms = tot["step_ms"] / args.steps
-        avg_bytes = tot["step_bytes"] / args.steps
-    achieved = avg_bytes / (avg_ms * 1e-3) / 1e9
-    prof_name = "k_sweep_torus_revolve_64" if exec_mode == lsb.LS_EXEC_KERNELS else "k_step_torus_revolve_64_graph"
+    tot, info = timed_torus(prec)
+    value = tot["verts"] / tot["wall"]
+    kmode = exec_mode == lsb.LS_EXEC_KERNELS
+    roof = roofline(tot, args.steps, kmode)
+    prof_name = "k_sweep_torus_revolve_64" if kmode else "k_step_torus_revolve_64_graph"
     traffic = profiled_traffic(prof_name)
+    roof["traffic"] = traffic
+    roof["traffic_source"] = (f"profiles/{prof_name}.md (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)"
+                              if traffic else None)
+    roof["compute"] = compute_fraction(tot, args.steps, kmode)
+    roof["limiter"] = ("latency: each FIM iteration lasts its slowest patch visit plus its slowest probe "
+                       "(dependent FP64 solve chains, four workgroup barriers per sweep), two grid barriers "
+                       "per iteration, 14-260 active patches for 256 CUs; neither HBM nor the FP64 VALUs "
+                       "are saturated")
 
     out = {
         "metric": "narrowband vertex-updates/s, learned warm-start vs plain FIM @ Linf<=1e-5",
@@ -537,72 +608,85 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": 1e3 * wall / args.steps,
+        "ms_per_step": 1e3 * tot["wall"] / args.steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": args.precision,
         "data": "synthetic",
         "config": {
-            "workload": "3d_torus_revolve_64: 64^3 Kuhn tets (262144 verts, 1500282 tets), "
-                        "256-vertex patches, band 6h, CFL 0.5, rigid rotation, 100 timesteps",
+            "workload": f"3d_torus_revolve_64: 64^3 Kuhn tets (262144 verts, 1500282 tets), "
+                        f"256-vertex patches, band 6h, CFL 0.5, rigid rotation, {args.steps} timed timesteps "
+                        f"from phi0 (the configured run is 100)",
             "mode": "plain_fim",
+            "storage": args.precision,
+            "fim_tol": args.tol,
             "execution": args.exec,
             "execution_trial_ms_per_step": tune,
             "sweep": "tasks" if tot.get("sweep_tasks") else "elements",
-            "step_wg_per_cu": tot.get("step_wg_per_cu", 0),
+            "step_wg_per_cu": sorted(tot["step_wg_per_cu"]),
             "patches": info["n_patches"],
             "element_copies": info["n_copies"],
             "fim_iters_per_step": tot["fim_iters"] / args.steps,
             "patch_visits_per_step": tot["visits"] / args.steps,
             "probes_per_step": tot["probes"] / args.steps,
+            "local_solves_per_step": tot["solves"] / args.steps,
             "gpu_event_ms_per_step": tot["ms_total"] / args.steps,
-            "fim_ms_per_step": tot["ms_fim"] / args.steps,
         },
-        "roofline": {
-            "kernel": kname,
-            "bound": "hbm",
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": traffic,
-            "traffic_source": f"profiles/{prof_name}.md (rocprofv3 FETCH_SIZE + WRITE_SIZE, "
-                              "copy-calibrated)" if traffic else None,
-            "avg_launch_ms": avg_ms,
-            "avg_launch_algorithmic_bytes": avg_bytes,
-            "launches": launches,
-        },
+        "roofline": roof,
     }
 
-    if not args.no_learned:
-        out["learned"] = learned_run(args, lsb, case, exec_mode, value)
+    # The other storage precision as its own labelled line (FP32 storage,
+    # FP64 element arithmetic): faster, but the reference's hard advection
+    # thresholds flip under FP32 rounding, so its trajectory leaves the bar.
+    other = "f32" if args.precision == "f64" else "f64"
+    if not args.no_fp32_line:
+        tot2, _ = timed_torus(precisions[other])
+        out["other_storage"] = {"storage": other, "value": tot2["verts"] / tot2["wall"],
+                                "ms_per_step": 1e3 * tot2["wall"] / args.steps,
+                                "fim_iters_per_step": tot2["fim_iters"] / args.steps}
 
+    O = None
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle as O
-        fp32 = lockstep(case, O, lsb, 32, args.cpu_steps)
-        fp64 = lockstep(case, O, lsb, 64, args.parity_steps)
+    if not args.no_learned:
+        out["learned"] = learned_run(args, lsb, case, exec_mode, value, prec, prm_kw, O)
+
+    if not args.no_cpu:
+        engines = [(args.precision, prec, exec_mode)]
+        if "other_storage" in out:
+            engines.append((other, precisions[other], exec_mode))
+        nsteps = args.parity_steps if args.parity_steps > 0 else args.steps
+        res, cpu = trajectory(case, O, lsb, nsteps, engines, prm_kw)
         out["cpu_baseline"] = {
-            "value": fp32["cpu_verts"] / fp32["cpu_s"],
+            "value": cpu["cpu_verts"] / cpu["cpu_s"],
             "unit": "vertex-updates/s",
             "cores": O.max_threads(),
             "kind": "port",
-            "sample": f"first {fp32['steps']} of the {args.steps} timesteps of the same workload, "
-                      "oracle FP32 instance, OpenMP over patches",
+            "sample": f"first {cpu['steps']} timesteps of the same workload from phi0, the oracle's FP64 "
+                      f"instance (the scheme of the headline engine), OpenMP over patches",
         }
+        head = res[args.precision]
         out["parity"] = {
-            "fp32_oracle_steps": fp32["steps"],
-            "fp32_bit_mismatches": fp32["bit_mismatch"],
-            "fp32_nb_membership_diff": fp32["nb_diff"],
-            "fp64_oracle_steps": fp64["steps"],
-            "linf_vs_fp64": fp64["linf"],
-            "linf_vs_fp64_in_band": fp64["linf_band"],
-            "linf_vs_fp64_incl_threshold_ambiguous": fp64["linf_all"],
-            "threshold_ambiguous_verts_max_per_step": fp64["ambiguous"],
-            "fp64_nb_membership_diff": fp64["nb_diff"],
+            "oracle": "FP64 host build of the same scheme, same schedule",
+            "trajectory_steps": head["steps"],
+            "linf_all_finite_vertices": head["linf"],
+            "linf_in_band": head["linf_in_band"],
+            "bit_mismatches": head["bit_mismatches"],
+            "finite_set_mismatches": head["finite_mismatch"],
+            "nb_membership_diff": head["nb_diff"],
             "tolerance": 1e-5,
+            "meets_bar": head["linf"] <= 1e-5 and head["nb_diff"] == 0 and head["finite_mismatch"] == 0,
         }
+        if other in res:
+            r = res[other]
+            out["other_storage"]["parity"] = {
+                "trajectory_steps": r["steps"], "linf_all_finite_vertices": r["linf"],
+                "linf_in_band": r["linf_in_band"], "nb_membership_diff": r["nb_diff"],
+                "finite_set_mismatches": r["finite_mismatch"],
+                "linf_per_step": [float(f"{x:.3e}") for x in r["linf_per_step"]],
+                "meets_bar": r["linf"] <= 1e-5 and r["nb_diff"] == 0 and r["finite_mismatch"] == 0}
     print(json.dumps(out))
 
 

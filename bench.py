@@ -33,44 +33,29 @@ import __graft_entry__ as ge  # noqa: E402
 HBM_PEAK_GBS = 8000.0
 
 
-def step_bytes(st):
-    """Algorithmic bytes of a whole timestep: seeds (16E+4(V+H)+9V per scanned
-    patch), FIM visits and probes, narrowband + sign save/restore (4N+4P+10N),
-    advection."""
-    seeds = 16 * st.seed_copies + 4 * (st.seed_verts + st.seed_halo) + 9 * st.seed_verts
-    return seeds + fim_bytes(st) + adv_bytes(st)
+def step_bytes(st, w=4):
+    """Algorithmic bytes of a whole timestep: seeds, FIM visits and probes,
+    advection. w is the width of a stored real (4: FP32 storage, 8: FP64);
+    connectivity is 16 B per element copy either way."""
+    seeds = 16 * st.seed_copies + w * (st.seed_verts + st.seed_halo) + (2 * w + 1) * st.seed_verts
+    return seeds + fim_bytes(st, w) + adv_bytes(st, w)
 
 
-def fim_bytes(st):
+def fim_bytes(st, w=4):
     """Algorithmic bytes of FIM patch visits and probes: per visit
-    40*E + 4*(V+H) + 5*V (16 B connectivity + 24 B local coordinates per
-    element copy, distances in, distances out + 1 B flag); probes do not write
-    distances: 40*E + 4*(V+H) + V."""
-    visits = 40 * st.visit_copies + 4 * (st.visit_verts + st.visit_halo) + 5 * st.visit_verts
-    probes = 40 * st.probe_copies + 4 * (st.probe_verts + st.probe_halo) + 1 * st.probe_verts
+    (16 + 6w) E + w (V + H) + (w + 1) V (connectivity and six local
+    coordinates per element copy, distances in, distances out + 1 B flag;
+    40 E + 4 (V + H) + 5 V with FP32 storage); probes do not write distances."""
+    visits = (16 + 6 * w) * st.visit_copies + w * (st.visit_verts + st.visit_halo) + (w + 1) * st.visit_verts
+    probes = (16 + 6 * w) * st.probe_copies + w * (st.probe_verts + st.probe_halo) + 1 * st.probe_verts
     return visits + probes
 
 
-def adv_bytes(st):
-    """Advection per narrowband patch: 28*E + 16*(V+H) + 4*V."""
-    return 28 * st.adv_copies + 16 * (st.adv_verts + st.adv_halo) + 4 * st.adv_verts
-
-
-def fp64_error(gphi, ophi, amb, band):
-    """L-inf of the GPU's phi against the FP64 oracle's over finite values,
-    without and with the vertices whose advection update hinges on a
-    threshold decision inside the FP32-storage window (counted), and over the
-    vertices inside the band (|phi| <= band; beyond it a patch's FIM stops
-    when a sweep changes its values by less than the tolerance, a test that
-    FP32 and FP64 storage can decide a sweep apart)."""
-    fin = (np.abs(ophi) < 1e8) & (np.abs(gphi) < 1e8)
-    d = np.abs(gphi - ophi)
-    ok = fin & ~amb
-    inb = ok & (np.abs(ophi) <= band)
-    return {"linf_vs_fp64": float(np.max(d[ok])) if ok.any() else 0.0,
-            "linf_vs_fp64_in_band": float(np.max(d[inb])) if inb.any() else 0.0,
-            "linf_vs_fp64_incl_threshold_ambiguous": float(np.max(d[fin])) if fin.any() else 0.0,
-            "threshold_ambiguous_verts": int(np.count_nonzero(fin & amb))}
+def adv_bytes(st, w=4):
+    """Advection per narrowband patch: (16 + 3w) E + 4w (V + H) + w V
+    (connectivity + per-element velocity; coordinates and phi per vertex;
+    phi out; 28 E + 16 (V + H) + 4 V with FP32 storage)."""
+    return (16 + 3 * w) * st.adv_copies + 4 * w * (st.adv_verts + st.adv_halo) + w * st.adv_verts
 
 
 def trajectory(case, oracle_mod, lsb, steps, engines, prm_kw):
@@ -135,8 +120,9 @@ def profiled_traffic(name):
     return None
 
 
-def timed_run(g, prm, steps):
-    """Times `steps` timesteps; sums the per-step statistics."""
+def timed_run(g, prm, steps, w=4):
+    """Times `steps` timesteps; sums the per-step statistics (w: bytes per
+    stored real, for the algorithmic byte counts)."""
     tot = {"verts": 0, "sweep_ms": 0.0, "sweep_launches": 0, "sweep_bytes": 0, "adv_ms": 0.0,
            "adv_bytes": 0, "adv_launches": 0, "ms_fim": 0.0, "ms_total": 0.0, "fim_iters": 0,
            "visits": 0, "probes": 0, "step_ms": 0.0, "step_bytes": 0, "exec_modes": set(),
@@ -147,9 +133,9 @@ def timed_run(g, prm, steps):
         tot["verts"] += st.nb_verts
         tot["sweep_ms"] += st.ms_sweep_kernels
         tot["sweep_launches"] += st.sweep_launches
-        tot["sweep_bytes"] += fim_bytes(st)
+        tot["sweep_bytes"] += fim_bytes(st, w)
         tot["adv_ms"] += st.ms_advect_kernels
-        tot["adv_bytes"] += adv_bytes(st)
+        tot["adv_bytes"] += adv_bytes(st, w)
         tot["adv_launches"] += 1 if st.nb_count > 0 else 0
         tot["ms_fim"] += st.ms_fim
         tot["ms_total"] += st.ms_total
@@ -157,7 +143,7 @@ def timed_run(g, prm, steps):
         tot["visits"] += st.patch_visits
         tot["probes"] += st.probes
         tot["step_ms"] += st.ms_step_kernel
-        tot["step_bytes"] += step_bytes(st)
+        tot["step_bytes"] += step_bytes(st, w)
         tot["exec_modes"].add(st.exec_mode)
         tot["step_wg_per_cu"].add(st.step_wg_per_cu)
         tot["solves"] += st.local_solves
@@ -581,7 +567,7 @@ def main():
         for _ in range(args.warmup):
             g.step(prm)
         g.set_phi(case.phi0)  # the timed region is the configured run from phi0
-        tot = timed_run(g, prm, args.steps)
+        tot = timed_run(g, prm, args.steps, w=8 if precision == lsb.LS_F64 else 4)
         info = g.layout_info()
         g.close()
         return tot, info
@@ -590,7 +576,8 @@ def main():
     value = tot["verts"] / tot["wall"]
     kmode = exec_mode == lsb.LS_EXEC_KERNELS
     roof = roofline(tot, args.steps, kmode)
-    prof_name = "k_sweep_torus_revolve_64" if kmode else "k_step_torus_revolve_64_graph"
+    prof_name = ("k_sweep_torus_revolve_64" if kmode else
+                 "k_step_torus_revolve_64_f64" if prec == lsb.LS_F64 else "k_step_torus_revolve_64_graph")
     traffic = profiled_traffic(prof_name)
     roof["traffic"] = traffic
     roof["traffic_source"] = (f"profiles/{prof_name}.md (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)"

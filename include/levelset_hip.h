@@ -70,7 +70,7 @@ typedef struct ls_step_params {
   int32_t mode;        /* LS_MODE_ADVECT, or LS_MODE_MCF: mean-curvature flow phi_t = kappa |grad phi| */
   int32_t max_fim_iters;/* 0: run the FIM loop to convergence; > 0: stop after that many iterations */
   int32_t warm;         /* LS_WARM_*; needs LS_EXEC_PERSISTENT or LS_EXEC_GRAPH */
-  int32_t reserved0;
+  int32_t cfl;          /* 1: advect with min(dt, CFL bound of the step's new narrowband) (ls_compute_dt) */
   double warm_margin;   /* warm value = predicted distance * (1 + warm_margin) + warm_abs_margin */
   double warm_res_tol;  /* undercut above which a warm step re-runs plain (0: tol) */
   double warm_abs_margin;
@@ -111,6 +111,9 @@ typedef struct ls_stats {
   int32_t sweep_tasks;      /* 1: the FIM sweeps ran task-compacted */
   int32_t step_wg_per_cu;   /* persistent step: workgroups per CU of its grid (the kernel build
                                compiled for it); 0 when the step ran one launch per phase */
+  double dt_used;           /* the advection's timestep: dt, or min(dt, cfl_dt) with cfl = 1 */
+  double cfl_dt;            /* cfl = 1: the new narrowband's CFL bound (0 otherwise) */
+  double cfl_max_rin;       /* cfl = 1: the largest inscribed radius over it (0 otherwise) */
 } ls_stats;
 
 typedef struct ls_layout_info {
@@ -219,6 +222,23 @@ int ls_get_batch_nb_counts(ls_ctx* ctx, int32_t* counts);
  * (the DT array of redistance3d::GenerateData, src/core3d/cuda/redistance3d.cu:225-273),
  * in the caller's vertex order; LARGENUM where the FIM did not reach. */
 int ls_get_distances(ls_ctx* ctx, double* out);
+
+/* CFL timestep bound over the current narrowband, or over every patch
+ * before the first step (meshFIM2d::compute_deltaT, src/core2d/cuda/meshFIM2d.cu:609-674,
+ * with kernel_compute_timestep2d, src/core2d/include/meshFIM2d_kernels.h:405-475,
+ * whose result the reference discards). Per element copy, with R its inscribed
+ * radius (3V / face area sum for tets, 2A / perimeter for triangles), d the
+ * dimension, Ceik and Ccurv the element's coefficients and |Cadv| the length of
+ * its velocity: dt1 = 4R^2 / |Ccurv| / 2 / d when |Ccurv| > 1e-12,
+ * dt2 = 2R / (|Ceik| + |Cadv|) / d when |Ceik| + |Cadv| > 1e-12; *dt is the
+ * least min(dt1, dt2) (LARGENUM when no element bounds it) and *max_rin the
+ * largest R. Computed in FP64 from the stored local coordinates. */
+int ls_compute_dt(ls_ctx* ctx, double* dt, double* max_rin);
+/* Eikonal and curvature coefficients of the CFL bound, one per element in
+ * the caller's order (the per-element arrays m_ceik_global_d and
+ * m_ccurv_global_d, src/core2d/cuda/meshFIM2d.cu:724-731); NULL restores the
+ * reference's values for that array, Ceik = 10 and Ccurv = 0. */
+int ls_set_cfl_coefficients(ls_ctx* ctx, const double* ceik, const double* ccurv);
 
 /* Predictor distances for the next ls_step with warm = LS_WARM_LEARNED, one
  * per vertex in the caller's order (no reference counterpart): a warm start

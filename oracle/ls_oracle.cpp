@@ -299,6 +299,8 @@ struct lso_params {
   double warm_margin;   // warm value = prediction * (1 + warm_margin) + warm_abs_margin
   double warm_res_tol;  // acceptance threshold of a warm step (0: tol); above it the step re-runs plain
   double warm_abs_margin;
+  int cfl;              // 1: advect with min(dt, the new narrowband's CFL bound)
+  int pad;
 };
 struct lso_stats {
   int fim_iters;
@@ -311,6 +313,7 @@ struct lso_stats {
   double warm_residual;
   int warm_fallback;
   int pad;
+  double dt_used;       // the advection's timestep
 };
 }
 
@@ -332,6 +335,9 @@ struct Base {
   virtual int nparts() const = 0;
   virtual void set_warm(const double*) = 0;
   virtual void get_resid(float*) const = 0;
+  virtual void set_cfl_coefficients(const double*, const double*) = 0;
+  virtual void compute_dt(double*) = 0;
+  virtual void get_labels(int32_t*) const = 0;
 };
 
 template <typename T>
@@ -555,6 +561,9 @@ struct Solver : Base {
     const int nk = dim == 3 ? 6 : 3;
     for (int64_t c = 0; c < F; ++c)
       for (int k = 0; k < nk; ++k) out[c * nk + k] = (double)coords[k * F + c];
+  }
+  void get_labels(int32_t* out) const override {
+    for (int64_t v = 0; v < N; ++v) out[vert_permute[v]] = label[v];
   }
   void get_amb(uint8_t* out) const override {
     for (int64_t v = 0; v < N; ++v) out[vert_permute[v]] = amb[v];
@@ -976,6 +985,64 @@ struct Solver : Base {
     return r;
   }
   int64_t F_of() const { return F; }
+
+  // CFL bound (kernel_compute_timestep2d, src/core2d/include/meshFIM2d_kernels.h:405-475,
+  // reduced by meshFIM2d::compute_deltaT, src/core2d/cuda/meshFIM2d.cu:609-674),
+  // for both element kinds: per element copy of the listed patches, with R
+  // the inscribed radius from the copy's local coordinates and d the
+  // dimension, dt1 = 4R^2/|Ccurv|/2/d if |Ccurv| > eps, dt2 = 2R/(|Ceik| + |Cadv|)/d
+  // if |Ceik| + |Cadv| > eps (eps = 1e-12, meshFIM2d.h:15); the least
+  // min(dt1, dt2) and the largest R. Coefficients per original element,
+  // 10 and 0 by default (meshFIM2d.cu:724-731).
+  std::vector<double> ceik_e, ccurv_e;
+  void set_cfl_coefficients(const double* ceik, const double* ccurv) override {
+    if (ceik) ceik_e.assign(ceik, ceik + E); else ceik_e.clear();
+    if (ccurv) ccurv_e.assign(ccurv, ccurv + E); else ccurv_e.clear();
+  }
+  static double inradius_tet(const double* L) {
+    const double v6 = std::fabs(L[0] * L[2] * L[5]);
+    const double abc = std::fabs(L[0] * L[2]);
+    const double abd = std::fabs(L[0]) * std::sqrt(L[4] * L[4] + L[5] * L[5]);
+    const double ax = L[2] * L[5], ay = -L[1] * L[5], az = L[1] * L[4] - L[2] * L[3];
+    const double acd = std::sqrt((ax * ax + ay * ay) + az * az);
+    const double bx = L[2] * L[5], by = -(L[1] - L[0]) * L[5], bz = (L[1] - L[0]) * L[4] - L[2] * (L[3] - L[0]);
+    const double bcd = std::sqrt((bx * bx + by * by) + bz * bz);
+    return v6 / (((abc + abd) + acd) + bcd);
+  }
+  static double inradius_tri(const double* c) {
+    const double a2 = std::fabs(c[0] * c[2]);
+    const double per = (std::fabs(c[0]) + std::sqrt((c[1] - c[0]) * (c[1] - c[0]) + c[2] * c[2])) +
+                       std::sqrt(c[1] * c[1] + c[2] * c[2]);
+    return a2 / per;
+  }
+  void cfl_bound(const std::vector<int32_t>& patches, double& dtmin, double& rmax) const {
+    const int d = dim, ncoord = dim == 3 ? 6 : 3;
+    dtmin = kLarge;
+    rmax = -1.0;
+    for (int32_t p : patches)
+      for (int32_t c = ele_offsets[p]; c < ele_offsets[p + 1]; ++c) {
+        double L[6];
+        for (int k = 0; k < ncoord; ++k) L[k] = (double)coords[k * F + c];
+        const double rin = dim == 3 ? inradius_tet(L) : inradius_tri(L);
+        double a2 = 0.0;
+        for (int k = 0; k < d; ++k) a2 += (double)sigma[k * F + c] * (double)sigma[k * F + c];
+        const int e = ele_permute[c];
+        const double ceik = std::fabs(ceik_e.empty() ? 10.0 : ceik_e[e]);
+        const double ccurv = std::fabs(ccurv_e.empty() ? 0.0 : ccurv_e[e]);
+        const double cadv = std::sqrt(a2);
+        double dt1 = kLarge, dt2 = kLarge;
+        if (ccurv > 1e-12) dt1 = (4.0 * rin * rin) / ccurv / 2.0 / (double)d;
+        if (ceik + cadv > 1e-12) dt2 = (rin * 2.0) / (ceik + cadv) / (double)d;
+        dtmin = std::fmin(dtmin, std::fmin(dt1, dt2));
+        rmax = std::fmax(rmax, rin);
+      }
+  }
+  // Over the current narrowband, or every patch before the first step.
+  void compute_dt(double* out) override {
+    std::vector<int32_t> all;
+    if (narrowband.empty()) { all.resize(P); std::iota(all.begin(), all.end(), 0); }
+    cfl_bound(narrowband.empty() ? all : narrowband, out[0], out[1]);
+  }
   void get_resid(float* out) const override {
     for (int64_t v = 0; v < N; ++v) out[vert_permute[v]] = resid_v.empty() ? 0.f : resid_v[v];
   }
@@ -1003,10 +1070,18 @@ struct Solver : Base {
       }
     }
     narrowband_and_restore(prm, st);
+    st.dt_used = prm.dt;
     if (narrowband.empty()) return 0;
+    lso_params aprm = prm;
+    if (prm.cfl) {
+      double b[2];
+      cfl_bound(narrowband, b[0], b[1]);
+      aprm.dt = std::fmin(prm.dt, b[0]);
+      st.dt_used = aprm.dt;
+    }
     for (int it = 0; it < prm.inside_niter; ++it) {
       #pragma omp parallel for schedule(dynamic, 1)
-      for (int64_t k = 0; k < (int64_t)narrowband.size(); ++k) advect_patch(narrowband[k], prm);
+      for (int64_t k = 0; k < (int64_t)narrowband.size(); ++k) advect_patch(narrowband[k], aprm);
       for (int32_t p : narrowband)  // CopyOutBack_levelset3d
         for (int32_t v = vert_offsets[p]; v < vert_offsets[p + 1]; ++v) phi[v] = phi_out[v];
     }
@@ -1044,6 +1119,17 @@ void lso_advect_all(void* h, double dt) { static_cast<Base*>(h)->advect_all(dt);
 void lso_set_warm(void* h, const double* warm) { static_cast<Base*>(h)->set_warm(warm); }
 // Per-vertex acceptance residuals of the last warm step (caller order).
 void lso_get_warm_residuals(void* h, float* out) { static_cast<Base*>(h)->get_resid(out); }
+void lso_set_cfl_coefficients(void* h, const double* ceik, const double* ccurv) {
+  static_cast<Base*>(h)->set_cfl_coefficients(ceik, ccurv);
+}
+// Vertex labels of the last step (SeedPoint 0 ... FarPoint 3), caller order.
+void lso_get_labels(void* h, int32_t* out) { static_cast<Base*>(h)->get_labels(out); }
+// out[0]: CFL bound, out[1]: largest inscribed radius.
+void lso_compute_dt(void* h, double* out) { static_cast<Base*>(h)->compute_dt(out); }
+// Inscribed radius from local coordinates (6 for a tet, 3 for a triangle).
+double lso_inradius(int dim, const double* L) {
+  return dim == 3 ? Solver<double>::inradius_tet(L) : Solver<double>::inradius_tri(L);
+}
 // Per-copy local coordinates in the storage type, copy-major.
 void lso_get_coords(void* h, double* out) { static_cast<Base*>(h)->get_coords(out); }
 void lso_get_layout(void* h, int32_t* vert_perm, int32_t* ele_perm, int32_t* vert_off,

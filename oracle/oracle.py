@@ -19,14 +19,15 @@ class Params(C.Structure):
     _fields_ = [("dt", C.c_double), ("bandwidth", C.c_double), ("tol", C.c_double),
                 ("lambda_eps", C.c_double), ("niter", C.c_int), ("inside_niter", C.c_int),
                 ("mode", C.c_int), ("nthreads", C.c_int), ("max_fim_iters", C.c_int), ("warm", C.c_int),
-                ("warm_margin", C.c_double), ("warm_res_tol", C.c_double), ("warm_abs_margin", C.c_double)]
+                ("warm_margin", C.c_double), ("warm_res_tol", C.c_double), ("warm_abs_margin", C.c_double),
+                ("cfl", C.c_int), ("pad", C.c_int)]
 
 
 class Stats(C.Structure):
     _fields_ = [("fim_iters", C.c_int), ("nb_count", C.c_int), ("patch_visits", C.c_int64),
                 ("probes", C.c_int64), ("nb_verts", C.c_int64), ("n_seed_patches", C.c_int),
                 ("warm_patches", C.c_int), ("warm_residual", C.c_double), ("warm_fallback", C.c_int),
-                ("pad", C.c_int)]
+                ("pad", C.c_int), ("dt_used", C.c_double)]
 
 
 def build() -> str:
@@ -69,14 +70,19 @@ def lib():
         L.lso_advect_all.argtypes = [vp, C.c_double]
         L.lso_set_warm.argtypes = [vp, C.c_void_p]
         L.lso_get_warm_residuals.argtypes = [vp, C.c_void_p]
+        L.lso_set_cfl_coefficients.argtypes = [vp, C.c_void_p, C.c_void_p]
+        L.lso_compute_dt.argtypes = [vp, C.c_void_p]
+        L.lso_get_labels.argtypes = [vp, C.c_void_p]
+        L.lso_inradius.restype = C.c_double
+        L.lso_inradius.argtypes = [C.c_int, dp]
         _lib = L
     return _lib
 
 
 def make_params(dt, bandwidth, niter, tol=1e-5, lambda_eps=1e-5, inside_niter=1, mode=0, nthreads=0,
-                max_fim_iters=0, warm=0, warm_margin=0.0, warm_res_tol=0.0, warm_abs_margin=0.0):
+                max_fim_iters=0, warm=0, warm_margin=0.0, warm_res_tol=0.0, warm_abs_margin=0.0, cfl=0):
     return Params(dt, bandwidth, tol, lambda_eps, niter, inside_niter, mode, nthreads, max_fim_iters, warm,
-                  warm_margin, warm_res_tol, warm_abs_margin)
+                  warm_margin, warm_res_tol, warm_abs_margin, cfl, 0)
 
 
 class Oracle:
@@ -139,6 +145,24 @@ class Oracle:
         self._warm = np.ascontiguousarray(warm, dtype=np.float64)
         self.L.lso_set_warm(self.h, self._warm.ctypes.data)
 
+    def set_cfl_coefficients(self, ceik=None, ccurv=None):
+        self._ceik = None if ceik is None else np.ascontiguousarray(ceik, dtype=np.float64)
+        self._ccurv = None if ccurv is None else np.ascontiguousarray(ccurv, dtype=np.float64)
+        self.L.lso_set_cfl_coefficients(self.h, None if self._ceik is None else self._ceik.ctypes.data,
+                                        None if self._ccurv is None else self._ccurv.ctypes.data)
+
+    def labels(self):
+        """Vertex labels after the last step (0: seed ... 3: far), caller order."""
+        out = np.empty(self.N, dtype=np.int32)
+        self.L.lso_get_labels(self.h, out.ctypes.data)
+        return out
+
+    def compute_dt(self):
+        """(CFL bound, largest inscribed radius) over the current narrowband."""
+        out = np.zeros(2)
+        self.L.lso_compute_dt(self.h, out.ctypes.data)
+        return float(out[0]), float(out[1])
+
     def warm_residuals(self):
         """Per-vertex acceptance residuals F_v - DT_v of the last warm step."""
         out = np.zeros(self.N, dtype=np.float32)
@@ -187,6 +211,11 @@ def tet_element_update(L6, vals, precision=64, lambda_eps=1e-5):
     v = np.array(vals, dtype=np.float64)
     lib().lso_tet_element_update(precision, np.asarray(L6, np.float64), v, lambda_eps)
     return v
+
+
+def inradius(dim, L):
+    """Inscribed radius of an element from its local coordinates."""
+    return lib().lso_inradius(dim, np.ascontiguousarray(L, np.float64))
 
 
 def max_threads():

@@ -262,7 +262,8 @@ LevelSet::LevelSet(bool isTriMesh, std::string fname, bool verbose)
       tetMesh_(nullptr),
       FIMPtr2d_(nullptr),
       FIMPtr3d_(nullptr),
-      isTriMesh_(isTriMesh) {}
+      isTriMesh_(isTriMesh),
+      cflTimeStep_(false) {}
 
 LevelSet::~LevelSet() {
   delete tetMesh_;
@@ -463,6 +464,7 @@ std::vector<std::vector<float>> generate(meshFIM* fim, int dim, const std::vecto
   p.inside_niter = o.insideIterations_;
   p.mode = LS_MODE_ADVECT;
   p.max_fim_iters = 0;
+  p.cfl = o.cflTimeStep_ ? 1 : 0;
 
   std::vector<std::vector<float>> ans;
   ans.push_back(vertT);
@@ -476,8 +478,8 @@ std::vector<std::vector<float>> generate(meshFIM* fim, int dim, const std::vecto
     if (ls_get_phi(c, vertT.data(), LS_F32) != LS_OK) die(c, "ls_get_phi");
     ans.push_back(vertT);
     if (o.verbose_)
-      printf("step %d: %d narrowband patches, %d FIM iterations, %.3f ms\n", s, st.nb_count, st.fim_iters,
-             st.ms_total);
+      printf("step %d: %d narrowband patches, %d FIM iterations, dt %g, %.3f ms\n", s, st.nb_count,
+             st.fim_iters, st.dt_used, st.ms_total);
   }
   return ans;
 }

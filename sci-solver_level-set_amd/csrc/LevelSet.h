@@ -97,4 +97,9 @@ class LevelSet {
   meshFIM2d* FIMPtr2d_;
   meshFIM3d* FIMPtr3d_;
   bool isTriMesh_;
+  // Opt-in (no reference counterpart; default off): advect each timestep
+  // with min(timeStep_, the CFL bound of the new narrowband), the bound the
+  // reference computes per step and discards (meshFIM2d::compute_deltaT,
+  // src/core2d/cuda/meshFIM2d.cu:609-674); see ls_compute_dt.
+  bool cflTimeStep_;
 };

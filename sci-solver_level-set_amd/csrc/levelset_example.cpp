@@ -26,6 +26,7 @@ static void usage() {
             << "   -n NSTEPS          # of steps to take of TIMESTEP amount.\n"
             << "   -t TIMESTEP        Duration of a timestep.\n"
             << "   -s INSIDE_NITER    # of inside iterations.\n"
+            << "   -c                 Bound each timestep by the narrowband's CFL condition.\n"
             << "   -d NSIDE           # of sides for Square partition type.\n"
             << "   -p PARTITION_TYPE  1 for Square, otherwise is it METIS.\n"
             << "   -b NUM_BLOCKS      # of blocks for Square partition type.\n"
@@ -64,6 +65,8 @@ int main(int argc, char* argv[]) {
       data.timeStep_ = atof(argv[++i]);
     } else if (!strcmp(argv[i], "-s")) {
       data.insideIterations_ = atoi(argv[++i]);
+    } else if (!strcmp(argv[i], "-c")) {
+      data.cflTimeStep_ = true;
     } else if (!strcmp(argv[i], "-d")) {
       data.sideLengths_ = atoi(argv[++i]);
     } else if (!strcmp(argv[i], "-p")) {

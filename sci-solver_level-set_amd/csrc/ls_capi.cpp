@@ -517,4 +517,20 @@ int ls_get_warm(ls_ctx* c, double* out) {
   return rc == LS_OK ? rc : fail(c, rc, err);
 }
 
+int ls_set_cfl_coefficients(ls_ctx* c, const double* ceik, const double* ccurv) {
+  if (!c) return LS_ERR_ARG;
+  if (int rc = engine_ready(c)) return rc;
+  std::string err;
+  const int rc = c->engine->set_cfl_coefficients(ceik, ccurv, c->E, err);
+  return rc == LS_OK ? rc : fail(c, rc, err);
+}
+
+int ls_compute_dt(ls_ctx* c, double* dt, double* max_rin) {
+  if (!c || !dt || !max_rin) return LS_ERR_ARG;
+  if (int rc = engine_ready(c)) return rc;
+  std::string err;
+  const int rc = c->engine->compute_dt(dt, max_rin, err);
+  return rc == LS_OK ? rc : fail(c, rc, err);
+}
+
 }  // extern "C"

@@ -35,6 +35,12 @@ struct DevMesh {
   // their own parameters; null: the step's uniform values.
   const double* patch_dt;      // P or null
   const T* patch_bw;           // P or null
+  // CFL coefficients (kernel_compute_timestep2d): per original element,
+  // looked up through copy_src, or the uniform values when null.
+  const int32_t* copy_src;     // F, original element of each copy
+  const double* ceik;          // E or null
+  const double* ccurv;         // E or null
+  double ceik0, ccurv0;
 };
 
 template <typename T>
@@ -92,6 +98,9 @@ struct DevState {
   // (ve_tag[p] != 0) starts such elements clean instead of dirty.
   uint8_t* ve_ok;             // F
   int32_t* ve_tag;            // P
+  // CFL bound of the last narrowband as order-preserving keys: [0] the least
+  // timestep, [1] the largest inscribed radius.
+  unsigned long long* cfl;
 };
 
 enum TickIndex : int { kTkSeed = 0, kTkVisit, kTkCopyback, kTkProbe, kTkNext, kTkNb, kTkAdvect, kNumTicks = 8 };

@@ -43,6 +43,9 @@ class Engine {
   virtual int get_distances(double* out_user, std::string& err) = 0;
   virtual int set_warm(const double* warm_user, std::string& err) = 0;  // caller order
   virtual int get_warm(double* out_user, std::string& err) = 0;         // caller order
+  // CFL coefficients per original element (E values each; null: defaults).
+  virtual int set_cfl_coefficients(const double* ceik, const double* ccurv, int64_t E, std::string& err) = 0;
+  virtual int compute_dt(double* dt, double* max_rin, std::string& err) = 0;
   virtual void set_exec(int mode) = 0;
   virtual int set_sweep_mode(int mode, std::string& err) = 0;  // LS_SWEEP_*
   virtual int sweep_tasks() const = 0;                         // 1: task-compacted sweeps in use

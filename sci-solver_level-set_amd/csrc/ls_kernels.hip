@@ -1287,13 +1287,14 @@ __device__ void curvature_patch(const DevMesh<T>& m, DevState<T>& s, int p, doub
 template <typename T, int NC, int BS>
 __device__ void advect_patch(const DevMesh<T>& m, DevState<T>& s, int p, double dt, unsigned char* smem);
 
+// dt is the patch's timestep (dt_of, bounded by the CFL bound when asked).
 template <typename T, int NC, int BS>
 __device__ __forceinline__ void update_patch(const DevMesh<T>& m, DevState<T>& s, int p, double dt, int mode,
                                              unsigned char* smem) {
   if (mode == LS_MODE_MCF)
-    curvature_patch<T, NC, BS>(m, s, p, dt_of(m, p, dt), smem);
+    curvature_patch<T, NC, BS>(m, s, p, dt, smem);
   else
-    advect_patch<T, NC, BS>(m, s, p, dt_of(m, p, dt), smem);
+    advect_patch<T, NC, BS>(m, s, p, dt, smem);
 }
 
 // Upwind residual-distribution (N-scheme) advection of one narrowband patch
@@ -1404,6 +1405,119 @@ __device__ void advect_patch(const DevMesh<T>& m, DevState<T>& s, int p, double 
     const double old = (double)s_vtx[v];
     s.phi_out[vs + v] = (T)((d_abs(dn) > 1e-8) ? old - dt * (up / dn) : old);
   }
+}
+
+// ---------------------------------------------------------------------------
+// CFL timestep bound (kernel_compute_timestep2d, meshFIM2d_kernels.h:405-475,
+// reduced by meshFIM2d::compute_deltaT, meshFIM2d.cu:609-674): per element copy
+// of the narrowband, with R its inscribed radius, d the dimension and the
+// element's eikonal, curvature and advection coefficients,
+//   dt1 = 4 R^2 / |Ccurv| / 2 / d     when |Ccurv| > eps,
+//   dt2 = 2 R / (|Ceik| + |Cadv|) / d  when |Ceik| + |Cadv| > eps,
+// and the bound is the least min(dt1, dt2); the largest R is reported beside
+// it. The reference computes this for triangles only and discards it; here
+// it is computed for both element kinds and can bound the step's advection.
+
+constexpr double kCflEps = 1e-12;  // _EPS, src/core2d/include/meshFIM2d.h:15
+// The reference fills every element's coefficients with Ceik = 10 and
+// Ccurv = 0 (src/core2d/cuda/meshFIM2d.cu:724-731).
+constexpr double kCeikDefault = 10.0, kCcurvDefault = 0.0;
+
+// Inscribed radius of a copy from its local coordinates (FP64): 3 V / (sum of
+// the face areas) for a tet, 2 A / perimeter for a triangle.
+template <int NC>
+struct Inradius;
+template <>
+struct Inradius<4> {
+  // A=(0,0,0), B=(L0,0,0), C=(L1,L2,0), D=(L3,L4,L5).
+  __device__ static double of(const double* L) {
+    const double v6 = d_abs(L[0] * L[2] * L[5]);
+    const double abc = d_abs(L[0] * L[2]);
+    const double abd = d_abs(L[0]) * d_sqrt(L[4] * L[4] + L[5] * L[5]);
+    const double ax = L[2] * L[5], ay = -L[1] * L[5], az = L[1] * L[4] - L[2] * L[3];
+    const double acd = d_sqrt((ax * ax + ay * ay) + az * az);
+    const double bx = L[2] * L[5], by = -(L[1] - L[0]) * L[5], bz = (L[1] - L[0]) * L[4] - L[2] * (L[3] - L[0]);
+    const double bcd = d_sqrt((bx * bx + by * by) + bz * bz);
+    // 3 V / (sum of areas) = (v6 / 2) / ((abc + abd + acd + bcd) / 2)
+    return v6 / (((abc + abd) + acd) + bcd);
+  }
+};
+template <>
+struct Inradius<3> {
+  // A=(0,0), B=(c0,0), C=(c1,c2).
+  __device__ static double of(const double* c) {
+    const double a2 = d_abs(c[0] * c[2]);  // twice the area
+    const double per = (d_abs(c[0]) + d_sqrt((c[1] - c[0]) * (c[1] - c[0]) + c[2] * c[2])) +
+                       d_sqrt(c[1] * c[1] + c[2] * c[2]);
+    return a2 / per;
+  }
+};
+
+// CFL bound and inscribed radius of element copy c.
+template <typename T, int NC>
+__device__ __forceinline__ void cfl_copy(const DevMesh<T>& m, int64_t c, double& dt, double& rin) {
+  constexpr int K = ElemSolve<T, NC>::kCoords;
+  constexpr int D = NC - 1;
+  double L[K];
+  const T* gc = m.coords + c * K;
+#pragma unroll
+  for (int k = 0; k < K; ++k) L[k] = (double)gc[k];
+  rin = Inradius<NC>::of(L);
+  const T* gs = m.sigma + c * D;
+  double a2 = 0.0;
+#pragma unroll
+  for (int k = 0; k < D; ++k) a2 += (double)gs[k] * (double)gs[k];
+  const int e = m.copy_src[c];
+  const double ceik = d_abs(m.ceik ? m.ceik[e] : m.ceik0);
+  const double ccurv = d_abs(m.ccurv ? m.ccurv[e] : m.ccurv0);
+  const double cadv = d_sqrt(a2);
+  double dt1 = kLargeNum, dt2 = kLargeNum;
+  if (ccurv > kCflEps) dt1 = (4.0 * rin * rin) / ccurv / 2.0 / (double)D;
+  if (ceik + cadv > kCflEps) dt2 = (rin * 2.0) / (ceik + cadv) / (double)D;
+  dt = d_min(dt1, dt2);
+}
+
+// One patch's bound, merged into s.cfl with order-preserving atomics. Every
+// thread of the workgroup calls it; red holds BS/64 doubles.
+template <typename T, int NC, int BS>
+__device__ void cfl_patch(const DevMesh<T>& m, DevState<T>& s, int p, double* red) {
+  const int es = m.ele_off[p], ee = m.ele_off[p + 1];
+  double dt = kLargeNum, rin = -1.0;
+  for (int64_t c = es + threadIdx.x; c < ee; c += BS) {
+    double d, r;
+    cfl_copy<T, NC>(m, c, d, r);
+    dt = d_min(dt, d);
+    rin = d_max(rin, r);
+  }
+  __syncthreads();
+  dt = block_min_n<double, BS>(dt, red);
+  rin = -block_min_n<double, BS>(-rin, red);
+  if (threadIdx.x == 0) {
+    atomicMin(&s.cfl[0], OrderedBits<double>::of(dt));
+    atomicMax(&s.cfl[1], OrderedBits<double>::of(rin));
+  }
+}
+
+__global__ void k_cfl_init(unsigned long long* cfl) {
+  if (threadIdx.x == 0) {
+    cfl[0] = OrderedBits<double>::of(kLargeNum);
+    cfl[1] = OrderedBits<double>::of(-1.0);
+  }
+}
+
+// One workgroup per listed patch (the narrowband), or per patch when list is null.
+template <typename T, int NC>
+__global__ __launch_bounds__(kBlock) void k_cfl(DevMesh<T> m, DevState<T> s, const int32_t* list) {
+  __shared__ double red[kBlock / 64];
+  cfl_patch<T, NC, kBlock>(m, s, list ? list[blockIdx.x] : (int)blockIdx.x, red);
+}
+
+// The advection timestep of patch p: the step's (or the patch's mesh's) dt,
+// bounded by the narrowband's CFL bound when the step asks for it.
+template <typename T>
+__device__ __forceinline__ double advect_dt(const DevMesh<T>& m, const DevState<T>& s, int p, double dt, int cfl) {
+  const double d = dt_of(m, p, dt);
+  return cfl ? d_min(d, OrderedBits<double>::value(s.cfl[0])) : d;
 }
 
 // ---------------------------------------------------------------------------
@@ -1518,9 +1632,10 @@ __global__ __launch_bounds__(kBlock) void k_narrowband(DevMesh<T> m, DevState<T>
 
 template <typename T, int NC>
 __global__ __launch_bounds__(kBlock) void k_advect(DevMesh<T> m, DevState<T> s, const int32_t* nb, double dt,
-                                                   int mode) {
+                                                   int mode, int cfl) {
   extern __shared__ __align__(16) unsigned char smem[];
-  update_patch<T, NC, kBlock>(m, s, nb[blockIdx.x], dt, mode, smem);
+  const int p = nb[blockIdx.x];
+  update_patch<T, NC, kBlock>(m, s, p, advect_dt(m, s, p, dt, cfl), mode, smem);
 }
 
 // CopyOutBack_levelset3d (meshFIM_kernels.h:22-34).
@@ -1534,10 +1649,11 @@ __global__ __launch_bounds__(kBlock) void k_copy_phi(DevMesh<T> m, DevState<T> s
 // the persistent step): blocks take list entries grid-stride.
 template <typename T, int NC>
 __global__ __launch_bounds__(kBlock) void k_advect_dev(DevMesh<T> m, DevState<T> s, const int32_t* nb,
-                                                       const int32_t* count, double dt, int mode) {
+                                                       const int32_t* count, double dt, int mode, int cfl) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int n = *count;
-  for (int i = blockIdx.x; i < n; i += gridDim.x) update_patch<T, NC, kBlock>(m, s, nb[i], dt, mode, smem);
+  for (int i = blockIdx.x; i < n; i += gridDim.x)
+    update_patch<T, NC, kBlock>(m, s, nb[i], advect_dt(m, s, nb[i], dt, cfl), mode, smem);
 }
 
 template <typename T>
@@ -1909,6 +2025,7 @@ struct StepArgs {
   double warm_margin;// warm value = prediction * (1 + warm_margin) + warm_abs
   double warm_abs;
   float res_tol;     // acceptance threshold above which a warm start falls back
+  int cfl;           // 1: the advection's dt is bounded by the new narrowband's CFL bound
 };
 
 // Counters are read right after a grid barrier, whose acquire fence already
@@ -1976,6 +2093,7 @@ __global__ __launch_bounds__(BS, WPE) void k_step(DevMesh<T> m, DevState<T> s, S
   __shared__ T red[BS / 64];
   __shared__ int s_ndirty[10];
   __shared__ int s_work[2];  // work draws: item, reserved second item
+  __shared__ double cred[BS / 64];  // CFL reductions
   const int G = gridDim.x, b = blockIdx.x, tid = threadIdx.x;
   const int64_t gtid = (int64_t)b * BS + tid, gsize = (int64_t)G * BS;
   const int nb_out = 1 - a.nb_in;
@@ -2013,6 +2131,10 @@ __global__ __launch_bounds__(BS, WPE) void k_step(DevMesh<T> m, DevState<T> s, S
     }
     s.pc[kPcOverflow] = 0;
     s.pc[kPcNb + nb_out] = 0;
+    if (a.cfl) {
+      s.cfl[0] = OrderedBits<double>::of(kLargeNum);
+      s.cfl[1] = OrderedBits<double>::of(-1.0);
+    }
     if (attempt == 0) {  // the first attempt's warm-start figures are reported
       s.pc[kPcResid] = 0;
       s.pc[kPcWarm] = 0;
@@ -2162,6 +2284,12 @@ __global__ __launch_bounds__(BS, WPE) void k_step(DevMesh<T> m, DevState<T> s, S
     }
   }
   grid_barrier(s.bar, G);
+  if (a.cfl) {
+    // Phase 4: CFL bound over the new narrowband, read by the advection launches.
+    const int nnb = ld_acquire(&s.pc[kPcNb + nb_out]);
+    for (int i = b; i < nnb; i += G) cfl_patch<T, NC, BS>(m, s, s.nb[nb_out][i], cred);
+    grid_barrier(s.bar, G);
+  }
   pcl.lap(clk, kTkNb);
 
   // Advection (or MCF) over the new narrowband follows as its own launches
@@ -2337,6 +2465,7 @@ class EngineImpl final : public Engine {
     LS_HIP(dev_alloc(&d_pc_, kNumPc));
     LS_HIP(dev_alloc(&d_ticks_, kNumTicks));
     LS_HIP(dev_alloc(&d_bar_, 2));
+    LS_HIP(dev_alloc(&d_cfl_, 2));
     LS_HIP(hipMemset(d_bar_, 0, 2 * sizeof(unsigned int)));
     {
       int khz = 0;
@@ -2708,6 +2837,36 @@ class EngineImpl final : public Engine {
                                                                             epoch_in, epoch_out);
   }
 
+  // CFL coefficients per original element (null: the uniform defaults).
+  int set_cfl_coefficients(const double* ceik, const double* ccurv, int64_t E, std::string& err) override {
+    if (!uploaded_) { err = "layout not built"; return LS_ERR_STATE; }
+    if (ceik) LS_HIP(dev_upload(&d_ceik_, ceik, (size_t)E));
+    else if (d_ceik_) { LS_HIP(hipFree(d_ceik_)); d_ceik_ = nullptr; }
+    if (ccurv) LS_HIP(dev_upload(&d_ccurv_, ccurv, (size_t)E));
+    else if (d_ccurv_) { LS_HIP(hipFree(d_ccurv_)); d_ccurv_ = nullptr; }
+    return LS_OK;
+  }
+  // CFL bound over the current narrowband (every patch before the first step).
+  int compute_dt(double* dt, double* max_rin, std::string& err) override {
+    if (!uploaded_) { err = "layout not built"; return LS_ERR_STATE; }
+    const int n = nnb_ > 0 ? nnb_ : P_;
+    k_cfl_init<<<1, 64, 0, stream_>>>(d_cfl_);
+    if (n > 0) k_cfl<T, NC><<<n, kBlock, 0, stream_>>>(mesh(), state(), nnb_ > 0 ? d_nb_cur() : nullptr);
+    LS_HIP(hipGetLastError());
+    unsigned long long k[2];
+    LS_HIP(hipMemcpyAsync(k, d_cfl_, sizeof(k), hipMemcpyDeviceToHost, stream_));
+    LS_HIP(hipStreamSynchronize(stream_));
+    *dt = key_value(k[0]);
+    *max_rin = key_value(k[1]);
+    return LS_OK;
+  }
+  static double key_value(unsigned long long k) {
+    const unsigned long long b = k ^ ((k >> 63) ? (1ull << 63) : ~0ull);
+    double d;
+    std::memcpy(&d, &b, sizeof(d));
+    return d;
+  }
+
   int set_patch_params(const std::vector<double>& dt, const std::vector<double>& bw, std::string& err) override {
     if (!uploaded_) { err = "layout not built"; return LS_ERR_STATE; }
     if (dt.empty() && bw.empty()) {
@@ -2754,7 +2913,7 @@ class EngineImpl final : public Engine {
     const DevMesh<T> m = mesh();
     const DevState<T> s = state();
     for (int pass = 0; pass < prm.inside_niter; ++pass) {
-      k_advect_dev<T, NC><<<adv_grid_, kBlock, adv_lds_, q>>>(m, s, nb, cnt, prm.dt, prm.mode);
+      k_advect_dev<T, NC><<<adv_grid_, kBlock, adv_lds_, q>>>(m, s, nb, cnt, prm.dt, prm.mode, prm.cfl);
       k_copy_phi_dev<T><<<adv_grid_, kBlock, 0, q>>>(m, s, nb, cnt);
     }
     return hipGetLastError();
@@ -2787,12 +2946,14 @@ class EngineImpl final : public Engine {
     a.warm = prm.warm;
     a.warm_margin = prm.warm_margin;
     a.warm_abs = prm.warm_abs_margin;
+    a.cfl = prm.cfl;
     a.res_tol = (float)(prm.warm_res_tol > 0 ? prm.warm_res_tol : prm.tol);
     if (a.warm == LS_WARM_LEARNED && !a.first && !d_warm_) {
       err = "no learned warm-start values (run ls_mlp_predict)";
       return LS_ERR_STATE;
     }
     if (a.warm < LS_WARM_NONE || a.warm > LS_WARM_LEARNED) { err = "bad warm mode"; return LS_ERR_ARG; }
+    if (int rc = check_cfl(prm, err)) return rc;
     hipStream_t q = stream_;
     void* args[] = {&m, &s, &a};
     if (!pc_synced_) {  // the previous narrowband size, when the host-driven path ran last
@@ -2840,6 +3001,8 @@ class EngineImpl final : public Engine {
     LS_HIP(hipMemcpyAsync(tk, d_ticks_, sizeof(tk), hipMemcpyDeviceToHost, q));
     LS_HIP(hipMemcpyAsync(pc, d_pc_, sizeof(pc), hipMemcpyDeviceToHost, q));
     LS_HIP(hipMemcpyAsync(sz, d_sizes_, sizeof(sz), hipMemcpyDeviceToHost, q));
+    unsigned long long ck[2] = {0, 0};
+    if (prm.cfl) LS_HIP(hipMemcpyAsync(ck, d_cfl_, sizeof(ck), hipMemcpyDeviceToHost, q));
     LS_HIP(hipStreamSynchronize(q));
     if (pc[kPcOverflow] && prm.max_fim_iters <= 0) {
       err = "FIM did not converge within " + std::to_string(kFimIterCap) + " iterations";
@@ -2854,6 +3017,7 @@ class EngineImpl final : public Engine {
     st.n_seed_patches = pc[kPcSeed];
     st.nb_count = nnb_;
     st.nb_verts = pc[kPcNbVerts];
+    fill_cfl(prm, ck, st);
     float ms = 0.f;
     LS_HIP(hipEventElapsedTime(&ms, ev_[0], ev_[4]));
     st.ms_total = ms;
@@ -2872,6 +3036,24 @@ class EngineImpl final : public Engine {
     return LS_OK;
   }
 
+  int check_cfl(const ls_step_params& prm, std::string& err) const {
+    if (prm.cfl != 0 && prm.cfl != 1) { err = "cfl must be 0 or 1"; return LS_ERR_ARG; }
+    if (prm.cfl && use_patch_prm_) {
+      err = "the CFL bound is per mesh: with per-mesh timesteps call ls_compute_dt per mesh instead";
+      return LS_ERR_ARG;
+    }
+    return LS_OK;
+  }
+  static void fill_cfl(const ls_step_params& prm, const unsigned long long* ck, ls_stats& st) {
+    st.dt_used = prm.dt;
+    st.cfl_dt = st.cfl_max_rin = 0.0;
+    if (prm.cfl && st.nb_count > 0) {
+      st.cfl_dt = key_value(ck[0]);
+      st.cfl_max_rin = key_value(ck[1]);
+      st.dt_used = std::min(prm.dt, st.cfl_dt);
+    }
+  }
+
   static void fill_sizes(ls_stats& st, const unsigned long long* sz) {
     st.visit_copies = sz[kSzVisit]; st.visit_verts = sz[kSzVisit + 1]; st.visit_halo = sz[kSzVisit + 2];
     st.probe_copies = sz[kSzProbe]; st.probe_verts = sz[kSzProbe + 1]; st.probe_halo = sz[kSzProbe + 2];
@@ -2883,6 +3065,7 @@ class EngineImpl final : public Engine {
   // One launch per phase; the FIM loop is driven from the host.
   int step_kernels(const ls_step_params& prm, ls_stats& st, std::string& err) {
     if (prm.niter <= 0 || prm.inside_niter < 0) { err = "niter must be > 0, inside_niter >= 0"; return LS_ERR_ARG; }
+    if (int rc = check_cfl(prm, err)) return rc;
     if (prm.mode != LS_MODE_ADVECT && prm.mode != LS_MODE_MCF) {
       err = "mode must be LS_MODE_ADVECT or LS_MODE_MCF";
       return LS_ERR_ARG;
@@ -2959,10 +3142,17 @@ class EngineImpl final : public Engine {
     st.nb_count = nnb_;
     st.nb_verts = cnt[kCntNBVerts];
     LS_HIP(hipEventRecord(ev_[3], q));
+    unsigned long long ck[2] = {0, 0};
+    if (nnb_ > 0 && prm.cfl) {
+      k_cfl_init<<<1, 64, 0, q>>>(d_cfl_);
+      k_cfl<T, NC><<<nnb_, kBlock, 0, q>>>(m, s, d_nb_cur());
+      LS_HIP(hipGetLastError());
+      LS_HIP(hipMemcpyAsync(ck, d_cfl_, sizeof(ck), hipMemcpyDeviceToHost, q));
+    }
     if (nnb_ > 0) {
       for (int it = 0; it < prm.inside_niter; ++it) {
         int ka = kev_begin(1, q);
-        k_advect<T, NC><<<nnb_, kBlock, adv_lds_, q>>>(m, s, d_nb_cur(), dt, prm.mode);
+        k_advect<T, NC><<<nnb_, kBlock, adv_lds_, q>>>(m, s, d_nb_cur(), dt, prm.mode, prm.cfl);
         kev_end(ka, q);
         k_copy_phi<T><<<nnb_, kBlock, 0, q>>>(m, s, d_nb_cur());
       }
@@ -2984,6 +3174,7 @@ class EngineImpl final : public Engine {
     unsigned long long sz[kNumSizes];
     LS_HIP(hipMemcpy(sz, d_sizes_, sizeof(sz), hipMemcpyDeviceToHost));
     fill_sizes(st, sz);
+    fill_cfl(prm, ck, st);
     return LS_OK;
   }
 
@@ -3013,6 +3204,11 @@ class EngineImpl final : public Engine {
     m.hgt = d_hgt_;
     m.patch_dt = use_patch_prm_ ? d_patch_dt_ : nullptr;
     m.patch_bw = use_patch_prm_ ? d_patch_bw_ : nullptr;
+    m.copy_src = d_copy_src_;
+    m.ceik = d_ceik_;
+    m.ccurv = d_ccurv_;
+    m.ceik0 = kCeikDefault;
+    m.ccurv0 = kCcurvDefault;
     return m;
   }
   DevState<T> state() const {
@@ -3035,6 +3231,7 @@ class EngineImpl final : public Engine {
     s.visitE = d_visitE_;
     s.ve_ok = d_ve_ok_;
     s.ve_tag = d_ve_tag_;
+    s.cfl = d_cfl_;
     return s;
   }
   int32_t* d_nb_cur() const { return nb_sel_ ? d_lists_[3] : d_nb_; }
@@ -3049,7 +3246,7 @@ class EngineImpl final : public Engine {
                     d_patch_dt_, d_patch_bw_, d_warm_, d_hgt_, d_mlp_prm_, d_mlp_m_, d_mlp_v_, d_mlp_grad_,
                     d_mlp_b_, d_mlp_w1_, d_mlp_w2_, d_mlp_w3_, d_ds_x_, d_ds_y_, d_pend_, d_xyz_,
                     d_copy_verts_, d_copy_src_, d_rs_slot_, d_rs_vval_, d_rs_dirty_, d_rs_epoch_, d_visitE_,
-                    d_vpatch_, d_ve_ok_, d_ve_tag_};
+                    d_vpatch_, d_ve_ok_, d_ve_tag_, d_cfl_, d_ceik_, d_ccurv_};
     if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
     if (graph_) (void)hipGraphDestroy(graph_);
     for (void* p : ptrs) if (p) (void)hipFree(p);
@@ -3091,6 +3288,9 @@ class EngineImpl final : public Engine {
   int32_t* d_copy_src_ = nullptr;    // F, original element of each copy
   __half* d_hgt_ = nullptr;  // tet copies: corner heights (F x 4), FP16 rounded toward zero
   T* d_warm_ = nullptr;  // learned warm-start distances (N), written by the predictor
+  unsigned long long* d_cfl_ = nullptr;  // CFL bound keys (2)
+  double* d_ceik_ = nullptr;   // per original element, or null (uniform default)
+  double* d_ccurv_ = nullptr;
   float *d_mlp_prm_ = nullptr, *d_mlp_m_ = nullptr, *d_mlp_v_ = nullptr, *d_mlp_grad_ = nullptr, *d_mlp_b_ = nullptr;
   uint16_t *d_mlp_w1_ = nullptr, *d_mlp_w2_ = nullptr, *d_mlp_w3_ = nullptr;
   bool mlp_ready_ = false;

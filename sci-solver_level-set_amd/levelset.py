@@ -26,7 +26,7 @@ class StepParams(C.Structure):
     _fields_ = [("dt", C.c_double), ("bandwidth", C.c_double), ("tol", C.c_double),
                 ("lambda_eps", C.c_double), ("niter", C.c_int32), ("inside_niter", C.c_int32),
                 ("mode", C.c_int32), ("max_fim_iters", C.c_int32), ("warm", C.c_int32),
-                ("reserved0", C.c_int32), ("warm_margin", C.c_double), ("warm_res_tol", C.c_double),
+                ("cfl", C.c_int32), ("warm_margin", C.c_double), ("warm_res_tol", C.c_double),
                 ("warm_abs_margin", C.c_double)]
 
 
@@ -45,7 +45,8 @@ class Stats(C.Structure):
                 ("ms_ph_probe", C.c_double), ("ms_ph_next", C.c_double), ("ms_ph_nb", C.c_double),
                 ("ms_ph_advect", C.c_double), ("elem_updates", C.c_int64), ("local_solves", C.c_int64),
                 ("warm_residual", C.c_double), ("warm_patches", C.c_int32), ("warm_fallback", C.c_int32),
-                ("sweep_tasks", C.c_int32), ("step_wg_per_cu", C.c_int32)]
+                ("sweep_tasks", C.c_int32), ("step_wg_per_cu", C.c_int32), ("dt_used", C.c_double),
+                ("cfl_dt", C.c_double), ("cfl_max_rin", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -67,7 +68,7 @@ EXPORTS = ["ls_create", "ls_destroy", "ls_last_error", "ls_set_mesh", "ls_set_pr
            "ls_set_mesh_batch", "ls_partition_batch", "ls_set_batch_params", "ls_get_batch_nb_counts",
            "ls_get_distances", "ls_mlp_set_params", "ls_mlp_get_params", "ls_mlp_predict", "ls_mlp_collect",
            "ls_mlp_dataset", "ls_mlp_gradient", "ls_mlp_train", "ls_set_warm",
-           "ls_get_warm"]
+           "ls_get_warm", "ls_compute_dt", "ls_set_cfl_coefficients"]
 MLP_PARAMS = 5892
 
 
@@ -117,6 +118,8 @@ def lib():
         L.ls_get_distances.argtypes = [vp, vp]
         L.ls_set_warm.argtypes = [vp, vp]
         L.ls_get_warm.argtypes = [vp, vp]
+        L.ls_compute_dt.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.ls_set_cfl_coefficients.argtypes = [vp, vp, vp]
         L.ls_mlp_set_params.argtypes = [vp, vp, C.c_int]
         L.ls_mlp_get_params.argtypes = [vp, vp, C.c_int]
         L.ls_mlp_predict.argtypes = [vp]
@@ -133,8 +136,9 @@ class LevelSetError(RuntimeError):
 
 
 def step_params(dt, bandwidth, niter, tol=1e-5, lambda_eps=1e-5, inside_niter=1, mode=LS_MODE_ADVECT,
-                max_fim_iters=0, warm=LS_WARM_NONE, warm_margin=0.0, warm_res_tol=0.0, warm_abs_margin=0.0):
-    return StepParams(dt, bandwidth, tol, lambda_eps, niter, inside_niter, mode, max_fim_iters, warm, 0,
+                max_fim_iters=0, warm=LS_WARM_NONE, warm_margin=0.0, warm_res_tol=0.0, warm_abs_margin=0.0,
+                cfl=0):
+    return StepParams(dt, bandwidth, tol, lambda_eps, niter, inside_niter, mode, max_fim_iters, warm, cfl,
                       warm_margin, warm_res_tol, warm_abs_margin)
 
 
@@ -283,6 +287,21 @@ class Solver:
         a = np.ascontiguousarray(warm, dtype=np.float64)
         self._check(self.L.ls_set_warm(self.ctx, a.ctypes.data))
 
+    def compute_dt(self):
+        """(CFL bound, largest inscribed radius) over the current narrowband."""
+        dt, rin = C.c_double(), C.c_double()
+        self._check(self.L.ls_compute_dt(self.ctx, C.byref(dt), C.byref(rin)))
+        return dt.value, rin.value
+
+    def set_cfl_coefficients(self, ceik=None, ccurv=None):
+        """Per-element eikonal and curvature coefficients of the CFL bound
+        (None: the reference's 10 and 0)."""
+        self._ceik = None if ceik is None else np.ascontiguousarray(ceik, dtype=np.float64)
+        self._ccurv = None if ccurv is None else np.ascontiguousarray(ccurv, dtype=np.float64)
+        self._check(self.L.ls_set_cfl_coefficients(
+            self.ctx, None if self._ceik is None else self._ceik.ctypes.data,
+            None if self._ccurv is None else self._ccurv.ctypes.data))
+
     def warm_values(self):
         """The predictor distances the context holds (caller order)."""
         out = np.empty(self.N, dtype=np.float64)
@@ -364,11 +383,12 @@ def mlp_init_params(seed=0):
                            np.zeros(4)]).astype(np.float32)
 
 
-def solver_for_batch(batch, device=0, execution=LS_EXEC_GRAPH, patch_verts=None, labels=None):
+def solver_for_batch(batch, device=0, execution=LS_EXEC_GRAPH, patch_verts=None, labels=None, precision=LS_F32):
     """A Solver holding a cases.Batch; patches from recursive bisection
     (patch_verts) or from caller labels (labels, nparts)."""
     s = Solver(device)
     s.set_mesh_batch(batch.dim, [(m.xyz, m.cells) for m in batch.meshes])
+    s.set_precision(precision)
     s.set_execution(execution)
     if labels is not None:
         s.partition_labels(labels[0], labels[1])

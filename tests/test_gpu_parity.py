@@ -7,7 +7,10 @@
 * FP32 engine against the FP64 oracle, one step at a time from the GPU's own
   state: L-inf <= 1e-5 on phi at every vertex whose advection update does not
   hinge on a threshold decision (|H| > 1e-8, |sum alpha~ vol| > 1e-8) inside
-  the FP32-storage window; those vertices are counted and bounded.
+  the FP32-storage window; those vertices are counted and their error bounded
+  by 1e-4 (one threshold flip), which is above the 1e-5 bar: the FP32 engine
+  is not the headline engine, the FP64 one is (bit-exact along whole
+  trajectories, tests/test_gpu_trajectory.py).
 """
 import numpy as np
 import pytest
@@ -160,7 +163,7 @@ def test_tet_fp32_vs_fp64_oracle_per_step(shape, n, block, oracle_mod):
         print(f"{shape}{n} step {k}: Linf {err_ok:.2e} (ambiguous: {int(amb.sum())} verts, {err_amb:.2e})")
         assert err_ok <= TOL_FP64
         assert amb.sum() <= 0.05 * gs.nb_verts
-        assert err_amb <= 1e-3
+        assert err_amb <= 1e-4
         phi_prev, nb_prev = gphi, g.narrowband()
     g.close()
     o.close()

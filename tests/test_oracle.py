@@ -98,7 +98,11 @@ def test_tri_solve_matches_brute_force_and_segment_distance(oracle_mod):
     assert got == pytest.approx(0.9, abs=1e-12)
 
 
-def test_element_update_is_monotone_and_idempotent(oracle_mod):
+def test_element_update_is_monotone_and_reaches_a_fixed_point(oracle_mod):
+    """Every corner takes min(own, solve from the other three), all reading
+    the values from before the update: values never rise, and repeating the
+    update reaches a fixed point within four applications (information
+    crosses the element corner by corner), where it is idempotent."""
     xyz, cells, h = cases.kuhn_grid(3)
     rng = np.random.default_rng(2)
     for t in cells[:12]:
@@ -110,10 +114,16 @@ def test_element_update_is_monotone_and_idempotent(oracle_mod):
         Y = np.cross(Z, X)
         L = [np.linalg.norm(AB), AC @ X, AC @ Y, AD @ X, AD @ Y, AD @ Z]
         v0 = rng.uniform(0, 1, 4)
+        v0[rng.integers(4)] = 0.0  # a seed-like corner, so solves lower the others
         v1 = oracle_mod.tet_element_update(L, v0, 64)
         assert np.all(v1 <= v0)
         v2 = oracle_mod.tet_element_update(L, v1, 64)
         assert np.all(v2 <= v1)
+        v = v2
+        for _ in range(3):
+            v = oracle_mod.tet_element_update(L, v, 64)
+        again = oracle_mod.tet_element_update(L, v, 64)
+        np.testing.assert_array_equal(again, v)
 
 
 def _run(oracle_mod, case, steps, precision=64, inside=1, leps=1e-5):
@@ -264,3 +274,34 @@ def test_narrowband_membership_is_the_brute_force_set(oracle_mod):
     nb = set(o.narrowband().tolist())
     expect = {p for p in range(case.nparts) if T[case.labels == p].min() <= case.bandwidth}
     assert nb == expect
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_seed_labels_and_distances_match_brute_force(oracle_mod, dim):
+    """Seeds (kernel_seedlabel, src/core3d/include/redistance_kernels3d.h:623-770):
+    every corner of an element whose values are neither all >= 0 nor all
+    <= 0, and every vertex with phi == 0; seeds keep DT = |phi|. Brute force
+    from the signs of phi0 over every element (the first step scans every
+    patch), with exact zeros planted on and off the front."""
+    case = cases.small3d(n=10, shape="sphere", block=(4, 4, 4)) if dim == 3 else cases.config1()
+    phi0 = case.phi0.copy()
+    rng = np.random.default_rng(4)
+    near = np.flatnonzero(np.abs(phi0) < 0.5 * case.h)
+    far = np.flatnonzero(np.abs(phi0) > 3 * case.h)
+    phi0[rng.choice(near, 3, replace=False)] = 0.0
+    phi0[rng.choice(far, 3, replace=False)] = 0.0  # isolated zeros: seeds through the phi == 0 rule alone
+    v = phi0[case.cells]
+    mixed = ~((v >= 0).all(axis=1) | (v <= 0).all(axis=1))
+    want = np.zeros(phi0.size, bool)
+    want[case.cells[mixed].ravel()] = True
+    want |= phi0 == 0.0
+    o = oracle_mod.Oracle(case.dim, case.xyz, case.cells, case.labels, case.nparts, 64)
+    o.set_phi(phi0)
+    o.set_velocity(case.vel)
+    o.step(oracle_mod.make_params(0.0, case.bandwidth, case.niter))  # dt = 0: phi = signed DT
+    got = o.labels() == 0
+    dist = np.abs(o.phi())
+    o.close()
+    np.testing.assert_array_equal(got, want)
+    np.testing.assert_array_equal(dist[want], np.abs(phi0[want]))
+    assert (~want).any() and np.all(dist[~want] > 0.0)

@@ -272,6 +272,15 @@ int ls_mlp_dataset(ls_ctx* ctx, float* X, float* Y, int64_t* n);
 /* Sum of squared errors over n caller samples (X n x 22, Y n x 4) and the
  * gradient of their mean (sum over the 4 outputs, mean over samples). */
 int ls_mlp_gradient(ls_ctx* ctx, const float* X, const float* Y, int64_t n, float* grad, double* loss);
+/* ls_mlp_gradient without any device allocation (no counterpart in the
+ * reference): for callers that cannot allocate device memory (code under
+ * stream capture, tight loops, hosts that forbid hipMalloc). The samples are
+ * staged and the sums accumulated in the library's static device scratch,
+ * one copy per GPU held until the process ends (about 2.3 MB), so n may not
+ * exceed 16384 (LS_ERR_LIMIT beyond). Calls that overlap on one GPU share
+ * that scratch and race. Same results as ls_mlp_gradient up to the order of
+ * floating-point sums. */
+int ls_mlp_gradient_noalloc(ls_ctx* ctx, const float* X, const float* Y, int64_t n, float* grad, double* loss);
 /* Adam on the GPU over the collected pairs (shuffled each epoch with seed);
  * *loss is the last epoch's mean squared error per output. */
 int ls_mlp_train(ls_ctx* ctx, int epochs, int batch, double lr, uint64_t seed, double* loss);

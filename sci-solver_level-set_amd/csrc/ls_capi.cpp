@@ -485,6 +485,14 @@ int ls_mlp_gradient(ls_ctx* c, const float* X, const float* Y, int64_t n, float*
   return rc == LS_OK ? rc : fail(c, rc, err);
 }
 
+int ls_mlp_gradient_noalloc(ls_ctx* c, const float* X, const float* Y, int64_t n, float* grad, double* loss) {
+  if (!c || !X || !Y || !grad || !loss) return LS_ERR_ARG;
+  if (int rc = engine_ready(c)) return rc;
+  std::string err;
+  const int rc = c->engine->mlp_gradient_noalloc(X, Y, n, grad, loss, err);
+  return rc == LS_OK ? rc : fail(c, rc, err);
+}
+
 int ls_mlp_train(ls_ctx* c, int epochs, int batch, double lr, uint64_t seed, double* loss) {
   if (!c || !loss) return LS_ERR_ARG;
   if (int rc = engine_ready(c)) return rc;

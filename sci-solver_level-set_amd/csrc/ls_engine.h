@@ -40,6 +40,9 @@ class Engine {
   virtual int mlp_gradient(const float* X, const float* Y, int64_t n, float* grad, double* loss,
                            std::string& err) = 0;
   virtual int mlp_train(int epochs, int batch, double lr, uint64_t seed, double* loss, std::string& err) = 0;
+  // mlp_gradient through the library's static scratch: no device allocation.
+  virtual int mlp_gradient_noalloc(const float* X, const float* Y, int64_t n, float* grad, double* loss,
+                                   std::string& err) = 0;
   virtual int get_distances(double* out_user, std::string& err) = 0;
   virtual int set_warm(const double* warm_user, std::string& err) = 0;  // caller order
   virtual int get_warm(double* out_user, std::string& err) = 0;         // caller order

@@ -23,6 +23,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "../../include/levelset_hip.h"
@@ -2004,6 +2005,23 @@ __global__ void k_mlp_pack(const float* prm, uint16_t* w1, uint16_t* w2, uint16_
   if (i < mlp::kOutPad) b3[i] = i < mlp::kOut ? prm[mlp::kOffB3 + i] : 0.f;
 }
 
+// Scratch of the allocation-free gradient entry point
+// (ls_mlp_gradient_noalloc): the staged samples and the gradient
+// accumulator, in one file-scope device array. Each GPU the library runs on
+// holds its own copy until the process ends; host code reaches it only
+// through the address hipGetSymbolAddress returns for that device
+// (scratch_address). Calls that overlap on one GPU share it and race.
+constexpr int64_t kScratchSamples = 16384;
+constexpr size_t kScratchX = (size_t)kScratchSamples * mlp::kInPad;  // padded inputs
+constexpr size_t kScratchY = (size_t)kScratchSamples * mlp::kOut;    // targets
+constexpr size_t kScratchG = (size_t)mlp::kNumParams + 1;             // gradient sums + loss
+static __device__ __align__(256) float scratch[kScratchX + kScratchY + kScratchG];
+
+__global__ void k_zero_f(float* a, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    a[i] = 0.f;
+}
+
 // ---------------------------------------------------------------------------
 // Persistent path: one cooperative launch per timestep. Phases are separated
 // by grid barriers; the FIM active-list loop runs on the device, so a step
@@ -2337,6 +2355,23 @@ static hipError_t dev_alloc(U** dptr, size_t n) {
   return hipMalloc((void**)dptr, std::max<size_t>(n, 1) * sizeof(U));
 }
 
+// Device address of `scratch` on each GPU, resolved once per device with
+// that device current (the array's name in host code is only a stand-in).
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[256] = {};
+static hipError_t scratch_address(int device, float** out) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (device < 0 || device >= 256) return hipErrorInvalidDevice;
+  if (!g_scratch_ptr[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_ptr[device] = p;
+  }
+  *out = static_cast<float*>(g_scratch_ptr[device]);
+  return hipSuccess;
+}
+
 template <typename T, int NC>
 class EngineImpl final : public Engine {
  public:
@@ -2348,6 +2383,7 @@ class EngineImpl final : public Engine {
 
   int upload(const Layout& L, DeviceLayout* D, std::string& err) override {
     LS_HIP(hipSetDevice(device_));
+    LS_HIP(scratch_address(device_, &scratch_));
     if (!stream_) LS_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
     if (!ev_[0]) for (auto& e : ev_) LS_HIP(hipEventCreate(&e));
     P_ = L.P; N_ = L.N; F_ = L.F;
@@ -2613,12 +2649,15 @@ class EngineImpl final : public Engine {
   size_t mlp_grad_lds() const {
     return (size_t)(kMlpBlock / 64) * mlp::kWaveTile * sizeof(float) + (size_t)mlp::kNumParams * sizeof(float);
   }
+  // Gradient sums into g[0, kNumParams) and the loss sum into g[kNumParams]
+  // (the context's accumulator when g is null).
   int mlp_grad_launch(const float* X, const float* Y, const int32_t* idx, int64_t n, float inv_total, hipStream_t q,
-                      std::string& err) {
+                      std::string& err, float* g = nullptr) {
+    if (!g) g = d_mlp_grad_;
     const int64_t tiles = (n + 15) / 16;
     const int grid = (int)std::max<int64_t>(1, std::min<int64_t>((tiles + 3) / 4, 1024));
-    k_mlp_grad<<<grid, kMlpBlock, mlp_grad_lds(), q>>>(X, Y, idx, n, inv_total, mlp_weights(), d_mlp_grad_,
-                                                       d_mlp_grad_ + mlp::kNumParams);
+    k_mlp_grad<<<grid, kMlpBlock, mlp_grad_lds(), q>>>(X, Y, idx, n, inv_total, mlp_weights(), g,
+                                                       g + mlp::kNumParams);
     LS_HIP(hipGetLastError());
     return LS_OK;
   }
@@ -2649,6 +2688,39 @@ class EngineImpl final : public Engine {
     *loss = g[mlp::kNumParams];
     return LS_OK;
   }
+  // The same loss and gradient with the samples staged in `scratch` and the
+  // sums accumulated there: no device allocation. The accumulator is cleared
+  // by a kernel, so the sequence stays graph-capturable.
+  int mlp_gradient_noalloc(const float* X, const float* Y, int64_t n, float* grad, double* loss,
+                           std::string& err) override {
+    if (int rc = mlp_tet_only(err)) return rc;
+    if (!mlp_ready_) { err = "no network parameters set"; return LS_ERR_STATE; }
+    if (n <= 0) { err = "n must be > 0"; return LS_ERR_ARG; }
+    if (n > kScratchSamples) {
+      err = "n = " + std::to_string(n) + " exceeds the " + std::to_string(kScratchSamples) +
+            " samples the allocation-free gradient's scratch holds";
+      return LS_ERR_LIMIT;
+    }
+    if (!scratch_) { err = "scratch not resolved"; return LS_ERR_STATE; }
+    float* dx = scratch_;
+    float* dy = dx + kScratchX;
+    float* dg = dy + kScratchY;
+    hipStream_t q = stream_;
+    k_zero_f<<<(int)((n * mlp::kInPad + 255) / 256), 256, 0, q>>>(dx, n * mlp::kInPad);  // padding columns
+    k_zero_f<<<(int)((kScratchG + 255) / 256), 256, 0, q>>>(dg, (int64_t)kScratchG);
+    LS_HIP(hipGetLastError());
+    LS_HIP(hipMemcpy2DAsync(dx, mlp::kInPad * sizeof(float), X, mlp::kIn * sizeof(float), mlp::kIn * sizeof(float),
+                            (size_t)n, hipMemcpyHostToDevice, q));
+    LS_HIP(hipMemcpyAsync(dy, Y, (size_t)n * mlp::kOut * sizeof(float), hipMemcpyHostToDevice, q));
+    if (int rc = mlp_grad_launch(dx, dy, nullptr, n, 1.0f / (float)n, q, err, dg)) return rc;
+    float lsum = 0.f;
+    LS_HIP(hipMemcpyAsync(grad, dg, mlp::kNumParams * sizeof(float), hipMemcpyDeviceToHost, q));
+    LS_HIP(hipMemcpyAsync(&lsum, dg + mlp::kNumParams, sizeof(float), hipMemcpyDeviceToHost, q));
+    LS_HIP(hipStreamSynchronize(q));
+    *loss = lsum;
+    return LS_OK;
+  }
+
   // Adam over the collected pairs, shuffled every epoch; returns the mean
   // squared error per output of the last epoch.
   int mlp_train(int epochs, int batch, double lr, uint64_t seed, double* loss, std::string& err) override {
@@ -3289,6 +3361,7 @@ class EngineImpl final : public Engine {
   __half* d_hgt_ = nullptr;  // tet copies: corner heights (F x 4), FP16 rounded toward zero
   T* d_warm_ = nullptr;  // learned warm-start distances (N), written by the predictor
   unsigned long long* d_cfl_ = nullptr;  // CFL bound keys (2)
+  float* scratch_ = nullptr;  // this device's `scratch` (scratch_address), never freed
   double* d_ceik_ = nullptr;   // per original element, or null (uniform default)
   double* d_ccurv_ = nullptr;
   float *d_mlp_prm_ = nullptr, *d_mlp_m_ = nullptr, *d_mlp_v_ = nullptr, *d_mlp_grad_ = nullptr, *d_mlp_b_ = nullptr;

@@ -68,7 +68,8 @@ EXPORTS = ["ls_create", "ls_destroy", "ls_last_error", "ls_set_mesh", "ls_set_pr
            "ls_set_mesh_batch", "ls_partition_batch", "ls_set_batch_params", "ls_get_batch_nb_counts",
            "ls_get_distances", "ls_mlp_set_params", "ls_mlp_get_params", "ls_mlp_predict", "ls_mlp_collect",
            "ls_mlp_dataset", "ls_mlp_gradient", "ls_mlp_train", "ls_set_warm",
-           "ls_get_warm", "ls_compute_dt", "ls_set_cfl_coefficients"]
+           "ls_get_warm", "ls_compute_dt", "ls_set_cfl_coefficients",
+           "ls_mlp_gradient_noalloc"]
 MLP_PARAMS = 5892
 
 
@@ -126,6 +127,7 @@ def lib():
         L.ls_mlp_collect.argtypes = [vp, C.c_int, C.c_int64]
         L.ls_mlp_dataset.argtypes = [vp, vp, vp, C.POINTER(C.c_int64)]
         L.ls_mlp_gradient.argtypes = [vp, vp, vp, C.c_int64, vp, C.POINTER(C.c_double)]
+        L.ls_mlp_gradient_noalloc.argtypes = [vp, vp, vp, C.c_int64, vp, C.POINTER(C.c_double)]
         L.ls_mlp_train.argtypes = [vp, C.c_int, C.c_int, C.c_double, C.c_uint64, C.POINTER(C.c_double)]
         _lib = L
     return _lib
@@ -334,13 +336,15 @@ class Solver:
             self._check(self.L.ls_mlp_dataset(self.ctx, X.ctypes.data, Y.ctypes.data, C.byref(n)))
         return X, Y
 
-    def mlp_gradient(self, X, Y):
+    def mlp_gradient(self, X, Y, noalloc=False):
+        """(gradient of the mean loss, loss sum) over caller samples; noalloc
+        uses ls_mlp_gradient_noalloc (no device allocation, n <= 16384)."""
         X = np.ascontiguousarray(X, np.float32)
         Y = np.ascontiguousarray(Y, np.float32)
         g = np.empty(MLP_PARAMS, np.float32)
         loss = C.c_double(0.0)
-        self._check(self.L.ls_mlp_gradient(self.ctx, X.ctypes.data, Y.ctypes.data, X.shape[0], g.ctypes.data,
-                                           C.byref(loss)))
+        fn = self.L.ls_mlp_gradient_noalloc if noalloc else self.L.ls_mlp_gradient
+        self._check(fn(self.ctx, X.ctypes.data, Y.ctypes.data, X.shape[0], g.ctypes.data, C.byref(loss)))
         return g, loss.value
 
     def mlp_train(self, epochs, batch, lr, seed=1):

@@ -101,6 +101,15 @@ struct DevState {
   // CFL bound of the last narrowband as order-preserving keys: [0] the least
   // timestep, [1] the largest inscribed radius.
   unsigned long long* cfl;
+  // Overlapped FIM iterations of the persistent step (one grid barrier per
+  // iteration): a second DT_out, visit tags and convergence flags, each
+  // pair indexed by iteration parity, and nine P-long lists indexed by
+  // iteration mod 3: probe marks [0..2], early members [3..5] (unconverged
+  // members of the previous iteration), members to copy back [6..8].
+  T* DT_out2;
+  int32_t* visitE2;
+  int32_t* bcon2;
+  int32_t* ov_lists;
 };
 
 enum TickIndex : int { kTkSeed = 0, kTkVisit, kTkCopyback, kTkProbe, kTkNext, kTkNb, kTkAdvect, kNumTicks = 8 };
@@ -123,7 +132,13 @@ enum PcIndex : int {
   kPcWorkP = 17,   // [17],[18]: next probe-list entry, by iteration parity
   kPcWorkS = 19,   // next patch to scan for seeds
   kPcWorkA = 20,   // next narrowband patch to advect (per pass)
-  kNumPc = 24
+  // Overlapped FIM iterations, each a triple indexed by iteration mod 3:
+  kPcOvF = 24,     // probe marks made in the iteration
+  kPcOvEl = 27,    // early members of the iteration (unconverged members of the one before)
+  kPcOvMl = 30,    // members of the iteration, copied back in the next one
+  kPcOvW = 33,     // next work item of the iteration's phase (early visits, then probes)
+  kPcOvVis = 36,   // visits of the iteration
+  kNumPc = 40
 };
 
 // Patch-size sums (element copies, owned, halo) per launch kind.

@@ -681,12 +681,33 @@ __device__ __forceinline__ void copy_to_lds(V* dst, const V* src, int n) {
 }
 
 // The patch's convergence flag and min distance, as sweep_patch stores them in
-// bcon / bmin, for every thread of the workgroup.
+// bcon / bmin, for every thread of the workgroup; visited: the call wrote the
+// patch's outputs (a visit, or a probe that continued as one).
 template <typename T>
 struct SweepResult {
   int con;
   T bmin;
+  int visited;
 };
+
+// Buffers of a sweep_patch call: patches whose tag in vtag equals the call's
+// `members` iteration are read from din (their values after that iteration,
+// not yet copied into DT), all others from DT; outputs go to dout and bcon.
+// probe_first: sweep 0 is a neighbour probe (no outputs); a converged probe
+// ends the call, an unconverged one continues from its state as the patch's
+// visit (needs niter >= 2).
+template <typename T>
+struct SweepIO {
+  const T* din;
+  const int32_t* vtag;
+  T* dout;
+  int32_t* bcon;
+  int probe_first;
+};
+template <typename T>
+__device__ __forceinline__ SweepIO<T> default_io(const DevState<T>& s) {
+  return SweepIO<T>{s.DT_out, s.visitE, s.DT_out, s.bcon, 0};
+}
 
 // Marks corner i of element e changed (s_dirty is 4-byte aligned in LDS;
 // lanes updating other elements of the same word do so atomically too).
@@ -722,7 +743,7 @@ constexpr int kOrgElements = 0, kOrgTasks = 1;
 template <typename T, int NC, int BS, int ORG>
 __device__ SweepResult<T> sweep_patch(const DevMesh<T>& m, DevState<T>& s, int p, int niter, int write_out,
                                       T tol, double leps, unsigned char* smem, T* red, int* s_ndirty /*[10]*/,
-                                      int epoch_in, int epoch_out, int members = 0) {
+                                      int epoch_in, int epoch_out, int members, const SweepIO<T> io) {
   constexpr bool tasks = ORG == kOrgTasks;
   constexpr int K = ElemSolve<T, NC>::kCoords;
   __syncthreads();
@@ -754,7 +775,7 @@ __device__ SweepResult<T> sweep_patch(const DevMesh<T>& m, DevState<T>& s, int p
   T* s_new = (T*)(s_task + NC * ne + (NC * ne) % 2);
   const int tid = threadIdx.x;
 
-  const T* own = (members > 0 && s.visitE[p] == members) ? s.DT_out : s.DT;
+  const T* own = (members > 0 && io.vtag[p] == members) ? io.din : s.DT;
   for (int i = tid; i < nv; i += BS) {
     const T v = own[vs + i];
     s_vtx[i] = v;
@@ -764,7 +785,7 @@ __device__ SweepResult<T> sweep_patch(const DevMesh<T>& m, DevState<T>& s, int p
   if (tid == 0) s_voff[nv] = (uint16_t)nslots;
   for (int i = tid; i < nh; i += BS) {
     const int h = m.halo_ids[hs + i];
-    s_vtx[nv + i] = (members > 0 && s.visitE[m.vpatch[h]] == members) ? s.DT_out[h] : s.DT[h];
+    s_vtx[nv + i] = (members > 0 && io.vtag[m.vpatch[h]] == members) ? io.din[h] : s.DT[h];
   }
   copy_to_lds<BS, 8>(s_vsl, m.vslots + sbase, nslots);
   // (A single-sweep visit recomputes the sweep: resuming would leave it none.)
@@ -803,6 +824,8 @@ __device__ SweepResult<T> sweep_patch(const DevMesh<T>& m, DevState<T>& s, int p
   int con_all = 1;
   T vmin = (T)kLargeNum;
   int n_upd = 0, n_solve = 0;
+  int probing = io.probe_first && niter > 1;  // sweep 0 decides a probe
+  int visited = write_out && !probing;
   for (int it = resume ? 1 : 0; it < niter; ++it) {
     if constexpr (tasks) {
       if (tid == 0) *s_ndirty = 0;
@@ -957,7 +980,7 @@ __device__ SweepResult<T> sweep_patch(const DevMesh<T>& m, DevState<T>& s, int p
       }
     }
     __syncthreads();
-    const bool last = (it == niter - 1);
+    const bool last = (it == niter - 1) || probing;
     // Every slot of an owned vertex equals the vertex value after this pass
     // and slots only decrease, so the minimum over the vertex's slots is the
     // running minimum the solves above left in s_vmin; only a vertex whose
@@ -991,15 +1014,27 @@ __device__ SweepResult<T> sweep_patch(const DevMesh<T>& m, DevState<T>& s, int p
         }
         // The patch's slots of v all hold s_vval[v] now; a fresh load gives
         // them outv.
-        if (save_clean) s_same[v] = (uint8_t)(d_min(old, mn) == outv);
-        if (write_out) s.DT_out[vs + v] = outv;
+        if (!probing) {
+          if (save_clean) s_same[v] = (uint8_t)(d_min(old, mn) == outv);
+          if (write_out) io.dout[vs + v] = outv;
+        }
         con_all &= c;
         vmin = d_min(vmin, outv);
       }
     }
+    if (probing) {
+      // The probe sweep decided: a converged probe ends here; an unconverged
+      // one continues as the visit from this state (what a visit resuming
+      // from the probe's saved state computes).
+      if (__syncthreads_and(con_all)) break;
+      probing = 0;
+      visited = write_out;
+      con_all = 1;
+      vmin = (T)kLargeNum;
+    }
   }
   con_all = __syncthreads_and(con_all);
-  if (save_clean) {
+  if (save_clean && visited) {
     for (int e = tid; e < ne; e += BS) {
       const ushort4 cl = m.copy_local[es + e];
       int ok = s_dirty[e] == 0;
@@ -1029,14 +1064,21 @@ __device__ SweepResult<T> sweep_patch(const DevMesh<T>& m, DevState<T>& s, int p
     atomicAdd(s.sizes + kSzSolve, (unsigned long long)n_solve);
   }
   if (tid == 0) {
-    s.bcon[p] = con_all;
-    if (write_out) s.bmin[p] = bm;
-    unsigned long long* sz = s.sizes + (write_out ? kSzVisit : kSzProbe);
-    atomicAdd(sz + 0, (unsigned long long)ne);
-    atomicAdd(sz + 1, (unsigned long long)nv);
-    atomicAdd(sz + 2, (unsigned long long)nh);
+    io.bcon[p] = con_all;
+    if (visited) s.bmin[p] = bm;
+    // byte accounting: a probe and, when it continued, the visit
+    if (io.probe_first || !write_out) {
+      atomicAdd(s.sizes + kSzProbe + 0, (unsigned long long)ne);
+      atomicAdd(s.sizes + kSzProbe + 1, (unsigned long long)nv);
+      atomicAdd(s.sizes + kSzProbe + 2, (unsigned long long)nh);
+    }
+    if (visited) {
+      atomicAdd(s.sizes + kSzVisit + 0, (unsigned long long)ne);
+      atomicAdd(s.sizes + kSzVisit + 1, (unsigned long long)nv);
+      atomicAdd(s.sizes + kSzVisit + 2, (unsigned long long)nh);
+    }
   }
-  return {con_all, bm};
+  return {con_all, bm, visited};
 }
 
 // Narrowband test of one patch (kernel_compute_new_narrowband,
@@ -1575,7 +1617,7 @@ __global__ __launch_bounds__(BS, 4) void k_sweep(DevMesh<T> m, DevState<T> s, co
   __shared__ T red[BS / 64];
   __shared__ int s_ndirty[10];
   sweep_patch<T, NC, BS, ORG>(m, s, list[blockIdx.x], niter, write_out, tol, leps, smem, red, s_ndirty, epoch_in,
-                              epoch_out);
+                              epoch_out, 0, default_io(s));
 }
 
 // CopyOutBack (redistance_kernels3d.h:772-783) plus the active-list label of
@@ -2133,6 +2175,7 @@ __global__ __launch_bounds__(BS, WPE) void k_step(DevMesh<T> m, DevState<T> s, S
     s.freshE[q] = 0;
     s.rs_epoch[q] = 0;
     s.visitE[q] = 0;
+    s.visitE2[q] = 0;
     s.ve_tag[q] = 0;
   }
   if (gtid < kNumSizes && attempt == 0) s.sizes[gtid] = 0ull;
@@ -2149,6 +2192,7 @@ __global__ __launch_bounds__(BS, WPE) void k_step(DevMesh<T> m, DevState<T> s, S
     }
     s.pc[kPcOverflow] = 0;
     s.pc[kPcNb + nb_out] = 0;
+    for (int k = kPcOvF; k < kNumPc; ++k) s.pc[k] = 0;
     if (a.cfl) {
       s.cfl[0] = OrderedBits<double>::of(kLargeNum);
       s.cfl[1] = OrderedBits<double>::of(-1.0);
@@ -2187,94 +2231,110 @@ __global__ __launch_bounds__(BS, WPE) void k_step(DevMesh<T> m, DevState<T> s, S
     grid_barrier(s.bar, G);
   }
   pcl.lap(clk, kTkSeed);
-  // Phase 2: FIM (redistance3d.cu:85-224). Epoch E tags the members of the
-  // current list (inA[p] == E) and the patches probed in this iteration
-  // (freshE[q] == E).
-  int32_t* cur = s.listA;
-  int32_t* nxt = s.listB;
-  int nA = ld_acquire(&s.pc[kPcSeed]);
-  int it = 0;
-  while (nA > 0) {
-    if (a.max_iters > 0 && it >= a.max_iters) {
-      if (gtid == 0) s.pc[kPcOverflow] = 1;
-      break;
-    }
-    const int E = it + 1, par = it & 1;
-    // Visits. Each visiting block then marks, for a probe, every neighbour
-    // of its patch when the patch converged below the band (its own flags
-    // are final once its sweep ends); whether a marked patch is an
-    // unconverged member, which is not probed, is decided in the probe phase.
-    // Patches are taken from a work counter, so a block that drew a cheap
-    // patch takes the next one.
-    for (int i = first_item(nA, &s.pc[kPcWorkV + par], s_work); i < nA;
-         i = later_item(nA, &s.pc[kPcWorkV + par], s_work)) {
-      const int p = cur[i];
-      const SweepResult<T> r =
-          sweep_patch<T, NC, BS, ORG>(m, s, p, a.niter, 1, a.tol, a.leps, smem, red, s_ndirty, E, 0);
-      if (r.con && r.bmin < band_of(m, p, a.bw)) {
-        for (int k = m.padj_off[p] + tid; k < m.padj_off[p + 1]; k += BS) {
-          const int q = m.padj[k];
-          if (atomicMax(&s.freshE[q], E) < E) s.listF[atomicAdd(&s.pc[kPcFresh + par], 1)] = q;
+  // Phase 2: FIM (redistance3d.cu:85-224) with overlapped iterations and one
+  // grid barrier per iteration. The phase of iteration E (after the barrier
+  // that ends iteration E-1) holds, as independent work items: the copy-back
+  // of E-1's members into DT; the visits of E's early members (E-1's
+  // unconverged members, or the first list); and the probes of the patches
+  // E-1's visits marked, each continuing at once as the patch's visit of E
+  // when it finds the patch unconverged. Every item reads the state after
+  // E-1: patches visited in E-1 from that iteration's DT_out (by parity),
+  // all others from DT, so the copy-back can run beside them; E's visits
+  // write the other DT_out. The sets of visits, probes and their inputs are
+  // those of the reference's loop; only the overlap differs.
+  {
+    const int P = m.P;
+    int32_t* ovF = s.ov_lists;          // [3][P] probe marks
+    int32_t* ovE = s.ov_lists + 3 * P;  // [3][P] early members
+    int32_t* ovM = s.ov_lists + 6 * P;  // [3][P] members to copy back
+    int iters = 0;
+    for (int E = 1;; ++E) {
+      const int sE = E % 3, sP = (E + 2) % 3, sN = (E + 1) % 3;
+      // parity buffers by selects (a runtime-indexed array would live in scratch)
+      const bool odd = E & 1;
+      T* const dcur = odd ? s.DT_out2 : s.DT_out;
+      const T* const dprev = odd ? s.DT_out : s.DT_out2;
+      int32_t* const vcur = odd ? s.visitE2 : s.visitE;
+      const int32_t* const vprev = odd ? s.visitE : s.visitE2;
+      int32_t* const bcur = odd ? s.bcon2 : s.bcon;
+      const int32_t* const bprev = odd ? s.bcon : s.bcon2;
+      const int nEarly = E == 1 ? ld_acquire(&s.pc[kPcSeed]) : ld_acquire(&s.pc[kPcOvEl + sE]);
+      const int nProbe = ld_acquire(&s.pc[kPcOvF + sP]);
+      const int nCopy = ld_acquire(&s.pc[kPcOvMl + sP]);
+      if (gtid == 0) {  // slots this phase neither reads nor writes; the next one uses them
+        s.pc[kPcOvF + sN] = 0;
+        s.pc[kPcOvEl + sP] = 0;
+        s.pc[kPcOvMl + sN] = 0;
+        s.pc[kPcOvW + sN] = 0;
+        s.pc[kPcOvVis + sN] = 0;
+      }
+      for (int i = b; i < nCopy; i += G) {  // CopyOutBack of E-1's members
+        const int p = ovM[sP * P + i];
+        const int vs = m.vert_off[p], ve = m.vert_off[p + 1];
+        for (int v = vs + tid; v < ve; v += BS) s.DT[v] = dprev[v];
+      }
+      const bool capped = a.max_iters > 0 && iters >= a.max_iters;
+      if ((nEarly == 0 && nProbe == 0) || capped) {
+        if (capped && nEarly + nProbe > 0 && gtid == 0) s.pc[kPcOverflow] = 1;
+        grid_barrier(s.bar, G);
+        pcl.lap(clk, kTkVisit);
+        break;
+      }
+      const int members = E - 1;
+      const int32_t* early = E == 1 ? s.listA : ovE + sE * P;
+      const int n = nEarly + nProbe;
+      for (int i = first_item(n, &s.pc[kPcOvW + sE], s_work); i < n; i = later_item(n, &s.pc[kPcOvW + sE], s_work)) {
+        // kind 0: a visit; 1: a probe that continues as the visit when the
+        // patch is unconverged; 2: a one-sweep probe (a one-sweep visit then
+        // recomputes its sweep from a fresh load).
+        int p, kind;
+        if (i < nEarly) {
+          p = early[i];
+          kind = 0;
+        } else {
+          p = ovF[sP * P + (i - nEarly)];
+          // an unconverged member of E-1 is one of E's early members
+          if (members > 0 && vprev[p] == members && !bprev[p]) continue;
+          if (tid == 0) atomicAdd(&s.pc[kPcProbes], 1);
+          kind = a.niter > 1 ? 1 : 2;
         }
-      }
-      if (tid == 0) s.visitE[p] = E;
-    }
-    grid_barrier(s.bar, G);
-    pcl.lap(clk, kTkVisit);
-    // Every block has read this iteration's list size, so the counters the
-    // next iteration appends to can be cleared.
-    if (gtid == 0) {
-      s.pc[kPcIters] += 1;
-      s.pc[kPcVisits] += nA;
-      s.pc[kPcFresh + (1 - par)] = 0;
-      s.pc[kPcNext + (1 - par)] = 0;
-      s.pc[kPcWorkV + (1 - par)] = 0;
-      s.pc[kPcWorkP + (1 - par)] = 0;
-    }
-    // CopyOutBack of the members, the probes and the next list in one
-    // phase. Probes read the members' vertices from DT_out while the copy
-    // runs. Next list: unconverged members that were not marked, marked
-    // unconverged members (kept without a probe), and each probed patch
-    // that stays unconverged, appended by the block that handled it. The
-    // list order is immaterial (patches of one iteration never read each
-    // other's outputs).
-    const int nF = ld_acquire(&s.pc[kPcFresh + par]);
-    for (int i = b; i < nA; i += G) {
-      const int p = cur[i];
-      const int vs = m.vert_off[p], ve = m.vert_off[p + 1];
-      for (int v = vs + tid; v < ve; v += BS) s.DT[v] = s.DT_out[v];
-      if (tid == 0 && s.freshE[p] != E && !s.bcon[p]) {
-        nxt[atomicAdd(&s.pc[kPcNext + par], 1)] = p;
-        s.inA[p] = E + 1;
-      }
-    }
-    for (int i = first_item(nF, &s.pc[kPcWorkP + par], s_work); i < nF;
-         i = later_item(nF, &s.pc[kPcWorkP + par], s_work)) {
-      const int q = s.listF[i];
-      if (s.visitE[q] == E && !s.bcon[q]) {  // unconverged member: not probed
+        SweepResult<T> r;
+        for (;;) {
+          r = sweep_patch<T, NC, BS, ORG>(m, s, p, kind == 2 ? 1 : a.niter, kind == 2 ? 0 : 1, a.tol, a.leps, smem,
+                                          red, s_ndirty, 0, 0, members,
+                                          SweepIO<T>{dprev, vprev, dcur, bcur, kind == 1 ? 1 : 0});
+          if (kind == 2 && !r.con) {
+            kind = 0;
+            continue;
+          }
+          break;
+        }
+        if (!r.visited) continue;
+        // p is a member of E: mark its neighbours for probes when it
+        // converged below the band, keep it for E+1 when it did not, and
+        // copy it back in E+1's phase.
+        if (r.con && r.bmin < band_of(m, p, a.bw)) {
+          for (int k = m.padj_off[p] + tid; k < m.padj_off[p + 1]; k += BS) {
+            const int q = m.padj[k];
+            if (atomicMax(&s.freshE[q], E) < E) ovF[sE * P + atomicAdd(&s.pc[kPcOvF + sE], 1)] = q;
+          }
+        }
         if (tid == 0) {
-          nxt[atomicAdd(&s.pc[kPcNext + par], 1)] = q;
-          s.inA[q] = E + 1;
+          vcur[p] = E;
+          ovM[sE * P + atomicAdd(&s.pc[kPcOvMl + sE], 1)] = p;
+          if (!r.con) ovE[sN * P + atomicAdd(&s.pc[kPcOvEl + sN], 1)] = p;
+          atomicAdd(&s.pc[kPcOvVis + sE], 1);
+          atomicAdd(&s.pc[kPcVisits], 1);
         }
-        continue;
       }
-      if (tid == 0) atomicAdd(&s.pc[kPcProbes], 1);
-      const SweepResult<T> r =
-          sweep_patch<T, NC, BS, ORG>(m, s, q, 1, 0, a.tol, a.leps, smem, red, s_ndirty, 0, E + 1, E);
-      if (tid == 0 && !r.con) {
-        nxt[atomicAdd(&s.pc[kPcNext + par], 1)] = q;
-        s.inA[q] = E + 1;
+      grid_barrier(s.bar, G);
+      pcl.lap(clk, kTkVisit);
+      if (ld_acquire(&s.pc[kPcOvVis + sE]) > 0) {
+        ++iters;
+        if (gtid == 0) s.pc[kPcIters] += 1;
       }
     }
-    grid_barrier(s.bar, G);
-    pcl.lap(clk, kTkProbe);
-    nA = ld_acquire(&s.pc[kPcNext + par]);
-    int32_t* t = cur;
-    cur = nxt;
-    nxt = t;
-    ++it;
   }
-
 
   if (!warm) break;
   // Phase 2b: acceptance over the patches that received warm values (the
@@ -2486,6 +2546,11 @@ class EngineImpl final : public Engine {
     LS_HIP(dev_alloc(&d_phi_out_, N_));
     LS_HIP(dev_alloc(&d_DT_, N_));
     LS_HIP(dev_alloc(&d_DT_out_, N_));
+    LS_HIP(dev_alloc(&d_DT_out2_, N_));
+    LS_HIP(dev_alloc(&d_visitE2_, (size_t)P_));
+    LS_HIP(hipMemset(d_visitE2_, 0, (size_t)P_ * sizeof(int32_t)));
+    LS_HIP(dev_alloc(&d_bcon2_, (size_t)P_));
+    LS_HIP(dev_alloc(&d_ov_lists_, (size_t)9 * P_));
     LS_HIP(dev_alloc(&d_label_, N_));
     LS_HIP(dev_alloc(&d_tsign_, N_));
     LS_HIP(dev_alloc(&d_bcon_, P_));
@@ -3304,6 +3369,10 @@ class EngineImpl final : public Engine {
     s.ve_ok = d_ve_ok_;
     s.ve_tag = d_ve_tag_;
     s.cfl = d_cfl_;
+    s.DT_out2 = d_DT_out2_;
+    s.visitE2 = d_visitE2_;
+    s.bcon2 = d_bcon2_;
+    s.ov_lists = d_ov_lists_;
     return s;
   }
   int32_t* d_nb_cur() const { return nb_sel_ ? d_lists_[3] : d_nb_; }
@@ -3318,7 +3387,8 @@ class EngineImpl final : public Engine {
                     d_patch_dt_, d_patch_bw_, d_warm_, d_hgt_, d_mlp_prm_, d_mlp_m_, d_mlp_v_, d_mlp_grad_,
                     d_mlp_b_, d_mlp_w1_, d_mlp_w2_, d_mlp_w3_, d_ds_x_, d_ds_y_, d_pend_, d_xyz_,
                     d_copy_verts_, d_copy_src_, d_rs_slot_, d_rs_vval_, d_rs_dirty_, d_rs_epoch_, d_visitE_,
-                    d_vpatch_, d_ve_ok_, d_ve_tag_, d_cfl_, d_ceik_, d_ccurv_};
+                    d_vpatch_, d_ve_ok_, d_ve_tag_, d_cfl_, d_ceik_, d_ccurv_, d_DT_out2_, d_visitE2_,
+                    d_bcon2_, d_ov_lists_};
     if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
     if (graph_) (void)hipGraphDestroy(graph_);
     for (void* p : ptrs) if (p) (void)hipFree(p);
@@ -3362,6 +3432,9 @@ class EngineImpl final : public Engine {
   T* d_warm_ = nullptr;  // learned warm-start distances (N), written by the predictor
   unsigned long long* d_cfl_ = nullptr;  // CFL bound keys (2)
   float* scratch_ = nullptr;  // this device's `scratch` (scratch_address), never freed
+  // overlapped FIM iterations of the persistent step (see DevState)
+  T* d_DT_out2_ = nullptr;
+  int32_t *d_visitE2_ = nullptr, *d_bcon2_ = nullptr, *d_ov_lists_ = nullptr;
   double* d_ceik_ = nullptr;   // per original element, or null (uniform default)
   double* d_ccurv_ = nullptr;
   float *d_mlp_prm_ = nullptr, *d_mlp_m_ = nullptr, *d_mlp_v_ = nullptr, *d_mlp_grad_ = nullptr, *d_mlp_b_ = nullptr;

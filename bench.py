@@ -583,10 +583,13 @@ def main():
     roof["traffic_source"] = (f"profiles/{prof_name}.md (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)"
                               if traffic else None)
     roof["compute"] = compute_fraction(tot, args.steps, kmode)
-    roof["limiter"] = ("latency: each FIM iteration lasts its slowest patch visit plus its slowest probe "
-                       "(dependent FP64 solve chains, four workgroup barriers per sweep), two grid barriers "
-                       "per iteration, 14-260 active patches for 256 CUs; neither HBM nor the FP64 VALUs "
-                       "are saturated")
+    roof["limiter"] = (("latency: each FIM launch lasts its slowest patch visit (dependent FP64 solve "
+                        "chains, four workgroup barriers per sweep), 14-260 active patches for 256 CUs; "
+                        "neither HBM nor the FP64 VALUs are saturated") if kmode else
+                       ("latency: each FIM iteration lasts its slowest patch visit or probe-then-visit "
+                        "(dependent FP64 solve chains, four workgroup barriers per sweep), one grid barrier "
+                        "per iteration, 14-260 active patches for 256 CUs; neither HBM nor the FP64 VALUs "
+                        "are saturated"))
 
     out = {
         "metric": "narrowband vertex-updates/s, learned warm-start vs plain FIM @ Linf<=1e-5",

@@ -9,14 +9,21 @@ phi for a step (the set CopyOutBack_levelset3d writes,
 src/core3d/include/meshFIM_kernels.h:22-34). One bench "step" is one full
 timestep (seeds, sign save, FIM, narrowband + sign restore, advection).
 
-Besides the JSON line's throughput it reports:
-  * roofline of the dominant kernel (the FIM patch sweep), from HIP-event
-    durations of its launches and the algorithmic bytes of the patches they
-    visited;
-  * the CPU baseline: the oracle's FP32 instance (same scheme, OpenMP over
-    patches, all host threads) on a bounded sample of the same run;
-  * parity: the GPU against the FP32 oracle (bit-exact) and the FP64 oracle
-    (L-inf on phi, narrowband membership diff) on the first timesteps.
+Storage is FP64 (the engine bit-exact with an FP64 host build); FIM
+tolerance 1e-6. Besides the JSON line's throughput it reports:
+  * roofline of one timestep's launches (the persistent seeds/FIM/narrowband
+    kernel and the advection after it), from HIP-event durations and the
+    algorithmic bytes of the patches they processed, with the FP64 compute
+    rate beside it and the counter-measured HBM traffic from profiles/;
+  * parity: the FP64 oracle run beside the engine from phi0 over the timed
+    steps, compared on every finite vertex, in the band and by narrowband;
+  * the CPU baseline: that oracle trajectory's time (OpenMP over patches,
+    all host threads);
+  * the FP32-storage engine on the same steps with its trajectory error, and
+    the learned and identity warm starts against plain FIM.
+--workload mcf / batch run the 192^3 mean-curvature flow and the 512-mesh
+batch (FP32 storage), checked along a trajectory against the oracle's FP32
+(bit for bit) and FP64 instances on a bounded sample.
 """
 import argparse
 import json
@@ -106,6 +113,81 @@ def trajectory(case, oracle_mod, lsb, steps, engines, prm_kw):
     return res, {"cpu_s": cpu_s, "cpu_verts": cpu_v, "steps": done}
 
 
+class SideBySide:
+    """An engine checked along a trajectory against the oracle, each stepping
+    from phi0 with no restart: bit for bit against the oracle instance of
+    the engine's storage type, and by L-inf against the FP64 instance over
+    every vertex both hold finite and inside the band (for an FP64 engine
+    the two are one instance). One entry per mesh; the time of the instance
+    of the engine's storage type is the CPU baseline."""
+
+    def __init__(self, O, steps, engine_bits=32):
+        self.O, self.steps, self.bits, self.meshes = O, steps, engine_bits, []
+        self.cpu_s, self.cpu_v = 0.0, 0
+        self.r = {"steps": 0, f"fp{engine_bits}_bit_mismatches": 0, f"fp{engine_bits}_nb_diff": 0,
+                  "linf_vs_fp64": 0.0, "linf_vs_fp64_in_band": 0.0, "fp64_finite_set_mismatches": 0,
+                  "fp64_nb_diff": 0, "linf_vs_fp64_per_step": []}
+
+    def add(self, dim, xyz, cells, labels, nparts, phi0, vel, params, bandwidth):
+        pair = []
+        for bits in sorted({self.bits, 64}):
+            o = self.O.Oracle(dim, xyz, cells, labels, nparts, bits)
+            o.set_phi(phi0)
+            o.set_velocity(vel)
+            pair.append(o)
+        self.meshes.append((pair[0], pair[-1], params, bandwidth))
+
+    def step(self, phis, nbs):
+        """One step of every oracle pair against the engine's phi and its
+        narrowband, per mesh: the patch ids, or the narrowband's size where
+        the engine numbers patches differently (batches)."""
+        step_linf = 0.0
+        for (o32, o64, prm, bw), phi, nb in zip(self.meshes, phis, nbs):
+            t0 = time.perf_counter()
+            st = o32.step(prm)
+            self.cpu_s += time.perf_counter() - t0
+            self.cpu_v += st.nb_verts
+            if o64 is not o32:
+                o64.step(prm)
+            key = f"fp{self.bits}_"
+            self.r[key + "bit_mismatches"] += int(np.count_nonzero(phi != o32.phi()) if self.bits == 64 else
+                                                  np.count_nonzero(phi.astype(np.float32) != o32.phi().astype(np.float32)))
+            self.r[key + "nb_diff"] += self._nb_diff(nb, o32.narrowband())
+            ophi = o64.phi()
+            ofin, gfin = np.abs(ophi) < 1e8, np.abs(phi) < 1e8
+            fin = ofin & gfin
+            d = np.abs(phi.astype(np.float64) - ophi)
+            inb = fin & (np.abs(ophi) <= bw)
+            if fin.any():
+                step_linf = max(step_linf, float(np.max(d[fin])))
+            if inb.any():
+                self.r["linf_vs_fp64_in_band"] = max(self.r["linf_vs_fp64_in_band"], float(np.max(d[inb])))
+            self.r["fp64_finite_set_mismatches"] += int(np.count_nonzero(ofin != gfin))
+            self.r["fp64_nb_diff"] += self._nb_diff(nb, o64.narrowband())
+        self.r["linf_vs_fp64"] = max(self.r["linf_vs_fp64"], step_linf)
+        self.r["linf_vs_fp64_per_step"].append(float(f"{step_linf:.4g}"))
+        self.r["steps"] += 1
+
+    @staticmethod
+    def _nb_diff(nb, onb):
+        if np.isscalar(nb):
+            return abs(int(nb) - int(onb.size))
+        return int(np.setxor1d(nb, onb).size)
+
+    def finish(self, sample):
+        for o32, o64, _, _ in self.meshes:
+            o32.close()
+            if o64 is not o32:
+                o64.close()
+        cpu = {"value": self.cpu_v / self.cpu_s if self.cpu_s > 0 else None, "unit": "vertex-updates/s",
+               "cores": self.O.max_threads(), "kind": "port", "sample": sample}
+        par = dict(self.r, oracle=("FP64 host build of the same scheme, from phi0" if self.bits == 64 else
+                                   "FP32-storage and FP64 host builds of the same scheme, each from phi0"),
+                   tolerance=1e-5, meets_bar=self.r[f"fp{self.bits}_bit_mismatches"] == 0
+                   and self.r["linf_vs_fp64"] <= 1e-5)
+        return cpu, par
+
+
 def profiled_traffic(name):
     """HBM bytes per launch that the committed rocprofv3 counter summary
     (profiles/<name>.md, FETCH_SIZE and WRITE_SIZE scaled by a 1 GiB copy)
@@ -191,23 +273,24 @@ def batch_workload(args, lsb, cases):
     patch size and execution mode chosen by the autotuner on a 64-mesh
     subset. CPU baseline and parity on a sample of the meshes."""
     b = cases.config4(steps=args.steps)
+    prec = lsb.LS_F64 if args.precision == "f64" else lsb.LS_F32
     print(f"batch: {len(b.meshes)} meshes, {b.n_verts} vertices, {b.n_cells} tets", file=sys.stderr, flush=True)
     if args.batch_patch != "auto" and args.exec != "auto":
         names = {"graph": lsb.LS_EXEC_GRAPH, "persistent": lsb.LS_EXEC_PERSISTENT, "kernels": lsb.LS_EXEC_KERNELS}
         cand, ex, table = int(args.batch_patch), names[args.exec], []
     else:
         sub = cases.Batch(b.name + "_tune", 3, b.meshes[:64], patch_verts=b.patch_verts)
-        cand, ex, ms, table = lsb.autotune_batch(sub, [64, 128, 256, 512], trial_steps=2,
+        cand, ex, ms, table = lsb.autotune_batch(sub, [64, 128, 256, 512], trial_steps=2, precision=prec,
                                                  log=lambda m: print(m, file=sys.stderr, flush=True))
     t0 = time.perf_counter()
-    g = lsb.solver_for_batch(b, execution=ex, patch_verts=cand)
+    g = lsb.solver_for_batch(b, execution=ex, patch_verts=cand, precision=prec)
     setup_s = time.perf_counter() - t0
     info = g.layout_info()
     prm = lsb.step_params(1.0, 1.0, 5)
     for _ in range(args.warmup):
         g.step(prm)
     g.set_phi(np.concatenate([m.phi0 for m in b.meshes]))
-    tot = timed_run(g, prm, args.steps)
+    tot = timed_run(g, prm, args.steps, w=8 if prec == lsb.LS_F64 else 4)
     g.close()
     out = {"workload": f"{b.name}: {len(b.meshes)} Kuhn meshes, {b.n_verts} verts, {b.n_cells} tets, "
                        f"ellipsoids, affine velocity, CFL 0.5, band 6h, {args.steps} timesteps",
@@ -221,57 +304,27 @@ def batch_workload(args, lsb, cases):
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle as O
         sample = cases.Batch(b.name + "_sample", 3, b.meshes[:24], patch_verts=cand)
-        gs = lsb.solver_for_batch(sample, execution=ex, patch_verts=cand)
-        orc = []
+        gs = lsb.solver_for_batch(sample, execution=ex, patch_verts=cand, precision=prec)
+        cmp = SideBySide(O, args.cpu_steps, engine_bits=64 if prec == lsb.LS_F64 else 32)
         for m in sample.meshes:
             nparts = -(-m.xyz.shape[0] // cand)
-            o = O.Oracle(3, m.xyz, m.cells, cases.partition_rcb(m.xyz, nparts), nparts, 32)
-            o.set_phi(m.phi0)
-            o.set_velocity(m.vel)
-            orc.append(o)
-        cpu_s, cpu_v, mism, nbd = 0.0, 0, 0, 0
+            cmp.add(3, m.xyz, m.cells, cases.partition_rcb(m.xyz, nparts), nparts, m.phi0, m.vel,
+                    O.make_params(m.dt, m.bandwidth, 5), m.bandwidth)
         for _ in range(args.cpu_steps):
             gs.step(prm)
-            counts = gs.batch_nb_counts()
-            for k, (o, phi) in enumerate(zip(orc, gs.split(gs.phi()))):
-                t0 = time.perf_counter()
-                ost = o.step(O.make_params(sample.meshes[k].dt, sample.meshes[k].bandwidth, 5))
-                cpu_s += time.perf_counter() - t0
-                cpu_v += ost.nb_verts
-                mism += int(np.count_nonzero(phi.astype(np.float32) != o.phi().astype(np.float32)))
-                nbd += abs(int(counts[k]) - ost.nb_count)
-        for o in orc:
-            o.close()
-        # One more step of every sample mesh checked against the FP64 oracle
-        # from the GPU's state (the oracle scans every patch for seeds).
-        o64s = []
-        for m, phi in zip(sample.meshes, gs.split(gs.phi())):
-            nparts = -(-m.xyz.shape[0] // cand)
-            o = O.Oracle(3, m.xyz, m.cells, cases.partition_rcb(m.xyz, nparts), nparts, 64)
-            o.set_velocity(m.vel)
-            o.set_phi(phi)
-            o64s.append(o)
-        gs.step(prm)
-        l64 = {"linf_vs_fp64": 0.0, "linf_vs_fp64_in_band": 0.0, "linf_vs_fp64_incl_threshold_ambiguous": 0.0,
-               "threshold_ambiguous_verts": 0}
-        for k, (o, phi) in enumerate(zip(o64s, gs.split(gs.phi()))):
-            o.step(O.make_params(sample.meshes[k].dt, sample.meshes[k].bandwidth, 5))
-            e = fp64_error(phi, o.phi(), o.ambiguous(), sample.meshes[k].bandwidth)
-            l64 = {key: max(l64[key], e[key]) if key != "threshold_ambiguous_verts" else l64[key] + e[key]
-                   for key in l64}
-            o.close()
+            cmp.step(gs.split(gs.phi()), [int(c) for c in gs.batch_nb_counts()])
         gs.close()
-        cpu = ({"value": cpu_v / cpu_s, "unit": "vertex-updates/s", "cores": O.max_threads(), "kind": "port",
-                "sample": f"first {args.cpu_steps} timesteps of the first 24 meshes, oracle FP32 instance, "
-                          "OpenMP over patches"},
-               dict({"fp32_oracle_meshes": 24, "fp32_oracle_steps": args.cpu_steps, "fp32_bit_mismatches": mism,
-                     "nb_count_diff": nbd, "fp64_oracle_steps": 1}, **l64, tolerance=1e-5))
+        cpu, parity = cmp.finish(f"first {args.cpu_steps} timesteps of the first 24 meshes from phi0, the "
+                                 f"oracle's {args.precision.upper()} instance (the scheme of this engine), "
+                                 "OpenMP over patches")
+        cpu = (cpu, dict(parity, sample_meshes=24))
     return out, tot, ex == lsb.LS_EXEC_KERNELS, cpu
 
 
 def mcf_workload(args, lsb, cases):
     """Mean-curvature flow of 8 spheres on a 192^3 Kuhn grid (41.8M tets)."""
     case = cases.config3(steps=args.steps)
+    prec = lsb.LS_F64 if args.precision == "f64" else lsb.LS_F32
     print(f"mcf: {case.xyz.shape[0]} vertices, {case.cells.shape[0]} tets", file=sys.stderr, flush=True)
     names = {"graph": lsb.LS_EXEC_GRAPH, "persistent": lsb.LS_EXEC_PERSISTENT, "kernels": lsb.LS_EXEC_KERNELS}
     prm = lsb.step_params(case.dt, case.bandwidth, case.niter, mode=lsb.LS_MODE_MCF)
@@ -282,7 +335,7 @@ def mcf_workload(args, lsb, cases):
         for blk in [(4, 4, 4), (8, 4, 4), (8, 8, 4), (8, 8, 8)]:
             case.labels, case.nparts = cases.partition_blocks(192, blk, 3)
             try:
-                gt = lsb.solver_for_case(case, execution=lsb.LS_EXEC_PERSISTENT)
+                gt = lsb.solver_for_case(case, precision=prec, execution=lsb.LS_EXEC_PERSISTENT)
             except lsb.LevelSetError as e:
                 print(f"mcf block {blk} skipped: {e}", file=sys.stderr, flush=True)
                 continue
@@ -296,7 +349,7 @@ def mcf_workload(args, lsb, cases):
         block = tuple(int(x) for x in args.mcf_block.split("x"))
     case.labels, case.nparts = cases.partition_blocks(192, block, 3)
     t0 = time.perf_counter()
-    g = lsb.solver_for_case(case)
+    g = lsb.solver_for_case(case, precision=prec)
     setup_s = time.perf_counter() - t0
     info = g.layout_info()
     tune = None
@@ -308,7 +361,7 @@ def mcf_workload(args, lsb, cases):
     for _ in range(args.warmup):
         g.step(prm)
     g.set_phi(case.phi0)
-    tot = timed_run(g, prm, args.steps)
+    tot = timed_run(g, prm, args.steps, w=8 if prec == lsb.LS_F64 else 4)
     g.close()
     out = {"workload": f"{case.name}: 192^3 Kuhn tets ({case.xyz.shape[0]} verts, {case.cells.shape[0]} tets), "
                        f"8 spheres, mean-curvature flow, dt = h^2/8, band 8h, {args.steps} timesteps",
@@ -319,33 +372,18 @@ def mcf_workload(args, lsb, cases):
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle as O
-        g = lsb.solver_for_case(case, execution=ex)
-        o = O.Oracle(3, case.xyz, case.cells, case.labels, case.nparts, 32)
-        o.set_phi(case.phi0)
-        o.set_velocity(case.vel)
-        cpu_s, cpu_v, mism = 0.0, 0, 0
-        for _ in range(args.cpu_steps):
+        nsteps = min(args.cpu_steps, 4)
+        g = lsb.solver_for_case(case, precision=prec, execution=ex)
+        cmp = SideBySide(O, nsteps, engine_bits=64 if prec == lsb.LS_F64 else 32)
+        cmp.add(case.dim, case.xyz, case.cells, case.labels, case.nparts, case.phi0, case.vel,
+                O.make_params(case.dt, case.bandwidth, case.niter, mode=1), case.bandwidth)
+        for _ in range(nsteps):
             g.step(prm)
-            t0 = time.perf_counter()
-            ost = o.step(O.make_params(case.dt, case.bandwidth, case.niter, mode=1))
-            cpu_s += time.perf_counter() - t0
-            cpu_v += ost.nb_verts
-            mism += int(np.count_nonzero(g.phi().astype(np.float32) != o.phi().astype(np.float32)))
-        o.close()
-        # One more step checked against the FP64 oracle from the GPU's state.
-        o64 = O.Oracle(3, case.xyz, case.cells, case.labels, case.nparts, 64)
-        o64.set_velocity(case.vel)
-        o64.set_phi(g.phi())
-        o64.set_narrowband(g.narrowband())
-        g.step(prm)
-        o64.step(O.make_params(case.dt, case.bandwidth, case.niter, mode=1))
-        l64 = fp64_error(g.phi(), o64.phi(), o64.ambiguous(), case.bandwidth)
-        o64.close()
+            cmp.step([g.phi()], [g.narrowband()])
         g.close()
-        cpu = ({"value": cpu_v / cpu_s, "unit": "vertex-updates/s", "cores": O.max_threads(), "kind": "port",
-                "sample": f"first {args.cpu_steps} timesteps of the same workload, oracle FP32 instance"},
-               dict({"fp32_oracle_steps": args.cpu_steps, "fp32_bit_mismatches": mism, "fp64_oracle_steps": 1},
-                    **l64, tolerance=1e-5))
+        cpu, parity = cmp.finish(f"first {nsteps} timesteps of the same workload from phi0, the oracle's "
+                                 f"{args.precision.upper()} instance (the scheme of this engine), OpenMP over patches")
+        cpu = (cpu, parity)
     return out, tot, ex == lsb.LS_EXEC_KERNELS, cpu
 
 
@@ -480,7 +518,7 @@ def main():
     ap.add_argument("--parity-steps", type=int, default=0,
                     help="timesteps of the trajectory compared with the FP64 oracle (0: the timed steps)")
     ap.add_argument("--precision", default="f64", choices=["f64", "f32"],
-                    help="storage of the torus engine: f64 (default; bit-exact with the FP64 oracle) or f32")
+                    help="storage of the engine: f64 (default; bit-exact with the FP64 oracle) or f32")
     ap.add_argument("--tol", type=float, default=1e-6,
                     help="FIM tolerance of the torus workload (1e-6: the plain method's own result lies "
                          "within 1e-6 of the scheme's fixed point inside the band)")
@@ -530,14 +568,15 @@ def main():
         out = {"metric": "narrowband vertex-updates/s, learned warm-start vs plain FIM @ Linf<=1e-5",
                "value": tot["verts"] / tot["wall"], "unit": "vertex-updates/s", "n_gpus": 1, "steps": args.steps,
                "warmup": args.warmup, "ms_per_step": 1e3 * tot["wall"] / args.steps, "higher_is_better": True,
-               "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+               "scaling": "weak", "vs_baseline": None, "dtype": args.precision, "data": "synthetic",
                "config": dict(cfg, mode="plain_fim", gpu_event_ms_per_step=tot["ms_total"] / args.steps,
                               fim_iters_per_step=tot["fim_iters"] / args.steps,
                               exec_modes_used=sorted(tot["exec_modes"]),
                               step_wg_per_cu=sorted(tot["step_wg_per_cu"])),
                "roofline": roofline(tot, args.steps, kmode)}
         if not kmode:
-            prof_name = "k_step_batch_512" if args.workload == "batch" else "k_step_mcf_spheres_192"
+            prof_name = ("k_step_batch_512" if args.workload == "batch" else "k_step_mcf_spheres_192") + \
+                ("_f64" if args.precision == "f64" else "")
             traffic = profiled_traffic(prof_name)
             if traffic is not None:
                 out["roofline"]["traffic"] = traffic

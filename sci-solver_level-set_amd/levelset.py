@@ -405,12 +405,12 @@ def solver_for_batch(batch, device=0, execution=LS_EXEC_GRAPH, patch_verts=None,
 
 
 def autotune_batch(batch, candidates, executions=(LS_EXEC_GRAPH, LS_EXEC_PERSISTENT, LS_EXEC_KERNELS),
-                   trial_steps=3, device=0, log=None):
+                   trial_steps=3, device=0, log=None, precision=LS_F32):
     """Times `trial_steps` steps of the batch for every patching candidate and
     execution mode and returns the fastest as (candidate, execution, ms/step,
     table). A candidate is either an int (recursive-bisection patch size) or
     a (name, labels, nparts) tuple (caller block partition). Every trial
-    starts from the batch's initial state."""
+    starts from the batch's initial state, with the given storage precision."""
     table = []
     best = None
     for cand in candidates:
@@ -418,7 +418,7 @@ def autotune_batch(batch, candidates, executions=(LS_EXEC_GRAPH, LS_EXEC_PERSIST
             kw = {"patch_verts": cand} if isinstance(cand, int) else {"labels": (cand[1], cand[2])}
             name = cand if isinstance(cand, int) else cand[0]
             try:
-                s = solver_for_batch(batch, device=device, execution=ex, **kw)
+                s = solver_for_batch(batch, device=device, execution=ex, precision=precision, **kw)
             except LevelSetError as e:  # a patch of this size exceeds the device's LDS
                 if "rc=-4" not in str(e):
                     raise

@@ -28,6 +28,7 @@ struct DevMesh {
   const T* coords;             // F x (6 | 3), local coordinates
   const T* sigma;              // F x (3 | 2), velocity in the local frame
   const __half* hgt;           // F x 4 corner heights over the opposite face, rounded toward zero (tets), or null
+  const float* pscale;         // P: largest coordinate scale of a patch's tet copies, rounded up (tets), or null
   const int32_t* padj_off;     // P+1
   const int32_t* padj;
   const int32_t* vpatch;       // N, patch of each renumbered vertex

@@ -276,16 +276,19 @@ struct ElemSolve<T, 4> {
     h[2] = d_abs(L[2] * L[5]) / d_sqrt(L[4] * L[4] + L[5] * L[5]);
     h[3] = d_abs(L[5]);
   }
+  // Sum of the absolute local coordinates: the size term of the skip's
+  // rounding margin.
+  __device__ static double scale(const double* L) {
+    return L[0] + d_abs(L[1]) + d_abs(L[2]) + d_abs(L[3]) + d_abs(L[4]) + d_abs(L[5]);
+  }
   // Bit k set: corner k must be solved (its value is not below the exact
   // lower bound min(other values) + height of corner k over its face).
-  // `hs` holds the copy's heights in FP16 rounded toward zero (a smaller
-  // height only solves more corners, so the skip stays exact).
-  __device__ static int need_mask(const T* Lt, const __half* hs, const T* v) {
-    double L[6];
-#pragma unroll
-    for (int k = 0; k < 6; ++k) L[k] = (double)Lt[k];
+  // `hs` holds the copy's heights in FP16 rounded toward zero and `scale` is
+  // at least the copy's coordinate scale (the patch's largest): a smaller
+  // height or a larger margin only solves more corners, so the skip stays
+  // exact, and the pass reads 8 bytes per copy instead of the coordinates.
+  __device__ static int need_mask(const __half* hs, const T* v, double scale) {
     const double Tv[4] = {(double)v[0], (double)v[1], (double)v[2], (double)v[3]};
-    const double scale = L[0] + d_abs(L[1]) + d_abs(L[2]) + d_abs(L[3]) + d_abs(L[4]) + d_abs(L[5]);
     int mask = 0;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
@@ -343,7 +346,7 @@ struct ElemSolve<T, 4> {
 template <typename T>
 struct ElemSolve<T, 3> {
   static constexpr int kCoords = 3;
-  __device__ static int need_mask(const T* ct, const __half*, const T* v) {
+  __device__ static int need_mask(const T* ct, const T* v) {
     const double c[3] = {(double)ct[0], (double)ct[1], (double)ct[2]};
     const double LenAB = d_abs(c[0]);
     const double LenBC = d_sqrt(c[2] * c[2] + (c[1] - c[0]) * (c[1] - c[0]));
@@ -389,10 +392,19 @@ struct ElemSolve<T, 3> {
   }
 };
 
-// Stored corner heights of a tet copy (null for triangles).
-template <typename T>
-__device__ __forceinline__ const __half* copy_heights(const DevMesh<T>& m, int64_t c) {
-  return m.hgt ? m.hgt + c * 4 : nullptr;
+// Need mask of element copy c of patch p with values x: tets from the
+// stored heights and the patch's scale bound, triangles from their
+// coordinates.
+template <typename T, int NC>
+__device__ __forceinline__ int copy_need_mask(const DevMesh<T>& m, int p, int64_t c, const T* x) {
+  if constexpr (NC == 4) {
+    return ElemSolve<T, 4>::need_mask(m.hgt + c * 4, x, (double)m.pscale[p]);
+  } else {
+    T L[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) L[q] = m.coords[c * 3 + q];
+    return ElemSolve<T, 3>::need_mask(L, x);
+  }
 }
 
 // Local frame and coordinates of one element copy in FP64 (scheme of
@@ -491,6 +503,28 @@ __global__ void k_heights(const T* coords, int64_t F, __half* hgt) {
   ElemSolve<T, 4>::heights(L, h);
 #pragma unroll
   for (int k = 0; k < 4; ++k) hgt[c * 4 + k] = __float2half_rz(__double2float_rz(h[k]));
+}
+
+// Per patch, the largest coordinate scale of its tet copies, rounded up to
+// FP32: the need-mask pass uses it as every copy's margin term.
+template <typename T>
+__global__ void k_patch_scale(const T* coords, const int32_t* ele_off, float* pscale) {
+  const int p = blockIdx.x;
+  double mx = 0.0;
+  for (int c = ele_off[p] + threadIdx.x; c < ele_off[p + 1]; c += blockDim.x) {
+    double L[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) L[k] = (double)coords[(int64_t)c * 6 + k];
+    mx = fmax(mx, ElemSolve<T, 4>::scale(L));
+  }
+  __shared__ double red[256];
+  red[threadIdx.x] = mx;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) pscale[p] = __double2float_ru(red[0]);
 }
 
 template <typename T>
@@ -850,13 +884,10 @@ __device__ SweepResult<T> sweep_patch(const DevMesh<T>& m, DevState<T>& s, int p
         int e = 0, mask = 0;
         if (k < nd) {
           e = s_work[k];
-          T x[NC], L[K];
+          T x[NC];
 #pragma unroll
           for (int i = 0; i < NC; ++i) x[i] = s_slot[e * NC + i];
-          const T* gc = m.coords + (int64_t)(es + e) * K;
-#pragma unroll
-          for (int q = 0; q < K; ++q) L[q] = gc[q];
-          mask = ElemSolve<T, NC>::need_mask(L, copy_heights(m, es + e), x) & unchanged_input_filter<NC>(s_dirty[e]);
+          mask = copy_need_mask<T, NC>(m, p, es + e, x) & unchanged_input_filter<NC>(s_dirty[e]);
           s_dirty[e] = 0;
           ++n_upd;
         }
@@ -916,14 +947,10 @@ __device__ SweepResult<T> sweep_patch(const DevMesh<T>& m, DevState<T>& s, int p
         const int e = base + tid;
         int c = 0;
         if (e < ne && s_dirty[e]) {
-          T x[NC], L[K];
+          T x[NC];
 #pragma unroll
           for (int i = 0; i < NC; ++i) x[i] = s_slot[e * NC + i];
-          const T* gc = m.coords + (int64_t)(es + e) * K;
-#pragma unroll
-          for (int q = 0; q < K; ++q) L[q] = gc[q];
-          const int mask =
-              ElemSolve<T, NC>::need_mask(L, copy_heights(m, es + e), x) & unchanged_input_filter<NC>(s_dirty[e]);
+          const int mask = copy_need_mask<T, NC>(m, p, es + e, x) & unchanged_input_filter<NC>(s_dirty[e]);
           s_dirty[e] = (uint8_t)(mask << 4);
           c = __popc(mask);
           n_solve += c;
@@ -2523,6 +2550,9 @@ class EngineImpl final : public Engine {
         LS_HIP(dev_alloc(&d_hgt_, (size_t)F_ * 4));
         k_heights<T><<<(int)((F_ + 255) / 256), 256, 0, stream_>>>(d_coords_, F_, d_hgt_);
         LS_HIP(hipGetLastError());
+        LS_HIP(dev_alloc(&d_pscale_, (size_t)P_));
+        k_patch_scale<T><<<P_, 256, 0, stream_>>>(d_coords_, d_ele_off_, d_pscale_);
+        LS_HIP(hipGetLastError());
         LS_HIP(hipStreamSynchronize(stream_));
       }
     }
@@ -3339,6 +3369,7 @@ class EngineImpl final : public Engine {
     m.coords = d_coords_; m.sigma = d_sigma_; m.padj_off = d_padj_off_; m.padj = d_padj_;
     m.vpatch = d_vpatch_;
     m.hgt = d_hgt_;
+    m.pscale = d_pscale_;
     m.patch_dt = use_patch_prm_ ? d_patch_dt_ : nullptr;
     m.patch_bw = use_patch_prm_ ? d_patch_bw_ : nullptr;
     m.copy_src = d_copy_src_;
@@ -3388,7 +3419,7 @@ class EngineImpl final : public Engine {
                     d_mlp_b_, d_mlp_w1_, d_mlp_w2_, d_mlp_w3_, d_ds_x_, d_ds_y_, d_pend_, d_xyz_,
                     d_copy_verts_, d_copy_src_, d_rs_slot_, d_rs_vval_, d_rs_dirty_, d_rs_epoch_, d_visitE_,
                     d_vpatch_, d_ve_ok_, d_ve_tag_, d_cfl_, d_ceik_, d_ccurv_, d_DT_out2_, d_visitE2_,
-                    d_bcon2_, d_ov_lists_};
+                    d_bcon2_, d_ov_lists_, d_pscale_};
     if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
     if (graph_) (void)hipGraphDestroy(graph_);
     for (void* p : ptrs) if (p) (void)hipFree(p);
@@ -3429,6 +3460,7 @@ class EngineImpl final : public Engine {
   int32_t* d_copy_verts_ = nullptr;  // F x NC, renumbered corner ids
   int32_t* d_copy_src_ = nullptr;    // F, original element of each copy
   __half* d_hgt_ = nullptr;  // tet copies: corner heights (F x 4), FP16 rounded toward zero
+  float* d_pscale_ = nullptr;  // tets: per-patch coordinate scale bound (P)
   T* d_warm_ = nullptr;  // learned warm-start distances (N), written by the predictor
   unsigned long long* d_cfl_ = nullptr;  // CFL bound keys (2)
   float* scratch_ = nullptr;  // this device's `scratch` (scratch_address), never freed

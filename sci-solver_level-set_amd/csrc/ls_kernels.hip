@@ -1599,6 +1599,11 @@ __device__ __forceinline__ double advect_dt(const DevMesh<T>& m, const DevState<
 // 192^3 MCF 15.1 vs 21.8 ms (per-phase path); with the persistent kernel,
 // torus 3.36 vs 3.38 ms and the 512-mesh batch 166 vs 257 ms per step.
 constexpr int kFimBlock = 512;
+// The persistent step kernel at one workgroup per CU runs 768 threads: three
+// waves per SIMD instead of two hide more of the sweep's dependent FP64
+// latency, and its ~160 VGPRs still fit without spilling (64^3 torus, FP64:
+// 2.28 against 2.36 ms per step with 512 threads).
+constexpr int kStepBlock1 = 768;
 constexpr int kFimIterCap = 100000;  // FIM iteration cap when none is requested
 
 template <typename T>
@@ -2172,8 +2177,8 @@ struct PhaseClock {
 // warm-start phases in: each variant carries only the code it runs. WPE is
 // the waves per SIMD the variant is compiled for: 4 keeps the register
 // demand within the 128 VGPRs that let two 512-thread workgroups share a CU;
-// 2 (one workgroup per CU) leaves 256, so nothing is spilled to scratch,
-// whose dirty lines every grid barrier would otherwise write back.
+// 3 (one 768-thread workgroup per CU) leaves 168, so nothing is spilled to
+// scratch, whose dirty lines every grid barrier would otherwise write back.
 template <typename T, int NC, int BS, int ORG, bool WARM, int WPE>
 __global__ __launch_bounds__(BS, WPE) void k_step(DevMesh<T> m, DevState<T> s, StepArgs<T> a) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -2956,7 +2961,7 @@ class EngineImpl final : public Engine {
     if (step_one_per_cu_) {
       for (int warm = 0; warm < 2; ++warm) {
         int n = 0;
-        LS_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, step_kernel(org_, warm), kFimBlock, step_lds_));
+        LS_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, step_kernel(org_, warm), step_block(), step_lds_));
         if (n < 1) per_cu = 0;
       }
     }
@@ -2971,13 +2976,15 @@ class EngineImpl final : public Engine {
   // The persistent kernel for the sweep organisation, warm-start support and
   // workgroups per CU chosen by size_kernels (the tasks organisation has
   // only the two-per-CU build).
+  // Threads per workgroup of the persistent kernel step_kernel() returns.
+  int step_block() const { return (step_one_per_cu_ && org_ != kOrgTasks) ? kStepBlock1 : kFimBlock; }
   const void* step_kernel(int org, int warm) const {
     if (org == kOrgTasks)
       return warm ? (const void*)k_step<T, NC, kFimBlock, kOrgTasks, true, 4>
                   : (const void*)k_step<T, NC, kFimBlock, kOrgTasks, false, 4>;
     if (step_one_per_cu_)
-      return warm ? (const void*)k_step<T, NC, kFimBlock, kOrgElements, true, 2>
-                  : (const void*)k_step<T, NC, kFimBlock, kOrgElements, false, 2>;
+      return warm ? (const void*)k_step<T, NC, kStepBlock1, kOrgElements, true, 3>
+                  : (const void*)k_step<T, NC, kStepBlock1, kOrgElements, false, 3>;
     return warm ? (const void*)k_step<T, NC, kFimBlock, kOrgElements, true, 4>
                 : (const void*)k_step<T, NC, kFimBlock, kOrgElements, false, 4>;
   }
@@ -3140,7 +3147,7 @@ class EngineImpl final : public Engine {
         if (graph_) { LS_HIP(hipGraphDestroy(graph_)); graph_ = nullptr; }
         LS_HIP(hipStreamBeginCapture(q, hipStreamCaptureModeThreadLocal));
         hipError_t le = hipLaunchCooperativeKernel(step_kernel(org_, a.warm != 0), dim3(coop_grid_),
-                                                   dim3(kFimBlock), args, step_lds_, q);
+                                                   dim3(step_block()), args, step_lds_, q);
         if (le == hipSuccess) le = launch_nb_advect(prm, q);
         hipError_t ce = hipStreamEndCapture(q, &graph_);
         if (le != hipSuccess || ce != hipSuccess) {
@@ -3157,7 +3164,7 @@ class EngineImpl final : public Engine {
       LS_HIP(ge);
     } else {
       const hipError_t le = hipLaunchCooperativeKernel(step_kernel(org_, a.warm != 0), dim3(coop_grid_),
-                                                       dim3(kFimBlock), args, step_lds_, q);
+                                                       dim3(step_block()), args, step_lds_, q);
       if (coop_refused(le)) return step_kernels(prm, st, err);
       LS_HIP(le);
       LS_HIP(launch_nb_advect(prm, q));

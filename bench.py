@@ -292,13 +292,16 @@ def batch_workload(args, lsb, cases):
     g.set_phi(np.concatenate([m.phi0 for m in b.meshes]))
     tot = timed_run(g, prm, args.steps, w=8 if prec == lsb.LS_F64 else 4)
     g.close()
+    learned = None
+    if not args.no_learned:
+        learned = learned_batch(args, lsb, b, cand, ex, prec, tot["verts"] / tot["wall"])
     out = {"workload": f"{b.name}: {len(b.meshes)} Kuhn meshes, {b.n_verts} verts, {b.n_cells} tets, "
                        f"ellipsoids, affine velocity, CFL 0.5, band 6h, {args.steps} timesteps",
            "patch_verts": cand, "execution": {lsb.LS_EXEC_GRAPH: "graph", lsb.LS_EXEC_PERSISTENT: "persistent",
                                               lsb.LS_EXEC_KERNELS: "kernels"}[ex],
            "autotune_ms_per_step": {f"{c}/{e}": m for c, e, m in table} or None,
            "patches": info["n_patches"], "element_copies": info["n_copies"],
-           "setup_s": setup_s}
+           "setup_s": setup_s, "learned": learned}
     cpu = None
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -509,6 +512,65 @@ def learned_run(args, lsb, case, exec_mode, plain_value, precision, prm_kw, orac
             "warm_margin": WARM_MARGIN, "warm_abs_margin_h": WARM_ABS_H, "parity": parity}
 
 
+def learned_batch(args, lsb, b, cand, ex, prec, plain_value):
+    """The learned warm start on the whole batch (one network for all
+    meshes): pairs from plain steps, GPU training, then the timed steps with
+    warm values from the network and with the identity predictor, from the
+    same initial state as the plain run. Each mesh accepts or rejects its
+    warm start on its own (fallback_mesh_steps counts the meshes that re-ran
+    plain, summed over the steps). The absolute margin is a quarter of
+    the coarsest mesh spacing (one margin serves every mesh of a batch, and
+    one that is small against a coarse mesh's spacing leaves warm values
+    below the result there, which the acceptance check rejects)."""
+    wex = ex if ex != lsb.LS_EXEC_KERNELS else lsb.LS_EXEC_GRAPH
+    g = lsb.solver_for_batch(b, execution=wex, patch_verts=cand, precision=prec)
+    g.mlp_set_params(lsb.mlp_init_params(5))
+    prm = lsb.step_params(1.0, 1.0, 5)
+    phi0 = np.concatenate([m.phi0 for m in b.meshes])
+    g.step(prm)
+    t0 = time.perf_counter()
+    for _ in range(args.train_steps):
+        g.mlp_collect(0, 12_000_000)
+        g.step(prm)
+        g.mlp_collect(1)
+    t1 = time.perf_counter()
+    loss = g.mlp_train(args.train_epochs, 4096, 2e-3, seed=7)
+    t2 = time.perf_counter()
+    abs_margin = WARM_ABS_H * max(m.h for m in b.meshes)
+
+    def timed(warm):
+        wprm = lsb.step_params(1.0, 1.0, 5, warm=warm, warm_margin=args.batch_warm_margin,
+                               warm_abs_margin=abs_margin)
+        g.set_phi(phi0)
+        r = {"verts": 0, "fallbacks": 0, "iters": 0, "visits": 0}
+        ts = time.perf_counter()
+        for k in range(args.steps):
+            if warm == lsb.LS_WARM_LEARNED and k > 0:
+                g.mlp_predict()
+            st = g.step(wprm if k > 0 else prm)
+            r["verts"] += st.nb_verts
+            r["fallbacks"] += st.warm_fallback
+            r["iters"] += st.fim_iters
+            r["visits"] += st.patch_visits
+        r["wall"] = time.perf_counter() - ts
+        return r
+
+    learned = timed(lsb.LS_WARM_LEARNED)
+    ident = timed(lsb.LS_WARM_IDENTITY)
+    g.close()
+    value = learned["verts"] / learned["wall"]
+    return {"value": value, "vs_plain_fim": value / plain_value, "ms_per_step": 1e3 * learned["wall"] / args.steps,
+            "fim_iters_per_step": learned["iters"] / args.steps,
+            "patch_visits_per_step": learned["visits"] / args.steps,
+            "fallback_mesh_steps": learned["fallbacks"], "mesh_steps": args.steps * len(b.meshes),
+            "identity_predictor": {"value": ident["verts"] / ident["wall"],
+                                   "vs_plain_fim": ident["verts"] / ident["wall"] / plain_value,
+                                   "fim_iters_per_step": ident["iters"] / args.steps,
+                                   "fallback_mesh_steps": ident["fallbacks"]},
+            "collect_s": t1 - t0, "train_s": t2 - t1, "train_loss": loss, "warm_margin": args.batch_warm_margin,
+            "warm_abs_margin": abs_margin}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -536,6 +598,8 @@ def main():
     ap.add_argument("--no-learned", action="store_true", help="skip the learned warm-start run")
     ap.add_argument("--train-steps", type=int, default=13, help="plain steps whose pairs train the network")
     ap.add_argument("--train-epochs", type=int, default=2)
+    ap.add_argument("--batch-warm-margin", type=float, default=WARM_MARGIN,
+                    help="relative warm-start margin of the batch workload's learned run")
     ap.add_argument("--learned-parity-steps", type=int, default=3,
                     help="learned warm steps compared with the FP64 oracle's plain step from the same input")
     ap.add_argument("--workload", default="torus", choices=["torus", "batch", "mcf"],
@@ -571,6 +635,7 @@ def main():
                "scaling": "weak", "vs_baseline": None, "dtype": args.precision, "data": "synthetic",
                "config": dict(cfg, mode="plain_fim", gpu_event_ms_per_step=tot["ms_total"] / args.steps,
                               fim_iters_per_step=tot["fim_iters"] / args.steps,
+                              patch_visits_per_step=tot["visits"] / args.steps,
                               exec_modes_used=sorted(tot["exec_modes"]),
                               step_wg_per_cu=sorted(tot["step_wg_per_cu"])),
                "roofline": roofline(tot, args.steps, kmode)}

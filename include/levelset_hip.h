@@ -52,7 +52,8 @@ enum { LS_LAYOUT_DEVICE = 0, LS_LAYOUT_HOST = 1 };
  * held: a step in which such a vertex inside the band lies more than
  * warm_res_tol below what its elements give it from the final distances
  * (an undercut) re-runs the plain method from the same phi in the same
- * launch, and returns the plain method's bits. */
+ * launch, and returns the plain method's bits; in a batch this holds per
+ * mesh (a mesh's undercut re-runs that mesh only). */
 enum { LS_WARM_NONE = 0, LS_WARM_IDENTITY = 1, LS_WARM_LEARNED = 2 };
 
 /* Per-step parameters. Replaces the argument list of
@@ -107,7 +108,8 @@ typedef struct ls_stats {
   double warm_residual;     /* warm start: largest undercut, max(F_v - DT_v) over vertices inside the band
                                still holding their warm value (0: none) */
   int32_t warm_patches;     /* warm start: patches that received warm values */
-  int32_t warm_fallback;    /* warm start: 1 when the residual check failed and the step re-ran plain */
+  int32_t warm_fallback;    /* warm start: meshes whose residual check failed and which re-ran plain
+                               (0 or 1; for a batch, each mesh is accepted or rejected on its own) */
   int32_t sweep_tasks;      /* 1: the FIM sweeps ran task-compacted */
   int32_t step_wg_per_cu;   /* persistent step: workgroups per CU of its grid (the kernel build
                                compiled for it); 0 when the step ran one launch per phase */

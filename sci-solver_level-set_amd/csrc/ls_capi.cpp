@@ -53,7 +53,8 @@ int apply_velocity(ls_ctx* c) {
   return rc == LS_OK ? rc : fail(c, rc, err);
 }
 
-// Per-patch timestep and band from the per-mesh values.
+// Per-patch timestep and band from the per-mesh values, and the mesh of
+// every patch (warm starts are accepted per mesh).
 int apply_batch_params(ls_ctx* c) {
   if (!c->engine || !c->have_layout) return LS_OK;
   std::vector<double> dt, bw;
@@ -67,7 +68,8 @@ int apply_batch_params(ls_ctx* c) {
     }
   }
   std::string err;
-  const int rc = c->engine->set_patch_params(dt, bw, err);
+  int rc = c->engine->set_patch_params(dt, bw, err);
+  if (rc == LS_OK) rc = c->engine->set_patch_groups(c->patch_mesh, std::max(c->n_meshes, 1), err);
   return rc == LS_OK ? rc : fail(c, rc, err);
 }
 

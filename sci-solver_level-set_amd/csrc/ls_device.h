@@ -32,6 +32,10 @@ struct DevMesh {
   const int32_t* padj_off;     // P+1
   const int32_t* padj;
   const int32_t* vpatch;       // N, patch of each renumbered vertex
+  // Mesh of each patch in a batch, or null (one mesh): a warm start is
+  // accepted or rejected per mesh.
+  const int32_t* pgroup;       // P or null
+  int ngroups;                 // meshes (1 when pgroup is null)
   // Per-patch timestep and narrowband width for batches of meshes with
   // their own parameters; null: the step's uniform values.
   const double* patch_dt;      // P or null
@@ -51,6 +55,10 @@ __device__ __forceinline__ T band_of(const DevMesh<T>& m, int p, T bw) {
 template <typename T>
 __device__ __forceinline__ double dt_of(const DevMesh<T>& m, int p, double dt) {
   return m.patch_dt ? m.patch_dt[p] : dt;
+}
+template <typename T>
+__device__ __forceinline__ int group_of(const DevMesh<T>& m, int p) {
+  return m.pgroup ? m.pgroup[p] : 0;
 }
 
 // Per-timestep state (renumbered vertex order).
@@ -102,6 +110,8 @@ struct DevState {
   // CFL bound of the last narrowband as order-preserving keys: [0] the least
   // timestep, [1] the largest inscribed radius.
   unsigned long long* cfl;
+  // Warm start: 1 for each mesh whose acceptance check failed this step.
+  int32_t* gfail;             // ngroups
   // Overlapped FIM iterations of the persistent step (one grid barrier per
   // iteration): a second DT_out, visit tags and convergence flags, each
   // pair indexed by iteration parity, and nine P-long lists indexed by

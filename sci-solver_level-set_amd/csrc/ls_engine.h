@@ -26,6 +26,9 @@ class Engine {
   // the step's uniform values.
   virtual int set_patch_params(const std::vector<double>& dt, const std::vector<double>& bw,
                                std::string& err) = 0;
+  // Mesh of each patch of a batch (P values in [0, ngroups)); a warm start is
+  // accepted or rejected per mesh. ngroups <= 1: one mesh.
+  virtual int set_patch_groups(const std::vector<int32_t>& group, int ngroups, std::string& err) = 0;
   virtual int step(const ls_step_params& prm, ls_stats& st, std::string& err) = 0;
   virtual int get_phi(double* phi_user, std::string& err) = 0;
   virtual int get_narrowband(std::vector<int32_t>& ids, std::string& err) = 0;

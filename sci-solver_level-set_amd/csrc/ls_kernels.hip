@@ -1198,10 +1198,11 @@ __device__ bool warm_patch(const DevMesh<T>& m, DevState<T>& s, int p, int mode,
 // minimum, since nothing lower around it can support it; the plain method's
 // output holds no warm values and always passes. The patch maximum of
 // F_v - DT_v is merged into pc[kPcResid] (non-negative floats order as their
-// bit patterns).
+// bit patterns); above res_tol it fails the patch's mesh (gfail), and the
+// first patch to fail a mesh counts it in pc[kPcFallback].
 template <typename T, int NC, int BS>
 __device__ void residual_patch(const DevMesh<T>& m, DevState<T>& s, int p, T bw, double leps, int mode,
-                               double margin, double abs_margin, unsigned char* smem, T* red) {
+                               double margin, double abs_margin, double res_tol, unsigned char* smem, T* red) {
   constexpr int K = ElemSolve<T, NC>::kCoords;
   const int vs = m.vert_off[p], ve = m.vert_off[p + 1];
   const int es = m.ele_off[p], hs = m.halo_off[p];
@@ -1234,6 +1235,8 @@ __device__ void residual_patch(const DevMesh<T>& m, DevState<T>& s, int p, T bw,
   }
   r = -block_min_n<float, BS>(-r, (float*)red);
   if (threadIdx.x == 0 && r > 0.f) atomicMax((unsigned int*)&s.pc[kPcResid], __float_as_uint(r));
+  if (threadIdx.x == 0 && (double)r > res_tol && atomicExch(&s.gfail[group_of(m, p)], 1) == 0)
+    atomicAdd(&s.pc[kPcFallback], 1);
 }
 
 // Shape-function gradients of a copy in its local frame (FP64).
@@ -2196,12 +2199,18 @@ __global__ __launch_bounds__(BS, WPE) void k_step(DevMesh<T> m, DevState<T> s, S
   if (tid == 0) s_work[0] = s_work[1] = -1;  // visible after the first grid barrier
 
   // Phases 0-2 run once, or twice when a warm start fails its residual
-  // check: the second attempt is the plain method from the same phi.
+  // check: the second attempt is the plain method from the same phi over
+  // the meshes that failed (every mesh of a batch is independent; the others
+  // keep their warm results).
   for (int attempt = 0; attempt < 2; ++attempt) {
     const bool warm = WARM && a.warm != 0 && !a.first && attempt == 0;
+    const bool redo = attempt == 1;
 
   // Phase 0: vertex reset and sign save; patch marks; counters.
-  for (int64_t v = gtid; v < m.N; v += gsize) init_vertex(s, v);
+  for (int64_t v = gtid; v < m.N; v += gsize)
+    if (!redo || s.gfail[group_of(m, m.vpatch[v])]) init_vertex(s, v);
+  if (!redo)
+    for (int64_t g = gtid; g < m.ngroups; g += gsize) s.gfail[g] = 0;
   for (int64_t q = gtid; q < m.P; q += gsize) {
     s.inA[q] = 0;
     s.freshE[q] = 0;
@@ -2241,6 +2250,7 @@ __global__ __launch_bounds__(BS, WPE) void k_step(DevMesh<T> m, DevState<T> s, S
   for (int i = first_item(nscan, &s.pc[kPcWorkS], s_work); i < nscan;
        i = later_item(nscan, &s.pc[kPcWorkS], s_work)) {
     const int p = a.first ? i : s.nb[a.nb_in][i];
+    if (redo && !s.gfail[group_of(m, p)]) continue;
     if (seeds_patch<T, NC, BS>(m, s, p, smem) && tid == 0) {
       s.listA[atomicAdd(&s.pc[kPcSeed], 1)] = p;
       s.inA[p] = 1;
@@ -2370,19 +2380,17 @@ __global__ __launch_bounds__(BS, WPE) void k_step(DevMesh<T> m, DevState<T> s, S
 
   if (!warm) break;
   // Phase 2b: acceptance over the patches that received warm values (the
-  // previous narrowband); above the threshold the warm start is rejected and
-  // the step re-runs plain.
+  // previous narrowband); a mesh with a patch above the threshold rejects its
+  // warm start and re-runs plain.
   if constexpr (WARM)
     for (int i = b; i < nscan; i += G) {
       const int p = s.nb[a.nb_in][i];
-      residual_patch<T, NC, BS>(m, s, p, band_of(m, p, a.bw), a.leps, a.warm, a.warm_margin, a.warm_abs, smem,
-                                red);
+      residual_patch<T, NC, BS>(m, s, p, band_of(m, p, a.bw), a.leps, a.warm, a.warm_margin, a.warm_abs,
+                                a.res_tol, smem, red);
     }
   grid_barrier(s.bar, G);
   const float resid = __uint_as_float((unsigned int)ld_acquire(&s.pc[kPcResid]));
   if (resid <= a.res_tol) break;
-  if (gtid == 0) s.pc[kPcFallback] = 1;
-  grid_barrier(s.bar, G);
   }
 
   // Phase 3: narrowband + sign restore over every patch.
@@ -2602,6 +2610,8 @@ class EngineImpl final : public Engine {
     LS_HIP(dev_alloc(&d_ticks_, kNumTicks));
     LS_HIP(dev_alloc(&d_bar_, 2));
     LS_HIP(dev_alloc(&d_cfl_, 2));
+    LS_HIP(dev_alloc(&d_gfail_, 1));
+    ngroups_ = 1;
     LS_HIP(hipMemset(d_bar_, 0, 2 * sizeof(unsigned int)));
     {
       int khz = 0;
@@ -3041,6 +3051,25 @@ class EngineImpl final : public Engine {
     return d;
   }
 
+  int set_patch_groups(const std::vector<int32_t>& group, int ngroups, std::string& err) override {
+    if (!uploaded_) { err = "layout not built"; return LS_ERR_STATE; }
+    if (ngroups <= 1) {
+      ngroups_ = 1;
+      return LS_OK;
+    }
+    if (group.size() != (size_t)P_) { err = "patch groups need P values"; return LS_ERR_ARG; }
+    for (int32_t g : group)
+      if (g < 0 || g >= ngroups) { err = "patch group out of range"; return LS_ERR_ARG; }
+    if (d_pgroup_) LS_HIP(hipFree(d_pgroup_));
+    if (d_gfail_) LS_HIP(hipFree(d_gfail_));
+    d_pgroup_ = nullptr;
+    d_gfail_ = nullptr;
+    LS_HIP(dev_upload(&d_pgroup_, group.data(), group.size()));
+    LS_HIP(dev_alloc(&d_gfail_, (size_t)ngroups));
+    LS_HIP(hipMemset(d_gfail_, 0, (size_t)ngroups * sizeof(int32_t)));
+    ngroups_ = ngroups;
+    return LS_OK;
+  }
   int set_patch_params(const std::vector<double>& dt, const std::vector<double>& bw, std::string& err) override {
     if (!uploaded_) { err = "layout not built"; return LS_ERR_STATE; }
     if (dt.empty() && bw.empty()) {
@@ -3382,6 +3411,8 @@ class EngineImpl final : public Engine {
     m.copy_src = d_copy_src_;
     m.ceik = d_ceik_;
     m.ccurv = d_ccurv_;
+    m.pgroup = d_pgroup_;
+    m.ngroups = ngroups_;
     m.ceik0 = kCeikDefault;
     m.ccurv0 = kCcurvDefault;
     return m;
@@ -3407,6 +3438,7 @@ class EngineImpl final : public Engine {
     s.ve_ok = d_ve_ok_;
     s.ve_tag = d_ve_tag_;
     s.cfl = d_cfl_;
+    s.gfail = d_gfail_;
     s.DT_out2 = d_DT_out2_;
     s.visitE2 = d_visitE2_;
     s.bcon2 = d_bcon2_;
@@ -3425,7 +3457,8 @@ class EngineImpl final : public Engine {
                     d_patch_dt_, d_patch_bw_, d_warm_, d_hgt_, d_mlp_prm_, d_mlp_m_, d_mlp_v_, d_mlp_grad_,
                     d_mlp_b_, d_mlp_w1_, d_mlp_w2_, d_mlp_w3_, d_ds_x_, d_ds_y_, d_pend_, d_xyz_,
                     d_copy_verts_, d_copy_src_, d_rs_slot_, d_rs_vval_, d_rs_dirty_, d_rs_epoch_, d_visitE_,
-                    d_vpatch_, d_ve_ok_, d_ve_tag_, d_cfl_, d_ceik_, d_ccurv_, d_DT_out2_, d_visitE2_,
+                    d_vpatch_, d_ve_ok_, d_ve_tag_, d_cfl_, d_gfail_, d_pgroup_, d_ceik_, d_ccurv_, d_DT_out2_,
+                    d_visitE2_,
                     d_bcon2_, d_ov_lists_, d_pscale_};
     if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
     if (graph_) (void)hipGraphDestroy(graph_);
@@ -3470,6 +3503,9 @@ class EngineImpl final : public Engine {
   float* d_pscale_ = nullptr;  // tets: per-patch coordinate scale bound (P)
   T* d_warm_ = nullptr;  // learned warm-start distances (N), written by the predictor
   unsigned long long* d_cfl_ = nullptr;  // CFL bound keys (2)
+  int32_t* d_gfail_ = nullptr;   // warm-start failure flag per mesh (ngroups_)
+  int32_t* d_pgroup_ = nullptr;  // mesh of each patch of a batch (P), or null
+  int ngroups_ = 1;
   float* scratch_ = nullptr;  // this device's `scratch` (scratch_address), never freed
   // overlapped FIM iterations of the persistent step (see DevState)
   T* d_DT_out2_ = nullptr;

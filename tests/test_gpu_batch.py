@@ -109,3 +109,58 @@ def test_autotune_picks_a_candidate_with_identical_results():
     np.testing.assert_array_equal(s.phi(), ref.phi())
     s.close()
     ref.close()
+
+
+def _alone_solver(mesh, patch_verts, precision):
+    s = lsb.Solver(device=0)
+    s.set_mesh(3, mesh.xyz, mesh.cells)
+    s.set_precision(precision)
+    s.set_execution(lsb.LS_EXEC_PERSISTENT)
+    s.partition_rcb(-(-mesh.xyz.shape[0] // patch_verts))
+    s.set_phi(mesh.phi0)
+    s.set_velocity(mesh.vel)
+    return s
+
+
+def test_batch_warm_start_is_accepted_or_rejected_per_mesh():
+    """Caller warm values above the plain result on meshes 0 and 2 and 10 %
+    below it on meshes 1 and 3: the batch step rejects exactly meshes 1 and 3,
+    which get the plain bits, while meshes 0 and 2 keep their warm results,
+    each identical to the same warm step of that mesh alone."""
+    b = cases.small_batch(n_meshes=4, seed=13)
+    prec = lsb.LS_F64
+    prm = lsb.step_params(1.0, 1.0, 5)
+    wprm = lsb.step_params(1.0, 1.0, 5, warm=lsb.LS_WARM_LEARNED, warm_margin=0.0, warm_abs_margin=0.0)
+    plain = lsb.solver_for_batch(b, execution=lsb.LS_EXEC_PERSISTENT, precision=prec)
+    for _ in range(3):
+        plain.step(prm)
+    ref_phi = plain.split(plain.phi())
+    D = plain.split(plain.distances())
+    plain.close()
+    low = {1, 3}
+    W = [np.where(d < 1e8, 0.9 * d if k in low else d + 1e-3, 1e9) for k, d in enumerate(D)]
+
+    s = lsb.solver_for_batch(b, execution=lsb.LS_EXEC_PERSISTENT, precision=prec)
+    for _ in range(2):
+        s.step(prm)
+    s.set_warm(np.concatenate(W))
+    st = s.step(wprm)
+    print(f"batch warm step: fallback meshes {st.warm_fallback}, residual {st.warm_residual:.3e}")
+    assert st.warm_fallback == len(low)
+    got = s.split(s.phi())
+    s.close()
+    for k, mesh in enumerate(b.meshes):
+        a = _alone_solver(mesh, b.patch_verts, prec)
+        mprm = lsb.step_params(mesh.dt, mesh.bandwidth, 5)
+        for _ in range(2):
+            a.step(mprm)
+        a.set_warm(W[k])
+        ast = a.step(lsb.step_params(mesh.dt, mesh.bandwidth, 5, warm=lsb.LS_WARM_LEARNED, warm_margin=0.0,
+                                     warm_abs_margin=0.0))
+        alone_phi = a.phi()
+        a.close()
+        print(f"mesh {k}: alone fallback {ast.warm_fallback}")
+        assert ast.warm_fallback == (1 if k in low else 0)
+        np.testing.assert_array_equal(got[k], alone_phi, err_msg=f"mesh {k}")
+        if k in low:
+            np.testing.assert_array_equal(got[k], ref_phi[k], err_msg=f"mesh {k} plain bits")

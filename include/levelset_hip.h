@@ -178,11 +178,19 @@ int ls_get_patch_lists(ls_ctx* ctx, int32_t* halo_off, int32_t* halo_ids, uint16
 /* Initial level set, one value per vertex in the caller's order
  * (LevelSet::initializeVertices, src/LevelSet.cu:52-80). dtype LS_F32/LS_F64. */
 int ls_set_phi(ls_ctx* ctx, const void* phi, int dtype);
+/* Velocity modes of ls_set_velocity: each element's own velocity
+ * (LS_VEL_ELEMENT); the first element copy's velocity for a whole patch, as
+ * the reference's 3D advection reads it (LS_VEL_PATCH_FIRST,
+ * src/core3d/include/meshFIM_kernels.h:51-53); or, triangles only, the
+ * reference's 2D advection, which ignores the given velocity and moves each
+ * element with (phi_A, phi_B) / |grad phi| in its local frame
+ * (LS_VEL_PHI_2D, src/core2d/include/meshFIM2d_kernels.h:133-134,178-181;
+ * an element with |grad phi| = 0 does not move). The CFL bound uses the
+ * given velocity in every mode, as the reference's does. */
+enum { LS_VEL_ELEMENT = 0, LS_VEL_PATCH_FIRST = 1, LS_VEL_PHI_2D = 2 };
 /* One velocity (x,y,z) per element (LevelSet::initializeAdvection,
- * src/LevelSet.cu:82-108). patch_sigma = 1 reproduces the reference's use of
- * the first element copy's velocity for a whole patch
- * (src/core3d/include/meshFIM_kernels.h:51-53); 0 uses each element's own. */
-int ls_set_velocity(ls_ctx* ctx, const double* vel, int patch_sigma);
+ * src/LevelSet.cu:82-108) and the velocity mode (LS_VEL_*). */
+int ls_set_velocity(ls_ctx* ctx, const double* vel, int mode);
 
 /* One timestep: seeds, sign save, FIM, narrowband + sign restore, advection
  * (loop body of meshFIM3d::GenerateData, src/core3d/cuda/meshFIM3d.cu:639-678;

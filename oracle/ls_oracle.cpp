@@ -354,6 +354,7 @@ struct Solver : Base {
   std::vector<T> coords;              // SoA local coordinates: coords[k*F + c]
   std::vector<double> frames;         // per copy, for velocity projection
   std::vector<T> sigma;               // SoA per-copy velocity in the local frame
+  bool sigma_phi = false;             // 2D: advect with the velocity from phi (mode 2)
   std::vector<std::vector<int32_t>> lmem;  // per renumbered vertex: patch-local slots
   std::vector<int32_t> bxadj, badj;   // patch adjacency
   // State (renumbered order).
@@ -517,9 +518,14 @@ struct Solver : Base {
     narrowband.clear();
     return 0;
   }
-  // patch_sigma = 1 reproduces the reference's use of the patch's first copy
-  // velocity for every element of the patch (meshFIM_kernels.h:51-53).
-  int set_velocity(const double* vel, int patch_sigma) override {
+  // mode 1 reproduces the reference's use of the patch's first copy velocity
+  // for every element of the patch (meshFIM_kernels.h:51-53); mode 2
+  // (triangles) its 2D advection's velocity from phi (meshFIM2d_kernels.h:
+  // 170-181), applied in advect_patch; the projected velocity still feeds
+  // the CFL bound.
+  int set_velocity(const double* vel, int mode) override {
+    const int patch_sigma = mode == 1;
+    sigma_phi = dim == 2 && mode == 2;
     const int ns = dim == 3 ? 3 : 2, nf = dim == 3 ? 9 : 6;
     std::vector<int32_t> first(F);
     for (int p = 0; p < P; ++p)
@@ -758,7 +764,7 @@ struct Solver : Base {
         eamb[c] = near_threshold(K, ph, 4, H);
       } else {
         const double c0 = coords[0 * F + g], c1 = coords[1 * F + g], c2 = coords[2 * F + g];
-        const double sg[2] = {sigma[0 * F + g], sigma[1 * F + g]};
+        double sg[2] = {sigma[0 * F + g], sigma[1 * F + g]};
         vol = std::fabs(c0 * c2) / 2.0;
         double gr[3][2];
         gr[1][0] = 1.0 / c0;
@@ -766,6 +772,13 @@ struct Solver : Base {
         gr[2][0] = 0.0;
         gr[2][1] = 1.0 / c2;
         for (int k = 0; k < 2; ++k) gr[0][k] = -(gr[1][k] + gr[2][k]);
+        if (sigma_phi) {  // (phi_A, phi_B) / |grad phi|, 0 where grad phi vanishes
+          const double gx = gr[0][0] * ph[0] + gr[1][0] * ph[1] + gr[2][0] * ph[2];
+          const double gy = gr[0][1] * ph[0] + gr[1][1] * ph[1] + gr[2][1] * ph[2];
+          const double n = std::sqrt(gx * gx + gy * gy);
+          sg[0] = n > 0.0 ? ph[0] / n : 0.0;
+          sg[1] = n > 0.0 ? ph[1] / n : 0.0;
+        }
         double K[3];
         for (int i = 0; i < 3; ++i) K[i] = vol * (sg[0] * gr[i][0] + sg[1] * gr[i][1]);
         nscheme(K, ph, 3, H, at);
@@ -1102,8 +1115,8 @@ void* lso_create(int dim, int precision, int64_t N, const double* xyz, int64_t E
 }
 void lso_destroy(void* h) { delete static_cast<Base*>(h); }
 int lso_set_phi(void* h, const double* phi) { return static_cast<Base*>(h)->set_phi(phi); }
-int lso_set_velocity(void* h, const double* vel, int patch_sigma) {
-  return static_cast<Base*>(h)->set_velocity(vel, patch_sigma);
+int lso_set_velocity(void* h, const double* vel, int mode) {
+  return static_cast<Base*>(h)->set_velocity(vel, mode);
 }
 int lso_step(void* h, const lso_params* prm, lso_stats* st) { return static_cast<Base*>(h)->step(*prm, *st); }
 void lso_get_phi(void* h, double* out) { static_cast<Base*>(h)->get_phi(out); }

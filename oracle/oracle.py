@@ -116,8 +116,10 @@ class Oracle:
     def set_phi(self, phi):
         self.L.lso_set_phi(self.h, np.ascontiguousarray(phi, dtype=np.float64))
 
-    def set_velocity(self, vel, patch_sigma=0):
-        self.L.lso_set_velocity(self.h, np.ascontiguousarray(vel, dtype=np.float64).ravel(), patch_sigma)
+    def set_velocity(self, vel, mode=0):
+        """mode 0: each element's velocity; 1: the patch's first copy's; 2
+        (triangles): the reference's 2D advection velocity from phi."""
+        self.L.lso_set_velocity(self.h, np.ascontiguousarray(vel, dtype=np.float64).ravel(), mode)
 
     def step(self, params: Params) -> Stats:
         st = Stats()

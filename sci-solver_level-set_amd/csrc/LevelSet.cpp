@@ -451,7 +451,7 @@ std::vector<std::vector<float>> generate(meshFIM* fim, int dim, const std::vecto
   std::vector<double> v(3 * ne, 0.0);
   for (size_t e = 0; e < ne && e < vel.size(); ++e)
     for (int a = 0; a < 3; ++a) v[3 * e + a] = vel[e][a];
-  if (ls_set_velocity(c, v.data(), 0) != LS_OK) die(c, "ls_set_velocity");
+  if (ls_set_velocity(c, v.data(), LS_VEL_ELEMENT) != LS_OK) die(c, "ls_set_velocity");
 
   ls_step_params p{};
   // 3D GenerateData takes the timestep and bandwidth as float

@@ -18,7 +18,7 @@ struct ls_ctx {
   std::vector<double> xyz;
   std::vector<int32_t> cells;
   std::vector<double> vel;   // per element, kept to re-project after re-partitioning
-  int patch_sigma = 0;
+  int patch_sigma = LS_VEL_ELEMENT;  // velocity mode (LS_VEL_*)
   int precision = LS_F32;  // tet storage precision
   int exec = LS_EXEC_PERSISTENT;
   int sweep = LS_SWEEP_AUTO;
@@ -44,8 +44,10 @@ int fail(ls_ctx* c, int code, const std::string& msg) {
 }
 
 // Uploads the per-element velocity; the engine projects it into every copy's
-// local frame. patch_sigma reproduces the reference's use of the patch's first
-// copy velocity for all of the patch's elements (meshFIM_kernels.h:51-53).
+// local frame. The mode (LS_VEL_*) selects each element's own velocity, the
+// patch's first copy velocity for all of the patch's elements
+// (meshFIM_kernels.h:51-53), or the 2D advection's velocity from phi
+// (meshFIM2d_kernels.h:178-181).
 int apply_velocity(ls_ctx* c) {
   if (!c->have_layout || !c->engine) return LS_OK;
   std::string err;
@@ -374,12 +376,15 @@ int ls_set_phi(ls_ctx* c, const void* phi, int dtype) {
   return rc == LS_OK ? rc : fail(c, rc, err);
 }
 
-int ls_set_velocity(ls_ctx* c, const double* vel, int patch_sigma) {
+int ls_set_velocity(ls_ctx* c, const double* vel, int mode) {
   if (!c) return LS_ERR_ARG;
   if (!c->N) return fail(c, LS_ERR_STATE, "set the mesh first");
   if (!vel) return fail(c, LS_ERR_ARG, "null velocity");
+  if (mode != LS_VEL_ELEMENT && mode != LS_VEL_PATCH_FIRST && mode != LS_VEL_PHI_2D)
+    return fail(c, LS_ERR_ARG, "unknown velocity mode");
+  if (mode == LS_VEL_PHI_2D && c->dim != 2) return fail(c, LS_ERR_ARG, "LS_VEL_PHI_2D is for triangle meshes");
   c->vel.assign(vel, vel + 3 * c->E);
-  c->patch_sigma = patch_sigma ? 1 : 0;
+  c->patch_sigma = mode;
   c->have_vel = true;
   return apply_velocity(c);
 }

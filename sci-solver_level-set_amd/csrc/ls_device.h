@@ -46,6 +46,9 @@ struct DevMesh {
   const double* ceik;          // E or null
   const double* ccurv;         // E or null
   double ceik0, ccurv0;
+  // Triangles: advect with (phi_A, phi_B) / |grad phi| in each copy's local
+  // frame instead of sigma (the reference's 2D advection).
+  int sigma_phi;
 };
 
 template <typename T>

@@ -20,8 +20,9 @@ class Engine {
   virtual int upload(const Layout& L, DeviceLayout* D, std::string& err) = 0;
   virtual int set_phi(const double* phi_user, std::string& err) = 0;
   // Velocity per original element (E x 3); projected on the device into every
-  // copy's local frame (patch_sigma: the patch's first copy's velocity).
-  virtual int set_velocity(const double* vel, int64_t E, int patch_sigma, std::string& err) = 0;
+  // copy's local frame. mode (LS_VEL_*): each element's own velocity, the
+  // patch's first copy's, or (triangles) the advection's velocity from phi.
+  virtual int set_velocity(const double* vel, int64_t E, int mode, std::string& err) = 0;
   // Per-patch timestep and narrowband width (P each); empty vectors restore
   // the step's uniform values.
   virtual int set_patch_params(const std::vector<double>& dt, const std::vector<double>& bw,

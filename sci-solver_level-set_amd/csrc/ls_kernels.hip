@@ -1414,6 +1414,16 @@ __device__ void advect_patch(const DevMesh<T>& m, DevState<T>& s, int p, double 
 #pragma unroll
     for (int k = 0; k < D; ++k) sg[k] = (double)gs[k];
     Grad<NC>::eval(L, g, vol);
+    if constexpr (NC == 3) {
+      if (m.sigma_phi) {
+        // (phi_A, phi_B) / |grad phi| (src/core2d/include/meshFIM2d_kernels.h:170-181)
+        const double gx = g[0][0] * ph[0] + g[1][0] * ph[1] + g[2][0] * ph[2];
+        const double gy = g[0][1] * ph[0] + g[1][1] * ph[1] + g[2][1] * ph[2];
+        const double n = sqrt(gx * gx + gy * gy);
+        sg[0] = n > 0.0 ? ph[0] / n : 0.0;
+        sg[1] = n > 0.0 ? ph[1] / n : 0.0;
+      }
+    }
     double Kc[NC], Kp[NC], Km[NC], at[NC];
 #pragma unroll
     for (int i = 0; i < NC; ++i) {
@@ -2917,8 +2927,10 @@ class EngineImpl final : public Engine {
     return LS_OK;
   }
 
-  int set_velocity(const double* vel, int64_t E, int patch_sigma, std::string& err) override {
+  int set_velocity(const double* vel, int64_t E, int mode, std::string& err) override {
     if (!uploaded_) { err = "layout not built"; return LS_ERR_STATE; }
+    sigma_phi_ = NC == 3 && mode == LS_VEL_PHI_2D;
+    const int patch_sigma = mode == LS_VEL_PATCH_FIRST;
     if (F_ == 0) return LS_OK;
     double* d_vel = nullptr;
     LS_HIP(dev_upload(&d_vel, vel, (size_t)E * 3));
@@ -3413,6 +3425,7 @@ class EngineImpl final : public Engine {
     m.ccurv = d_ccurv_;
     m.pgroup = d_pgroup_;
     m.ngroups = ngroups_;
+    m.sigma_phi = sigma_phi_ ? 1 : 0;
     m.ceik0 = kCeikDefault;
     m.ccurv0 = kCcurvDefault;
     return m;
@@ -3504,6 +3517,7 @@ class EngineImpl final : public Engine {
   T* d_warm_ = nullptr;  // learned warm-start distances (N), written by the predictor
   unsigned long long* d_cfl_ = nullptr;  // CFL bound keys (2)
   int32_t* d_gfail_ = nullptr;   // warm-start failure flag per mesh (ngroups_)
+  bool sigma_phi_ = false;       // triangles advect with the velocity from phi (LS_VEL_PHI_2D)
   int32_t* d_pgroup_ = nullptr;  // mesh of each patch of a batch (P), or null
   int ngroups_ = 1;
   float* scratch_ = nullptr;  // this device's `scratch` (scratch_address), never freed

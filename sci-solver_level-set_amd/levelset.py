@@ -18,6 +18,7 @@ LS_F32, LS_F64 = 0, 1
 LS_MODE_ADVECT, LS_MODE_MCF = 0, 1
 LS_EXEC_PERSISTENT, LS_EXEC_KERNELS, LS_EXEC_GRAPH = 0, 1, 2
 LS_WARM_NONE, LS_WARM_IDENTITY, LS_WARM_LEARNED = 0, 1, 2
+LS_VEL_ELEMENT, LS_VEL_PATCH_FIRST, LS_VEL_PHI_2D = 0, 1, 2
 LS_SWEEP_AUTO, LS_SWEEP_ELEMENTS, LS_SWEEP_TASKS = 0, 1, 2
 LS_LAYOUT_DEVICE, LS_LAYOUT_HOST = 0, 1
 
@@ -233,9 +234,9 @@ class Solver:
         a = np.ascontiguousarray(phi, dtype=np.float64)
         self._check(self.L.ls_set_phi(self.ctx, a.ctypes.data, LS_F64))
 
-    def set_velocity(self, vel, patch_sigma=0):
+    def set_velocity(self, vel, mode=LS_VEL_ELEMENT):
         a = np.ascontiguousarray(vel, dtype=np.float64)
-        self._check(self.L.ls_set_velocity(self.ctx, a.ctypes.data, int(patch_sigma)))
+        self._check(self.L.ls_set_velocity(self.ctx, a.ctypes.data, int(mode)))
 
     def step(self, params: StepParams) -> Stats:
         st = Stats()

@@ -378,3 +378,24 @@ def test_near_degenerate_tets_bit_exact(engine, prec, oracle_mod):
             np.testing.assert_array_equal(gphi.astype(np.float32), ophi.astype(np.float32), err_msg=f"step {k}")
         else:
             np.testing.assert_array_equal(gphi, ophi, err_msg=f"step {k}")
+
+
+@pytest.mark.parametrize("which,mode", [("tri", lsb.LS_VEL_PHI_2D), ("tet", lsb.LS_VEL_PATCH_FIRST)])
+@pytest.mark.parametrize("execution", [lsb.LS_EXEC_PERSISTENT, lsb.LS_EXEC_KERNELS])
+def test_velocity_modes_bit_exact_with_oracle(oracle_mod, which, mode, execution):
+    """The reference's velocity quirks as opt-in modes: the 2D advection's
+    velocity from phi (meshFIM2d_kernels.h:178-181) and the patch's first
+    copy velocity (meshFIM_kernels.h:51-53), each step against the oracle."""
+    case = cases.config1() if which == "tri" else cases.small3d(n=16, shape="torus", block=(4, 4, 4))
+    g = lsb.solver_for_case(case, precision=lsb.LS_F64, execution=execution)
+    g.set_velocity(case.vel, mode)
+    o = oracle_mod.Oracle(case.dim, case.xyz, case.cells, case.labels, case.nparts, 64)
+    o.set_phi(case.phi0)
+    o.set_velocity(case.vel, mode)
+    for k in range(4):
+        g.step(lsb.step_params(case.dt, case.bandwidth, case.niter))
+        o.step(oracle_mod.make_params(case.dt, case.bandwidth, case.niter))
+        np.testing.assert_array_equal(g.narrowband(), o.narrowband())
+        np.testing.assert_array_equal(g.phi(), o.phi(), err_msg=f"step {k}")
+    g.close()
+    o.close()

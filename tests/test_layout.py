@@ -109,6 +109,11 @@ def test_errors_are_returned_not_raised_across_abi():
     s.set_mesh(3, xyz, cells)
     with pytest.raises(lsb.LevelSetError, match="out of range"):
         s.partition_labels(np.full(xyz.shape[0], 9, np.int32), 2)
+    vel = np.zeros((cells.shape[0], 3))
+    with pytest.raises(lsb.LevelSetError, match="triangle meshes"):
+        s.set_velocity(vel, lsb.LS_VEL_PHI_2D)
+    with pytest.raises(lsb.LevelSetError, match="unknown velocity mode"):
+        s.set_velocity(vel, 7)
     bad = cells.copy()
     bad[0, 0] = 10 ** 6
     with pytest.raises(lsb.LevelSetError, match="out of range"):

@@ -259,10 +259,12 @@ int ls_set_warm(ls_ctx* ctx, const double* warm);
  * ls_set_warm, rounded to the working type), in the caller's order. */
 int ls_get_warm(ls_ctx* ctx, double* out);
 
-/* Learned warm start (no reference counterpart; tetrahedral meshes). A
+/* Learned warm start (no reference counterpart; tet and triangle meshes). A
  * 22-64-64-4 ReLU MLP per element copy: inputs are the four corner phi values
  * and the six edge vectors in the copy's local frame, divided by |AB|;
  * outputs y_i = (D_i - |phi_i|) / |AB| with D_i the redistanced distance.
+ * A triangle copy fills the same inputs with its three corners and edges
+ * AB, AC, BC (the rest zero) and uses three outputs.
  * Inference and training run BF16 MFMA with FP32 accumulation. Parameters
  * are a flat FP32 vector: W1 (22x64, by input), b1 (64), W2 (64x64), b2,
  * W3 (64x4), b3 (4). */

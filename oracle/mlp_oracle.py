@@ -1,8 +1,8 @@
 """CPU oracle of the learned warm start's network (test infrastructure only).
 
 A numpy FP64 restatement of the 22-64-64-4 ReLU MLP the GPU runs with BF16
-MFMA: the per-copy features, the forward pass, the mean squared error and
-its analytic gradient. `bf16=True` rounds every matrix-product operand to
+MFMA: the per-copy features (tets, and triangles in the same layout), the
+forward pass, the mean squared error and its analytic gradient. `bf16=True` rounds every matrix-product operand to
 BF16 (round to nearest even) as the GPU's MFMA inputs are, so GPU results
 can be compared at FP32-accumulation tolerance; `bf16=False` is the exact
 FP64 network that central finite differences pin.
@@ -62,6 +62,34 @@ def features(ph, L):
     AD = np.stack([L[:, 3], L[:, 4], L[:, 5]], 1)
     X = np.concatenate([ph, AB, AC, AD, AC - AB, AD - AB, AD - AC], 1) / s[:, None]
     return X, s
+
+
+def features_tri(ph, c):
+    """Features of triangle copies: ph (n,3) corner phi, c (n,3) local
+    coordinates A=(0,0), B=(c0,0), C=(c1,c2), in the tet layout with the
+    fourth corner's phi and the edges AD, BD, CD zero."""
+    ph = np.asarray(ph, np.float64)
+    c = np.asarray(c, np.float64)
+    s = c[:, 0]
+    z = np.zeros_like(s)
+    AB = np.stack([c[:, 0], z, z], 1)
+    AC = np.stack([c[:, 1], c[:, 2], z], 1)
+    Z3 = np.zeros((s.size, 3))
+    X = np.concatenate([ph, z[:, None], AB, AC, Z3, AC - AB, Z3, Z3], 1) / s[:, None]
+    return X, s
+
+
+def local_coords_tri(xyz, tris):
+    """Local coordinates of triangles (rows A,B,C): |AB|, AC.X, AC.Y with X
+    along AB and Y the in-plane normal to it towards C."""
+    A, B, Cc = (np.asarray(xyz, np.float64)[tris[:, i]] for i in range(3))
+    ab = B - A
+    lab = np.linalg.norm(ab, axis=1)
+    ex = ab / lab[:, None]
+    ac = Cc - A
+    acx = np.einsum("ij,ij->i", ac, ex)
+    yv = ac - acx[:, None] * ex
+    return np.stack([lab, acx, np.linalg.norm(yv, axis=1)], 1)
 
 
 def forward(p, X, bf16_ops=False):

@@ -34,7 +34,7 @@ class Engine {
   virtual int get_phi(double* phi_user, std::string& err) = 0;
   virtual int get_narrowband(std::vector<int32_t>& ids, std::string& err) = 0;
   virtual int working_bytes() const = 0;  // sizeof working real type
-  // Learned warm start (tetrahedral engines).
+  // Learned warm start (tet and triangle engines).
   virtual int mlp_set_params(const float* prm, std::string& err) = 0;
   virtual int mlp_get_params(float* prm, std::string& err) = 0;
   virtual int mlp_predict(std::string& err) = 0;

@@ -1781,9 +1781,10 @@ __device__ inline int copy_patch(const DevMesh<T>& m, int64_t c) {
 
 // Warm values of one narrowband patch: per element copy the predicted
 // corner distances, per owned vertex their mean over its copies.
-template <typename T>
+template <typename T, int NC>
 __global__ __launch_bounds__(kMlpBlock) void k_mlp_predict(DevMesh<T> m, DevState<T> s, const int32_t* nb,
                                                            mlp::Weights w, T* warm) {
+  constexpr int K = ElemSolve<T, NC>::kCoords;
   extern __shared__ __align__(16) unsigned char smem[];
   const int p = nb[blockIdx.x];
   const int vs = m.vert_off[p], ve = m.vert_off[p + 1];
@@ -1796,8 +1797,8 @@ __global__ __launch_bounds__(kMlpBlock) void k_mlp_predict(DevMesh<T> m, DevStat
   float* H1 = X + 16 * mlp::kInPad;
   float* H2 = H1 + 16 * mlp::kHid;
   float* O = H2 + 16 * mlp::kHid;
-  float* s_pred = (float*)smem + (kMlpBlock / 64) * mlp::kWaveTile;  // 4 per copy
-  float* s_vtx = s_pred + 4 * ne;
+  float* s_pred = (float*)smem + (kMlpBlock / 64) * mlp::kWaveTile;  // NC per copy (one per slot)
+  float* s_vtx = s_pred + NC * ne;
   for (int i = threadIdx.x; i < nv; i += kMlpBlock) s_vtx[i] = (float)s.phi[vs + i];
   for (int i = threadIdx.x; i < nh; i += kMlpBlock) s_vtx[nv + i] = (float)s.phi[m.halo_ids[hs + i]];
   __syncthreads();
@@ -1811,10 +1812,10 @@ __global__ __launch_bounds__(kMlpBlock) void k_mlp_predict(DevMesh<T> m, DevStat
         const ushort4 c = m.copy_local[es + e];
         float L[6];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) ph[i] = s_vtx[corner(c, i)];
+        for (int i = 0; i < NC; ++i) ph[i] = s_vtx[corner(c, i)];
 #pragma unroll
-        for (int k = 0; k < 6; ++k) L[k] = (float)m.coords[(int64_t)(es + e) * 6 + k];
-        sc = mlp::features(ph, L, x);
+        for (int k = 0; k < K; ++k) L[k] = (float)m.coords[(int64_t)(es + e) * K + k];
+        sc = mlp::copy_features<NC>(ph, L, x);
       } else {
 #pragma unroll
         for (int k = 0; k < mlp::kIn; ++k) x[k] = 0.f;
@@ -1826,9 +1827,9 @@ __global__ __launch_bounds__(kMlpBlock) void k_mlp_predict(DevMesh<T> m, DevStat
     mlp::forward16(w, X, H1, H2, O, lane);
     if (lane < 16 && e < ne) {
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
+      for (int i = 0; i < NC; ++i) {
         const float d = fabsf(ph[i]) + sc * O[r * mlp::kOutPad + i];
-        s_pred[e * 4 + i] = d > 0.f ? d : 0.f;
+        s_pred[e * NC + i] = d > 0.f ? d : 0.f;
       }
     }
     mlp::wave_sync();
@@ -1844,9 +1845,10 @@ __global__ __launch_bounds__(kMlpBlock) void k_mlp_predict(DevMesh<T> m, DevStat
 
 // Training inputs of the copies listed in `copies`: padded features from
 // the current phi (before the step) into X[i][kInPad].
-template <typename T>
+template <typename T, int NC>
 __global__ __launch_bounds__(kBlock) void k_mlp_collect_x(DevMesh<T> m, DevState<T> s, const int32_t* copies,
                                                           int64_t n, float* X) {
+  constexpr int K = ElemSolve<T, NC>::kCoords;
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
   const int64_t c = copies[i];
@@ -1856,31 +1858,36 @@ __global__ __launch_bounds__(kBlock) void k_mlp_collect_x(DevMesh<T> m, DevState
   const ushort4 cl = m.copy_local[c];
   float ph[4], L[6], x[mlp::kIn];
 #pragma unroll
-  for (int k = 0; k < 4; ++k) {
+  for (int k = 0; k < NC; ++k) {
     const int lv = corner(cl, k);
     ph[k] = (float)s.phi[lv < nv ? vs + lv : m.halo_ids[hs + lv - nv]];
   }
 #pragma unroll
-  for (int k = 0; k < 6; ++k) L[k] = (float)m.coords[c * 6 + k];
-  mlp::features(ph, L, x);
+  for (int k = 0; k < K; ++k) L[k] = (float)m.coords[c * K + k];
+  mlp::copy_features<NC>(ph, L, x);
 #pragma unroll
   for (int k = 0; k < mlp::kInPad; ++k) X[i * mlp::kInPad + k] = k < mlp::kIn ? x[k] : 0.f;
 }
 
 // Training targets after the step: y_k = D_k / s - |x_k| with D the
 // redistanced distances (DT) at the copy's corners.
-template <typename T>
+template <typename T, int NC>
 __global__ __launch_bounds__(kBlock) void k_mlp_collect_y(DevMesh<T> m, DevState<T> s, const int32_t* copies,
                                                           int64_t n, const float* X, float* Y) {
+  constexpr int K = ElemSolve<T, NC>::kCoords;
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
   const int64_t c = copies[i];
   const int p = copy_patch(m, c);
   const int vs = m.vert_off[p], nv = m.vert_off[p + 1] - vs, hs = m.halo_off[p];
   const ushort4 cl = m.copy_local[c];
-  const float inv = 1.0f / (float)m.coords[c * 6];
+  const float inv = 1.0f / (float)m.coords[c * K];
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
+    if (k >= NC) {  // triangles: no fourth corner
+      Y[i * mlp::kOut + k] = 0.f;
+      continue;
+    }
     const int lv = corner(cl, k);
     const float D = (float)s.DT[lv < nv ? vs + lv : m.halo_ids[hs + lv - nv]];
     Y[i * mlp::kOut + k] = D < 0.5f * (float)kLargeNum ? D * inv - fabsf(X[i * mlp::kInPad + k]) : 0.f;
@@ -2669,12 +2676,7 @@ class EngineImpl final : public Engine {
     LS_HIP(hipGetLastError());
     return LS_OK;
   }
-  int mlp_tet_only(std::string& err) const {
-    if (NC != 4) { err = "the learned warm start supports tetrahedral meshes"; return LS_ERR_ARG; }
-    return LS_OK;
-  }
   int mlp_set_params(const float* prm, std::string& err) override {
-    if (int rc = mlp_tet_only(err)) return rc;
     if (!uploaded_) { err = "layout not built"; return LS_ERR_STATE; }
     if (int rc = mlp_alloc(err)) return rc;
     LS_HIP(hipMemcpy(d_mlp_prm_, prm, mlp::kNumParams * sizeof(float), hipMemcpyHostToDevice));
@@ -2698,7 +2700,6 @@ class EngineImpl final : public Engine {
   // Warm values for the next step from the current phi over the current
   // narrowband (the next step's previous narrowband).
   int mlp_predict(std::string& err) override {
-    if (int rc = mlp_tet_only(err)) return rc;
     if (!mlp_ready_) { err = "no network parameters set"; return LS_ERR_STATE; }
     if (nnb_ == 0) { err = "no narrowband yet: the first step has nothing to warm-start"; return LS_ERR_STATE; }
     if (!d_warm_) {
@@ -2706,7 +2707,7 @@ class EngineImpl final : public Engine {
       std::vector<T> big(N_, (T)kLargeNum);
       LS_HIP(hipMemcpy(d_warm_, big.data(), N_ * sizeof(T), hipMemcpyHostToDevice));
     }
-    k_mlp_predict<T><<<nnb_, kMlpBlock, mlp_predict_lds(), stream_>>>(mesh(), state(), d_nb_cur(), mlp_weights(),
+    k_mlp_predict<T, NC><<<nnb_, kMlpBlock, mlp_predict_lds(), stream_>>>(mesh(), state(), d_nb_cur(), mlp_weights(),
                                                                       d_warm_);
     LS_HIP(hipGetLastError());
     LS_HIP(hipStreamSynchronize(stream_));
@@ -2715,7 +2716,6 @@ class EngineImpl final : public Engine {
   // Training pairs: phase 0 before a step records the inputs of every copy
   // of the current narrowband; phase 1 after the step records their targets.
   int mlp_collect(int phase, int64_t capacity, std::string& err) override {
-    if (int rc = mlp_tet_only(err)) return rc;
     if (phase == 0) {
       if (nnb_ == 0) { err = "no narrowband yet"; return LS_ERR_STATE; }
       if (!d_ds_x_) {
@@ -2741,14 +2741,14 @@ class EngineImpl final : public Engine {
         pend_cap_ = n;
       }
       LS_HIP(hipMemcpy(d_pend_, copies.data(), n * sizeof(int32_t), hipMemcpyHostToDevice));
-      k_mlp_collect_x<T><<<(int)((n + kBlock - 1) / kBlock), kBlock, 0, stream_>>>(
+      k_mlp_collect_x<T, NC><<<(int)((n + kBlock - 1) / kBlock), kBlock, 0, stream_>>>(
           mesh(), state(), d_pend_, n, d_ds_x_ + ds_n_ * mlp::kInPad);
       LS_HIP(hipGetLastError());
       LS_HIP(hipStreamSynchronize(stream_));
       return LS_OK;
     }
     if (pend_n_ <= 0) return LS_OK;
-    k_mlp_collect_y<T><<<(int)((pend_n_ + kBlock - 1) / kBlock), kBlock, 0, stream_>>>(
+    k_mlp_collect_y<T, NC><<<(int)((pend_n_ + kBlock - 1) / kBlock), kBlock, 0, stream_>>>(
         mesh(), state(), d_pend_, pend_n_, d_ds_x_ + ds_n_ * mlp::kInPad, d_ds_y_ + ds_n_ * mlp::kOut);
     LS_HIP(hipGetLastError());
     LS_HIP(hipStreamSynchronize(stream_));
@@ -2783,7 +2783,6 @@ class EngineImpl final : public Engine {
   }
   // Loss sum and the gradient of the mean loss over n caller samples.
   int mlp_gradient(const float* X, const float* Y, int64_t n, float* grad, double* loss, std::string& err) override {
-    if (int rc = mlp_tet_only(err)) return rc;
     if (!mlp_ready_) { err = "no network parameters set"; return LS_ERR_STATE; }
     if (n <= 0) { err = "n must be > 0"; return LS_ERR_ARG; }
     std::vector<float> xp((size_t)n * mlp::kInPad, 0.f);
@@ -2813,7 +2812,6 @@ class EngineImpl final : public Engine {
   // by a kernel, so the sequence stays graph-capturable.
   int mlp_gradient_noalloc(const float* X, const float* Y, int64_t n, float* grad, double* loss,
                            std::string& err) override {
-    if (int rc = mlp_tet_only(err)) return rc;
     if (!mlp_ready_) { err = "no network parameters set"; return LS_ERR_STATE; }
     if (n <= 0) { err = "n must be > 0"; return LS_ERR_ARG; }
     if (n > kScratchSamples) {
@@ -2844,7 +2842,6 @@ class EngineImpl final : public Engine {
   // Adam over the collected pairs, shuffled every epoch; returns the mean
   // squared error per output of the last epoch.
   int mlp_train(int epochs, int batch, double lr, uint64_t seed, double* loss, std::string& err) override {
-    if (int rc = mlp_tet_only(err)) return rc;
     if (!mlp_ready_) { err = "no network parameters set"; return LS_ERR_STATE; }
     if (ds_n_ == 0) { err = "no training pairs collected"; return LS_ERR_STATE; }
     if (epochs <= 0 || batch <= 0) { err = "epochs and batch must be > 0"; return LS_ERR_ARG; }

@@ -7,7 +7,10 @@
 // frame, all divided by the copy's scale s = |AB|. Output (4 values, padded
 // to 16): y_i = (D_i - |phi_i|) / s, D_i the corner's distance after
 // redistancing. The predicted distance is max(0, |phi_i| + s * y_i); a
-// vertex's warm value is the mean over its element copies.
+// vertex's warm value is the mean over its element copies. A triangle copy
+// uses the same layout with its three corners and edges AB, AC, BC, the
+// fourth corner's phi and edges zero and its output unused (a context holds
+// one kind of mesh, so its network sees one kind of input).
 //
 // MFMA operand layout (64 lanes, g = lane / 16, r = lane % 16):
 //   A (16x16, rows m, cols k):   lane holds A[r][4g + j], j = 0..3
@@ -78,6 +81,32 @@ __host__ __device__ inline float features(const float* ph, const float* L, float
     x[19 + k] = (AD[k] - AC[k]) * inv;
   }
   return s;
+}
+
+// Triangle copy, local frame A=(0,0), B=(c0,0), C=(c1,c2).
+__host__ __device__ inline float features_tri(const float* ph, const float* c, float* x) {
+  const float s = c[0];
+  const float inv = 1.0f / s;
+  for (int i = 0; i < 3; ++i) x[i] = ph[i] * inv;
+  x[3] = 0.f;
+  const float AB[3] = {c[0], 0.f, 0.f}, AC[3] = {c[1], c[2], 0.f};
+  for (int k = 0; k < 3; ++k) {
+    x[4 + k] = AB[k] * inv;
+    x[7 + k] = AC[k] * inv;
+    x[10 + k] = 0.f;
+    x[13 + k] = (AC[k] - AB[k]) * inv;
+    x[16 + k] = 0.f;
+    x[19 + k] = 0.f;
+  }
+  return s;
+}
+
+// Features of an element copy with NC corners (ph: NC values, L: its local
+// coordinates).
+template <int NC>
+__host__ __device__ inline float copy_features(const float* ph, const float* L, float* x) {
+  if constexpr (NC == 4) return features(ph, L, x);
+  else return features_tri(ph, L, x);
 }
 
 __device__ inline bf16x4 pack4(float a, float b, float c, float d) {

@@ -292,3 +292,93 @@ def test_learned_warm_start_ablation(mlp_oracle, oracle_mod):
         o.close()
         table[name] = rows
         print(name, "iters/visits/fallback/in-band Linf per step:", rows)
+
+
+def _tri_case():
+    import dataclasses
+    case = dataclasses.replace(cases.config1(n=65), labels=None, nparts=None)
+    lab = cases.partition_rcb(case.xyz, 64)
+    return dataclasses.replace(case, labels=lab, nparts=64)
+
+
+def test_triangle_pairs_and_predictions_match_network_oracle(mlp_oracle):
+    """The network on triangles (the 2D circle workload): collected inputs
+    and targets against the host features, and the per-vertex warm values
+    against the numpy forward pass with BF16 operands, averaged over each
+    vertex's copies in its own patch."""
+    M = mlp_oracle
+    case = _tri_case()
+    p = _params(M)
+    s = lsb.solver_for_case(case, precision=lsb.LS_F64)
+    s.mlp_set_params(p)
+    prm = lsb.step_params(case.dt, case.bandwidth, case.niter)
+    s.step(prm)
+    nb = s.narrowband()
+    phi_before = s.phi()
+    s.mlp_collect(0, 100000)
+    s.step(prm)
+    s.mlp_collect(1)
+    X, Y = s.mlp_dataset()
+    dist = s.distances()
+    lay = s.layout()
+    eo = lay["ele_off"]
+    copies = np.concatenate([np.arange(eo[q], eo[q + 1]) for q in nb])
+    assert X.shape[0] == copies.size
+    tris = case.cells[lay["copy_src"][copies]]
+    want, sc = M.features_tri(phi_before[tris].astype(np.float32), M.local_coords_tri(case.xyz, tris))
+    np.testing.assert_allclose(X, want, rtol=2e-5, atol=2e-6)
+    D = dist[tris]
+    wy = np.zeros((copies.size, 4))
+    wy[:, :3] = np.where(D < 5e8, D / sc[:, None] - np.abs(want[:, :3]), 0.0)
+    np.testing.assert_allclose(Y, wy, rtol=1e-4, atol=1e-4)
+
+    # predictions from the current state over the current narrowband
+    nb = s.narrowband()
+    phi = s.phi()
+    s.mlp_predict()
+    W = s.warm_values()
+    s.close()
+    copies = np.concatenate([np.arange(eo[q], eo[q + 1]) for q in nb])
+    cpatch = np.repeat(nb, [eo[q + 1] - eo[q] for q in nb])
+    tris = case.cells[lay["copy_src"][copies]]
+    Xn, scn = M.features_tri(phi[tris].astype(np.float32), M.local_coords_tri(case.xyz, tris))
+    out = M.forward(p.astype(np.float64), Xn, bf16_ops=True)[0]
+    pred = np.maximum(np.abs(phi[tris].astype(np.float32)) + scn[:, None] * out[:, :3], 0.0)
+    acc = np.zeros(case.xyz.shape[0])
+    cnt = np.zeros(case.xyz.shape[0])
+    for k in range(3):
+        own = case.labels[tris[:, k]] == cpatch
+        np.add.at(acc, tris[own, k], pred[own, k])
+        np.add.at(cnt, tris[own, k], 1)
+    v = cnt > 0
+    print(f"predicted vertices {int(v.sum())}, max |diff| {np.max(np.abs(W[v] - acc[v] / cnt[v])):.2e}")
+    np.testing.assert_allclose(W[v], acc[v] / cnt[v], rtol=1e-4, atol=1e-6)
+
+
+def test_learned_warm_start_on_triangles_bit_exact_with_oracle(mlp_oracle, oracle_mod):
+    """The network's warm values on triangles: the oracle takes the GPU's
+    predictions through set_warm and reproduces every step and decision."""
+    case = _tri_case()
+    g = lsb.solver_for_case(case, precision=lsb.LS_F64)
+    g.mlp_set_params(_params(mlp_oracle))
+    o = oracle_mod.Oracle(2, case.xyz, case.cells, case.labels, case.nparts, 64)
+    o.set_phi(case.phi0)
+    o.set_velocity(case.vel)
+    rows = []
+    for k in range(5):
+        kw = {"tol": 1e-6}
+        if k > 0:
+            g.mlp_predict()
+            o.set_warm(g.warm_values())
+            kw.update(warm=lsb.LS_WARM_LEARNED, warm_margin=0.03, warm_abs_margin=0.25 * case.h)
+        gs = g.step(lsb.step_params(case.dt, case.bandwidth, case.niter, **kw))
+        os_ = o.step(oracle_mod.make_params(case.dt, case.bandwidth, case.niter, **kw))
+        rows.append((gs.fim_iters, gs.warm_patches, gs.warm_fallback))
+        assert (gs.warm_fallback, gs.fim_iters, gs.warm_patches) == (os_.warm_fallback, os_.fim_iters,
+                                                                      os_.warm_patches), k
+        np.testing.assert_array_equal(g.narrowband(), o.narrowband())
+        np.testing.assert_array_equal(g.phi(), o.phi(), err_msg=f"phi step {k}")
+    print("iters / warm patches / fallback per step:", rows)
+    assert rows[1][1] > 0
+    g.close()
+    o.close()

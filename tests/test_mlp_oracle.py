@@ -45,3 +45,26 @@ def test_local_frame_matches_layout(oracle_mod, mlp_oracle):
     o.close()
     mine = mlp_oracle.local_coords(case.xyz, case.cells[lay["copy_src"]])
     np.testing.assert_allclose(mine, L, rtol=1e-12, atol=1e-14)
+
+
+def test_triangle_features_closed_form(mlp_oracle):
+    """A triangle copy fills the tet layout: its three corners' phi, edges AB,
+    AC, BC, everything else zero, all divided by |AB|."""
+    c = np.array([[2.0, 0.5, 1.5]])
+    ph = np.array([[0.2, -0.4, 0.6]])
+    X, s = mlp_oracle.features_tri(ph, c)
+    assert s[0] == 2.0
+    B, C = np.array([2.0, 0, 0]), np.array([0.5, 1.5, 0])
+    z3 = np.zeros(3)
+    want = np.concatenate([ph[0], [0.0], B, C, z3, C - B, z3, z3]) / 2.0
+    np.testing.assert_allclose(X[0], want, rtol=0, atol=1e-15)
+
+
+def test_triangle_local_frame_matches_layout(oracle_mod, mlp_oracle):
+    case = cases.config1()
+    o = oracle_mod.Oracle(2, case.xyz, case.cells, case.labels, case.nparts, 64)
+    lay = o.layout()
+    L = o.coords()
+    o.close()
+    mine = mlp_oracle.local_coords_tri(case.xyz, case.cells[lay["copy_src"]])
+    np.testing.assert_allclose(mine, L, rtol=1e-12, atol=1e-14)

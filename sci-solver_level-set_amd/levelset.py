@@ -328,6 +328,11 @@ class Solver:
     def mlp_collect(self, phase, capacity=0):
         self._check(self.L.ls_mlp_collect(self.ctx, int(phase), int(capacity)))
 
+    def mlp_dataset_size(self):
+        n = C.c_int64(0)
+        self._check(self.L.ls_mlp_dataset(self.ctx, None, None, C.byref(n)))
+        return n.value
+
     def mlp_dataset(self):
         n = C.c_int64(0)
         self._check(self.L.ls_mlp_dataset(self.ctx, None, None, C.byref(n)))

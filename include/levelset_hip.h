@@ -36,8 +36,12 @@ enum { LS_EXEC_PERSISTENT = 0, LS_EXEC_KERNELS = 1, LS_EXEC_GRAPH = 2 };
 /* FIM sweep organisation (results are identical): one thread per element
  * copy, or the (element, corner) solves left by the exact lower-bound skip
  * spread one per thread (needs more LDS). AUTO (default) is per element,
- * the faster of the two on MI355X in measurements so far. */
-enum { LS_SWEEP_AUTO = 0, LS_SWEEP_ELEMENTS = 1, LS_SWEEP_TASKS = 2 };
+ * the faster of the two on MI355X in measurements so far. By default a
+ * visit stages each element copy's corner ids and heights in LDS when that
+ * costs the persistent kernel no workgroups per CU; OR-ing
+ * LS_SWEEP_UNSTAGED into the mode makes every sweep read them from memory
+ * (same results). */
+enum { LS_SWEEP_AUTO = 0, LS_SWEEP_ELEMENTS = 1, LS_SWEEP_TASKS = 2, LS_SWEEP_UNSTAGED = 4 };
 /* Where the patch layout is built: on the context's GPU (default; radix
  * sorts, identical arrays) or on the host. Host-only contexts always build
  * on the host. */
@@ -96,7 +100,7 @@ typedef struct ls_stats {
   int64_t seed_copies, seed_verts, seed_halo;
   double ms_step_kernel;    /* duration of the persistent step kernel (LS_EXEC_PERSISTENT/GRAPH) */
   int32_t exec_mode;        /* execution mode used for the step */
-  int32_t reserved;
+  int32_t sweep_staged;     /* 1: the FIM sweeps staged corner ids and heights in LDS (LS_SWEEP_UNSTAGED) */
   /* Persistent kernel phase times (device wall clock, barrier to barrier):
    * reset+seeds, FIM visits with neighbour activation, (unused: 0),
    * copyback with the probes and the next list, (unused: 0), narrowband,

@@ -300,6 +300,14 @@ struct ElemSolve<T, 4> {
     }
     return mask;
   }
+  // The same from the copy's four FP16 heights packed in 8 bytes.
+  __device__ static int need_mask(const uint2 hw, const T* v, double scale) {
+    const __half hs[4] = {__ushort_as_half((unsigned short)(hw.x & 0xffffu)),
+                          __ushort_as_half((unsigned short)(hw.x >> 16)),
+                          __ushort_as_half((unsigned short)(hw.y & 0xffffu)),
+                          __ushort_as_half((unsigned short)(hw.y >> 16))};
+    return need_mask(hs, v, scale);
+  }
   // Corner k's updated value min(v_k, solve_k) from the element's values v.
   __device__ static T corner_update(const T* Lt, const T* v, int k, double leps) {
     double L[6];
@@ -405,6 +413,17 @@ __device__ __forceinline__ int copy_need_mask(const DevMesh<T>& m, int p, int64_
     for (int q = 0; q < 3; ++q) L[q] = m.coords[c * 3 + q];
     return ElemSolve<T, 3>::need_mask(L, x);
   }
+}
+
+// copy_need_mask for element e of a patch whose first copy is es, with the
+// heights staged in LDS (s_hgt, 8 bytes per copy) when `stage` is set.
+template <typename T, int NC>
+__device__ __forceinline__ int staged_need_mask(const DevMesh<T>& m, int p, int64_t es, int e, const T* x,
+                                                bool stage, const uint2* s_hgt) {
+  if constexpr (NC == 4) {
+    if (stage) return ElemSolve<T, 4>::need_mask(s_hgt[e], x, (double)m.pscale[p]);
+  }
+  return copy_need_mask<T, NC>(m, p, es + e, x);
 }
 
 // Local frame and coordinates of one element copy in FP64 (scheme of
@@ -743,6 +762,22 @@ __device__ __forceinline__ SweepIO<T> default_io(const DevState<T>& s) {
   return SweepIO<T>{s.DT_out, s.visitE, s.DT_out, s.bcon, 0};
 }
 
+// Per-workgroup statistics in LDS: kNumSizes patch-size sums, then the
+// visit and probe counts of the persistent step.
+constexpr int kAccVisits = kNumSizes, kAccProbes = kNumSizes + 1, kNumAcc = kNumSizes + 2;
+__device__ __forceinline__ void clear_stats(unsigned long long* acc) {
+  if ((int)threadIdx.x < kNumAcc) acc[threadIdx.x] = 0ull;
+}
+// Adds the workgroup's statistics to the device counters (after a barrier
+// that ends the workgroup's last update of them).
+template <typename T>
+__device__ __forceinline__ void flush_stats(const DevState<T>& s, const unsigned long long* acc) {
+  const int t = threadIdx.x;
+  if (t < kNumSizes && acc[t]) atomicAdd(s.sizes + t, acc[t]);
+  if (t == kAccVisits && acc[t]) atomicAdd(&s.pc[kPcVisits], (int32_t)acc[t]);
+  if (t == kAccProbes && acc[t]) atomicAdd(&s.pc[kPcProbes], (int32_t)acc[t]);
+}
+
 // Marks corner i of element e changed (s_dirty is 4-byte aligned in LDS;
 // lanes updating other elements of the same word do so atomically too).
 __device__ __forceinline__ void mark_changed(uint8_t* s_dirty, int e, int i) {
@@ -774,10 +809,14 @@ constexpr int kOrgElements = 0, kOrgTasks = 1;
 // patch stays unconverged. The resumed visit computes the same bits.
 // members > 0: vertices of patches visited in FIM iteration `members` are
 // read from DT_out (their copy into DT runs concurrently), all others from DT.
+// acc: the workgroup's statistics in LDS (kNumSizes patch-size sums, then
+// visits and probes), added to the device counters once per workgroup by
+// flush_stats, so visits contend on no global counter for them.
 template <typename T, int NC, int BS, int ORG>
 __device__ SweepResult<T> sweep_patch(const DevMesh<T>& m, DevState<T>& s, int p, int niter, int write_out,
                                       T tol, double leps, unsigned char* smem, T* red, int* s_ndirty /*[10]*/,
-                                      int epoch_in, int epoch_out, int members, const SweepIO<T> io) {
+                                      int epoch_in, int epoch_out, int members, const SweepIO<T> io,
+                                      unsigned long long* acc, int stage) {
   constexpr bool tasks = ORG == kOrgTasks;
   constexpr int K = ElemSolve<T, NC>::kCoords;
   __syncthreads();
@@ -804,8 +843,13 @@ __device__ SweepResult<T> sweep_patch(const DevMesh<T>& m, DevState<T>& s, int p
   using Bits = typename OrderedBits<T>::type;
   const int o_vmin = (o_dirty + ne + 7) & ~7;
   Bits* s_vmin = (Bits*)(smem + o_vmin);
+  // Staged element data (stage set): each copy's corner ids and FP16
+  // heights, read by every sweep of the visit from LDS instead of memory.
+  const int o_stage = (o_vmin + nv * (int)sizeof(Bits) + 7) & ~7;
+  ushort4* s_cl = (ushort4*)(smem + o_stage);
+  uint2* s_hgt = (uint2*)(smem + o_stage + 8 * ne);
   // Task mode: (element, corner) solves and their results.
-  uint32_t* s_task = (uint32_t*)(smem + ((o_vmin + nv * (int)sizeof(Bits) + 7) & ~7));
+  uint32_t* s_task = (uint32_t*)(smem + o_stage + (stage ? 16 * ne : 0));
   T* s_new = (T*)(s_task + NC * ne + (NC * ne) % 2);
   const int tid = threadIdx.x;
 
@@ -822,6 +866,10 @@ __device__ SweepResult<T> sweep_patch(const DevMesh<T>& m, DevState<T>& s, int p
     s_vtx[nv + i] = (members > 0 && io.vtag[m.vpatch[h]] == members) ? io.din[h] : s.DT[h];
   }
   copy_to_lds<BS, 8>(s_vsl, m.vslots + sbase, nslots);
+  if (stage) {
+    copy_to_lds<BS, 4>(s_cl, m.copy_local + es, ne);
+    if constexpr (NC == 4) copy_to_lds<BS, 4>(s_hgt, (const uint2*)m.hgt + es, ne);
+  }
   // (A single-sweep visit recomputes the sweep: resuming would leave it none.)
   const bool resume = niter > 1 && epoch_in > 0 && s.rs_epoch[p] == epoch_in;
   if (resume) {
@@ -836,7 +884,7 @@ __device__ SweepResult<T> sweep_patch(const DevMesh<T>& m, DevState<T>& s, int p
     ushort4 c[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u)
-      if (base + u * BS < ne) c[u] = m.copy_local[es + base + u * BS];
+      if (base + u * BS < ne) c[u] = stage ? s_cl[base + u * BS] : m.copy_local[es + base + u * BS];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       const int e = base + u * BS;
@@ -887,7 +935,7 @@ __device__ SweepResult<T> sweep_patch(const DevMesh<T>& m, DevState<T>& s, int p
           T x[NC];
 #pragma unroll
           for (int i = 0; i < NC; ++i) x[i] = s_slot[e * NC + i];
-          mask = copy_need_mask<T, NC>(m, p, es + e, x) & unchanged_input_filter<NC>(s_dirty[e]);
+          mask = staged_need_mask<T, NC>(m, p, es, e, x, stage, s_hgt) & unchanged_input_filter<NC>(s_dirty[e]);
           s_dirty[e] = 0;
           ++n_upd;
         }
@@ -930,7 +978,7 @@ __device__ SweepResult<T> sweep_patch(const DevMesh<T>& m, DevState<T>& s, int p
         if (y != s_slot[e * NC + j]) {
           s_slot[e * NC + j] = y;
           mark_changed(s_dirty, e, j);
-          const int lv = corner(m.copy_local[es + e], j);
+          const int lv = corner(stage ? s_cl[e] : m.copy_local[es + e], j);
           if (lv < nv) atomicMin(&s_vmin[lv], OrderedBits<T>::of(y));
         }
       }
@@ -950,7 +998,8 @@ __device__ SweepResult<T> sweep_patch(const DevMesh<T>& m, DevState<T>& s, int p
           T x[NC];
 #pragma unroll
           for (int i = 0; i < NC; ++i) x[i] = s_slot[e * NC + i];
-          const int mask = copy_need_mask<T, NC>(m, p, es + e, x) & unchanged_input_filter<NC>(s_dirty[e]);
+          const int mask =
+              staged_need_mask<T, NC>(m, p, es, e, x, stage, s_hgt) & unchanged_input_filter<NC>(s_dirty[e]);
           s_dirty[e] = (uint8_t)(mask << 4);
           c = __popc(mask);
           n_solve += c;
@@ -993,7 +1042,7 @@ __device__ SweepResult<T> sweep_patch(const DevMesh<T>& m, DevState<T>& s, int p
         for (int q = 0; q < K; ++q) L[q] = gc[q];
         ElemSolve<T, NC>::solve_mask(L, s_dirty[e] >> 4, y, leps);
         int changed = 0;
-        const ushort4 cl = m.copy_local[es + e];
+        const ushort4 cl = stage ? s_cl[e] : m.copy_local[es + e];
 #pragma unroll
         for (int i = 0; i < NC; ++i) {
           if (y[i] != x[i]) {
@@ -1063,7 +1112,7 @@ __device__ SweepResult<T> sweep_patch(const DevMesh<T>& m, DevState<T>& s, int p
   con_all = __syncthreads_and(con_all);
   if (save_clean && visited) {
     for (int e = tid; e < ne; e += BS) {
-      const ushort4 cl = m.copy_local[es + e];
+      const ushort4 cl = stage ? s_cl[e] : m.copy_local[es + e];
       int ok = s_dirty[e] == 0;
 #pragma unroll
       for (int i = 0; i < NC; ++i) {
@@ -1087,22 +1136,22 @@ __device__ SweepResult<T> sweep_patch(const DevMesh<T>& m, DevState<T>& s, int p
     n_solve += __shfl_xor(n_solve, off, 64);
   }
   if ((tid & 63) == 0 && (n_upd | n_solve)) {
-    atomicAdd(s.sizes + kSzElem, (unsigned long long)n_upd);
-    atomicAdd(s.sizes + kSzSolve, (unsigned long long)n_solve);
+    atomicAdd(acc + kSzElem, (unsigned long long)n_upd);
+    atomicAdd(acc + kSzSolve, (unsigned long long)n_solve);
   }
   if (tid == 0) {
     io.bcon[p] = con_all;
     if (visited) s.bmin[p] = bm;
     // byte accounting: a probe and, when it continued, the visit
     if (io.probe_first || !write_out) {
-      atomicAdd(s.sizes + kSzProbe + 0, (unsigned long long)ne);
-      atomicAdd(s.sizes + kSzProbe + 1, (unsigned long long)nv);
-      atomicAdd(s.sizes + kSzProbe + 2, (unsigned long long)nh);
+      acc[kSzProbe + 0] += (unsigned long long)ne;
+      acc[kSzProbe + 1] += (unsigned long long)nv;
+      acc[kSzProbe + 2] += (unsigned long long)nh;
     }
     if (visited) {
-      atomicAdd(s.sizes + kSzVisit + 0, (unsigned long long)ne);
-      atomicAdd(s.sizes + kSzVisit + 1, (unsigned long long)nv);
-      atomicAdd(s.sizes + kSzVisit + 2, (unsigned long long)nh);
+      acc[kSzVisit + 0] += (unsigned long long)ne;
+      acc[kSzVisit + 1] += (unsigned long long)nv;
+      acc[kSzVisit + 2] += (unsigned long long)nh;
     }
   }
   return {con_all, bm, visited};
@@ -1657,12 +1706,16 @@ __global__ __launch_bounds__(kBlock) void k_seeds(DevMesh<T> m, DevState<T> s, c
 template <typename T, int NC, int BS, int ORG>
 __global__ __launch_bounds__(BS, 4) void k_sweep(DevMesh<T> m, DevState<T> s, const int32_t* list,
                                               int niter, int write_out, T tol, double leps, int epoch_in,
-                                              int epoch_out) {
+                                              int epoch_out, int stage) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ T red[BS / 64];
   __shared__ int s_ndirty[10];
+  __shared__ unsigned long long s_acc[kNumAcc];
+  clear_stats(s_acc);  // sweep_patch starts with a barrier
   sweep_patch<T, NC, BS, ORG>(m, s, list[blockIdx.x], niter, write_out, tol, leps, smem, red, s_ndirty, epoch_in,
-                              epoch_out, 0, default_io(s));
+                              epoch_out, 0, default_io(s), s_acc, stage);
+  __syncthreads();
+  flush_stats(s, s_acc);
 }
 
 // CopyOutBack (redistance_kernels3d.h:772-783) plus the active-list label of
@@ -2138,6 +2191,7 @@ struct StepArgs {
   double warm_abs;
   float res_tol;     // acceptance threshold above which a warm start falls back
   int cfl;           // 1: the advection's dt is bounded by the new narrowband's CFL bound
+  int stage;         // 1: sweeps stage each copy's corner ids and heights in LDS
 };
 
 // Counters are read right after a grid barrier, whose acquire fence already
@@ -2206,6 +2260,7 @@ __global__ __launch_bounds__(BS, WPE) void k_step(DevMesh<T> m, DevState<T> s, S
   __shared__ int s_ndirty[10];
   __shared__ int s_work[2];  // work draws: item, reserved second item
   __shared__ double cred[BS / 64];  // CFL reductions
+  __shared__ unsigned long long s_acc[kNumAcc];  // statistics, flushed at the end
   const int G = gridDim.x, b = blockIdx.x, tid = threadIdx.x;
   const int64_t gtid = (int64_t)b * BS + tid, gsize = (int64_t)G * BS;
   const int nb_out = 1 - a.nb_in;
@@ -2214,6 +2269,7 @@ __global__ __launch_bounds__(BS, WPE) void k_step(DevMesh<T> m, DevState<T> s, S
   PhaseClock pcl{tk};
   pcl.start(clk);
   if (tid == 0) s_work[0] = s_work[1] = -1;  // visible after the first grid barrier
+  clear_stats(s_acc);
 
   // Phases 0-2 run once, or twice when a warm start fails its residual
   // check: the second attempt is the plain method from the same phi over
@@ -2354,14 +2410,15 @@ __global__ __launch_bounds__(BS, WPE) void k_step(DevMesh<T> m, DevState<T> s, S
           p = ovF[sP * P + (i - nEarly)];
           // an unconverged member of E-1 is one of E's early members
           if (members > 0 && vprev[p] == members && !bprev[p]) continue;
-          if (tid == 0) atomicAdd(&s.pc[kPcProbes], 1);
+          if (tid == 0) s_acc[kAccProbes] += 1ull;
           kind = a.niter > 1 ? 1 : 2;
         }
         SweepResult<T> r;
         for (;;) {
           r = sweep_patch<T, NC, BS, ORG>(m, s, p, kind == 2 ? 1 : a.niter, kind == 2 ? 0 : 1, a.tol, a.leps, smem,
                                           red, s_ndirty, 0, 0, members,
-                                          SweepIO<T>{dprev, vprev, dcur, bcur, kind == 1 ? 1 : 0});
+                                          SweepIO<T>{dprev, vprev, dcur, bcur, kind == 1 ? 1 : 0}, s_acc,
+                                          a.stage);
           if (kind == 2 && !r.con) {
             kind = 0;
             continue;
@@ -2383,7 +2440,7 @@ __global__ __launch_bounds__(BS, WPE) void k_step(DevMesh<T> m, DevState<T> s, S
           ovM[sE * P + atomicAdd(&s.pc[kPcOvMl + sE], 1)] = p;
           if (!r.con) ovE[sN * P + atomicAdd(&s.pc[kPcOvEl + sN], 1)] = p;
           atomicAdd(&s.pc[kPcOvVis + sE], 1);
-          atomicAdd(&s.pc[kPcVisits], 1);
+          s_acc[kAccVisits] += 1ull;
         }
       }
       grid_barrier(s.bar, G);
@@ -2432,6 +2489,8 @@ __global__ __launch_bounds__(BS, WPE) void k_step(DevMesh<T> m, DevState<T> s, S
   pcl.lap(clk, kTkAdvect);
   if (clk)
     for (int i = 0; i < kNumTicks; ++i) s.ticks[i] = tk[i];
+  __syncthreads();
+  flush_stats(s, s_acc);
 }
 
 }  // namespace kern
@@ -2514,7 +2573,7 @@ class EngineImpl final : public Engine {
     LS_HIP(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, device_));
     // Exact per-patch LDS needs.
     fim_lds_ = adv_lds_ = seed_lds_ = 0;
-    size_t fim_task_lds = 0;
+    size_t fim_task_lds = 0, fim_stage_lds = 0, fim_task_stage_lds = 0;
     for (int p = 0; p < P_; ++p) {
       const size_t ne = L.ele_off[p + 1] - L.ele_off[p], nv = L.vert_off[p + 1] - L.vert_off[p];
       const size_t nh = L.halo_off[p + 1] - L.halo_off[p];
@@ -2523,6 +2582,9 @@ class EngineImpl final : public Engine {
                           ~size_t(7)) + nv * sizeof(T);  // + running vertex minima
       const size_t fim_tasks = ((fim + 7) & ~size_t(7)) + 4 * (NC * ne + (NC * ne) % 2) + NC * ne * sizeof(T);
       fim_task_lds = std::max(fim_task_lds, fim_tasks);
+      // Staged corner ids and heights: 16 bytes per copy after the vertex minima.
+      fim_stage_lds = std::max(fim_stage_lds, ((fim + 7) & ~size_t(7)) + 16 * ne);
+      fim_task_stage_lds = std::max(fim_task_stage_lds, fim_tasks + 16 * ne);
       // Advection: per-slot products (FP32) or per-element records (FP64);
       // curvature: per-slot terms plus two per-element values.
       const size_t adv = ((sizeof(T) == 8 ? (NC + 2) * ne : 2 * NC * ne) + nv + nh) * sizeof(T);
@@ -2534,6 +2596,8 @@ class EngineImpl final : public Engine {
     // it does not fit.
     fim_elem_lds_ = (fim_lds_ + 15) & ~size_t(15);
     fim_task_lds_ = (fim_task_lds + 15) & ~size_t(15);
+    fim_elem_stage_lds_ = (fim_stage_lds + 15) & ~size_t(15);
+    fim_task_stage_lds_ = (fim_task_stage_lds + 15) & ~size_t(15);
     adv_lds_ = (adv_lds_ + 15) & ~size_t(15);
     seed_lds_ = (seed_lds_ + 15) & ~size_t(15);
     lds_max_ = lds_max;
@@ -2943,8 +3007,9 @@ class EngineImpl final : public Engine {
 
   // LDS of the FIM sweep for the sweep mode, and the persistent grid size.
   int size_kernels(std::string& err) {
+    const int org_mode = sweep_mode_ & ~LS_SWEEP_UNSTAGED;
     const bool fits = fim_task_lds_ <= (size_t)lds_max_;
-    if (sweep_mode_ == LS_SWEEP_TASKS && !fits) {
+    if (org_mode == LS_SWEEP_TASKS && !fits) {
       err = "the task-compacted sweep needs " + std::to_string(fim_task_lds_) + " B of LDS, device allows " +
             std::to_string(lds_max_) + " B";
       return LS_ERR_LIMIT;
@@ -2953,8 +3018,9 @@ class EngineImpl final : public Engine {
     // task-compacted sweep for 256-vertex patches, whose extra listing pass
     // and per-task reloads of the element geometry outweigh the divergence
     // they remove.
-    org_ = sweep_mode_ == LS_SWEEP_TASKS ? kOrgTasks : kOrgElements;
+    org_ = org_mode == LS_SWEEP_TASKS ? kOrgTasks : kOrgElements;
     fim_lds_ = org_ == kOrgTasks ? fim_task_lds_ : fim_elem_lds_;
+    stage_ = 0;
     step_lds_ = std::max(fim_lds_, seed_lds_);
     step_lds_ = (step_lds_ + 15) & ~size_t(15);
     const size_t need = std::max(step_lds_, adv_lds_);
@@ -2982,6 +3048,26 @@ class EngineImpl final : public Engine {
         int n = 0;
         LS_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, step_kernel(org_, warm), step_block(), step_lds_));
         if (n < 1) per_cu = 0;
+      }
+    }
+    // Sweeps stage each copy's corner ids and heights in LDS (16 bytes per
+    // copy) unless that lowers the persistent grid's workgroups per CU: the
+    // need-mask and solve passes of every sweep then read them from LDS, not
+    // again from L2 or HBM.
+    if (!(sweep_mode_ & LS_SWEEP_UNSTAGED) && per_cu > 0) {
+      const size_t fim_s = org_ == kOrgTasks ? fim_task_stage_lds_ : fim_elem_stage_lds_;
+      const size_t step_s = (std::max(fim_s, seed_lds_) + 15) & ~size_t(15);
+      bool ok = step_s <= (size_t)lds_max_;
+      const int want = step_one_per_cu_ ? 1 : per_cu;
+      for (int warm = 0; warm < 2 && ok; ++warm) {
+        int n = 0;
+        LS_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, step_kernel(org_, warm), step_block(), step_s));
+        ok = n >= want;
+      }
+      if (ok) {
+        stage_ = 1;
+        fim_lds_ = fim_s;
+        step_lds_ = step_s;
       }
     }
     coop_grid_ = cus * (step_one_per_cu_ ? std::min(per_cu, 1) : per_cu);
@@ -3024,10 +3110,10 @@ class EngineImpl final : public Engine {
                     int write_out, T tol, double leps, int epoch_in, int epoch_out) {
     if (org_ == kOrgTasks)
       k_sweep<T, NC, kFimBlock, kOrgTasks><<<n, kFimBlock, fim_lds_, q>>>(m, s, list, niter, write_out, tol, leps,
-                                                                         epoch_in, epoch_out);
+                                                                         epoch_in, epoch_out, stage_);
     else
       k_sweep<T, NC, kFimBlock, kOrgElements><<<n, kFimBlock, fim_lds_, q>>>(m, s, list, niter, write_out, tol, leps,
-                                                                            epoch_in, epoch_out);
+                                                                            epoch_in, epoch_out, stage_);
   }
 
   // CFL coefficients per original element (null: the uniform defaults).
@@ -3140,6 +3226,7 @@ class EngineImpl final : public Engine {
     std::memset(&st, 0, sizeof(st));
     st.exec_mode = exec_;
     st.sweep_tasks = org_ == kOrgTasks ? 1 : 0;
+    st.sweep_staged = stage_;
     st.step_wg_per_cu = step_one_per_cu_ ? 1 : coop_grid_ / std::max(1, cus_);
     DevMesh<T> m = mesh();
     DevState<T> s = state();
@@ -3159,6 +3246,7 @@ class EngineImpl final : public Engine {
     a.warm_margin = prm.warm_margin;
     a.warm_abs = prm.warm_abs_margin;
     a.cfl = prm.cfl;
+    a.stage = stage_;
     a.res_tol = (float)(prm.warm_res_tol > 0 ? prm.warm_res_tol : prm.tol);
     if (a.warm == LS_WARM_LEARNED && !a.first && !d_warm_) {
       err = "no learned warm-start values (run ls_mlp_predict)";
@@ -3289,6 +3377,7 @@ class EngineImpl final : public Engine {
     std::memset(&st, 0, sizeof(st));
     st.exec_mode = LS_EXEC_KERNELS;
     st.sweep_tasks = org_ == kOrgTasks ? 1 : 0;
+    st.sweep_staged = stage_;
     st.step_wg_per_cu = 0;
     pc_synced_ = false;
     const DevMesh<T> m = mesh();
@@ -3489,7 +3578,8 @@ class EngineImpl final : public Engine {
   int org_ = kOrgElements;  // FIM sweep organisation in use (kOrg*)
   int adv_grid_ = 1;          // blocks of the narrowband advection after a persistent step
   int sweep_mode_ = LS_SWEEP_AUTO;
-  size_t fim_elem_lds_ = 0, fim_task_lds_ = 0;
+  size_t fim_elem_lds_ = 0, fim_task_lds_ = 0, fim_elem_stage_lds_ = 0, fim_task_stage_lds_ = 0;
+  int stage_ = 0;  // sweeps stage corner ids and heights in LDS (size_kernels)
   int lds_max_ = 0;
   std::vector<int32_t> vert_off_h_, vert_perm_h_;
   int nnb_ = 0, nb_sel_ = 0;

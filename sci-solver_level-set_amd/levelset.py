@@ -19,7 +19,7 @@ LS_MODE_ADVECT, LS_MODE_MCF = 0, 1
 LS_EXEC_PERSISTENT, LS_EXEC_KERNELS, LS_EXEC_GRAPH = 0, 1, 2
 LS_WARM_NONE, LS_WARM_IDENTITY, LS_WARM_LEARNED = 0, 1, 2
 LS_VEL_ELEMENT, LS_VEL_PATCH_FIRST, LS_VEL_PHI_2D = 0, 1, 2
-LS_SWEEP_AUTO, LS_SWEEP_ELEMENTS, LS_SWEEP_TASKS = 0, 1, 2
+LS_SWEEP_AUTO, LS_SWEEP_ELEMENTS, LS_SWEEP_TASKS, LS_SWEEP_UNSTAGED = 0, 1, 2, 4
 LS_LAYOUT_DEVICE, LS_LAYOUT_HOST = 0, 1
 
 
@@ -41,7 +41,7 @@ class Stats(C.Structure):
                 ("probe_copies", C.c_int64), ("probe_verts", C.c_int64), ("probe_halo", C.c_int64),
                 ("adv_copies", C.c_int64), ("adv_verts", C.c_int64), ("adv_halo", C.c_int64),
                 ("seed_copies", C.c_int64), ("seed_verts", C.c_int64), ("seed_halo", C.c_int64),
-                ("ms_step_kernel", C.c_double), ("exec_mode", C.c_int32), ("reserved", C.c_int32),
+                ("ms_step_kernel", C.c_double), ("exec_mode", C.c_int32), ("sweep_staged", C.c_int32),
                 ("ms_ph_seed", C.c_double), ("ms_ph_visit", C.c_double), ("ms_ph_copyback", C.c_double),
                 ("ms_ph_probe", C.c_double), ("ms_ph_next", C.c_double), ("ms_ph_nb", C.c_double),
                 ("ms_ph_advect", C.c_double), ("elem_updates", C.c_int64), ("local_solves", C.c_int64),

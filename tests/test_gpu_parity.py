@@ -264,17 +264,24 @@ def test_unstructured_fp32_vs_fp64_oracle(oracle_mod):
 
 @pytest.mark.parametrize("execution", [lsb.LS_EXEC_PERSISTENT, lsb.LS_EXEC_KERNELS])
 def test_sweep_modes_bit_exact(execution, oracle_mod):
-    # Per-element and task-compacted FIM sweeps give the same bits as the
-    # oracle (tasks only reorganise which thread runs which corner solve).
+    # Per-element and task-compacted FIM sweeps, each with the element
+    # copies' corner ids and heights staged in LDS or read from memory in
+    # every sweep, give the same bits as the oracle (they only reorganise
+    # which thread runs which corner solve and where its inputs come from).
     case = cases.small3d(n=32, shape="torus", block=(8, 8, 4), steps=4)
     runs = {}
-    for mode in (lsb.LS_SWEEP_ELEMENTS, lsb.LS_SWEEP_TASKS):
+    for mode in (lsb.LS_SWEEP_ELEMENTS, lsb.LS_SWEEP_TASKS,
+                 lsb.LS_SWEEP_ELEMENTS | lsb.LS_SWEEP_UNSTAGED, lsb.LS_SWEEP_TASKS | lsb.LS_SWEEP_UNSTAGED):
         g = lsb.solver_for_case(case, execution=execution)
         g.set_sweep(mode)
         out = []
         for _ in range(4):
             st = g.step(lsb.step_params(case.dt, case.bandwidth, case.niter))
-            assert st.sweep_tasks == (1 if mode == lsb.LS_SWEEP_TASKS else 0)
+            assert st.sweep_tasks == (1 if mode & 3 == lsb.LS_SWEEP_TASKS else 0)
+            # staging is on unless turned off, where its LDS fits (the task
+            # organisation's 256-vertex patches leave no room for it)
+            assert st.sweep_staged == (0 if mode & lsb.LS_SWEEP_UNSTAGED else
+                                       1 if mode == lsb.LS_SWEEP_ELEMENTS else st.sweep_staged)
             out.append(g.phi())
         g.close()
         runs[mode] = out

@@ -120,6 +120,10 @@ typedef struct ls_stats {
   double dt_used;           /* the advection's timestep: dt, or min(dt, cfl_dt) with cfl = 1 */
   double cfl_dt;            /* cfl = 1: the new narrowband's CFL bound (0 otherwise) */
   double cfl_max_rin;       /* cfl = 1: the largest inscribed radius over it (0 otherwise) */
+  int32_t fast_len_roots;   /* tets: 1 when every element's least height is at least 2^-350 and
+                               2^-20 of its size, so the local solves' length roots skip sqrt's
+                               small-argument rescaling (same bits); 0 on meshes with flatter elements */
+  int32_t reserved2;
 } ls_stats;
 
 typedef struct ls_layout_info {

@@ -49,6 +49,9 @@ struct DevMesh {
   // Triangles: advect with (phi_A, phi_B) / |grad phi| in each copy's local
   // frame instead of sigma (the reference's 2D advection).
   int sigma_phi;
+  // Tets: every element copy's least height is at least 2^-350 and 2^-20 of
+  // its coordinate scale, so the solve's length roots use sqrt_len.
+  int fast_len;
 };
 
 template <typename T>

@@ -47,6 +47,32 @@ __device__ __forceinline__ double d_max(double a, double b) { return fmax(a, b);
 __device__ __forceinline__ float d_abs(float a) { return fabsf(a); }
 __device__ __forceinline__ double d_abs(double a) { return fabs(a); }
 
+// Square root of a squared length of a mesh whose element heights all
+// exceed 2^-350 (checked when the heights are computed, DevMesh::fast_len):
+// the distances the tet solve takes roots of are at least a height, so the
+// argument is a normal number of at least 2^-700. This is the correctly
+// rounded sequence the compiler emits for sqrt on gfx950 (a reciprocal
+// square-root seed and two Newton-Raphson refinements, Markstein's scheme)
+// without its rescaling of arguments below 2^-767 and its zero/infinity
+// selects, which such arguments never take: the same bits as sqrt, in 10
+// instead of 18 instructions.
+__device__ __forceinline__ double sqrt_len(double x) {
+  const double y = __builtin_amdgcn_rsq(x);
+  double g = x * y, h = y * 0.5;
+  const double r = fma(-h, g, 0.5);
+  g = fma(g, r, g);
+  h = fma(h, r, h);
+  double d = fma(-g, g, x);
+  g = fma(d, h, g);
+  d = fma(-g, g, x);
+  return fma(d, h, g);
+}
+template <bool FAST>
+__device__ __forceinline__ double len_sqrt(double x) {
+  if constexpr (FAST) return sqrt_len(x);
+  else return sqrt(x);
+}
+
 // A discriminant within this fraction of q^2 + |4pr| of zero is a double
 // root: equal values at both ends of an edge, or a front parallel to a face,
 // make it exactly zero, and rounding alone must not decide whether the
@@ -71,7 +97,7 @@ __device__ __forceinline__ bool quad_roots(T p, T q, T r, T& x1, T& x2) {
 
 // Tet Hopf-Lax solve for the fourth corner (scheme of
 // src/core3d/include/redistance_kernels3d.h:124-285).
-template <typename T>
+template <bool FAST, typename T>
 __device__ T tet_solve(T TA, T TB, T TC, T ACAC, T ACBC, T ACCD, T BCBC, T BCCD, T CDCD, T leps) {
   const T big = (T)kLargeNum;
   if (TA >= big && TB >= big && TC >= big) return big;
@@ -90,11 +116,11 @@ __device__ T tet_solve(T TA, T TB, T TC, T ACAC, T ACBC, T ACCD, T BCBC, T BCCD,
         const T l3 = (T)1 - l2;
         if (l2 >= (T)0 && l2 <= (T)1) {
           const T TE = TB * l2 + TC * l3;
-          FaceTBC = d_min(FaceTBC, TE + d_sqrt((l2 * BCBC + BCCD) * l2 + (l2 * BCCD + CDCD)));
+          FaceTBC = d_min(FaceTBC, TE + len_sqrt<FAST>((l2 * BCBC + BCCD) * l2 + (l2 * BCCD + CDCD)));
         }
       }
     }
-    FaceTBC = d_min(FaceTBC, d_min(TB + d_sqrt((BCBC + BCCD) + (BCCD + CDCD)), TC + d_sqrt(CDCD)));
+    FaceTBC = d_min(FaceTBC, d_min(TB + len_sqrt<FAST>((BCBC + BCCD) + (BCCD + CDCD)), TC + len_sqrt<FAST>(CDCD)));
   }
   {
     const T gx = ACAC - ACBC, gy = ACBC - BCBC, gz = ACCD - BCCD;
@@ -110,14 +136,14 @@ __device__ T tet_solve(T TA, T TB, T TC, T ACAC, T ACBC, T ACCD, T BCBC, T BCCD,
         const T l2 = (T)1 - l1;
         if (l1 >= (T)0 && l1 <= (T)1) {
           const T TE = TA * l1 + TB * l2;
-          FaceTAB = d_min(FaceTAB, TE + d_sqrt((l1 * ACAC + l2 * ACBC + ACCD) * l1 +
+          FaceTAB = d_min(FaceTAB, TE + len_sqrt<FAST>((l1 * ACAC + l2 * ACBC + ACCD) * l1 +
                                                (l1 * ACBC + l2 * BCBC + BCCD) * l2 +
                                                (l1 * ACCD + l2 * BCCD + CDCD)));
         }
       }
     }
-    FaceTAB = d_min(FaceTAB, d_min(TB + d_sqrt((BCBC + BCCD) + (BCCD + CDCD)),
-                                   TA + d_sqrt((ACAC + ACCD) + (ACCD + CDCD))));
+    FaceTAB = d_min(FaceTAB, d_min(TB + len_sqrt<FAST>((BCBC + BCCD) + (BCCD + CDCD)),
+                                   TA + len_sqrt<FAST>((ACAC + ACCD) + (ACCD + CDCD))));
   }
   {
     const T p = ACAC * TAC * TAC - ACAC * ACAC;
@@ -130,11 +156,11 @@ __device__ T tet_solve(T TA, T TB, T TC, T ACAC, T ACBC, T ACCD, T BCBC, T BCCD,
         const T l3 = (T)1 - l1;
         if (l1 >= (T)0 && l1 <= (T)1) {
           const T TE = TA * l1 + TC * l3;
-          FaceTAC = d_min(FaceTAC, TE + d_sqrt((l1 * ACAC + ACCD) * l1 + (l1 * ACCD + CDCD)));
+          FaceTAC = d_min(FaceTAC, TE + len_sqrt<FAST>((l1 * ACAC + ACCD) * l1 + (l1 * ACCD + CDCD)));
         }
       }
     }
-    FaceTAC = d_min(FaceTAC, d_min(TA + d_sqrt((ACAC + ACCD) + (ACCD + CDCD)), TC + d_sqrt(CDCD)));
+    FaceTAC = d_min(FaceTAC, d_min(TA + len_sqrt<FAST>((ACAC + ACCD) + (ACCD + CDCD)), TC + len_sqrt<FAST>(CDCD)));
   }
   {
     const T s = TAC * ACBC - TBC * ACAC;
@@ -159,7 +185,7 @@ __device__ T tet_solve(T TA, T TB, T TC, T ACAC, T ACBC, T ACCD, T BCBC, T BCCD,
         const T l3 = (T)1 - l1 - l2;
         if (l1 >= (T)0 && l1 <= (T)1 && l2 >= (T)0 && l2 <= (T)1 && l3 >= (T)0 && l3 <= (T)1) {
           const T TE = TA * l1 + TB * l2 + TC * l3;
-          TD = d_min(TD, TE + d_sqrt((l1 * ACAC + l2 * ACBC + ACCD) * l1 +
+          TD = d_min(TD, TE + len_sqrt<FAST>((l1 * ACAC + l2 * ACBC + ACCD) * l1 +
                                      (l1 * ACBC + l2 * BCBC + BCCD) * l2 +
                                      (l1 * ACCD + l2 * BCCD + CDCD)));
         }
@@ -247,6 +273,7 @@ struct TetDots {
   //   k=0: (TB, TD, TC; BCBC, -BCCD, -ACBC, CDCD,  ACCD, ACAC)
   //   k=1: (TA, TD, TC; ACAC, -ACCD, -ACBC, CDCD,  BCCD, BCBC)
   //   k=2: (TA, TB, TD; ADAD,  ADBD, -ADCD, BDBD, -CDBD, CDCD)
+  template <bool FAST>
   __device__ double solve(int k, const double* Tv, double leps) const {
     const bool k0 = k == 0, k1 = k == 1, k2 = k == 2, k01 = k0 || k1;
     const double a = k0 ? Tv[1] : Tv[0];
@@ -258,7 +285,7 @@ struct TetDots {
     const double d4 = k01 ? CDCD : (k2 ? BDBD : BCBC);
     const double d5 = k0 ? ACCD : (k2 ? -CDBD : BCCD);
     const double d6 = k0 ? ACAC : (k1 ? BCBC : CDCD);
-    return tet_solve(a, b, c, d1, d2, d3, d4, d5, d6, leps);
+    return tet_solve<FAST>(a, b, c, d1, d2, d3, d4, d5, d6, leps);
   }
 };
 
@@ -309,7 +336,10 @@ struct ElemSolve<T, 4> {
     return need_mask(hs, v, scale);
   }
   // Corner k's updated value min(v_k, solve_k) from the element's values v.
-  __device__ static T corner_update(const T* Lt, const T* v, int k, double leps) {
+  // fast: the mesh allows sqrt_len for lengths (DevMesh::fast_len); the
+  // uniform branch picks one of two copies of the solve, same bits.
+  template <bool FAST>
+  __device__ static T corner_update_t(const T* Lt, const T* v, int k, double leps) {
     double L[6];
 #pragma unroll
     for (int j = 0; j < 6; ++j) L[j] = (double)Lt[j];
@@ -317,10 +347,14 @@ struct ElemSolve<T, 4> {
     // Selects instead of Tv[k]: a runtime index would put Tv in scratch memory.
     const double Tk = k == 0 ? Tv[0] : (k == 1 ? Tv[1] : (k == 2 ? Tv[2] : Tv[3]));
     const TetDots d(L);
-    return (T)d_min(Tk, d.solve(k, Tv, leps));
+    return (T)d_min(Tk, d.template solve<FAST>(k, Tv, leps));
+  }
+  __device__ static T corner_update(const T* Lt, const T* v, int k, double leps, bool fast) {
+    return fast ? corner_update_t<true>(Lt, v, k, leps) : corner_update_t<false>(Lt, v, k, leps);
   }
   // Solves the corners of `mask` (from need_mask) into v.
-  __device__ static void solve_mask(const T* Lt, int mask, T* v, double leps) {
+  template <bool FAST>
+  __device__ static void solve_mask_t(const T* Lt, int mask, T* v, double leps) {
     double L[6];
 #pragma unroll
     for (int k = 0; k < 6; ++k) L[k] = (double)Lt[k];
@@ -332,22 +366,26 @@ struct ElemSolve<T, 4> {
     for (int m = mask; m; m &= m - 1) {
       const int k = __ffs(m) - 1;
       const double Tk = k == 0 ? Tv[0] : (k == 1 ? Tv[1] : (k == 2 ? Tv[2] : Tv[3]));
-      const T y = (T)d_min(Tk, d.solve(k, Tv, leps));
+      const T y = (T)d_min(Tk, d.template solve<FAST>(k, Tv, leps));
       if (k == 0) v[0] = y;
       else if (k == 1) v[1] = y;
       else if (k == 2) v[2] = y;
       else v[3] = y;
     }
   }
+  __device__ static void solve_mask(const T* Lt, int mask, T* v, double leps, bool fast) {
+    if (fast) solve_mask_t<true>(Lt, mask, v, leps);
+    else solve_mask_t<false>(Lt, mask, v, leps);
+  }
   // The value the element gives corner k from the other three (no min with
   // corner k's own value, no skip): what update() stores when it solves k.
-  __device__ static T corner_value(const T* Lt, const T* v, int k, double leps) {
+  __device__ static T corner_value(const T* Lt, const T* v, int k, double leps, bool fast) {
     double L[6];
 #pragma unroll
     for (int j = 0; j < 6; ++j) L[j] = (double)Lt[j];
     const double Tv[4] = {(double)v[0], (double)v[1], (double)v[2], (double)v[3]};
     const TetDots d(L);
-    return (T)d_min(kLargeNum, d.solve(k, Tv, leps));
+    return (T)d_min(kLargeNum, fast ? d.template solve<true>(k, Tv, leps) : d.template solve<false>(k, Tv, leps));
   }
 };
 
@@ -372,11 +410,11 @@ struct ElemSolve<T, 3> {
     }
     return mask;
   }
-  __device__ static T corner_update(const T* ct, const T* v, int k, double leps) {
+  __device__ static T corner_update(const T* ct, const T* v, int k, double leps, bool) {
     const double vk = k == 0 ? (double)v[0] : (k == 1 ? (double)v[1] : (double)v[2]);
-    return (T)d_min(vk, (double)corner_value(ct, v, k, leps));
+    return (T)d_min(vk, (double)corner_value(ct, v, k, leps, false));
   }
-  __device__ static void solve_mask(const T* ct, int mask, T* v, double leps) {
+  __device__ static void solve_mask(const T* ct, int mask, T* v, double leps, bool) {
     const double c[3] = {(double)ct[0], (double)ct[1], (double)ct[2]};
     const double LenAB = d_abs(c[0]);
     const double LenBC = d_sqrt(c[2] * c[2] + (c[1] - c[0]) * (c[1] - c[0]));
@@ -386,7 +424,7 @@ struct ElemSolve<T, 3> {
     if (mask & 1) v[0] = (T)d_min(TA, tri_solve(TB, TC, LenBC, LenCA, LenAB));
     if (mask & 2) v[1] = (T)d_min(TB, tri_solve(TC, TA, LenCA, LenAB, LenBC));
   }
-  __device__ static T corner_value(const T* ct, const T* v, int k, double) {
+  __device__ static T corner_value(const T* ct, const T* v, int k, double, bool) {
     const double c[3] = {(double)ct[0], (double)ct[1], (double)ct[2]};
     const double LenAB = d_abs(c[0]);
     const double LenBC = d_sqrt(c[2] * c[2] + (c[1] - c[0]) * (c[1] - c[0]));
@@ -512,8 +550,14 @@ __global__ void k_project_velocity(const int32_t* cv, const double* xyz, const i
 
 // Corner heights of every tet copy, computed once at upload with the same
 // code the sweeps used to evaluate per visit; FP32 storage rounds toward zero.
+// `slow` is set when a copy's least height is below 2^-350 or below 2^-20 of
+// its coordinate scale (the sum of its coordinates' magnitudes, which bounds
+// its edge lengths): the lengths the tet solve takes roots of are at least a
+// height, with rounding errors of a few ulps of the squared scale, so on
+// meshes without such copies their squares stay normal numbers of at least
+// 2^-700 and sqrt_len gives sqrt's bits.
 template <typename T>
-__global__ void k_heights(const T* coords, int64_t F, __half* hgt) {
+__global__ void k_heights(const T* coords, int64_t F, __half* hgt, int32_t* slow) {
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= F) return;
   double L[6], h[4];
@@ -522,6 +566,8 @@ __global__ void k_heights(const T* coords, int64_t F, __half* hgt) {
   ElemSolve<T, 4>::heights(L, h);
 #pragma unroll
   for (int k = 0; k < 4; ++k) hgt[c * 4 + k] = __float2half_rz(__double2float_rz(h[k]));
+  const double hmin = d_min(d_min(h[0], h[1]), d_min(h[2], h[3]));
+  if (!(hmin >= 0x1p-350 && hmin >= 0x1p-20 * ElemSolve<T, 4>::scale(L))) *slow = 1;
 }
 
 // Per patch, the largest coordinate scale of its tet copies, rounded up to
@@ -968,7 +1014,7 @@ __device__ SweepResult<T> sweep_patch(const DevMesh<T>& m, DevState<T>& s, int p
         const T* gc = m.coords + (int64_t)(es + e) * K;
 #pragma unroll
         for (int q = 0; q < K; ++q) L[q] = gc[q];
-        s_new[t] = ElemSolve<T, NC>::corner_update(L, x, j, leps);
+        s_new[t] = ElemSolve<T, NC>::corner_update(L, x, j, leps, m.fast_len != 0);
       }
       __syncthreads();
       for (int t = tid; t < nt; t += BS) {
@@ -1040,7 +1086,7 @@ __device__ SweepResult<T> sweep_patch(const DevMesh<T>& m, DevState<T>& s, int p
         const T* gc = m.coords + (int64_t)(es + e) * K;
 #pragma unroll
         for (int q = 0; q < K; ++q) L[q] = gc[q];
-        ElemSolve<T, NC>::solve_mask(L, s_dirty[e] >> 4, y, leps);
+        ElemSolve<T, NC>::solve_mask(L, s_dirty[e] >> 4, y, leps, m.fast_len != 0);
         int changed = 0;
         const ushort4 cl = stage ? s_cl[e] : m.copy_local[es + e];
 #pragma unroll
@@ -1278,7 +1324,7 @@ __device__ void residual_patch(const DevMesh<T>& m, DevState<T>& s, int p, T bw,
       const T* gc = m.coords + c * K;
 #pragma unroll
       for (int q = 0; q < K; ++q) L[q] = gc[q];
-      f = d_min(f, (double)ElemSolve<T, NC>::corner_value(L, val, sl % NC, leps));
+      f = d_min(f, (double)ElemSolve<T, NC>::corner_value(L, val, sl % NC, leps, m.fast_len != 0));
     }
     r = fmaxf(r, (float)(f - (double)dv));
   }
@@ -2642,8 +2688,19 @@ class EngineImpl final : public Engine {
       LS_HIP(hipGetLastError());
       if (NC == 4 && F_ > 0) {
         LS_HIP(dev_alloc(&d_hgt_, (size_t)F_ * 4));
-        k_heights<T><<<(int)((F_ + 255) / 256), 256, 0, stream_>>>(d_coords_, F_, d_hgt_);
-        LS_HIP(hipGetLastError());
+        int32_t* d_slow = nullptr;
+        LS_HIP(dev_alloc(&d_slow, 1));
+        hipError_t e = hipMemsetAsync(d_slow, 0, sizeof(int32_t), stream_);
+        if (e == hipSuccess) {
+          k_heights<T><<<(int)((F_ + 255) / 256), 256, 0, stream_>>>(d_coords_, F_, d_hgt_, d_slow);
+          e = hipGetLastError();
+        }
+        int32_t slow = 1;
+        if (e == hipSuccess) e = hipMemcpyAsync(&slow, d_slow, sizeof(int32_t), hipMemcpyDeviceToHost, stream_);
+        if (e == hipSuccess) e = hipStreamSynchronize(stream_);
+        (void)hipFree(d_slow);
+        LS_HIP(e);
+        fast_len_ = slow == 0;
         LS_HIP(dev_alloc(&d_pscale_, (size_t)P_));
         k_patch_scale<T><<<P_, 256, 0, stream_>>>(d_coords_, d_ele_off_, d_pscale_);
         LS_HIP(hipGetLastError());
@@ -3227,6 +3284,7 @@ class EngineImpl final : public Engine {
     st.exec_mode = exec_;
     st.sweep_tasks = org_ == kOrgTasks ? 1 : 0;
     st.sweep_staged = stage_;
+    st.fast_len_roots = fast_len_ ? 1 : 0;
     st.step_wg_per_cu = step_one_per_cu_ ? 1 : coop_grid_ / std::max(1, cus_);
     DevMesh<T> m = mesh();
     DevState<T> s = state();
@@ -3378,6 +3436,7 @@ class EngineImpl final : public Engine {
     st.exec_mode = LS_EXEC_KERNELS;
     st.sweep_tasks = org_ == kOrgTasks ? 1 : 0;
     st.sweep_staged = stage_;
+    st.fast_len_roots = fast_len_ ? 1 : 0;
     st.step_wg_per_cu = 0;
     pc_synced_ = false;
     const DevMesh<T> m = mesh();
@@ -3512,6 +3571,7 @@ class EngineImpl final : public Engine {
     m.pgroup = d_pgroup_;
     m.ngroups = ngroups_;
     m.sigma_phi = sigma_phi_ ? 1 : 0;
+    m.fast_len = fast_len_ ? 1 : 0;
     m.ceik0 = kCeikDefault;
     m.ccurv0 = kCcurvDefault;
     return m;
@@ -3600,6 +3660,7 @@ class EngineImpl final : public Engine {
   int32_t* d_copy_verts_ = nullptr;  // F x NC, renumbered corner ids
   int32_t* d_copy_src_ = nullptr;    // F, original element of each copy
   __half* d_hgt_ = nullptr;  // tet copies: corner heights (F x 4), FP16 rounded toward zero
+  bool fast_len_ = false;    // the tet solve's length roots use sqrt_len (k_heights)
   float* d_pscale_ = nullptr;  // tets: per-patch coordinate scale bound (P)
   T* d_warm_ = nullptr;  // learned warm-start distances (N), written by the predictor
   unsigned long long* d_cfl_ = nullptr;  // CFL bound keys (2)

@@ -47,7 +47,8 @@ class Stats(C.Structure):
                 ("ms_ph_advect", C.c_double), ("elem_updates", C.c_int64), ("local_solves", C.c_int64),
                 ("warm_residual", C.c_double), ("warm_patches", C.c_int32), ("warm_fallback", C.c_int32),
                 ("sweep_tasks", C.c_int32), ("step_wg_per_cu", C.c_int32), ("dt_used", C.c_double),
-                ("cfl_dt", C.c_double), ("cfl_max_rin", C.c_double)]
+                ("cfl_dt", C.c_double), ("cfl_max_rin", C.c_double), ("fast_len_roots", C.c_int32),
+                ("reserved2", C.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -387,6 +387,32 @@ def test_near_degenerate_tets_bit_exact(engine, prec, oracle_mod):
             np.testing.assert_array_equal(gphi, ophi, err_msg=f"step {k}")
 
 
+@pytest.mark.parametrize("flat", [False, True])
+def test_length_roots_path_bit_exact(oracle_mod, flat):
+    """The local solves take the roots of squared lengths with sqrt's own
+    sequence minus its small-argument rescaling when every element's least
+    height is at least 2^-350 and 2^-20 of its size; one flat element (height
+    1e-9 of its edges, away from the front) sends the whole mesh through
+    sqrt. Both paths are bit-exact with the FP64 oracle."""
+    case = _custom_case(12, 0.1, lambda x: cases.sdf_sphere(x, r=0.3))
+    if flat:
+        import dataclasses
+        h = case.h
+        a = np.array([0.9, 0.9, 0.9])
+        extra = np.array([a, a + [h, 0, 0], a + [0, h, 0], a + [h / 3, h / 3, 1e-9 * h]])
+        n0 = case.xyz.shape[0]
+        xyz = np.vstack([case.xyz, extra])
+        cells = np.vstack([case.cells, np.arange(n0, n0 + 4, dtype=case.cells.dtype)[None, :]])
+        case = dataclasses.replace(case, xyz=xyz, cells=cells, phi0=cases.sdf_sphere(xyz, r=0.3),
+                                   vel=cases.velocity_rotation_z(xyz, cells),
+                                   labels=np.concatenate([case.labels, np.zeros(4, case.labels.dtype)]))
+    for k, gs, os_, gphi, ophi, gnb, onb in _lockstep(case, 3, oracle_mod, 64, engine=lsb.LS_F64):
+        assert gs.fast_len_roots == (0 if flat else 1)
+        np.testing.assert_array_equal(gnb, onb, err_msg=f"narrowband step {k}")
+        assert _same_schedule(gs, os_), f"schedule step {k}"
+        np.testing.assert_array_equal(gphi, ophi, err_msg=f"step {k}")
+
+
 @pytest.mark.parametrize("which,mode", [("tri", lsb.LS_VEL_PHI_2D), ("tet", lsb.LS_VEL_PATCH_FIRST)])
 @pytest.mark.parametrize("execution", [lsb.LS_EXEC_PERSISTENT, lsb.LS_EXEC_KERNELS])
 def test_velocity_modes_bit_exact_with_oracle(oracle_mod, which, mode, execution):

@@ -626,7 +626,8 @@ def learned_batch(args, lsb, b, cand, ex, prec, plain_value):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed timesteps (default: the configured run, 100 for the torus, 50 for mcf and batch)")
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-steps", type=int, default=8, help="timesteps of the CPU-baseline sample")
     ap.add_argument("--parity-steps", type=int, default=0,
@@ -660,6 +661,8 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device-to-device (a reference kernel for HBM byte counters)")
     args = ap.parse_args()
+    if args.steps is None:
+        args.steps = 100 if args.workload == "torus" else 50
     if args.copy_calibration:
         import torch
         a = torch.empty(1 << 28, dtype=torch.float32, device="cuda").fill_(1.0)

@@ -607,8 +607,32 @@ def learned_batch(args, lsb, b, cand, ex, prec, plain_value):
         r["wall"] = time.perf_counter() - ts
         return r
 
+    def timed_perfect():
+        # warm values from the plain step's own distances on the same input
+        # (a second context, outside the timed region); see learned_run
+        g2 = lsb.solver_for_batch(b, execution=wex, patch_verts=cand, precision=prec)
+        wprm = lsb.step_params(1.0, 1.0, 5, warm=lsb.LS_WARM_LEARNED, warm_margin=0.0,
+                               warm_abs_margin=PERFECT_ABS, warm_res_tol=PERFECT_ABS)
+        g.set_phi(phi0)
+        r = {"verts": 0, "fallbacks": 0, "iters": 0, "visits": 0, "wall": 0.0}
+        for k in range(args.steps):
+            if k > 0:
+                g2.set_phi(g.phi())
+                g2.step(prm)
+                g.set_warm(g2.distances())
+            ts = time.perf_counter()
+            st = g.step(wprm if k > 0 else prm)
+            r["wall"] += time.perf_counter() - ts
+            r["verts"] += st.nb_verts
+            r["fallbacks"] += st.warm_fallback
+            r["iters"] += st.fim_iters
+            r["visits"] += st.patch_visits
+        g2.close()
+        return r
+
     learned = timed(lsb.LS_WARM_LEARNED)
     ident = timed(lsb.LS_WARM_IDENTITY)
+    perfect = timed_perfect()
     g.close()
     value = learned["verts"] / learned["wall"]
     return {"value": value, "vs_plain_fim": value / plain_value, "ms_per_step": 1e3 * learned["wall"] / args.steps,
@@ -619,6 +643,15 @@ def learned_batch(args, lsb, b, cand, ex, prec, plain_value):
                                    "vs_plain_fim": ident["verts"] / ident["wall"] / plain_value,
                                    "fim_iters_per_step": ident["iters"] / args.steps,
                                    "fallback_mesh_steps": ident["fallbacks"]},
+            "perfect_predictor": {"what": "warm values = the plain method's own distances of the same step "
+                                          f"from the same input + {PERFECT_ABS:g}, accepted up to an undercut "
+                                          f"of {PERFECT_ABS:g} (the ceiling of any warm start on this scheme); "
+                                          "only the warm step is timed, per step",
+                                  "value": perfect["verts"] / perfect["wall"],
+                                  "vs_plain_fim": perfect["verts"] / perfect["wall"] / plain_value,
+                                  "fim_iters_per_step": perfect["iters"] / args.steps,
+                                  "patch_visits_per_step": perfect["visits"] / args.steps,
+                                  "fallback_mesh_steps": perfect["fallbacks"]},
             "collect_s": t1 - t0, "train_s": t2 - t1, "train_loss": loss, "warm_margin": args.batch_warm_margin,
             "warm_abs_margin": abs_margin}
 

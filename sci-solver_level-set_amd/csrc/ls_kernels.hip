@@ -47,10 +47,11 @@ __device__ __forceinline__ double d_max(double a, double b) { return fmax(a, b);
 __device__ __forceinline__ float d_abs(float a) { return fabsf(a); }
 __device__ __forceinline__ double d_abs(double a) { return fabs(a); }
 
-// Square root of a squared length of a mesh whose element heights all
-// exceed 2^-350 (checked when the heights are computed, DevMesh::fast_len):
-// the distances the tet solve takes roots of are at least a height, so the
-// argument is a normal number of at least 2^-700. This is the correctly
+// Square root of a squared length of a mesh whose elements' least heights
+// are at least 2^-350 and 2^-20 of their size (checked when the heights are
+// computed, DevMesh::fast_len): the distances the tet solve takes roots of
+// are at least a height, computed with errors far below it, so the argument
+// is a normal number of at least 2^-700. This is the correctly
 // rounded sequence the compiler emits for sqrt on gfx950 (a reciprocal
 // square-root seed and two Newton-Raphson refinements, Markstein's scheme)
 // without its rescaling of arguments below 2^-767 and its zero/infinity

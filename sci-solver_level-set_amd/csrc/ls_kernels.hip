@@ -2741,14 +2741,19 @@ class EngineImpl final : public Engine {
     LS_HIP(dev_alloc(&d_fresh_, P_));
     LS_HIP(dev_alloc(&d_seedflag_, P_));
     LS_HIP(dev_alloc(&d_counters_, kNumCounters));
-    LS_HIP(dev_alloc(&d_sizes_, kNumSizes));
+
     LS_HIP(dev_alloc(&d_inA_, P_));
     LS_HIP(dev_alloc(&d_freshE_, P_));
     LS_HIP(dev_alloc(&d_listF_, P_));
-    LS_HIP(dev_alloc(&d_pc_, kNumPc));
-    LS_HIP(dev_alloc(&d_ticks_, kNumTicks));
+    // Phase ticks, size sums, CFL keys and step counters in one block, read
+    // back after a persistent step with one copy into pinned host memory.
+    LS_HIP(dev_alloc(&d_stat_, kStatWords));
+    d_ticks_ = d_stat_;
+    d_sizes_ = d_ticks_ + kNumTicks;
+    d_cfl_ = d_sizes_ + kNumSizes;
+    d_pc_ = reinterpret_cast<int32_t*>(d_cfl_ + 2);
+    if (!h_stat_) LS_HIP(hipHostMalloc((void**)&h_stat_, kStatWords * sizeof(unsigned long long), hipHostMallocDefault));
     LS_HIP(dev_alloc(&d_bar_, 2));
-    LS_HIP(dev_alloc(&d_cfl_, 2));
     LS_HIP(dev_alloc(&d_gfail_, 1));
     ngroups_ = 1;
     LS_HIP(hipMemset(d_bar_, 0, 2 * sizeof(unsigned int)));
@@ -3355,14 +3360,14 @@ class EngineImpl final : public Engine {
       LS_HIP(launch_nb_advect(prm, q));
     }
     LS_HIP(hipEventRecord(ev_[4], q));
-    int32_t pc[kNumPc];
-    unsigned long long sz[kNumSizes], tk[kNumTicks];
-    LS_HIP(hipMemcpyAsync(tk, d_ticks_, sizeof(tk), hipMemcpyDeviceToHost, q));
-    LS_HIP(hipMemcpyAsync(pc, d_pc_, sizeof(pc), hipMemcpyDeviceToHost, q));
-    LS_HIP(hipMemcpyAsync(sz, d_sizes_, sizeof(sz), hipMemcpyDeviceToHost, q));
-    unsigned long long ck[2] = {0, 0};
-    if (prm.cfl) LS_HIP(hipMemcpyAsync(ck, d_cfl_, sizeof(ck), hipMemcpyDeviceToHost, q));
+    LS_HIP(hipMemcpyAsync(h_stat_, d_stat_, kStatWords * sizeof(unsigned long long), hipMemcpyDeviceToHost, q));
     LS_HIP(hipStreamSynchronize(q));
+    const unsigned long long* tk = h_stat_;
+    const unsigned long long* sz = tk + kNumTicks;
+    unsigned long long ck[2] = {0, 0};
+    if (prm.cfl) { ck[0] = sz[kNumSizes]; ck[1] = sz[kNumSizes + 1]; }
+    int32_t pc[kNumPc];
+    std::memcpy(pc, sz + kNumSizes + 2, sizeof(pc));
     if (pc[kPcOverflow] && prm.max_fim_iters <= 0) {
       err = "FIM did not converge within " + std::to_string(kFimIterCap) + " iterations";
       return LS_ERR_INTERNAL;
@@ -3612,17 +3617,18 @@ class EngineImpl final : public Engine {
     void* ptrs[] = {d_vert_off_, d_ele_off_, d_halo_off_, d_halo_ids_, d_copy_local_, d_vslot_off_,
                     d_vslots_, d_padj_off_, d_padj_, d_coords_, d_sigma_, d_phi_, d_phi_out_, d_DT_,
                     d_DT_out_, d_label_, d_tsign_, d_bcon_, d_bmin_, d_act_, d_fresh_, d_seedflag_,
-                    d_counters_, d_sizes_, d_nb_, d_scan_all_, d_lists_[0], d_lists_[1], d_lists_[2],
-                    d_lists_[3], d_inA_, d_freshE_, d_listF_, d_pc_, d_ticks_, d_bar_,
+                    d_counters_, d_nb_, d_scan_all_, d_lists_[0], d_lists_[1], d_lists_[2],
+                    d_lists_[3], d_inA_, d_freshE_, d_listF_, d_stat_, d_bar_,
                     d_patch_dt_, d_patch_bw_, d_warm_, d_hgt_, d_mlp_prm_, d_mlp_m_, d_mlp_v_, d_mlp_grad_,
                     d_mlp_b_, d_mlp_w1_, d_mlp_w2_, d_mlp_w3_, d_ds_x_, d_ds_y_, d_pend_, d_xyz_,
                     d_copy_verts_, d_copy_src_, d_rs_slot_, d_rs_vval_, d_rs_dirty_, d_rs_epoch_, d_visitE_,
-                    d_vpatch_, d_ve_ok_, d_ve_tag_, d_cfl_, d_gfail_, d_pgroup_, d_ceik_, d_ccurv_, d_DT_out2_,
+                    d_vpatch_, d_ve_ok_, d_ve_tag_, d_gfail_, d_pgroup_, d_ceik_, d_ccurv_, d_DT_out2_,
                     d_visitE2_,
                     d_bcon2_, d_ov_lists_, d_pscale_};
     if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
     if (graph_) (void)hipGraphDestroy(graph_);
     for (void* p : ptrs) if (p) (void)hipFree(p);
+    if (h_stat_) (void)hipHostFree(h_stat_);
     for (auto& e : ev_) if (e) (void)hipEventDestroy(e);
     for (auto& e : kev_) if (e) (void)hipEventDestroy(e);
     if (stream_) (void)hipStreamDestroy(stream_);
@@ -3690,7 +3696,12 @@ class EngineImpl final : public Engine {
   T *d_phi_ = nullptr, *d_phi_out_ = nullptr, *d_DT_ = nullptr, *d_DT_out_ = nullptr, *d_bmin_ = nullptr;
   int8_t *d_label_ = nullptr, *d_tsign_ = nullptr;
   int32_t *d_bcon_ = nullptr, *d_act_ = nullptr, *d_fresh_ = nullptr, *d_seedflag_ = nullptr, *d_counters_ = nullptr;
-  unsigned long long* d_sizes_ = nullptr;
+  unsigned long long* d_sizes_ = nullptr;  // in d_stat_
+  // d_ticks_, d_sizes_, d_cfl_ and d_pc_ live in one block, d_stat_, so a
+  // persistent step reads them back with one copy into h_stat_ (pinned).
+  static constexpr int kStatWords = kNumTicks + kNumSizes + 2 + (kNumPc + 1) / 2;
+  unsigned long long* d_stat_ = nullptr;
+  unsigned long long* h_stat_ = nullptr;
   std::vector<hipEvent_t> kev_;  // kernel-timing events, pairs (start, end)
   std::vector<int> kev_kind_;    // 0 sweep, 1 advect, per pair
   size_t kev_used_ = 0;

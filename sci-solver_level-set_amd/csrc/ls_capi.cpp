@@ -178,7 +178,9 @@ int ls_set_mesh(ls_ctx* c, int dim, int64_t n_verts, const double* xyz, int64_t 
 int ls_set_mesh_batch(ls_ctx* c, int dim, int n_meshes, const int64_t* vert_off, const double* xyz,
                       const int64_t* cell_off, const int32_t* cells) {
   if (!c) return LS_ERR_ARG;
+  if (dim != 2 && dim != 3) return fail(c, LS_ERR_ARG, "dim must be 2 or 3");
   if (n_meshes <= 0 || !vert_off || !cell_off) return fail(c, LS_ERR_ARG, "n_meshes > 0 and offsets required");
+  if (!xyz || !cells) return fail(c, LS_ERR_ARG, "empty mesh");
   if (vert_off[0] != 0 || cell_off[0] != 0) return fail(c, LS_ERR_ARG, "offsets must start at 0");
   for (int m = 0; m < n_meshes; ++m)
     if (vert_off[m + 1] <= vert_off[m] || cell_off[m + 1] <= cell_off[m])
@@ -186,7 +188,7 @@ int ls_set_mesh_batch(ls_ctx* c, int dim, int n_meshes, const int64_t* vert_off,
   const int nc = dim + 1;
   for (int m = 0; m < n_meshes; ++m)
     for (int64_t t = cell_off[m] * nc; t < cell_off[m + 1] * nc; ++t)
-      if (cells && (cells[t] < vert_off[m] || cells[t] >= vert_off[m + 1]))
+      if (cells[t] < vert_off[m] || cells[t] >= vert_off[m + 1])
         return fail(c, LS_ERR_ARG, "a cell of mesh " + std::to_string(m) + " names a vertex of another mesh");
   const int rc = ls_set_mesh(c, dim, vert_off[n_meshes], xyz, cell_off[n_meshes], cells);
   if (rc != LS_OK) return rc;

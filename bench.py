@@ -417,6 +417,33 @@ def autotune_exec(lsb, case, names, steps=6, g=None, prm=None, precision=None, p
 
 WARM_MARGIN, WARM_ABS_H = 0.03, 0.25  # warm value = prediction * 1.03 + h / 4
 PERFECT_ABS = 1e-5  # the best-case predictor: the plain step's own distances + 1e-5
+BF16_MFMA_PEAK_TFLOPS = 2500.0  # MI355X dense BF16 matrix peak
+MLP_FWD_FLOPS = 2 * (22 * 64 + 64 * 64 + 64 * 4)  # per element copy (11,520)
+
+
+def mlp_throughput(g, n_train, train_s, epochs):
+    """The network's inference over the current narrowband: ls_mlp_predict
+    (one launch, synchronised) timed over repeated calls, its algorithmic
+    FLOPs (22-64-64-4 per element copy) against the BF16 MFMA peak; and the
+    training rate end to end (forward + backward, about 3x the forward
+    FLOPs per pair, host round trips per batch included)."""
+    nb = g.narrowband()
+    eo = g.layout()["ele_off"]
+    copies = int(np.sum(eo[nb + 1] - eo[nb]))
+    g.mlp_predict()
+    reps = 20
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        g.mlp_predict()
+    ms = 1e3 * (time.perf_counter() - t0) / reps
+    tf = copies * MLP_FWD_FLOPS / (ms * 1e-3) / 1e12
+    train_tf = epochs * n_train * 3 * MLP_FWD_FLOPS / train_s / 1e12 if train_s > 0 else None
+    return {"predict": {"element_copies": copies, "ms": ms, "tflops": tf, "peak_tflops": BF16_MFMA_PEAK_TFLOPS,
+                        "frac_of_bf16_mfma_peak": tf / BF16_MFMA_PEAK_TFLOPS,
+                        "note": "host-timed launch + synchronisation; a small dense MLP per copy whose inputs "
+                                "are gathered from the mesh, so memory and launch latency, not the MFMA rate, "
+                                "bound it"},
+            "train": {"pairs_x_epochs": epochs * n_train, "s": train_s, "tflops_end_to_end": train_tf}}
 
 
 def learned_run(args, lsb, case, exec_mode, plain_value, precision, prm_kw, oracle_mod=None):
@@ -454,6 +481,7 @@ def learned_run(args, lsb, case, exec_mode, plain_value, precision, prm_kw, orac
         g.step(prm)
         g.mlp_collect(1)
     X, Y = g.mlp_dataset()
+    mlp = mlp_throughput(g, n_train, t2 - t1, args.train_epochs)
     held = slice(n_train, X.shape[0])
     _, sse = g.mlp_gradient(X[held], Y[held])
     trained_mse = sse / max(Y[held].size, 1)
@@ -561,7 +589,7 @@ def learned_run(args, lsb, case, exec_mode, plain_value, precision, prm_kw, orac
             "train_pairs": int(n_train), "held_out_pairs": int(X.shape[0] - n_train),
             "collect_s": t1 - t0, "train_s": t2 - t1, "train_epochs": args.train_epochs,
             "train_loss": loss, "held_out_mse": trained_mse, "held_out_identity_mse": identity_mse,
-            "warm_margin": WARM_MARGIN, "warm_abs_margin_h": WARM_ABS_H, "parity": parity}
+            "warm_margin": WARM_MARGIN, "warm_abs_margin_h": WARM_ABS_H, "mlp": mlp, "parity": parity}
 
 
 def learned_batch(args, lsb, b, cand, ex, prec, plain_value):

@@ -593,11 +593,48 @@ __global__ void k_patch_scale(const T* coords, const int32_t* ele_off, float* ps
   if (threadIdx.x == 0) pscale[p] = __double2float_ru(red[0]);
 }
 
+// Wave reductions by xor butterflies on ds_swizzle (bit mode: lane ^ M
+// within each half of the wave, the pattern an immediate) and a final
+// readlane of both halves: unlike ds_bpermute they keep no per-lane address
+// registers live, which the two-workgroups-per-CU builds would spill. The
+// results are exact (min, integer sums), so the order does not matter.
+template <int M>
+__device__ __forceinline__ int swz_xor(int v) {
+  return __builtin_amdgcn_ds_swizzle(v, (M << 10) | 0x1F);
+}
+template <int M>
+__device__ __forceinline__ double swz_xor(double v) {
+  const int lo = swz_xor<M>(__double2loint(v)), hi = swz_xor<M>(__double2hiint(v));
+  return __hiloint2double(hi, lo);
+}
+template <int M>
+__device__ __forceinline__ float swz_xor(float v) {
+  return __int_as_float(swz_xor<M>(__float_as_int(v)));
+}
+__device__ __forceinline__ double lane_value(double v, int l) {
+  return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), l),
+                          __builtin_amdgcn_readlane(__double2loint(v), l));
+}
+__device__ __forceinline__ float lane_value(float v, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+// Every lane of the wave must be active; the result is uniform.
 template <typename T>
 __device__ __forceinline__ T wave_min(T v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v = d_min(v, __shfl_xor(v, off, 64));
-  return v;
+  v = d_min(v, swz_xor<1>(v));
+  v = d_min(v, swz_xor<2>(v));
+  v = d_min(v, swz_xor<4>(v));
+  v = d_min(v, swz_xor<8>(v));
+  v = d_min(v, swz_xor<16>(v));
+  return d_min(lane_value(v, 0), lane_value(v, 32));
+}
+__device__ __forceinline__ int wave_sum(int v) {
+  v += swz_xor<1>(v);
+  v += swz_xor<2>(v);
+  v += swz_xor<4>(v);
+  v += swz_xor<8>(v);
+  v += swz_xor<16>(v);
+  return __builtin_amdgcn_readlane(v, 0) + __builtin_amdgcn_readlane(v, 32);
 }
 
 // Block-wide min; every thread must call it. Returns the result in all threads.
@@ -967,7 +1004,7 @@ __device__ SweepResult<T> sweep_patch(const DevMesh<T>& m, DevState<T>& s, int p
         const int lane = tid & 63;
         int off = 0;
         if (lane == 0 && mask) off = atomicAdd(s_ndirty, __popcll(mask));
-        off = __shfl(off, 0, 64);
+        off = __builtin_amdgcn_readlane(off, 0);
         if (d) s_work[off + __popcll(mask & ((1ull << lane) - 1ull))] = (uint16_t)e;
       }
       if (tid == 0) s_ndirty[1] = 0;
@@ -995,10 +1032,10 @@ __device__ SweepResult<T> sweep_patch(const DevMesh<T>& m, DevState<T>& s, int p
           const int t = __shfl_up(incl, off, 64);
           if (lane >= off) incl += t;
         }
-        const int total = __shfl(incl, 63, 64);
+        const int total = __builtin_amdgcn_readlane(incl, 63);
         int wbase = 0;
         if (lane == 0 && total) wbase = atomicAdd(&s_ndirty[1], total);
-        wbase = __shfl(wbase, 0, 64);
+        wbase = __builtin_amdgcn_readlane(wbase, 0);
         int o = wbase + incl - c;
 #pragma unroll
         for (int j = 0; j < NC; ++j)
@@ -1074,7 +1111,7 @@ __device__ SweepResult<T> sweep_patch(const DevMesh<T>& m, DevState<T>& s, int p
           if (!b) continue;
           int off = 0;
           if (lane == 0) off = atomicAdd(&s_ndirty[5 + j], __popcll(b));
-          off = __shfl(off, 0, 64);
+          off = __builtin_amdgcn_readlane(off, 0);
           if (c == j) s_work[start[j] + off + __popcll(b & ((1ull << lane) - 1ull))] = (uint16_t)e;
         }
       }
@@ -1177,11 +1214,8 @@ __device__ SweepResult<T> sweep_patch(const DevMesh<T>& m, DevState<T>& s, int p
     if (tid == 0) s.rs_epoch[p] = epoch_out;
   }
   const T bm = block_min_n<T, BS>(vmin, red);
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    n_upd += __shfl_xor(n_upd, off, 64);
-    n_solve += __shfl_xor(n_solve, off, 64);
-  }
+  n_upd = wave_sum(n_upd);
+  n_solve = wave_sum(n_solve);
   if ((tid & 63) == 0 && (n_upd | n_solve)) {
     atomicAdd(acc + kSzElem, (unsigned long long)n_upd);
     atomicAdd(acc + kSzSolve, (unsigned long long)n_solve);

@@ -896,12 +896,14 @@ constexpr int kOrgElements = 0, kOrgTasks = 1;
 // acc: the workgroup's statistics in LDS (kNumSizes patch-size sums, then
 // visits and probes), added to the device counters once per workgroup by
 // flush_stats, so visits contend on no global counter for them.
-template <typename T, int NC, int BS, int ORG>
+// STG fixes the staging choice at compile time (0 or 1); -1 takes `stage_arg`.
+template <typename T, int NC, int BS, int ORG, int STG = -1>
 __device__ SweepResult<T> sweep_patch(const DevMesh<T>& m, DevState<T>& s, int p, int niter, int write_out,
                                       T tol, double leps, unsigned char* smem, T* red, int* s_ndirty /*[10]*/,
                                       int epoch_in, int epoch_out, int members, const SweepIO<T> io,
-                                      unsigned long long* acc, int stage) {
+                                      unsigned long long* acc, int stage_arg) {
   constexpr bool tasks = ORG == kOrgTasks;
+  const int stage = STG < 0 ? stage_arg : STG;
   constexpr int K = ElemSolve<T, NC>::kCoords;
   __syncthreads();
   const int vs = m.vert_off[p], ve = m.vert_off[p + 1];
@@ -2334,7 +2336,9 @@ struct PhaseClock {
 // demand within the 128 VGPRs that let two 512-thread workgroups share a CU;
 // 3 (one 768-thread workgroup per CU) leaves 168, so nothing is spilled to
 // scratch, whose dirty lines every grid barrier would otherwise write back.
-template <typename T, int NC, int BS, int ORG, bool WARM, int WPE>
+// STG >= 0 compiles one staging choice only (a.stage must equal it), which
+// frees the registers the other path's addresses held.
+template <typename T, int NC, int BS, int ORG, bool WARM, int WPE, int STG = -1>
 __global__ __launch_bounds__(BS, WPE) void k_step(DevMesh<T> m, DevState<T> s, StepArgs<T> a) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ T red[BS / 64];
@@ -2496,10 +2500,10 @@ __global__ __launch_bounds__(BS, WPE) void k_step(DevMesh<T> m, DevState<T> s, S
         }
         SweepResult<T> r;
         for (;;) {
-          r = sweep_patch<T, NC, BS, ORG>(m, s, p, kind == 2 ? 1 : a.niter, kind == 2 ? 0 : 1, a.tol, a.leps, smem,
-                                          red, s_ndirty, 0, 0, members,
-                                          SweepIO<T>{dprev, vprev, dcur, bcur, kind == 1 ? 1 : 0}, s_acc,
-                                          a.stage);
+          r = sweep_patch<T, NC, BS, ORG, STG>(m, s, p, kind == 2 ? 1 : a.niter, kind == 2 ? 0 : 1, a.tol, a.leps,
+                                               smem, red, s_ndirty, 0, 0, members,
+                                               SweepIO<T>{dprev, vprev, dcur, bcur, kind == 1 ? 1 : 0},
+                                               s_acc, a.stage);
           if (kind == 2 && !r.con) {
             kind = 0;
             continue;
@@ -3132,7 +3136,7 @@ class EngineImpl final : public Engine {
     step_one_per_cu_ = false;
     for (int warm = 0; warm < 2; ++warm) {
       int n = 0;
-      LS_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, step_kernel(org_, warm), kFimBlock, step_lds_));
+      LS_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, step_kernel(org_, warm, 0), kFimBlock, step_lds_));
       per_cu = std::min(per_cu, n);
     }
     // Two workgroups per CU only pay when patches outnumber CUs several times
@@ -3143,7 +3147,7 @@ class EngineImpl final : public Engine {
     if (step_one_per_cu_) {
       for (int warm = 0; warm < 2; ++warm) {
         int n = 0;
-        LS_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, step_kernel(org_, warm), step_block(), step_lds_));
+        LS_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, step_kernel(org_, warm, 0), step_block(), step_lds_));
         if (n < 1) per_cu = 0;
       }
     }
@@ -3158,7 +3162,7 @@ class EngineImpl final : public Engine {
       const int want = step_one_per_cu_ ? 1 : per_cu;
       for (int warm = 0; warm < 2 && ok; ++warm) {
         int n = 0;
-        LS_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, step_kernel(org_, warm), step_block(), step_s));
+        LS_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, step_kernel(org_, warm, 1), step_block(), step_s));
         ok = n >= want;
       }
       if (ok) {
@@ -3175,20 +3179,25 @@ class EngineImpl final : public Engine {
     if (graph_) { LS_HIP(hipGraphDestroy(graph_)); graph_ = nullptr; }
     return LS_OK;
   }
-  // The persistent kernel for the sweep organisation, warm-start support and
-  // workgroups per CU chosen by size_kernels (the tasks organisation has
-  // only the two-per-CU build).
   // Threads per workgroup of the persistent kernel step_kernel() returns.
   int step_block() const { return (step_one_per_cu_ && org_ != kOrgTasks) ? kStepBlock1 : kFimBlock; }
-  const void* step_kernel(int org, int warm) const {
+  // The persistent kernel for the sweep organisation, warm-start support,
+  // staging and workgroups per CU chosen by size_kernels (the tasks
+  // organisation has only the two-per-CU build). The two-per-CU elements
+  // build is compiled once per staging choice: at 128 VGPRs, the registers
+  // the unused path holds would be spilled.
+  const void* step_kernel(int org, int warm, int stage) const {
     if (org == kOrgTasks)
       return warm ? (const void*)k_step<T, NC, kFimBlock, kOrgTasks, true, 4>
                   : (const void*)k_step<T, NC, kFimBlock, kOrgTasks, false, 4>;
     if (step_one_per_cu_)
       return warm ? (const void*)k_step<T, NC, kStepBlock1, kOrgElements, true, 3>
                   : (const void*)k_step<T, NC, kStepBlock1, kOrgElements, false, 3>;
-    return warm ? (const void*)k_step<T, NC, kFimBlock, kOrgElements, true, 4>
-                : (const void*)k_step<T, NC, kFimBlock, kOrgElements, false, 4>;
+    if (stage)
+      return warm ? (const void*)k_step<T, NC, kFimBlock, kOrgElements, true, 4, 1>
+                  : (const void*)k_step<T, NC, kFimBlock, kOrgElements, false, 4, 1>;
+    return warm ? (const void*)k_step<T, NC, kFimBlock, kOrgElements, true, 4, 0>
+                : (const void*)k_step<T, NC, kFimBlock, kOrgElements, false, 4, 0>;
   }
   int set_sweep_mode(int mode, std::string& err) override {
     const int old = sweep_mode_;
@@ -3370,7 +3379,7 @@ class EngineImpl final : public Engine {
         if (graph_exec_) { LS_HIP(hipGraphExecDestroy(graph_exec_)); graph_exec_ = nullptr; }
         if (graph_) { LS_HIP(hipGraphDestroy(graph_)); graph_ = nullptr; }
         LS_HIP(hipStreamBeginCapture(q, hipStreamCaptureModeThreadLocal));
-        hipError_t le = hipLaunchCooperativeKernel(step_kernel(org_, a.warm != 0), dim3(coop_grid_),
+        hipError_t le = hipLaunchCooperativeKernel(step_kernel(org_, a.warm != 0, stage_), dim3(coop_grid_),
                                                    dim3(step_block()), args, step_lds_, q);
         if (le == hipSuccess) le = launch_nb_advect(prm, q);
         hipError_t ce = hipStreamEndCapture(q, &graph_);
@@ -3387,7 +3396,7 @@ class EngineImpl final : public Engine {
       if (coop_refused(ge)) return step_kernels(prm, st, err);
       LS_HIP(ge);
     } else {
-      const hipError_t le = hipLaunchCooperativeKernel(step_kernel(org_, a.warm != 0), dim3(coop_grid_),
+      const hipError_t le = hipLaunchCooperativeKernel(step_kernel(org_, a.warm != 0, stage_), dim3(coop_grid_),
                                                        dim3(step_block()), args, step_lds_, q);
       if (coop_refused(le)) return step_kernels(prm, st, err);
       LS_HIP(le);

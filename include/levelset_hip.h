@@ -40,8 +40,13 @@ enum { LS_EXEC_PERSISTENT = 0, LS_EXEC_KERNELS = 1, LS_EXEC_GRAPH = 2 };
  * visit stages each element copy's corner ids and heights in LDS when that
  * costs the persistent kernel no workgroups per CU; OR-ing
  * LS_SWEEP_UNSTAGED into the mode makes every sweep read them from memory
- * (same results). */
-enum { LS_SWEEP_AUTO = 0, LS_SWEEP_ELEMENTS = 1, LS_SWEEP_TASKS = 2, LS_SWEEP_UNSTAGED = 4 };
+ * (same results). Tet corners whose solve cannot lower their value are
+ * skipped by an exact lower bound: the height bound, and by default also a
+ * supporting-plane bound that removes about a third of the remaining solves
+ * on the torus; LS_SWEEP_NO_PLANE_BOUND keeps the height bound only (same
+ * results, more solves). */
+enum { LS_SWEEP_AUTO = 0, LS_SWEEP_ELEMENTS = 1, LS_SWEEP_TASKS = 2, LS_SWEEP_UNSTAGED = 4,
+       LS_SWEEP_NO_PLANE_BOUND = 8 };
 /* Where the patch layout is built: on the context's GPU (default; radix
  * sorts, identical arrays) or on the host. Host-only contexts always build
  * on the host. */
@@ -123,7 +128,9 @@ typedef struct ls_stats {
   int32_t fast_len_roots;   /* tets: 1 when every element's least height is at least 2^-350 and
                                2^-20 of its size, so the local solves' length roots skip sqrt's
                                small-argument rescaling (same bits); 0 on meshes with flatter elements */
-  int32_t reserved2;
+  int32_t plane_bound;      /* tets: 1 when the sweeps skipped solves by the supporting-plane bound
+                               (default; LS_SWEEP_NO_PLANE_BOUND, or element sizes outside
+                               [2^-48, 2^48] where its FP32 evaluation is not used, give 0) */
 } ls_stats;
 
 typedef struct ls_layout_info {

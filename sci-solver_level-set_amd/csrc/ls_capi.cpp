@@ -268,10 +268,11 @@ int ls_set_layout_builder(ls_ctx* c, int builder) {
 
 int ls_set_sweep(ls_ctx* c, int mode) {
   if (!c) return LS_ERR_ARG;
-  const int org = mode & ~LS_SWEEP_UNSTAGED;
+  const int org = mode & ~(LS_SWEEP_UNSTAGED | LS_SWEEP_NO_PLANE_BOUND);
   if (mode < 0 || (org != LS_SWEEP_AUTO && org != LS_SWEEP_ELEMENTS && org != LS_SWEEP_TASKS))
     return fail(c, LS_ERR_ARG,
-                "mode must be LS_SWEEP_AUTO, LS_SWEEP_ELEMENTS or LS_SWEEP_TASKS, optionally with LS_SWEEP_UNSTAGED");
+                "mode must be LS_SWEEP_AUTO, LS_SWEEP_ELEMENTS or LS_SWEEP_TASKS, optionally with "
+                "LS_SWEEP_UNSTAGED and LS_SWEEP_NO_PLANE_BOUND");
   c->sweep = mode;
   if (!c->engine) return LS_OK;
   std::string err;

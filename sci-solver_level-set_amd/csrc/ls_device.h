@@ -52,6 +52,9 @@ struct DevMesh {
   // Tets: every element copy's least height is at least 2^-350 and 2^-20 of
   // its coordinate scale, so the solve's length roots use sqrt_len.
   int fast_len;
+  // Tets: the need-mask pass also applies the supporting-plane bound
+  // (ElemSolve<T, 4>::plane_mask; element sizes allow its FP32 evaluation).
+  int plane;
 };
 
 template <typename T>
@@ -150,12 +153,13 @@ enum PcIndex : int {
   kPcWorkS = 19,   // next patch to scan for seeds
   kPcWorkA = 20,   // next narrowband patch to advect (per pass)
   // Overlapped FIM iterations, each a triple indexed by iteration mod 3:
-  kPcOvF = 24,     // probe marks made in the iteration
+  kPcOvF = 24,     // probe marks made in the iteration (patches visited earlier in the step)
   kPcOvEl = 27,    // early members of the iteration (unconverged members of the one before)
   kPcOvMl = 30,    // members of the iteration, copied back in the next one
   kPcOvW = 33,     // next work item of the iteration's phase (early visits, then probes)
   kPcOvVis = 36,   // visits of the iteration
-  kNumPc = 40
+  kPcOvFn = 39,    // probe marks made in the iteration (patches not yet visited in the step)
+  kNumPc = 42
 };
 
 // Patch-size sums (element copies, owned, halo) per launch kind.

@@ -328,6 +328,73 @@ struct ElemSolve<T, 4> {
     }
     return mask;
   }
+  // Supporting-plane bound, applied to the corners `mask` leaves. Corner
+  // k's solve is the least of f(E) = T(E) + |D - E| over points E of the
+  // opposite face (T linear on it, D the corner): every candidate the solve
+  // evaluates is f at a point of the face, the roots' rounding only moving
+  // that point. f is convex, so for any face point E0 and every E,
+  // f(E) >= f(E0) + grad f(E0).(E - E0), which is linear in E; over the
+  // face it is least at a face corner c, where it equals T_c + (c - D).w,
+  // w the unit vector from D towards E0. With E0 the foot of the
+  // perpendicular this is the height bound; here E0 runs over the three
+  // face corners and the centroid, and the bound is the largest of the
+  // four. The products (c - D).(c' - D) come from squared edge lengths (law
+  // of cosines), evaluated in FP32: a margin of 1e-5 (max other value +
+  // scale + 1) lies far above their rounding (below 1e-6 of the same, as
+  // every edge is at most `scale` long) and the FP64 rounding of the
+  // candidates, so a corner is dropped only when its solve cannot go below
+  // its value and the result is bit-identical to solving it. DevMesh::plane
+  // is set when element sizes keep the squared lengths normal FP32 numbers.
+  // On the 64^3 torus it removes about 31 % of the solves the height bound
+  // leaves.
+  __device__ static float plane_bound(float T0, float T1, float T2, float qk0, float qk1, float qk2, float q01,
+                                      float q02, float q12, float n0, float n1, float n2, float r0, float r1,
+                                      float r2) {
+    const float G01 = 0.5f * (qk0 + qk1 - q01), G02 = 0.5f * (qk0 + qk2 - q02), G12 = 0.5f * (qk1 + qk2 - q12);
+    const float b0 = fminf(T0 + n0, fminf(fmaf(G01, r0, T1), fmaf(G02, r0, T2)));
+    const float b1 = fminf(T1 + n1, fminf(fmaf(G01, r1, T0), fmaf(G12, r1, T2)));
+    const float b2 = fminf(T2 + n2, fminf(fmaf(G02, r2, T0), fmaf(G12, r2, T1)));
+    const float S0 = qk0 + G01 + G02, S1 = qk1 + G01 + G12, S2 = qk2 + G02 + G12;
+    const float rg = __builtin_amdgcn_rsqf(S0 + S1 + S2);
+    const float bg = fminf(fmaf(S0, rg, T0), fminf(fmaf(S1, rg, T1), fmaf(S2, rg, T2)));
+    return fmaxf(fmaxf(b0, b1), fmaxf(b2, bg));
+  }
+  __device__ static int plane_mask(const T* Lt, const T* v, int mask, float scale) {
+    float L[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) L[k] = (float)Lt[k];
+    // Squared edge lengths in the local frame A=(0,0,0), B=(L0,0,0),
+    // C=(L1,L2,0), D=(L3,L4,L5), their roots and reciprocal roots.
+    const float d10 = L[1] - L[0], d30 = L[3] - L[0], d31 = L[3] - L[1], d42 = L[4] - L[2];
+    const float qAB = L[0] * L[0], qAC = L[1] * L[1] + L[2] * L[2], qAD = L[3] * L[3] + L[4] * L[4] + L[5] * L[5];
+    const float qBC = d10 * d10 + L[2] * L[2], qBD = d30 * d30 + L[4] * L[4] + L[5] * L[5];
+    const float qCD = d31 * d31 + d42 * d42 + L[5] * L[5];
+    const float nAB = __builtin_amdgcn_sqrtf(qAB), nAC = __builtin_amdgcn_sqrtf(qAC);
+    const float nAD = __builtin_amdgcn_sqrtf(qAD), nBC = __builtin_amdgcn_sqrtf(qBC);
+    const float nBD = __builtin_amdgcn_sqrtf(qBD), nCD = __builtin_amdgcn_sqrtf(qCD);
+    const float rAB = __builtin_amdgcn_rcpf(nAB), rAC = __builtin_amdgcn_rcpf(nAC);
+    const float rAD = __builtin_amdgcn_rcpf(nAD), rBC = __builtin_amdgcn_rcpf(nBC);
+    const float rBD = __builtin_amdgcn_rcpf(nBD), rCD = __builtin_amdgcn_rcpf(nCD);
+    const float TA = (float)v[0], TB = (float)v[1], TC = (float)v[2], TD = (float)v[3];
+    int out = mask;
+    if (mask & 1) {  // A over face B, C, D
+      const float b = plane_bound(TB, TC, TD, qAB, qAC, qAD, qBC, qBD, qCD, nAB, nAC, nAD, rAB, rAC, rAD);
+      if (b - 1e-5f * (fmaxf(TB, fmaxf(TC, TD)) + scale + 1.0f) > TA) out &= ~1;
+    }
+    if (mask & 2) {  // B over face A, C, D
+      const float b = plane_bound(TA, TC, TD, qAB, qBC, qBD, qAC, qAD, qCD, nAB, nBC, nBD, rAB, rBC, rBD);
+      if (b - 1e-5f * (fmaxf(TA, fmaxf(TC, TD)) + scale + 1.0f) > TB) out &= ~2;
+    }
+    if (mask & 4) {  // C over face A, B, D
+      const float b = plane_bound(TA, TB, TD, qAC, qBC, qCD, qAB, qAD, qBD, nAC, nBC, nCD, rAC, rBC, rCD);
+      if (b - 1e-5f * (fmaxf(TA, fmaxf(TB, TD)) + scale + 1.0f) > TC) out &= ~4;
+    }
+    if (mask & 8) {  // D over face A, B, C
+      const float b = plane_bound(TA, TB, TC, qAD, qBD, qCD, qAB, qAC, qBC, nAD, nBD, nCD, rAD, rBD, rCD);
+      if (b - 1e-5f * (fmaxf(TA, fmaxf(TB, TC)) + scale + 1.0f) > TD) out &= ~8;
+    }
+    return out;
+  }
   // The same from the copy's four FP16 heights packed in 8 bytes.
   __device__ static int need_mask(const uint2 hw, const T* v, double scale) {
     const __half hs[4] = {__ushort_as_half((unsigned short)(hw.x & 0xffffu)),
@@ -568,7 +635,10 @@ __global__ void k_heights(const T* coords, int64_t F, __half* hgt, int32_t* slow
 #pragma unroll
   for (int k = 0; k < 4; ++k) hgt[c * 4 + k] = __float2half_rz(__double2float_rz(h[k]));
   const double hmin = d_min(d_min(h[0], h[1]), d_min(h[2], h[3]));
-  if (!(hmin >= 0x1p-350 && hmin >= 0x1p-20 * ElemSolve<T, 4>::scale(L))) *slow = 1;
+  const double sc = ElemSolve<T, 4>::scale(L);
+  if (!(hmin >= 0x1p-350 && hmin >= 0x1p-20 * sc)) slow[0] = 1;
+  // The plane bound's FP32 squared lengths stay normal numbers.
+  if (!(hmin >= 0x1p-48 && sc <= 0x1p48)) slow[1] = 1;
 }
 
 // Per patch, the largest coordinate scale of its tet copies, rounded up to
@@ -1022,6 +1092,8 @@ __device__ SweepResult<T> sweep_patch(const DevMesh<T>& m, DevState<T>& s, int p
 #pragma unroll
           for (int i = 0; i < NC; ++i) x[i] = s_slot[e * NC + i];
           mask = staged_need_mask<T, NC>(m, p, es, e, x, stage, s_hgt) & unchanged_input_filter<NC>(s_dirty[e]);
+          if constexpr (NC == 4)
+            if (m.plane && mask) mask = ElemSolve<T, 4>::plane_mask(m.coords + (int64_t)(es + e) * K, x, mask, m.pscale[p]);
           s_dirty[e] = 0;
           ++n_upd;
         }
@@ -1084,8 +1156,9 @@ __device__ SweepResult<T> sweep_patch(const DevMesh<T>& m, DevState<T>& s, int p
           T x[NC];
 #pragma unroll
           for (int i = 0; i < NC; ++i) x[i] = s_slot[e * NC + i];
-          const int mask =
-              staged_need_mask<T, NC>(m, p, es, e, x, stage, s_hgt) & unchanged_input_filter<NC>(s_dirty[e]);
+          int mask = staged_need_mask<T, NC>(m, p, es, e, x, stage, s_hgt) & unchanged_input_filter<NC>(s_dirty[e]);
+          if constexpr (NC == 4)
+            if (m.plane && mask) mask = ElemSolve<T, 4>::plane_mask(m.coords + (int64_t)(es + e) * K, x, mask, m.pscale[p]);
           s_dirty[e] = (uint8_t)(mask << 4);
           c = __popc(mask);
           n_solve += c;
@@ -2444,9 +2517,10 @@ __global__ __launch_bounds__(BS, WPE) void k_step(DevMesh<T> m, DevState<T> s, S
   // those of the reference's loop; only the overlap differs.
   {
     const int P = m.P;
-    int32_t* ovF = s.ov_lists;          // [3][P] probe marks
+    int32_t* ovF = s.ov_lists;          // [3][P] probe marks, patches visited earlier in the step
     int32_t* ovE = s.ov_lists + 3 * P;  // [3][P] early members
     int32_t* ovM = s.ov_lists + 6 * P;  // [3][P] members to copy back
+    int32_t* ovN = s.ov_lists + 9 * P;  // [3][P] probe marks, patches not yet visited in the step
     int iters = 0;
     for (int E = 1;; ++E) {
       const int sE = E % 3, sP = (E + 2) % 3, sN = (E + 1) % 3;
@@ -2459,10 +2533,12 @@ __global__ __launch_bounds__(BS, WPE) void k_step(DevMesh<T> m, DevState<T> s, S
       int32_t* const bcur = odd ? s.bcon2 : s.bcon;
       const int32_t* const bprev = odd ? s.bcon : s.bcon2;
       const int nEarly = E == 1 ? ld_acquire(&s.pc[kPcSeed]) : ld_acquire(&s.pc[kPcOvEl + sE]);
-      const int nProbe = ld_acquire(&s.pc[kPcOvF + sP]);
+      const int nNew = ld_acquire(&s.pc[kPcOvFn + sP]);
+      const int nProbe = ld_acquire(&s.pc[kPcOvF + sP]) + nNew;
       const int nCopy = ld_acquire(&s.pc[kPcOvMl + sP]);
       if (gtid == 0) {  // slots this phase neither reads nor writes; the next one uses them
         s.pc[kPcOvF + sN] = 0;
+        s.pc[kPcOvFn + sN] = 0;
         s.pc[kPcOvEl + sP] = 0;
         s.pc[kPcOvMl + sN] = 0;
         s.pc[kPcOvW + sN] = 0;
@@ -2483,16 +2559,26 @@ __global__ __launch_bounds__(BS, WPE) void k_step(DevMesh<T> m, DevState<T> s, S
       const int members = E - 1;
       const int32_t* early = E == 1 ? s.listA : ovE + sE * P;
       const int n = nEarly + nProbe;
+      // With under two items per workgroup the slowest workgroup's items set
+      // the phase's length, so the first-visit probes go first (see below);
+      // with more, the phase is work-bound and the early visits go first.
+      const int nFirst = n < 2 * G ? nNew : 0;
       for (int i = first_item(n, &s.pc[kPcOvW + sE], s_work); i < n; i = later_item(n, &s.pc[kPcOvW + sE], s_work)) {
         // kind 0: a visit; 1: a probe that continues as the visit when the
         // patch is unconverged; 2: a one-sweep probe (a one-sweep visit then
-        // recomputes its sweep from a fresh load).
+        // recomputes its sweep from a fresh load). In a short phase items go
+        // longest first as far as that is known: probes of patches the front
+        // reaches for the first time in the step (they continue as full
+        // visits), the early visits, then the other probes (most end after
+        // one sweep); otherwise the early visits, the first-visit probes, the
+        // others. The order decides only which workgroup runs an item and when.
         int p, kind;
-        if (i < nEarly) {
-          p = early[i];
+        const int j = i < nFirst ? nEarly + i : (i < nFirst + nEarly ? i - nFirst : i);
+        if (j < nEarly) {
+          p = early[j];
           kind = 0;
         } else {
-          p = ovF[sP * P + (i - nEarly)];
+          p = j < nEarly + nNew ? ovN[sP * P + (j - nEarly)] : ovF[sP * P + (j - nEarly - nNew)];
           // an unconverged member of E-1 is one of E's early members
           if (members > 0 && vprev[p] == members && !bprev[p]) continue;
           if (tid == 0) s_acc[kAccProbes] += 1ull;
@@ -2517,7 +2603,12 @@ __global__ __launch_bounds__(BS, WPE) void k_step(DevMesh<T> m, DevState<T> s, S
         if (r.con && r.bmin < band_of(m, p, a.bw)) {
           for (int k = m.padj_off[p] + tid; k < m.padj_off[p + 1]; k += BS) {
             const int q = m.padj[k];
-            if (atomicMax(&s.freshE[q], E) < E) ovF[sE * P + atomicAdd(&s.pc[kPcOvF + sE], 1)] = q;
+            if (atomicMax(&s.freshE[q], E) < E) {
+              // visit tags are zero until the patch's first visit in the step
+              // (a tag being written concurrently only changes the list)
+              if (s.visitE[q] == 0 && s.visitE2[q] == 0) ovN[sE * P + atomicAdd(&s.pc[kPcOvFn + sE], 1)] = q;
+              else ovF[sE * P + atomicAdd(&s.pc[kPcOvF + sE], 1)] = q;
+            }
           }
         }
         if (tid == 0) {
@@ -2728,18 +2819,19 @@ class EngineImpl final : public Engine {
       if (NC == 4 && F_ > 0) {
         LS_HIP(dev_alloc(&d_hgt_, (size_t)F_ * 4));
         int32_t* d_slow = nullptr;
-        LS_HIP(dev_alloc(&d_slow, 1));
-        hipError_t e = hipMemsetAsync(d_slow, 0, sizeof(int32_t), stream_);
+        LS_HIP(dev_alloc(&d_slow, 2));
+        hipError_t e = hipMemsetAsync(d_slow, 0, 2 * sizeof(int32_t), stream_);
         if (e == hipSuccess) {
           k_heights<T><<<(int)((F_ + 255) / 256), 256, 0, stream_>>>(d_coords_, F_, d_hgt_, d_slow);
           e = hipGetLastError();
         }
-        int32_t slow = 1;
-        if (e == hipSuccess) e = hipMemcpyAsync(&slow, d_slow, sizeof(int32_t), hipMemcpyDeviceToHost, stream_);
+        int32_t slow[2] = {1, 1};
+        if (e == hipSuccess) e = hipMemcpyAsync(slow, d_slow, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, stream_);
         if (e == hipSuccess) e = hipStreamSynchronize(stream_);
         (void)hipFree(d_slow);
         LS_HIP(e);
-        fast_len_ = slow == 0;
+        fast_len_ = slow[0] == 0;
+        plane_ok_ = slow[1] == 0;
         LS_HIP(dev_alloc(&d_pscale_, (size_t)P_));
         k_patch_scale<T><<<P_, 256, 0, stream_>>>(d_coords_, d_ele_off_, d_pscale_);
         LS_HIP(hipGetLastError());
@@ -2770,7 +2862,7 @@ class EngineImpl final : public Engine {
     LS_HIP(dev_alloc(&d_visitE2_, (size_t)P_));
     LS_HIP(hipMemset(d_visitE2_, 0, (size_t)P_ * sizeof(int32_t)));
     LS_HIP(dev_alloc(&d_bcon2_, (size_t)P_));
-    LS_HIP(dev_alloc(&d_ov_lists_, (size_t)9 * P_));
+    LS_HIP(dev_alloc(&d_ov_lists_, (size_t)12 * P_));
     LS_HIP(dev_alloc(&d_label_, N_));
     LS_HIP(dev_alloc(&d_tsign_, N_));
     LS_HIP(dev_alloc(&d_bcon_, P_));
@@ -3108,7 +3200,7 @@ class EngineImpl final : public Engine {
 
   // LDS of the FIM sweep for the sweep mode, and the persistent grid size.
   int size_kernels(std::string& err) {
-    const int org_mode = sweep_mode_ & ~LS_SWEEP_UNSTAGED;
+    const int org_mode = sweep_mode_ & ~(LS_SWEEP_UNSTAGED | LS_SWEEP_NO_PLANE_BOUND);
     const bool fits = fim_task_lds_ <= (size_t)lds_max_;
     if (org_mode == LS_SWEEP_TASKS && !fits) {
       err = "the task-compacted sweep needs " + std::to_string(fim_task_lds_) + " B of LDS, device allows " +
@@ -3212,6 +3304,8 @@ class EngineImpl final : public Engine {
     return rc;
   }
   int sweep_tasks() const override { return org_ == kOrgTasks ? 1 : 0; }
+  // The sweeps apply the supporting-plane bound (tets, sizes allow, not switched off).
+  bool plane_on() const { return NC == 4 && plane_ok_ && !(sweep_mode_ & LS_SWEEP_NO_PLANE_BOUND); }
   void launch_sweep(int n, hipStream_t q, const DevMesh<T>& m, const DevState<T>& s, const int32_t* list, int niter,
                     int write_out, T tol, double leps, int epoch_in, int epoch_out) {
     if (org_ == kOrgTasks)
@@ -3334,6 +3428,7 @@ class EngineImpl final : public Engine {
     st.sweep_tasks = org_ == kOrgTasks ? 1 : 0;
     st.sweep_staged = stage_;
     st.fast_len_roots = fast_len_ ? 1 : 0;
+    st.plane_bound = plane_on() ? 1 : 0;
     st.step_wg_per_cu = step_one_per_cu_ ? 1 : coop_grid_ / std::max(1, cus_);
     DevMesh<T> m = mesh();
     DevState<T> s = state();
@@ -3486,6 +3581,7 @@ class EngineImpl final : public Engine {
     st.sweep_tasks = org_ == kOrgTasks ? 1 : 0;
     st.sweep_staged = stage_;
     st.fast_len_roots = fast_len_ ? 1 : 0;
+    st.plane_bound = plane_on() ? 1 : 0;
     st.step_wg_per_cu = 0;
     pc_synced_ = false;
     const DevMesh<T> m = mesh();
@@ -3621,6 +3717,7 @@ class EngineImpl final : public Engine {
     m.ngroups = ngroups_;
     m.sigma_phi = sigma_phi_ ? 1 : 0;
     m.fast_len = fast_len_ ? 1 : 0;
+    m.plane = plane_on() ? 1 : 0;
     m.ceik0 = kCeikDefault;
     m.ccurv0 = kCcurvDefault;
     return m;
@@ -3711,6 +3808,7 @@ class EngineImpl final : public Engine {
   int32_t* d_copy_src_ = nullptr;    // F, original element of each copy
   __half* d_hgt_ = nullptr;  // tet copies: corner heights (F x 4), FP16 rounded toward zero
   bool fast_len_ = false;    // the tet solve's length roots use sqrt_len (k_heights)
+  bool plane_ok_ = false;    // element sizes allow the FP32 plane bound (k_heights)
   float* d_pscale_ = nullptr;  // tets: per-patch coordinate scale bound (P)
   T* d_warm_ = nullptr;  // learned warm-start distances (N), written by the predictor
   unsigned long long* d_cfl_ = nullptr;  // CFL bound keys (2)

@@ -20,6 +20,7 @@ LS_EXEC_PERSISTENT, LS_EXEC_KERNELS, LS_EXEC_GRAPH = 0, 1, 2
 LS_WARM_NONE, LS_WARM_IDENTITY, LS_WARM_LEARNED = 0, 1, 2
 LS_VEL_ELEMENT, LS_VEL_PATCH_FIRST, LS_VEL_PHI_2D = 0, 1, 2
 LS_SWEEP_AUTO, LS_SWEEP_ELEMENTS, LS_SWEEP_TASKS, LS_SWEEP_UNSTAGED = 0, 1, 2, 4
+LS_SWEEP_NO_PLANE_BOUND = 8
 LS_LAYOUT_DEVICE, LS_LAYOUT_HOST = 0, 1
 
 
@@ -48,7 +49,7 @@ class Stats(C.Structure):
                 ("warm_residual", C.c_double), ("warm_patches", C.c_int32), ("warm_fallback", C.c_int32),
                 ("sweep_tasks", C.c_int32), ("step_wg_per_cu", C.c_int32), ("dt_used", C.c_double),
                 ("cfl_dt", C.c_double), ("cfl_max_rin", C.c_double), ("fast_len_roots", C.c_int32),
-                ("reserved2", C.c_int32)]
+                ("plane_bound", C.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -296,6 +296,48 @@ def test_sweep_modes_bit_exact(execution, oracle_mod):
     o.close()
 
 
+@pytest.mark.parametrize("execution", [lsb.LS_EXEC_PERSISTENT, lsb.LS_EXEC_KERNELS])
+@pytest.mark.parametrize("precision", [lsb.LS_F32, lsb.LS_F64])
+def test_plane_bound_skips_solves_bit_exact(execution, precision, oracle_mod):
+    # The supporting-plane bound only drops corner solves that cannot lower
+    # a value: with and without it the steps give the same bits, the same
+    # schedule and the oracle's result, with fewer local solves.
+    case = cases.small3d(n=32, shape="torus", block=(8, 8, 4), steps=4)
+    prm = lsb.step_params(case.dt, case.bandwidth, case.niter)
+    # (the task organisation's LDS holds FP32 patches of this size only)
+    pairs = [(lsb.LS_SWEEP_AUTO, lsb.LS_SWEEP_NO_PLANE_BOUND)]
+    if precision == lsb.LS_F32:
+        pairs.append((lsb.LS_SWEEP_TASKS, lsb.LS_SWEEP_TASKS | lsb.LS_SWEEP_NO_PLANE_BOUND))
+    runs = {}
+    for mode in [m for pair in pairs for m in pair]:
+        g = lsb.solver_for_case(case, execution=execution, precision=precision)
+        g.set_sweep(mode)
+        out = []
+        for _ in range(4):
+            st = g.step(prm)
+            assert st.plane_bound == (0 if mode & lsb.LS_SWEEP_NO_PLANE_BOUND else 1)
+            out.append((g.phi(), st.fim_iters, st.patch_visits, st.probes, st.local_solves, st.elem_updates))
+        g.close()
+        runs[mode] = out
+    for on, off in pairs:
+        for k in range(4):
+            a, b = runs[on][k], runs[off][k]
+            np.testing.assert_array_equal(a[0], b[0], err_msg=f"mode {on} step {k}")
+            assert a[1:4] == b[1:4]
+        solves_on = sum(r[4] for r in runs[on])
+        solves_off = sum(r[4] for r in runs[off])
+        assert solves_on < 0.9 * solves_off, (solves_on, solves_off)
+    o = oracle_mod.Oracle(3, case.xyz, case.cells, case.labels, case.nparts,
+                          32 if precision == lsb.LS_F32 else 64)
+    o.set_phi(case.phi0)
+    o.set_velocity(case.vel)
+    for k in range(4):
+        o.step(oracle_mod.make_params(case.dt, case.bandwidth, case.niter))
+        ref = o.phi()
+        np.testing.assert_array_equal(runs[lsb.LS_SWEEP_AUTO][k][0], ref, err_msg=f"step {k}")
+    o.close()
+
+
 def test_patch_too_large_is_reported_and_recoverable(oracle_mod):
     # One patch over a 12^3 Kuhn grid (7986 element copies) needs ~230 KB of
     # LDS, more than a workgroup has: partitioning reports LS_ERR_LIMIT, and

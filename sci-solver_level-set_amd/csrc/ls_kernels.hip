@@ -841,9 +841,9 @@ struct OrderedBits<double> {
 // drawn last, s_draw[1] a second item reserved with it (-1: none). While at
 // least 16 draws per workgroup remain a draw reserves two consecutive items,
 // halving the counter round trips; near the end of the phase it reserves one,
-// so a workgroup done early still finds work. Every counter value is one
-// item, taken once. A phase's loop ends only on a draw past n, which finds no
-// reserved item, so the pair state starts every phase empty.
+// so a workgroup done early still finds work. The counter numbers the items
+// after the G each workgroup took first (first_item), each value one item,
+// taken once.
 __device__ __forceinline__ int draw_items(int n, int32_t* counter, int* s_draw) {
   __syncthreads();  // every thread has read the previous item
   if (threadIdx.x == 0) {
@@ -852,7 +852,7 @@ __device__ __forceinline__ int draw_items(int n, int32_t* counter, int* s_draw) 
       s_draw[1] = -1;
     } else {
       const bool pair = s_draw[0] + 16 * (int)gridDim.x < n;
-      const int i = atomicAdd(counter, pair ? 2 : 1);
+      const int i = (int)gridDim.x + atomicAdd(counter, pair ? 2 : 1);
       s_draw[0] = i;
       s_draw[1] = pair && i + 1 < n ? i + 1 : -1;
     }
@@ -861,13 +861,18 @@ __device__ __forceinline__ int draw_items(int n, int32_t* counter, int* s_draw) 
   return s_draw[0];
 }
 
-// Work items of a phase with n items: with no more items than workgroups
-// every workgroup takes the item of its own index once (all G indices are
-// taken, so every item is), which keeps the counter's atomic round trips off
-// the phase's critical path; otherwise items are drawn from the counter, so a
-// workgroup done early takes the next one. n means no further item.
+// Work items of a phase with n items: every workgroup first takes the item
+// of its own index (all G indices are taken, so every item below G is),
+// which keeps a counter round trip off the start of the phase; with more
+// items than workgroups the rest are drawn from the counter, so a workgroup
+// done early takes the next one. n means no further item.
 __device__ __forceinline__ int first_item(int n, int32_t* counter, int* s_draw) {
-  return n <= (int)gridDim.x ? ((int)blockIdx.x < n ? (int)blockIdx.x : n) : draw_items(n, counter, s_draw);
+  (void)counter;
+  if (threadIdx.x == 0) {  // the draw state; draw_items synchronises before reading it
+    s_draw[0] = (int)blockIdx.x;
+    s_draw[1] = -1;
+  }
+  return (int)blockIdx.x < n ? (int)blockIdx.x : n;
 }
 __device__ __forceinline__ int later_item(int n, int32_t* counter, int* s_draw) {
   return n <= (int)gridDim.x ? n : draw_items(n, counter, s_draw);
@@ -2544,13 +2549,19 @@ __global__ __launch_bounds__(BS, WPE) void k_step(DevMesh<T> m, DevState<T> s, S
         s.pc[kPcOvW + sN] = 0;
         s.pc[kPcOvVis + sN] = 0;
       }
-      for (int i = b; i < nCopy; i += G) {  // CopyOutBack of E-1's members
+      const bool capped = a.max_iters > 0 && iters >= a.max_iters;
+      const bool done = (nEarly == 0 && nProbe == 0) || capped;
+      const int n = done ? 0 : nEarly + nProbe;
+      // CopyOutBack of E-1's members (this phase's visits read them from
+      // dprev; the next phase reads DT). With fewer items than workgroups the
+      // workgroups without an item do it, so it delays no item; otherwise
+      // every workgroup does its share first, while dprev is still in cache.
+      for (int i = n < G ? b - n : b; i >= 0 && i < nCopy; i += n < G ? G - n : G) {
         const int p = ovM[sP * P + i];
         const int vs = m.vert_off[p], ve = m.vert_off[p + 1];
         for (int v = vs + tid; v < ve; v += BS) s.DT[v] = dprev[v];
       }
-      const bool capped = a.max_iters > 0 && iters >= a.max_iters;
-      if ((nEarly == 0 && nProbe == 0) || capped) {
+      if (done) {
         if (capped && nEarly + nProbe > 0 && gtid == 0) s.pc[kPcOverflow] = 1;
         grid_barrier(s.bar, G);
         pcl.lap(clk, kTkVisit);
@@ -2558,7 +2569,6 @@ __global__ __launch_bounds__(BS, WPE) void k_step(DevMesh<T> m, DevState<T> s, S
       }
       const int members = E - 1;
       const int32_t* early = E == 1 ? s.listA : ovE + sE * P;
-      const int n = nEarly + nProbe;
       // With under two items per workgroup the slowest workgroup's items set
       // the phase's length, so the first-visit probes go first (see below);
       // with more, the phase is work-bound and the early visits go first.

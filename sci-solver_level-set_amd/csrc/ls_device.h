@@ -22,6 +22,7 @@ struct DevMesh {
   const int32_t* ele_off;      // P+1
   const int32_t* halo_off;     // P+1
   const int32_t* halo_ids;     // renumbered global ids
+  const int32_t* halo_patch;   // patch owning each halo entry's vertex (vpatch[halo_ids[i]])
   const ushort4* copy_local;   // F, patch-local corner ids
   const int32_t* vslot_off;    // N+1
   const uint16_t* vslots;      // patch-local slot ids per owned vertex

@@ -991,7 +991,11 @@ __device__ SweepResult<T> sweep_patch(const DevMesh<T>& m, DevState<T>& s, int p
   T* s_vtx = s_slot + NC * ne;
   T* s_vval = s_vtx + nv + nh;
   uint16_t* s_voff = (uint16_t*)(s_vval + nv);
-  uint16_t* s_vsl = s_voff + nv + 1;
+  // The slot lists and staged element data start at their sources' offsets
+  // modulo 16 bytes (device arrays are 256-byte aligned), so 16-byte loads
+  // and stores copy them.
+  const int o_vsl = ((int)((unsigned char*)(s_voff + nv + 1) - smem) + 15 & ~15) + ((2 * sbase) & 15);
+  uint16_t* s_vsl = (uint16_t*)(smem + o_vsl);
   uint16_t* s_work = s_vsl + nslots;            // compacted dirty elements
   // Aligned regions are carved by byte offsets from smem so that every
   // access stays an LDS access. s_dirty[e]: bits 0..NC-1 mark the corner
@@ -1006,30 +1010,100 @@ __device__ SweepResult<T> sweep_patch(const DevMesh<T>& m, DevState<T>& s, int p
   Bits* s_vmin = (Bits*)(smem + o_vmin);
   // Staged element data (stage set): each copy's corner ids and FP16
   // heights, read by every sweep of the visit from LDS instead of memory.
-  const int o_stage = (o_vmin + nv * (int)sizeof(Bits) + 7) & ~7;
+  const int o_end = (o_vmin + nv * (int)sizeof(Bits) + 7) & ~7;
+  const int o_stage = ((o_end + 15) & ~15) + 8 * (es & 1);
   ushort4* s_cl = (ushort4*)(smem + o_stage);
-  uint2* s_hgt = (uint2*)(smem + o_stage + 8 * ne);
+  const int o_hgt = ((o_stage + 8 * ne + 15) & ~15) + 8 * (es & 1);
+  uint2* s_hgt = (uint2*)(smem + o_hgt);
   // Task mode: (element, corner) solves and their results.
-  uint32_t* s_task = (uint32_t*)(smem + o_stage + (stage ? 16 * ne : 0));
+  uint32_t* s_task = (uint32_t*)(smem + (stage ? o_hgt + 8 * ne : o_end));
   T* s_new = (T*)(s_task + NC * ne + (NC * ne) % 2);
   const int tid = threadIdx.x;
 
-  const T* own = (members > 0 && io.vtag[p] == members) ? io.din : s.DT;
-  for (int i = tid; i < nv; i += BS) {
-    const T v = own[vs + i];
-    s_vtx[i] = v;
-    s_vval[i] = v;
-    s_voff[i] = (uint16_t)(m.vslot_off[vs + i] - sbase);
+  // The patch load issues every independent load of a thread before its
+  // stores (offsets; owned values, halo ids and owners, 16-byte pieces of
+  // the slot lists and staged element data, clean flags; then the selected
+  // halo values), so it waits for memory about twice instead of once per
+  // array. The unaligned ends of the 16-byte copies go element by element.
+  const int tag_p = io.vtag[p], ve_tag_p = s.ve_tag[p];
+  const T* own = (members > 0 && tag_p == members) ? io.din : s.DT;
+  const uint16_t* vsl_src = m.vslots + sbase;
+  const int vsl_head = min(nslots, (int)(((16u - ((2u * (unsigned)sbase) & 15u)) & 15u) / 2u));
+  const int vsl_nvec = (nslots - vsl_head) / 8;
+  const int vsl_tail0 = vsl_head + 8 * vsl_nvec;  // fewer than 8 elements follow
+  const int st_head = stage ? min(ne, es & 1) : 0;
+  const int st_nvec = stage ? (ne - st_head) / 2 : 0;
+  const int st_tail0 = st_head + 2 * st_nvec;     // at most one element follows
+  const uint2* hgt_src = NC == 4 ? (const uint2*)m.hgt + es : nullptr;
+  uint16_t e_vh = 0, e_vt = 0;
+  ushort4 e_ch, e_ct;
+  uint2 e_hh, e_ht;
+  const bool do_vh = tid < vsl_head, do_vt = tid < nslots - vsl_tail0;
+  const bool do_sh = tid == 0 && st_head > 0, do_st = tid == 0 && stage && st_tail0 < ne;
+  if (do_vh) e_vh = vsl_src[tid];
+  if (do_vt) e_vt = vsl_src[vsl_tail0 + tid];
+  if (do_sh) {
+    e_ch = m.copy_local[es];
+    if constexpr (NC == 4) e_hh = hgt_src[0];
+  }
+  if (do_st) {
+    e_ct = m.copy_local[es + st_tail0];
+    if constexpr (NC == 4) e_ht = hgt_src[st_tail0];
+  }
+  // The first chunk's clean flags (read by the slot initialisation below).
+  // (A visit that may resume from a probe's state reads them later.)
+  const bool okb_valid = !(niter > 1 && epoch_in > 0);
+  uint8_t okb[4] = {0, 0, 0, 0};
+  if (okb_valid && ve_tag_p != 0) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (tid + u * BS < ne) okb[u] = s.ve_ok[es + tid + u * BS];
+  }
+  const int nwork = max(max(nv, nh), max(vsl_nvec, st_nvec));
+  for (int i = tid; i < nwork; i += BS) {
+    int h = 0, q = 0, vo = 0;
+    T ov = (T)0;
+    uint4 w0, w1, w2;
+    if (i < nh) {
+      h = m.halo_ids[hs + i];
+      q = m.halo_patch[hs + i];
+    }
+    if (i < nv) {
+      ov = own[vs + i];
+      vo = m.vslot_off[vs + i];
+    }
+    if (i < vsl_nvec) w0 = reinterpret_cast<const uint4*>(vsl_src + vsl_head)[i];
+    if (i < st_nvec) {
+      w1 = reinterpret_cast<const uint4*>(m.copy_local + es + st_head)[i];
+      if constexpr (NC == 4) w2 = reinterpret_cast<const uint4*>(hgt_src + st_head)[i];
+    }
+    if (i < nh) {
+      // both candidates loaded beside the owner's tag, then selected
+      const T a = io.din[h], d = s.DT[h];
+      const int t = io.vtag[q];
+      s_vtx[nv + i] = (members > 0 && t == members) ? a : d;
+    }
+    if (i < nv) {
+      s_vtx[i] = ov;
+      s_vval[i] = ov;
+      s_voff[i] = (uint16_t)(vo - sbase);
+    }
+    if (i < vsl_nvec) reinterpret_cast<uint4*>(s_vsl + vsl_head)[i] = w0;
+    if (i < st_nvec) {
+      reinterpret_cast<uint4*>(s_cl + st_head)[i] = w1;
+      if constexpr (NC == 4) reinterpret_cast<uint4*>(s_hgt + st_head)[i] = w2;
+    }
   }
   if (tid == 0) s_voff[nv] = (uint16_t)nslots;
-  for (int i = tid; i < nh; i += BS) {
-    const int h = m.halo_ids[hs + i];
-    s_vtx[nv + i] = (members > 0 && io.vtag[m.vpatch[h]] == members) ? io.din[h] : s.DT[h];
+  if (do_vh) s_vsl[tid] = e_vh;
+  if (do_vt) s_vsl[vsl_tail0 + tid] = e_vt;
+  if (do_sh) {
+    s_cl[0] = e_ch;
+    if constexpr (NC == 4) s_hgt[0] = e_hh;
   }
-  copy_to_lds<BS, 8>(s_vsl, m.vslots + sbase, nslots);
-  if (stage) {
-    copy_to_lds<BS, 4>(s_cl, m.copy_local + es, ne);
-    if constexpr (NC == 4) copy_to_lds<BS, 4>(s_hgt, (const uint2*)m.hgt + es, ne);
+  if (do_st) {
+    s_cl[st_tail0] = e_ct;
+    if constexpr (NC == 4) s_hgt[st_tail0] = e_ht;
   }
   // (A single-sweep visit recomputes the sweep: resuming would leave it none.)
   const bool resume = niter > 1 && epoch_in > 0 && s.rs_epoch[p] == epoch_in;
@@ -1039,7 +1113,7 @@ __device__ SweepResult<T> sweep_patch(const DevMesh<T>& m, DevState<T>& s, int p
     copy_to_lds<BS, 4>(s_vval, s.rs_vval + vs, nv);
   }
   __syncthreads();
-  const bool saved = !resume && s.ve_tag[p] != 0;
+  const bool saved = !resume && ve_tag_p != 0;
   if (!resume)
   for (int base = tid; base < ne; base += 4 * BS) {
     ushort4 c[4];
@@ -1052,7 +1126,8 @@ __device__ SweepResult<T> sweep_patch(const DevMesh<T>& m, DevState<T>& s, int p
       if (e < ne) {
 #pragma unroll
         for (int i = 0; i < NC; ++i) s_slot[e * NC + i] = s_vtx[corner(c[u], i)];
-        s_dirty[e] = (saved && s.ve_ok[es + e]) ? (uint8_t)0 : (uint8_t)((1 << NC) - 1);
+        const bool ok = base == tid && okb_valid ? okb[u] != 0 : (saved && s.ve_ok[es + e]);
+        s_dirty[e] = ok ? (uint8_t)0 : (uint8_t)((1 << NC) - 1);
       }
     }
   }
@@ -1849,6 +1924,15 @@ __global__ void k_vertex_patch(const int32_t* vert_off, int P, int32_t* vpatch) 
   const int p = blockIdx.x;
   if (p >= P) return;
   for (int v = vert_off[p] + threadIdx.x; v < vert_off[p + 1]; v += blockDim.x) vpatch[v] = p;
+}
+
+// Owner patch of every halo entry, so a visit's halo gather needs no
+// dependent vpatch lookup.
+__global__ void k_halo_patch(const int32_t* halo_off, const int32_t* halo_ids, const int32_t* vpatch, int P,
+                             int32_t* halo_patch) {
+  const int p = blockIdx.x;
+  if (p >= P) return;
+  for (int i = halo_off[p] + threadIdx.x; i < halo_off[p + 1]; i += blockDim.x) halo_patch[i] = vpatch[halo_ids[i]];
 }
 
 __global__ void k_clear_sizes(unsigned long long* c) {
@@ -2764,13 +2848,15 @@ class EngineImpl final : public Engine {
       const size_t ne = L.ele_off[p + 1] - L.ele_off[p], nv = L.vert_off[p + 1] - L.vert_off[p];
       const size_t nh = L.halo_off[p + 1] - L.halo_off[p];
       const size_t ns = L.vslot_off[L.vert_off[p + 1]] - L.vslot_off[L.vert_off[p]];
-      const size_t fim = ((NC * ne * sizeof(T) + (2 * nv + nh) * sizeof(T) + 2 * (nv + 1 + ns) + 2 * ne + 3 + ne + 7) &
-                          ~size_t(7)) + nv * sizeof(T);  // + running vertex minima
+      // (+32: the slot lists' 16-byte alignment shift)
+      const size_t fim = ((NC * ne * sizeof(T) + (2 * nv + nh) * sizeof(T) + 2 * (nv + 1 + ns) + 32 + 2 * ne + 3 +
+                           ne + 7) & ~size_t(7)) + nv * sizeof(T);  // + running vertex minima
       const size_t fim_tasks = ((fim + 7) & ~size_t(7)) + 4 * (NC * ne + (NC * ne) % 2) + NC * ne * sizeof(T);
       fim_task_lds = std::max(fim_task_lds, fim_tasks);
-      // Staged corner ids and heights: 16 bytes per copy after the vertex minima.
-      fim_stage_lds = std::max(fim_stage_lds, ((fim + 7) & ~size_t(7)) + 16 * ne);
-      fim_task_stage_lds = std::max(fim_task_stage_lds, fim_tasks + 16 * ne);
+      // Staged corner ids and heights: 16 bytes per copy after the vertex
+      // minima, each array shifted to its source's 16-byte alignment.
+      fim_stage_lds = std::max(fim_stage_lds, ((fim + 7) & ~size_t(7)) + 16 * ne + 64);
+      fim_task_stage_lds = std::max(fim_task_stage_lds, fim_tasks + 16 * ne + 64);
       // Advection: per-slot products (FP32) or per-element records (FP64);
       // curvature: per-slot terms plus two per-element values.
       const size_t adv = ((sizeof(T) == 8 ? (NC + 2) * ne : 2 * NC * ne) + nv + nh) * sizeof(T);
@@ -2861,6 +2947,14 @@ class EngineImpl final : public Engine {
     LS_HIP(dev_alloc(&d_vpatch_, (size_t)N_));
     if (P_ > 0) k_vertex_patch<<<P_, kBlock, 0, stream_>>>(d_vert_off_, P_, d_vpatch_);
     LS_HIP(hipGetLastError());
+    {
+      int32_t H = 0;
+      if (P_ > 0) LS_HIP(hipMemcpy(&H, d_halo_off_ + P_, sizeof(int32_t), hipMemcpyDeviceToHost));
+      LS_HIP(dev_alloc(&d_halo_patch_, (size_t)H));
+      if (P_ > 0 && H > 0)
+        k_halo_patch<<<P_, kBlock, 0, stream_>>>(d_halo_off_, d_halo_ids_, d_vpatch_, P_, d_halo_patch_);
+      LS_HIP(hipGetLastError());
+    }
     LS_HIP(hipStreamSynchronize(stream_));
     LS_HIP(dev_alloc(&d_sigma_, (size_t)F_ * (NC - 1)));
     LS_HIP(hipMemset(d_sigma_, 0, (size_t)F_ * (NC - 1) * sizeof(T)));
@@ -3713,6 +3807,7 @@ class EngineImpl final : public Engine {
     DevMesh<T> m;
     m.P = P_; m.N = N_; m.F = F_;
     m.vert_off = d_vert_off_; m.ele_off = d_ele_off_; m.halo_off = d_halo_off_; m.halo_ids = d_halo_ids_;
+    m.halo_patch = d_halo_patch_;
     m.copy_local = d_copy_local_; m.vslot_off = d_vslot_off_; m.vslots = d_vslots_;
     m.coords = d_coords_; m.sigma = d_sigma_; m.padj_off = d_padj_off_; m.padj = d_padj_;
     m.vpatch = d_vpatch_;
@@ -3774,7 +3869,7 @@ class EngineImpl final : public Engine {
                     d_copy_verts_, d_copy_src_, d_rs_slot_, d_rs_vval_, d_rs_dirty_, d_rs_epoch_, d_visitE_,
                     d_vpatch_, d_ve_ok_, d_ve_tag_, d_gfail_, d_pgroup_, d_ceik_, d_ccurv_, d_DT_out2_,
                     d_visitE2_,
-                    d_bcon2_, d_ov_lists_, d_pscale_};
+                    d_bcon2_, d_ov_lists_, d_pscale_, d_halo_patch_};
     if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
     if (graph_) (void)hipGraphDestroy(graph_);
     for (void* p : ptrs) if (p) (void)hipFree(p);
@@ -3814,6 +3909,7 @@ class EngineImpl final : public Engine {
   uint8_t* d_ve_ok_ = nullptr;   // F, element copies a visit left clean
   int32_t* d_ve_tag_ = nullptr;  // P, the patch has ve_ok from this FIM run
   int32_t* d_vpatch_ = nullptr;  // N, patch of each renumbered vertex
+  int32_t* d_halo_patch_ = nullptr;  // halo entries, patch owning each one's vertex
   int32_t* d_copy_verts_ = nullptr;  // F x NC, renumbered corner ids
   int32_t* d_copy_src_ = nullptr;    // F, original element of each copy
   __half* d_hgt_ = nullptr;  // tet copies: corner heights (F x 4), FP16 rounded toward zero

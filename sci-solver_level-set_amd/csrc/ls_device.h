@@ -23,6 +23,9 @@ struct DevMesh {
   const int32_t* halo_off;     // P+1
   const int32_t* halo_ids;     // renumbered global ids
   const int32_t* halo_patch;   // patch owning each halo entry's vertex (vpatch[halo_ids[i]])
+  // Per patch, two int4: (vert_off[p], vert_off[p+1], ele_off[p], ele_off[p+1]),
+  // (halo_off[p], halo_off[p+1], vslot_off[vert_off[p]], vslot_off[vert_off[p+1]]).
+  const int4* phdr;
   const ushort4* copy_local;   // F, patch-local corner ids
   const int32_t* vslot_off;    // N+1
   const uint16_t* vslots;      // patch-local slot ids per owned vertex

@@ -981,12 +981,12 @@ __device__ SweepResult<T> sweep_patch(const DevMesh<T>& m, DevState<T>& s, int p
   const int stage = STG < 0 ? stage_arg : STG;
   constexpr int K = ElemSolve<T, NC>::kCoords;
   __syncthreads();
-  const int vs = m.vert_off[p], ve = m.vert_off[p + 1];
-  const int es = m.ele_off[p], ee = m.ele_off[p + 1];
-  const int hs = m.halo_off[p], he = m.halo_off[p + 1];
+  // The patch's offsets in one load (no dependent vslot_off lookup).
+  const int4 h0 = m.phdr[2 * p], h1 = m.phdr[2 * p + 1];
+  const int vs = h0.x, ve = h0.y, es = h0.z, ee = h0.w, hs = h1.x, he = h1.y;
   const int nv = ve - vs, ne = ee - es, nh = he - hs;
-  const int sbase = m.vslot_off[vs];
-  const int nslots = m.vslot_off[ve] - sbase;
+  const int sbase = h1.z;
+  const int nslots = h1.w - sbase;
   T* s_slot = (T*)smem;
   T* s_vtx = s_slot + NC * ne;
   T* s_vval = s_vtx + nv + nh;
@@ -1935,6 +1935,15 @@ __global__ void k_halo_patch(const int32_t* halo_off, const int32_t* halo_ids, c
   for (int i = halo_off[p] + threadIdx.x; i < halo_off[p + 1]; i += blockDim.x) halo_patch[i] = vpatch[halo_ids[i]];
 }
 
+// Per-patch offset header (DevMesh::phdr).
+__global__ void k_patch_hdr(const int32_t* vert_off, const int32_t* ele_off, const int32_t* halo_off,
+                            const int32_t* vslot_off, int P, int4* hdr) {
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < P; p += gridDim.x * blockDim.x) {
+    hdr[2 * p] = make_int4(vert_off[p], vert_off[p + 1], ele_off[p], ele_off[p + 1]);
+    hdr[2 * p + 1] = make_int4(halo_off[p], halo_off[p + 1], vslot_off[vert_off[p]], vslot_off[vert_off[p + 1]]);
+  }
+}
+
 __global__ void k_clear_sizes(unsigned long long* c) {
   if ((int)threadIdx.x < kNumSizes) c[threadIdx.x] = 0ull;
 }
@@ -2621,6 +2630,9 @@ __global__ __launch_bounds__(BS, WPE) void k_step(DevMesh<T> m, DevState<T> s, S
       const int32_t* const vprev = odd ? s.visitE : s.visitE2;
       int32_t* const bcur = odd ? s.bcon2 : s.bcon;
       const int32_t* const bprev = odd ? s.bcon : s.bcon2;
+      // E-1's phase counted as an iteration when it visited a patch (read
+      // here with this phase's counters, not as a load of its own).
+      const int visPrev = E == 1 ? 0 : ld_acquire(&s.pc[kPcOvVis + sP]);
       const int nEarly = E == 1 ? ld_acquire(&s.pc[kPcSeed]) : ld_acquire(&s.pc[kPcOvEl + sE]);
       const int nNew = ld_acquire(&s.pc[kPcOvFn + sP]);
       const int nProbe = ld_acquire(&s.pc[kPcOvF + sP]) + nNew;
@@ -2632,6 +2644,10 @@ __global__ __launch_bounds__(BS, WPE) void k_step(DevMesh<T> m, DevState<T> s, S
         s.pc[kPcOvMl + sN] = 0;
         s.pc[kPcOvW + sN] = 0;
         s.pc[kPcOvVis + sN] = 0;
+      }
+      if (visPrev > 0) {
+        ++iters;
+        if (gtid == 0) s.pc[kPcIters] += 1;
       }
       const bool capped = a.max_iters > 0 && iters >= a.max_iters;
       const bool done = (nEarly == 0 && nProbe == 0) || capped;
@@ -2715,10 +2731,6 @@ __global__ __launch_bounds__(BS, WPE) void k_step(DevMesh<T> m, DevState<T> s, S
       }
       grid_barrier(s.bar, G);
       pcl.lap(clk, kTkVisit);
-      if (ld_acquire(&s.pc[kPcOvVis + sE]) > 0) {
-        ++iters;
-        if (gtid == 0) s.pc[kPcIters] += 1;
-      }
     }
   }
 
@@ -2953,6 +2965,11 @@ class EngineImpl final : public Engine {
       LS_HIP(dev_alloc(&d_halo_patch_, (size_t)H));
       if (P_ > 0 && H > 0)
         k_halo_patch<<<P_, kBlock, 0, stream_>>>(d_halo_off_, d_halo_ids_, d_vpatch_, P_, d_halo_patch_);
+      LS_HIP(hipGetLastError());
+      LS_HIP(dev_alloc(&d_phdr_, (size_t)2 * P_));
+      if (P_ > 0)
+        k_patch_hdr<<<(P_ + kBlock - 1) / kBlock, kBlock, 0, stream_>>>(d_vert_off_, d_ele_off_, d_halo_off_,
+                                                                          d_vslot_off_, P_, d_phdr_);
       LS_HIP(hipGetLastError());
     }
     LS_HIP(hipStreamSynchronize(stream_));
@@ -3808,6 +3825,7 @@ class EngineImpl final : public Engine {
     m.P = P_; m.N = N_; m.F = F_;
     m.vert_off = d_vert_off_; m.ele_off = d_ele_off_; m.halo_off = d_halo_off_; m.halo_ids = d_halo_ids_;
     m.halo_patch = d_halo_patch_;
+    m.phdr = d_phdr_;
     m.copy_local = d_copy_local_; m.vslot_off = d_vslot_off_; m.vslots = d_vslots_;
     m.coords = d_coords_; m.sigma = d_sigma_; m.padj_off = d_padj_off_; m.padj = d_padj_;
     m.vpatch = d_vpatch_;
@@ -3869,7 +3887,7 @@ class EngineImpl final : public Engine {
                     d_copy_verts_, d_copy_src_, d_rs_slot_, d_rs_vval_, d_rs_dirty_, d_rs_epoch_, d_visitE_,
                     d_vpatch_, d_ve_ok_, d_ve_tag_, d_gfail_, d_pgroup_, d_ceik_, d_ccurv_, d_DT_out2_,
                     d_visitE2_,
-                    d_bcon2_, d_ov_lists_, d_pscale_, d_halo_patch_};
+                    d_bcon2_, d_ov_lists_, d_pscale_, d_halo_patch_, d_phdr_};
     if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
     if (graph_) (void)hipGraphDestroy(graph_);
     for (void* p : ptrs) if (p) (void)hipFree(p);
@@ -3910,6 +3928,7 @@ class EngineImpl final : public Engine {
   int32_t* d_ve_tag_ = nullptr;  // P, the patch has ve_ok from this FIM run
   int32_t* d_vpatch_ = nullptr;  // N, patch of each renumbered vertex
   int32_t* d_halo_patch_ = nullptr;  // halo entries, patch owning each one's vertex
+  int4* d_phdr_ = nullptr;           // 2P, per-patch offset header
   int32_t* d_copy_verts_ = nullptr;  // F x NC, renumbered corner ids
   int32_t* d_copy_src_ = nullptr;    // F, original element of each copy
   __half* d_hgt_ = nullptr;  // tet copies: corner heights (F x 4), FP16 rounded toward zero

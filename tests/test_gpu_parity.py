@@ -455,6 +455,36 @@ def test_length_roots_path_bit_exact(oracle_mod, flat):
         np.testing.assert_array_equal(gphi, ophi, err_msg=f"step {k}")
 
 
+@pytest.mark.parametrize("scale", [1.0, 2.0 ** -50])
+def test_plane_bound_size_guard_bit_exact(oracle_mod, scale):
+    """The supporting-plane bound is evaluated in FP32 only where element
+    sizes keep squared lengths normal FP32 numbers (least height at least
+    2^-48, size at most 2^48); the same mesh scaled by 2^-50 (heights about
+    3e-17) runs without it. Both are bit-exact with the FP64 oracle of the
+    scaled mesh (distances, band, tolerance and lambda scaled alike)."""
+    import dataclasses
+    case = _custom_case(12, 0.1, lambda x: cases.sdf_sphere(x, r=0.3))
+    case = dataclasses.replace(case, xyz=case.xyz * scale, phi0=case.phi0 * scale, vel=case.vel * scale,
+                               bandwidth=case.bandwidth * scale, h=case.h * scale)
+    g = lsb.solver_for_case(case, precision=lsb.LS_F64)
+    o = oracle_mod.Oracle(3, case.xyz, case.cells, case.labels, case.nparts, 64)
+    o.set_phi(case.phi0)
+    o.set_velocity(case.vel)
+    gp = lsb.step_params(case.dt, case.bandwidth, case.niter, tol=1e-6 * scale, lambda_eps=1e-5 * scale)
+    op = oracle_mod.make_params(case.dt, case.bandwidth, case.niter, tol=1e-6 * scale, lambda_eps=1e-5 * scale)
+    for k in range(3):
+        gs = g.step(gp)
+        os_ = o.step(op)
+        assert gs.plane_bound == (1 if scale == 1.0 else 0)
+        assert gs.fast_len_roots == 1
+        assert gs.nb_count > 0
+        np.testing.assert_array_equal(g.narrowband(), o.narrowband(), err_msg=f"narrowband step {k}")
+        assert _same_schedule(gs, os_), f"schedule step {k}"
+        np.testing.assert_array_equal(g.phi(), o.phi(), err_msg=f"step {k}")
+    g.close()
+    o.close()
+
+
 @pytest.mark.parametrize("which,mode", [("tri", lsb.LS_VEL_PHI_2D), ("tet", lsb.LS_VEL_PATCH_FIRST)])
 @pytest.mark.parametrize("execution", [lsb.LS_EXEC_PERSISTENT, lsb.LS_EXEC_KERNELS])
 def test_velocity_modes_bit_exact_with_oracle(oracle_mod, which, mode, execution):

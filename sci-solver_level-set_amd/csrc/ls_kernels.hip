@@ -337,16 +337,17 @@ struct ElemSolve<T, 4> {
   // face it is least at a face corner c, where it equals T_c + (c - D).w,
   // w the unit vector from D towards E0. With E0 the foot of the
   // perpendicular this is the height bound; here E0 runs over the three
-  // face corners and the centroid, and the bound is the largest of the
-  // four. The products (c - D).(c' - D) come from squared edge lengths (law
+  // face corners and the bound is the largest of the three (the centroid as
+  // a fourth point pruned 5 % more solves in a model but cost more VALU time
+  // than they take). The products (c - D).(c' - D) come from squared edge lengths (law
   // of cosines), evaluated in FP32: a margin of 1e-5 (max other value +
   // scale + 1) lies far above their rounding (below 1e-6 of the same, as
   // every edge is at most `scale` long) and the FP64 rounding of the
   // candidates, so a corner is dropped only when its solve cannot go below
   // its value and the result is bit-identical to solving it. DevMesh::plane
   // is set when element sizes keep the squared lengths normal FP32 numbers.
-  // On the 64^3 torus it removes about 31 % of the solves the height bound
-  // leaves.
+  // On the 64^3 torus it removes about a quarter of the solves the height
+  // bound leaves.
   __device__ static float plane_bound(float T0, float T1, float T2, float qk0, float qk1, float qk2, float q01,
                                       float q02, float q12, float n0, float n1, float n2, float r0, float r1,
                                       float r2) {
@@ -354,10 +355,7 @@ struct ElemSolve<T, 4> {
     const float b0 = fminf(T0 + n0, fminf(fmaf(G01, r0, T1), fmaf(G02, r0, T2)));
     const float b1 = fminf(T1 + n1, fminf(fmaf(G01, r1, T0), fmaf(G12, r1, T2)));
     const float b2 = fminf(T2 + n2, fminf(fmaf(G02, r2, T0), fmaf(G12, r2, T1)));
-    const float S0 = qk0 + G01 + G02, S1 = qk1 + G01 + G12, S2 = qk2 + G02 + G12;
-    const float rg = __builtin_amdgcn_rsqf(S0 + S1 + S2);
-    const float bg = fminf(fmaf(S0, rg, T0), fminf(fmaf(S1, rg, T1), fmaf(S2, rg, T2)));
-    return fmaxf(fmaxf(b0, b1), fmaxf(b2, bg));
+    return fmaxf(fmaxf(b0, b1), b2);
   }
   __device__ static int plane_mask(const T* Lt, const T* v, int mask, float scale) {
     float L[6];

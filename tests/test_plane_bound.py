@@ -31,10 +31,7 @@ def _corner_bound(T, qk, q, n, r, wrong_sign=False):
     b0 = min(T[0] + n[0], T[1] + G01 * r[0], T[2] + G02 * r[0])
     b1 = min(T[1] + n[1], T[0] + G01 * r[1], T[2] + G12 * r[1])
     b2 = min(T[2] + n[2], T[0] + G02 * r[2], T[1] + G12 * r[2])
-    S = [qk[0] + G01 + G02, qk[1] + G01 + G12, qk[2] + G02 + G12]
-    rg = f(1.0) / np.sqrt(f(S[0] + S[1] + S[2]))
-    bg = min(T[0] + S[0] * rg, T[1] + S[1] * rg, T[2] + S[2] * rg)
-    return f(max(b0, b1, b2, bg))
+    return f(max(b0, b1, b2))
 
 
 def plane_bounds(L, v, wrong_sign=False):

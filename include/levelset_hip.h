@@ -42,8 +42,8 @@ enum { LS_EXEC_PERSISTENT = 0, LS_EXEC_KERNELS = 1, LS_EXEC_GRAPH = 2 };
  * LS_SWEEP_UNSTAGED into the mode makes every sweep read them from memory
  * (same results). Tet corners whose solve cannot lower their value are
  * skipped by an exact lower bound: the height bound, and by default also a
- * supporting-plane bound that removes about a third of the remaining solves
- * on the torus; LS_SWEEP_NO_PLANE_BOUND keeps the height bound only (same
+ * supporting-plane bound that removes about a quarter of the remaining
+ * solves on the torus; LS_SWEEP_NO_PLANE_BOUND keeps the height bound only (same
  * results, more solves). */
 enum { LS_SWEEP_AUTO = 0, LS_SWEEP_ELEMENTS = 1, LS_SWEEP_TASKS = 2, LS_SWEEP_UNSTAGED = 4,
        LS_SWEEP_NO_PLANE_BOUND = 8 };

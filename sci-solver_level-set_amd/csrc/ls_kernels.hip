@@ -1242,31 +1242,43 @@ __device__ SweepResult<T> sweep_patch(const DevMesh<T>& m, DevState<T>& s, int p
           n_solve += c;
           ++n_upd;
         }
+        // The wave's counts per solve number c = 1..NC, 16 bits each (a
+        // patch has far fewer than 65536 element copies), in one atomic.
+        unsigned long long pk = 0;
 #pragma unroll
-        for (int j = 1; j <= NC; ++j) {
-          const unsigned long long b = __ballot(c == j);
-          if ((tid & 63) == 0 && b) atomicAdd(&s_ndirty[1 + j], __popcll(b));
-        }
+        for (int j = 1; j <= NC; ++j) pk |= (unsigned long long)__popcll(__ballot(c == j)) << (16 * (j - 1));
+        if ((tid & 63) == 0 && pk) atomicAdd(reinterpret_cast<unsigned long long*>(s_ndirty + 2), pk);
       }
       __syncthreads();
+      const unsigned long long cnt = *reinterpret_cast<const unsigned long long*>(s_ndirty + 2);
       int start[NC + 2];
       start[NC] = 0;
 #pragma unroll
-      for (int j = NC - 1; j >= 1; --j) start[j] = start[j + 1] + s_ndirty[2 + j];
-      const int nd = start[1] + s_ndirty[2];
+      for (int j = NC - 1; j >= 1; --j) start[j] = start[j + 1] + (int)((cnt >> (16 * j)) & 0xFFFFull);
+      const int nd = start[1] + (int)(cnt & 0xFFFFull);
       for (int base = 0; base < ne; base += BS) {
         const int e = base + tid;
         const int c = e < ne ? __popc(s_dirty[e] >> 4) : 0;
         const int lane = tid & 63;
+        // Cursors per solve number, packed like the counts: one atomic per wave.
+        unsigned long long bj[NC];
+        unsigned long long pc = 0;
 #pragma unroll
         for (int j = 1; j <= NC; ++j) {
-          const unsigned long long b = __ballot(c == j);
-          if (!b) continue;
-          int off = 0;
-          if (lane == 0) off = atomicAdd(&s_ndirty[5 + j], __popcll(b));
-          off = __builtin_amdgcn_readlane(off, 0);
-          if (c == j) s_work[start[j] + off + __popcll(b & ((1ull << lane) - 1ull))] = (uint16_t)e;
+          bj[j - 1] = __ballot(c == j);
+          pc |= (unsigned long long)__popcll(bj[j - 1]) << (16 * (j - 1));
         }
+        unsigned long long old = 0;
+        if (lane == 0 && pc) old = atomicAdd(reinterpret_cast<unsigned long long*>(s_ndirty + 6), pc);
+        const unsigned int olo = __builtin_amdgcn_readlane((unsigned int)old, 0);
+        const unsigned int ohi = __builtin_amdgcn_readlane((unsigned int)(old >> 32), 0);
+        old = ((unsigned long long)ohi << 32) | olo;
+#pragma unroll
+        for (int j = 1; j <= NC; ++j)
+          if (c == j) {
+            const int off = (int)((old >> (16 * (j - 1))) & 0xFFFFull);
+            s_work[start[j] + off + __popcll(bj[j - 1] & ((1ull << lane) - 1ull))] = (uint16_t)e;
+          }
       }
       __syncthreads();
       for (int k = tid; k < nd; k += BS) {
@@ -1961,7 +1973,7 @@ __global__ __launch_bounds__(BS, 4) void k_sweep(DevMesh<T> m, DevState<T> s, co
                                               int epoch_out, int stage) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ T red[BS / 64];
-  __shared__ int s_ndirty[10];
+  __shared__ __align__(8) int s_ndirty[10];
   __shared__ unsigned long long s_acc[kNumAcc];
   clear_stats(s_acc);  // sweep_patch starts with a barrier
   sweep_patch<T, NC, BS, ORG>(m, s, list[blockIdx.x], niter, write_out, tol, leps, smem, red, s_ndirty, epoch_in,
@@ -2511,7 +2523,7 @@ template <typename T, int NC, int BS, int ORG, bool WARM, int WPE, int STG = -1>
 __global__ __launch_bounds__(BS, WPE) void k_step(DevMesh<T> m, DevState<T> s, StepArgs<T> a) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ T red[BS / 64];
-  __shared__ int s_ndirty[10];
+  __shared__ __align__(8) int s_ndirty[10];
   __shared__ int s_work[2];  // work draws: item, reserved second item
   __shared__ double cred[BS / 64];  // CFL reductions
   __shared__ unsigned long long s_acc[kNumAcc];  // statistics, flushed at the end
